@@ -1,0 +1,22 @@
+"""caustic-tracer hot path on MI355X (gfx950).
+
+The public names mirror the reference's barrel (src/tracer/index.ts:1-20) for
+the path this package accelerates: packScene / buildBvh (pack.ts, bvh.ts),
+intersectClosest / occluded (intersect.ts), accumulatePass / renderImage
+(tracer.ts) and cameraFrame (camera.ts).  All tracing and the BVH build run in
+HIP kernels through libcaustic_amd.so; importing works without a GPU, and any
+call that needs the library fails loudly when it is missing.
+"""
+from . import synth
+from ._capi import CTError, DeviceBuffer, init, sync
+from .pack import PackedScene, buildBvh, packScene, primBounds
+from .tracer import (T_MIN, WALK_ORDERED, WALK_STACKLESS, DeviceScene, Tracer, accumulatePass,
+                     cameraFrame, closest_hit, closest_hit_counts, intersectClosest, occluded,
+                     renderImage)
+
+__all__ = [
+    "synth", "CTError", "DeviceBuffer", "init", "sync", "PackedScene", "buildBvh", "packScene",
+    "primBounds", "T_MIN", "WALK_ORDERED", "WALK_STACKLESS", "DeviceScene", "Tracer",
+    "accumulatePass", "cameraFrame", "closest_hit", "closest_hit_counts", "intersectClosest",
+    "occluded", "renderImage",
+]

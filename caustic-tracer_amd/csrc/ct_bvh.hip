@@ -1,0 +1,931 @@
+// GPU binned-SAH BVH builder, bit-identical to the reference's recursive
+// builder (bvh.ts:72-301).
+//
+// The reference builds depth first: buildNode(start,end) reduces the node's
+// FP64 bounds, makes a leaf at <= 4 prims, else bins centroids into 12 buckets
+// per axis, sweeps the SAH cost, partitions order[start,end) in place with a
+// two-pointer loop (or stable-sorts it when every centroid axis is
+// degenerate) and recurses left then right, numbering nodes in DFS pre-order.
+//
+// Here the same tree is built breadth first, one level per round of kernels,
+// over node records that are kept in position order:
+//   bounds     segmented min/max of prim and centroid bounds (order-independent,
+//              exact through sortable-integer atomics), wave/LDS aggregated
+//   setup      leaf / SAH / median decision per node, bucket scale per axis
+//   bin        LDS histograms per (node, axis, bucket) for the first nodes a
+//              block touches, global atomics for the rest
+//   sweep      the reference's suffix/prefix sweeps per node (one lane per node)
+//   partition  the permutation of the two-pointer loop from prefix counts:
+//              L-goods stay, the k-th R-good fills the k-th L-bad's slot, the
+//              k-th L-bad lands at end-2-b_{k-1}, R-bads move one slot inward,
+//              up to the meeting point K = max{k : a_k + b_k + 2 <= n}
+//   median     stable bitonic sort by (centroid, position) in LDS
+//   children   child records appended in parent order (exclusive scan)
+// After the last level, subtree sizes (bottom-up) give DFS indices and skip
+// links (top-down), and each record is written as one 8-float node.
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "ct_internal.h"
+
+namespace ct {
+
+constexpr int BB = 256;          // block size of position kernels
+constexpr int CHUNK = BB * 8;    // positions per block in the aggregated kernels
+constexpr int LSLOTS = 4;        // nodes per block aggregated in LDS
+constexpr int MAX_BINS = 32;
+constexpr int MEDIAN_MAX = 2048;  // largest all-degenerate node the LDS sort takes
+
+enum : int32_t { KIND_LEAF = 0, KIND_SAH = 1, KIND_MEDIAN = 2 };
+
+// Sortable 64-bit image of a double: unsigned order == numeric order, -0 < +0.
+__device__ __forceinline__ unsigned long long dkey(double x) {
+  unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double dval(unsigned long long k) {
+  unsigned long long u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __longlong_as_double((long long)u);
+}
+#define KEY_POS_INF 0xfff0000000000000ull  // dkey(+inf)
+#define KEY_NEG_INF 0x000fffffffffffffull  // dkey(-inf)
+
+struct Build {
+  // inputs
+  const double* bmin;
+  const double* bmax;
+  const double* bcen;
+  uint32_t n;
+  int bins;
+  // position arrays
+  uint32_t* order;      // current
+  uint32_t* orderNext;  // next level
+  int32_t* nodeOf;      // record of each position, -1 once in a finished leaf
+  uint32_t* good;       // partition flag per position (0/1)
+  uint32_t* goodScan;   // exclusive scan of good, n+1 entries
+  uint32_t* A;          // per node: positions of bads from the left (relative)
+  uint32_t* B;          // per node: R-indices of goods from the right
+  // records (capacity 2n)
+  uint32_t* recStart;
+  uint32_t* recEnd;
+  int32_t* recLeft;
+  int32_t* recRight;
+  double* recBox;       // 6 per record: min.xyz, max.xyz (un-nudged)
+  uint32_t* recSize;
+  uint32_t* recIdx;
+  // per active node of the current level (local index j)
+  unsigned long long* keys;  // 12 per node: bmin3 bmax3 cmin3 cmax3
+  int32_t* kind;
+  int32_t* axis;             // SAH best axis / median axis
+  int32_t* split;            // SAH best split bucket
+  double* origin;            // SAH bestOrigin (cmin of axis)
+  double* scale;             // SAH bestScale (bins / extent)
+  double* cmin3;             // 3 per node
+  double* tob3;              // 3 per node: bins/extent or 0 when the axis is skipped
+  uint32_t* flag;            // scan input (several uses)
+  uint32_t* flagScan;        // scan output (m+1)
+  uint32_t* binSlot;         // scan of SAH flags
+  uint32_t* mid;             // split position per node
+  uint32_t* meetX;           // L-read count of the partition per node
+  // bins per SAH node: 3 * bins * (count + 6 keys)
+  uint32_t* binCount;
+  unsigned long long* binKey;
+  uint32_t* levelInfo;       // [0] nSplit [1] nMedian [2] nSah
+  uint32_t* scanTmp;         // block sums for scans
+};
+
+// ---------------------------------------------------------------------------
+// exclusive scan of uint32 (out has n+1 entries; out[n] = total)
+
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+
+__global__ void __launch_bounds__(SCAN_BLOCK)
+k_scan_tile(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n,
+            uint32_t* __restrict__ tileSums) {
+  __shared__ uint32_t s[SCAN_BLOCK];
+  uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  uint32_t v[SCAN_ITEMS];
+  uint32_t sum = 0;
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    uint32_t i = base + k;
+    v[k] = i < n ? in[i] : 0;
+    sum += v[k];
+  }
+  s[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 1; off < SCAN_BLOCK; off <<= 1) {
+    uint32_t t = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  uint32_t run = s[threadIdx.x] - sum;  // exclusive prefix of this thread
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    uint32_t i = base + k;
+    if (i < n) out[i] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == SCAN_BLOCK - 1) tileSums[blockIdx.x] = s[SCAN_BLOCK - 1];
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK)
+k_scan_add(uint32_t* __restrict__ out, uint32_t n, const uint32_t* __restrict__ tileOffsets) {
+  uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  uint32_t add = tileOffsets[blockIdx.x];
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    uint32_t i = base + k;
+    if (i < n) out[i] += add;
+  }
+}
+
+__global__ void k_store_u32(uint32_t* p, uint32_t v) { *p = v; }
+
+__global__ void k_scan_total(uint32_t* out, uint32_t n, const uint32_t* in) {
+  // out[n] = out[n-1] + in[n-1]  (n >= 1)
+  out[n] = out[n - 1] + in[n - 1];
+}
+
+// out[0..n] = exclusive scan of in[0..n-1]; tmp needs room for the tile sums of
+// every recursion level (about n / 1000 entries).
+static int scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp,
+                    hipStream_t st) {
+  if (n == 0) {
+    hipLaunchKernelGGL(k_store_u32, dim3(1), dim3(1), 0, st, out, 0u);
+    CT_LAUNCH_CHECK("k_store_u32");
+    return CT_OK;
+  }
+  uint32_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  hipLaunchKernelGGL(k_scan_tile, dim3(tiles), dim3(SCAN_BLOCK), 0, st, in, out, n, tmp);
+  CT_LAUNCH_CHECK("k_scan_tile");
+  if (tiles > 1) {
+    uint32_t* offs = tmp + tiles;  // tiles+1 entries
+    int rc = scan_u32(tmp, offs, tiles, offs + tiles + 1, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_scan_add, dim3(tiles), dim3(SCAN_BLOCK), 0, st, out, n, offs);
+    CT_LAUNCH_CHECK("k_scan_add");
+  }
+  hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(1), 0, st, out, n, in);
+  CT_LAUNCH_CHECK("k_scan_total");
+  return CT_OK;
+}
+
+static size_t scan_tmp_entries(size_t n) {
+  size_t total = 0;
+  while (n > 1) {
+    size_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    total += 2 * tiles + 2;
+    n = tiles;
+  }
+  return total + 4;
+}
+
+// ---------------------------------------------------------------------------
+// level kernels
+
+__global__ void k_init_root(Build b) {
+  b.recStart[0] = 0;
+  b.recEnd[0] = b.n;
+  b.recLeft[0] = -1;
+  b.recRight[0] = -1;
+}
+
+__global__ void k_init_positions(Build b) {
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += stride) {
+    b.order[i] = i;
+    b.nodeOf[i] = 0;
+  }
+}
+
+__global__ void k_level_init(Build b, uint32_t m) {
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += stride) {
+    for (int k = 0; k < 12; k++)
+      b.keys[12 * (size_t)j + k] = (k < 3 || (k >= 6 && k < 9)) ? KEY_POS_INF : KEY_NEG_INF;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 2) b.levelInfo[threadIdx.x] = 0;
+}
+
+// First active local node index in this block's chunk (or m if none).
+__device__ uint32_t block_first_node(Build& b, uint32_t c0, uint32_t lvStart, uint32_t m,
+                                     uint32_t* sFirst) {
+  if (threadIdx.x == 0) *sFirst = m;
+  __syncthreads();
+  uint32_t mine = m;
+  for (uint32_t i = c0 + threadIdx.x; i < c0 + CHUNK && i < b.n; i += BB) {
+    int32_t r = b.nodeOf[i];
+    if (r >= 0) {
+      mine = (uint32_t)r - lvStart;
+      break;
+    }
+  }
+  if (mine < m) atomicMin(sFirst, mine);
+  __syncthreads();
+  return *sFirst;
+}
+
+// Bounds of every active node: box of prim bounds and box of centroids.
+__global__ void __launch_bounds__(BB) k_bounds(Build b, uint32_t lvStart, uint32_t m) {
+  __shared__ unsigned long long sKeys[LSLOTS * 12];
+  __shared__ uint32_t sFirst;
+  uint32_t c0 = blockIdx.x * CHUNK;
+  for (int k = threadIdx.x; k < LSLOTS * 12; k += BB) {
+    int f = k % 12;
+    sKeys[k] = (f < 3 || (f >= 6 && f < 9)) ? KEY_POS_INF : KEY_NEG_INF;
+  }
+  uint32_t j0 = block_first_node(b, c0, lvStart, m, &sFirst);
+  for (uint32_t i = c0 + threadIdx.x; i < c0 + CHUNK && i < b.n; i += BB) {
+    int32_t r = b.nodeOf[i];
+    if (r < 0) continue;
+    uint32_t j = (uint32_t)r - lvStart;
+    size_t o = 3 * (size_t)b.order[i];
+    unsigned long long v[12];
+    for (int k = 0; k < 3; k++) {
+      v[k] = dkey(b.bmin[o + k]);
+      v[3 + k] = dkey(b.bmax[o + k]);
+      v[6 + k] = dkey(b.bcen[o + k]);
+      v[9 + k] = v[6 + k];
+    }
+    if (j - j0 < (uint32_t)LSLOTS) {
+      unsigned long long* s = sKeys + 12 * (j - j0);
+      for (int k = 0; k < 3; k++) {
+        atomicMin(s + k, v[k]);
+        atomicMax(s + 3 + k, v[3 + k]);
+        atomicMin(s + 6 + k, v[6 + k]);
+        atomicMax(s + 9 + k, v[9 + k]);
+      }
+    } else {
+      unsigned long long* g = b.keys + 12 * (size_t)j;
+      for (int k = 0; k < 3; k++) {
+        atomicMin(g + k, v[k]);
+        atomicMax(g + 3 + k, v[3 + k]);
+        atomicMin(g + 6 + k, v[6 + k]);
+        atomicMax(g + 9 + k, v[9 + k]);
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < LSLOTS * 12; k += BB) {
+    uint32_t j = j0 + k / 12;
+    if (j >= m) continue;
+    int f = k % 12;
+    unsigned long long x = sKeys[k];
+    bool isMin = f < 3 || (f >= 6 && f < 9);
+    if (x == (isMin ? KEY_POS_INF : KEY_NEG_INF)) continue;
+    unsigned long long* g = b.keys + 12 * (size_t)j + f;
+    if (isMin) atomicMin(g, x);
+    else atomicMax(g, x);
+  }
+}
+
+// Leaf / SAH / median decision (bvh.ts:149-160, 187-192, 284-289).
+__global__ void k_setup(Build b, uint32_t lvStart, uint32_t m) {
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += stride) {
+    uint32_t rec = lvStart + j;
+    const unsigned long long* kk = b.keys + 12 * (size_t)j;
+    for (int k = 0; k < 6; k++) b.recBox[6 * (size_t)rec + k] = dval(kk[k]);
+    uint32_t count = b.recEnd[rec] - b.recStart[rec];
+    int32_t kd;
+    if (count <= 4) {
+      kd = KIND_LEAF;
+    } else {
+      bool any = false;
+      for (int a = 0; a < 3; a++) {
+        double cmin = dval(kk[6 + a]), cmax = dval(kk[9 + a]);
+        double extent = cmax - cmin;
+        b.cmin3[3 * (size_t)j + a] = cmin;
+        if (extent > 1e-12) {
+          b.tob3[3 * (size_t)j + a] = b.bins / extent;
+          any = true;
+        } else {
+          b.tob3[3 * (size_t)j + a] = 0;
+        }
+      }
+      if (any) {
+        kd = KIND_SAH;
+      } else {
+        kd = KIND_MEDIAN;
+        double ex = dval(kk[9]) - dval(kk[6]);
+        double ey = dval(kk[10]) - dval(kk[7]);
+        double ez = dval(kk[11]) - dval(kk[8]);
+        b.axis[j] = ex >= ey && ex >= ez ? 0 : ey >= ez ? 1 : 2;
+        atomicAdd(b.levelInfo + 1, 1u);
+      }
+    }
+    b.kind[j] = kd;
+    b.flag[j] = kd == KIND_SAH ? 1u : 0u;
+  }
+}
+
+__global__ void k_bin_init(Build b, uint32_t m) {
+  uint32_t stride = gridDim.x * blockDim.x;
+  const int per = 3 * b.bins;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += stride) {
+    if (b.kind[j] != KIND_SAH) continue;
+    size_t s = (size_t)b.binSlot[j] * per;
+    for (int q = 0; q < per; q++) {
+      b.binCount[s + q] = 0;
+      for (int k = 0; k < 3; k++) {
+        b.binKey[6 * (s + q) + k] = KEY_POS_INF;
+        b.binKey[6 * (s + q) + 3 + k] = KEY_NEG_INF;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ int bucket_of(double c, double cmin, double tob, int bins) {
+  double fb = floor((c - cmin) * tob);  // bvh.ts:200-202
+  if (fb >= bins) return bins - 1;
+  if (fb < 0) return 0;
+  return (int)fb;
+}
+
+// SAH bucket histograms (bvh.ts:198-211).
+__global__ void __launch_bounds__(BB) k_bin(Build b, uint32_t lvStart, uint32_t m) {
+  __shared__ uint32_t sCount[LSLOTS * 3 * MAX_BINS];
+  __shared__ unsigned long long sKey[LSLOTS * 3 * MAX_BINS * 6];
+  __shared__ uint32_t sFirst;
+  const int bins = b.bins;
+  const int per = 3 * bins;
+  uint32_t c0 = blockIdx.x * CHUNK;
+  for (int k = threadIdx.x; k < LSLOTS * per; k += BB) {
+    sCount[k] = 0;
+    for (int f = 0; f < 3; f++) {
+      sKey[6 * k + f] = KEY_POS_INF;
+      sKey[6 * k + 3 + f] = KEY_NEG_INF;
+    }
+  }
+  uint32_t j0 = block_first_node(b, c0, lvStart, m, &sFirst);
+  for (uint32_t i = c0 + threadIdx.x; i < c0 + CHUNK && i < b.n; i += BB) {
+    int32_t r = b.nodeOf[i];
+    if (r < 0) continue;
+    uint32_t j = (uint32_t)r - lvStart;
+    if (b.kind[j] != KIND_SAH) continue;
+    size_t o = 3 * (size_t)b.order[i];
+    unsigned long long mn[3], mx[3];
+    for (int k = 0; k < 3; k++) {
+      mn[k] = dkey(b.bmin[o + k]);
+      mx[k] = dkey(b.bmax[o + k]);
+    }
+    bool local = j - j0 < (uint32_t)LSLOTS;
+    size_t gs = (size_t)b.binSlot[j] * per;
+    for (int a = 0; a < 3; a++) {
+      double tob = b.tob3[3 * (size_t)j + a];
+      if (tob == 0) continue;
+      int bk = bucket_of(b.bcen[o + a], b.cmin3[3 * (size_t)j + a], tob, bins);
+      if (local) {
+        int q = (int)(j - j0) * per + a * bins + bk;
+        atomicAdd(sCount + q, 1u);
+        for (int k = 0; k < 3; k++) {
+          atomicMin(sKey + 6 * q + k, mn[k]);
+          atomicMax(sKey + 6 * q + 3 + k, mx[k]);
+        }
+      } else {
+        size_t q = gs + a * bins + bk;
+        atomicAdd(b.binCount + q, 1u);
+        for (int k = 0; k < 3; k++) {
+          atomicMin(b.binKey + 6 * q + k, mn[k]);
+          atomicMax(b.binKey + 6 * q + 3 + k, mx[k]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < LSLOTS * per; k += BB) {
+    uint32_t c = sCount[k];
+    if (c == 0) continue;
+    uint32_t j = j0 + k / per;
+    if (j >= m || b.kind[j] != KIND_SAH) continue;
+    size_t q = (size_t)b.binSlot[j] * per + (k % per);
+    atomicAdd(b.binCount + q, c);
+    for (int f = 0; f < 3; f++) {
+      atomicMin(b.binKey + 6 * q + f, sKey[6 * k + f]);
+      atomicMax(b.binKey + 6 * q + 3 + f, sKey[6 * k + 3 + f]);
+    }
+  }
+}
+
+__device__ __forceinline__ double surface_area(double dx, double dy, double dz) {
+  return 2 * (dx * dy + dy * dz + dz * dx);  // bvh.ts:70
+}
+
+// Suffix/prefix sweeps and the strict-'<' argmin (bvh.ts:213-257).
+__global__ void k_sweep(Build b, uint32_t m) {
+  uint32_t stride = gridDim.x * blockDim.x;
+  const int bins = b.bins;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += stride) {
+    if (b.kind[j] != KIND_SAH) continue;
+    size_t base = (size_t)b.binSlot[j] * 3 * bins;
+    double bestCost = __builtin_inf();
+    int bestAxis = -1, bestSplit = -1;
+    double bestOrigin = 0, bestScale = 0;
+    double rightArea[MAX_BINS];
+    uint32_t rightCount[MAX_BINS];
+    for (int a = 0; a < 3; a++) {
+      double tob = b.tob3[3 * (size_t)j + a];
+      if (tob == 0) continue;
+      const uint32_t* cnt = b.binCount + base + a * bins;
+      const unsigned long long* key = b.binKey + 6 * (base + a * bins);
+      double rx0 = __builtin_inf(), ry0 = rx0, rz0 = rx0;
+      double rx1 = -__builtin_inf(), ry1 = rx1, rz1 = rx1;
+      uint32_t rc = 0;
+      for (int s = bins - 1; s >= 1; s--) {
+        if (cnt[s] > 0) {
+          double v;
+          v = dval(key[6 * s + 0]); if (v < rx0) rx0 = v;
+          v = dval(key[6 * s + 1]); if (v < ry0) ry0 = v;
+          v = dval(key[6 * s + 2]); if (v < rz0) rz0 = v;
+          v = dval(key[6 * s + 3]); if (v > rx1) rx1 = v;
+          v = dval(key[6 * s + 4]); if (v > ry1) ry1 = v;
+          v = dval(key[6 * s + 5]); if (v > rz1) rz1 = v;
+          rc += cnt[s];
+        }
+        rightCount[s - 1] = rc;
+        rightArea[s - 1] = rc > 0 ? surface_area(rx1 - rx0, ry1 - ry0, rz1 - rz0) : 0;
+      }
+      double lx0 = __builtin_inf(), ly0 = lx0, lz0 = lx0;
+      double lx1 = -__builtin_inf(), ly1 = lx1, lz1 = lx1;
+      uint32_t lc = 0;
+      for (int s = 0; s < bins - 1; s++) {
+        if (cnt[s] > 0) {
+          double v;
+          v = dval(key[6 * s + 0]); if (v < lx0) lx0 = v;
+          v = dval(key[6 * s + 1]); if (v < ly0) ly0 = v;
+          v = dval(key[6 * s + 2]); if (v < lz0) lz0 = v;
+          v = dval(key[6 * s + 3]); if (v > lx1) lx1 = v;
+          v = dval(key[6 * s + 4]); if (v > ly1) ly1 = v;
+          v = dval(key[6 * s + 5]); if (v > lz1) lz1 = v;
+          lc += cnt[s];
+        }
+        uint32_t rcs = rightCount[s];
+        if (lc == 0 || rcs == 0) continue;
+        double cost = (double)lc * surface_area(lx1 - lx0, ly1 - ly0, lz1 - lz0) +
+                      (double)rcs * rightArea[s];
+        if (cost < bestCost) {
+          bestCost = cost;
+          bestAxis = a;
+          bestSplit = s;
+          bestOrigin = b.cmin3[3 * (size_t)j + a];
+          bestScale = tob;
+        }
+      }
+    }
+    if (bestAxis < 0) {
+      // no split with both sides non-empty: fall back to the median split
+      const unsigned long long* kk = b.keys + 12 * (size_t)j;
+      double ex = dval(kk[9]) - dval(kk[6]);
+      double ey = dval(kk[10]) - dval(kk[7]);
+      double ez = dval(kk[11]) - dval(kk[8]);
+      b.axis[j] = ex >= ey && ex >= ez ? 0 : ey >= ez ? 1 : 2;
+      b.kind[j] = KIND_MEDIAN;
+      atomicAdd(b.levelInfo + 1, 1u);
+    } else {
+      b.axis[j] = bestAxis;
+      b.split[j] = bestSplit;
+      b.origin[j] = bestOrigin;
+      b.scale[j] = bestScale;
+    }
+  }
+}
+
+// Partition flag of each position of a SAH node (bvh.ts:264-269).
+__global__ void k_flags(Build b, uint32_t lvStart) {
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += stride) {
+    uint32_t g = 0;
+    int32_t r = b.nodeOf[i];
+    if (r >= 0) {
+      uint32_t j = (uint32_t)r - lvStart;
+      if (b.kind[j] == KIND_SAH) {
+        double c = b.bcen[3 * (size_t)b.order[i] + b.axis[j]];
+        int bk = bucket_of(c, b.origin[j], b.scale[j], b.bins);
+        g = bk <= b.split[j] ? 1u : 0u;
+      }
+    }
+    b.good[i] = g;
+  }
+}
+
+// a_k (bads from the left) and b_k (goods from the right) per node.
+__global__ void k_ab(Build b, uint32_t lvStart) {
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += stride) {
+    int32_t r = b.nodeOf[i];
+    if (r < 0) continue;
+    uint32_t j = (uint32_t)r - lvStart;
+    if (b.kind[j] != KIND_SAH) continue;
+    uint32_t start = b.recStart[r], end = b.recEnd[r];
+    uint32_t rel = i - start;
+    uint32_t gBefore = b.goodScan[i] - b.goodScan[start];
+    uint32_t gTotal = b.goodScan[end] - b.goodScan[start];
+    if (b.good[i]) {
+      uint32_t gAfter = gTotal - gBefore - 1;
+      if (gAfter < end - start) b.B[start + gAfter] = (end - 1) - i;
+    } else {
+      uint32_t badBefore = rel - gBefore;
+      if (badBefore < end - start) b.A[start + badBefore] = rel;
+    }
+  }
+}
+
+// Meeting point of the two-pointer loop per SAH node.
+__global__ void k_meet(Build b, uint32_t lvStart, uint32_t m) {
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += stride) {
+    if (b.kind[j] != KIND_SAH) continue;
+    uint32_t rec = lvStart + j;
+    uint32_t start = b.recStart[rec], end = b.recEnd[rec];
+    uint32_t n = end - start;
+    uint32_t nGood = b.goodScan[end] - b.goodScan[start];
+    uint32_t nBad = n - nGood;
+    const uint32_t* A = b.A + start;
+    const uint32_t* B = b.B + start;
+    // K = number of complete (L-bad, R-good) pairs: max k with a_k + b_k + 2 <= n
+    uint32_t lo = 0, hi = nBad < nGood ? nBad : nGood;
+    while (lo < hi) {
+      uint32_t mk = (lo + hi + 1) >> 1;  // test k = mk (1-based)
+      if ((uint64_t)A[mk - 1] + B[mk - 1] + 2 <= n) lo = mk;
+      else hi = mk - 1;
+    }
+    uint32_t K = lo;
+    int64_t bK = K > 0 ? (int64_t)B[K - 1] : -1;
+    uint32_t x;
+    if (K < nBad && (int64_t)A[K] + bK + 2 <= (int64_t)n) x = A[K] + 1;
+    else x = (uint32_t)((int64_t)n - (bK + 1));
+    b.meetX[j] = x;
+    b.mid[j] = start + nGood;
+  }
+}
+
+// Scatter every position to its place after the partition (SAH nodes) or keep it.
+__global__ void k_scatter(Build b, uint32_t lvStart) {
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += stride) {
+    int32_t r = b.nodeOf[i];
+    uint32_t dst = i;
+    if (r >= 0) {
+      uint32_t j = (uint32_t)r - lvStart;
+      int32_t kd = b.kind[j];
+      if (kd == KIND_MEDIAN) continue;  // written by k_median
+      if (kd == KIND_SAH) {
+        uint32_t start = b.recStart[r], end = b.recEnd[r];
+        uint32_t n = end - start, rel = i - start;
+        uint32_t gBefore = b.goodScan[i] - b.goodScan[start];
+        uint32_t gTotal = b.goodScan[end] - b.goodScan[start];
+        bool g = b.good[i] != 0;
+        uint32_t x = b.meetX[j];
+        uint32_t d;
+        if (rel < x) {
+          if (g) {
+            d = rel;
+          } else {
+            uint32_t k = rel - gBefore;  // rank among L-bads
+            int64_t bprev = k > 0 ? (int64_t)b.B[start + k - 1] : -1;
+            d = (uint32_t)((int64_t)n - 2 - bprev);
+          }
+        } else {
+          uint32_t rr = n - 1 - rel;  // R-index
+          if (g) {
+            uint32_t k = gTotal - gBefore - 1;  // rank among R-goods
+            d = b.A[start + k];
+          } else {
+            d = n - 2 - rr;
+          }
+        }
+        if (d >= n) {  // cannot happen for a consistent partition; never write out of range
+          atomicOr(b.levelInfo + 2, 1u);
+          continue;
+        }
+        dst = start + d;
+      }
+    }
+    b.orderNext[dst] = b.order[i];
+  }
+}
+
+// Stable sort of an all-degenerate node by centroid (bvh.ts:284-296): bitonic
+// sort of (centroid, position) pairs, ties keep position order.
+__global__ void __launch_bounds__(BB) k_median(Build b, uint32_t lvStart, uint32_t m) {
+  __shared__ double sKey[MEDIAN_MAX];
+  __shared__ uint32_t sPos[MEDIAN_MAX];
+  for (uint32_t j = blockIdx.x; j < m; j += gridDim.x) {
+    if (b.kind[j] != KIND_MEDIAN) continue;
+    uint32_t rec = lvStart + j;
+    uint32_t start = b.recStart[rec], end = b.recEnd[rec];
+    uint32_t n = end - start;
+    int axis = b.axis[j];
+    uint32_t P2 = 1;
+    while (P2 < n) P2 <<= 1;
+    for (uint32_t t = threadIdx.x; t < P2; t += BB) {
+      if (t < n) {
+        sKey[t] = b.bcen[3 * (size_t)b.order[start + t] + axis];
+        sPos[t] = t;
+      } else {
+        sKey[t] = __builtin_inf();
+        sPos[t] = 0xffffffffu;
+      }
+    }
+    __syncthreads();
+    for (uint32_t k = 2; k <= P2; k <<= 1) {
+      for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+        for (uint32_t t = threadIdx.x; t < P2; t += BB) {
+          uint32_t u = t ^ jj;
+          if (u > t) {
+            double ka = sKey[t], kb = sKey[u];
+            uint32_t pa = sPos[t], pb = sPos[u];
+            bool aFirst = (ka < kb) || (!(kb < ka) && pa < pb);
+            bool up = (t & k) == 0;
+            if (up != aFirst) {
+              sKey[t] = kb; sKey[u] = ka;
+              sPos[t] = pb; sPos[u] = pa;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (uint32_t t = threadIdx.x; t < n; t += BB)
+      b.orderNext[start + t] = b.order[start + sPos[t]];
+    if (threadIdx.x == 0) b.mid[j] = (start + end) >> 1;
+    __syncthreads();
+  }
+}
+
+__global__ void k_split_flags(Build b, uint32_t m) {
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += stride)
+    b.flag[j] = b.kind[j] != KIND_LEAF ? 1u : 0u;
+}
+
+// Children of split nodes appended at lvEnd in parent order.
+__global__ void k_children(Build b, uint32_t lvStart, uint32_t lvEnd, uint32_t m) {
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += stride) {
+    uint32_t rec = lvStart + j;
+    if (b.kind[j] == KIND_LEAF) {
+      b.recLeft[rec] = -1;
+      b.recRight[rec] = -1;
+      continue;
+    }
+    uint32_t L = lvEnd + 2 * b.flagScan[j];
+    uint32_t R = L + 1;
+    uint32_t mid = b.mid[j];
+    b.recLeft[rec] = (int32_t)L;
+    b.recRight[rec] = (int32_t)R;
+    b.recStart[L] = b.recStart[rec];
+    b.recEnd[L] = mid;
+    b.recStart[R] = mid;
+    b.recEnd[R] = b.recEnd[rec];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) b.levelInfo[0] = b.flagScan[m];
+}
+
+__global__ void k_node_of(Build b, uint32_t lvStart) {
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += stride) {
+    int32_t r = b.nodeOf[i];
+    if (r < 0) continue;
+    uint32_t j = (uint32_t)r - lvStart;
+    if (b.kind[j] == KIND_LEAF) {
+      b.nodeOf[i] = -1;
+    } else {
+      b.nodeOf[i] = i < b.mid[j] ? b.recLeft[r] : b.recRight[r];
+    }
+  }
+}
+
+// subtree node counts, bottom-up per level
+__global__ void k_sizes(Build b, uint32_t r0, uint32_t r1) {
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t r = r0 + blockIdx.x * blockDim.x + threadIdx.x; r < r1; r += stride) {
+    int32_t L = b.recLeft[r];
+    b.recSize[r] = L < 0 ? 1u : 1u + b.recSize[L] + b.recSize[b.recRight[r]];
+  }
+}
+
+// DFS pre-order indices, top-down per level (bvh.ts:111-112, 161-164)
+__global__ void k_dfs(Build b, uint32_t r0, uint32_t r1) {
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t r = r0 + blockIdx.x * blockDim.x + threadIdx.x; r < r1; r += stride) {
+    int32_t L = b.recLeft[r];
+    if (L < 0) continue;
+    uint32_t idx = b.recIdx[r];
+    b.recIdx[L] = idx + 1;
+    b.recIdx[b.recRight[r]] = idx + 1 + b.recSize[L];
+  }
+}
+
+// One 8-float node per record (bvh.ts:140-164).
+__global__ void k_emit(Build b, uint32_t nrec, float* __restrict__ nodes) {
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nrec; r += stride) {
+    const double* bx = b.recBox + 6 * (size_t)r;
+    uint32_t idx = b.recIdx[r];
+    float* nd = nodes + 8 * (size_t)idx;
+    nd[0] = (float)(bx[0] - fabs(bx[0]) * 1.2e-7 - 1e-9);
+    nd[1] = (float)(bx[1] - fabs(bx[1]) * 1.2e-7 - 1e-9);
+    nd[2] = (float)(bx[2] - fabs(bx[2]) * 1.2e-7 - 1e-9);
+    nd[4] = (float)(bx[3] + fabs(bx[3]) * 1.2e-7 + 1e-9);
+    nd[5] = (float)(bx[4] + fabs(bx[4]) * 1.2e-7 + 1e-9);
+    nd[6] = (float)(bx[5] + fabs(bx[5]) * 1.2e-7 + 1e-9);
+    nd[3] = (float)(idx + b.recSize[r]);
+    if (b.recLeft[r] < 0) {
+      uint32_t start = b.recStart[r], count = b.recEnd[r] - start;
+      nd[7] = (float)((double)start * 8 + count);  // encodeLeaf (bvh.ts:60-62)
+    } else {
+      nd[7] = -1.0f;
+    }
+  }
+}
+
+__global__ void k_copy_u32(uint32_t* dst, const uint32_t* src, uint32_t n) {
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+
+static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
+
+}  // namespace ct
+
+using namespace ct;
+
+extern "C" int ct_bvh_build(const double* d_min, const double* d_max, const double* d_cen,
+                            uint32_t n, int32_t bins, float* d_nodes, uint32_t* h_nodeCount,
+                            uint32_t* d_order, void* stream) {
+  if (!d_nodes || !h_nodeCount || (n && (!d_min || !d_max || !d_cen || !d_order)))
+    return set_error(CT_EINVAL, "ct_bvh_build: null argument");
+  if (bins < 2 || bins > MAX_BINS)
+    return set_error(CT_EINVAL, "ct_bvh_build: bins must be in [2, 32]");
+  if (n > (1u << 30)) return set_error(CT_ELIMIT, "ct_bvh_build: more than 2^30 primitives");
+  hipStream_t st = (hipStream_t)stream;
+  if (n == 0) {
+    // a single empty leaf (bvh.ts:93-100)
+    float node[8] = {0, 0, 0, 1, 0, 0, 0, 0};
+    CT_HIP(hipMemcpyAsync(d_nodes, node, sizeof node, hipMemcpyHostToDevice, st));
+    CT_HIP(hipStreamSynchronize(st));
+    *h_nodeCount = 1;
+    return CT_OK;
+  }
+  const size_t N = n, NR = 2 * N;           // records: 2n-1 at most
+  const size_t M = N;                       // active nodes per level: at most n
+  const size_t NS = N / 5 + 1;              // SAH nodes per level: each holds >= 5 prims
+  const size_t scanTmp = scan_tmp_entries(N + 1);
+  size_t sz = 0;
+  auto reserve = [&](size_t bytes) { size_t o = sz; sz += al(bytes); return o; };
+  size_t oOrderB = reserve(N * 4), oNodeOf = reserve(N * 4), oGood = reserve(N * 4),
+         oGoodScan = reserve((N + 1) * 4), oA = reserve(N * 4), oB = reserve(N * 4),
+         oRecStart = reserve(NR * 4), oRecEnd = reserve(NR * 4), oRecLeft = reserve(NR * 4),
+         oRecRight = reserve(NR * 4), oRecBox = reserve(NR * 48), oRecSize = reserve(NR * 4),
+         oRecIdx = reserve(NR * 4), oKeys = reserve(M * 96), oKind = reserve(M * 4),
+         oAxis = reserve(M * 4), oSplit = reserve(M * 4), oOrigin = reserve(M * 8),
+         oScale = reserve(M * 8), oCmin3 = reserve(M * 24), oTob3 = reserve(M * 24),
+         oFlag = reserve(M * 4), oFlagScan = reserve((M + 1) * 4), oBinSlot = reserve((M + 1) * 4),
+         oMid = reserve(M * 4), oMeetX = reserve(M * 4),
+         oBinCount = reserve(NS * 3 * bins * 4), oBinKey = reserve(NS * 3 * bins * 48),
+         oLevelInfo = reserve(16), oScanTmp = reserve(scanTmp * 4);
+  char* mem = nullptr;
+  CT_HIP(hipMallocAsync((void**)&mem, sz, st));
+  Build b;
+  b.bmin = d_min;
+  b.bmax = d_max;
+  b.bcen = d_cen;
+  b.n = n;
+  b.bins = bins;
+  b.order = d_order;
+  b.orderNext = (uint32_t*)(mem + oOrderB);
+  b.nodeOf = (int32_t*)(mem + oNodeOf);
+  b.good = (uint32_t*)(mem + oGood);
+  b.goodScan = (uint32_t*)(mem + oGoodScan);
+  b.A = (uint32_t*)(mem + oA);
+  b.B = (uint32_t*)(mem + oB);
+  b.recStart = (uint32_t*)(mem + oRecStart);
+  b.recEnd = (uint32_t*)(mem + oRecEnd);
+  b.recLeft = (int32_t*)(mem + oRecLeft);
+  b.recRight = (int32_t*)(mem + oRecRight);
+  b.recBox = (double*)(mem + oRecBox);
+  b.recSize = (uint32_t*)(mem + oRecSize);
+  b.recIdx = (uint32_t*)(mem + oRecIdx);
+  b.keys = (unsigned long long*)(mem + oKeys);
+  b.kind = (int32_t*)(mem + oKind);
+  b.axis = (int32_t*)(mem + oAxis);
+  b.split = (int32_t*)(mem + oSplit);
+  b.origin = (double*)(mem + oOrigin);
+  b.scale = (double*)(mem + oScale);
+  b.cmin3 = (double*)(mem + oCmin3);
+  b.tob3 = (double*)(mem + oTob3);
+  b.flag = (uint32_t*)(mem + oFlag);
+  b.flagScan = (uint32_t*)(mem + oFlagScan);
+  b.binSlot = (uint32_t*)(mem + oBinSlot);
+  b.mid = (uint32_t*)(mem + oMid);
+  b.meetX = (uint32_t*)(mem + oMeetX);
+  b.binCount = (uint32_t*)(mem + oBinCount);
+  b.binKey = (unsigned long long*)(mem + oBinKey);
+  b.levelInfo = (uint32_t*)(mem + oLevelInfo);
+  b.scanTmp = (uint32_t*)(mem + oScanTmp);
+
+  int rc = CT_OK;
+  std::vector<std::pair<uint32_t, uint32_t>> levels;
+  auto fail = [&](int code) {
+    (void)hipFreeAsync(mem, st);
+    (void)hipStreamSynchronize(st);
+    return code;
+  };
+#define BVH_LAUNCH(kern, grid, block, ...)                              \
+  do {                                                                  \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, __VA_ARGS__); \
+    hipError_t e_ = hipGetLastError();                                  \
+    if (e_ != hipSuccess) return fail(check_hip(e_, #kern));            \
+  } while (0)
+
+  const unsigned gN = grid_for(N, BB);
+  const unsigned gChunks = (unsigned)((N + CHUNK - 1) / CHUNK);
+  BVH_LAUNCH(k_init_root, 1, 1, b);
+  BVH_LAUNCH(k_set_u32, 1, 1, b.levelInfo + 2, 0u);
+  BVH_LAUNCH(k_init_positions, gN, BB, b);
+  uint32_t lvStart = 0, lvEnd = 1;
+  while (lvEnd > lvStart) {
+    uint32_t m = lvEnd - lvStart;
+    if (levels.size() >= 8192)
+      return fail(set_error(CT_ELIMIT, "ct_bvh_build: tree deeper than 8192 levels"));
+    levels.push_back({lvStart, lvEnd});
+    unsigned gM = grid_for(m, BB);
+    BVH_LAUNCH(k_level_init, gM, BB, b, m);
+    BVH_LAUNCH(k_bounds, gChunks, BB, b, lvStart, m);
+    BVH_LAUNCH(k_setup, gM, BB, b, lvStart, m);
+    rc = scan_u32(b.flag, b.binSlot, m, b.scanTmp, st);
+    if (rc) return fail(rc);
+    BVH_LAUNCH(k_bin_init, gM, BB, b, m);
+    BVH_LAUNCH(k_bin, gChunks, BB, b, lvStart, m);
+    BVH_LAUNCH(k_sweep, gM, BB, b, m);
+    BVH_LAUNCH(k_flags, gN, BB, b, lvStart);
+    rc = scan_u32(b.good, b.goodScan, n, b.scanTmp, st);
+    if (rc) return fail(rc);
+    BVH_LAUNCH(k_ab, gN, BB, b, lvStart);
+    BVH_LAUNCH(k_meet, gM, BB, b, lvStart, m);
+    BVH_LAUNCH(k_scatter, gN, BB, b, lvStart);
+    // median nodes are rare; find out whether this level has any
+    uint32_t info[3] = {0, 0, 0};
+    {
+      hipError_t e = hipMemcpyAsync(info, b.levelInfo, sizeof info, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      if (e != hipSuccess) return fail(check_hip(e, "ct_bvh_build: level info"));
+    }
+    if (info[1]) {
+      // every median node must fit the LDS sort
+      std::vector<int32_t> kinds(m);
+      std::vector<uint32_t> st0(m), en0(m);
+      hipError_t e = hipMemcpyAsync(kinds.data(), b.kind, m * 4, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipMemcpyAsync(st0.data(), b.recStart + lvStart, m * 4, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipMemcpyAsync(en0.data(), b.recEnd + lvStart, m * 4, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      if (e != hipSuccess) return fail(check_hip(e, "ct_bvh_build: median check"));
+      for (uint32_t j = 0; j < m; j++)
+        if (kinds[j] == KIND_MEDIAN && en0[j] - st0[j] > (uint32_t)MEDIAN_MAX)
+          return fail(set_error(CT_ELIMIT,
+                                "ct_bvh_build: more than 2048 primitives with coincident "
+                                "centroids in one node"));
+      unsigned gMed = (unsigned)std::min<uint32_t>(m, 4096u);
+      BVH_LAUNCH(k_median, gMed, BB, b, lvStart, m);
+    }
+    BVH_LAUNCH(k_split_flags, gM, BB, b, m);
+    rc = scan_u32(b.flag, b.flagScan, m, b.scanTmp, st);
+    if (rc) return fail(rc);
+    BVH_LAUNCH(k_children, gM, BB, b, lvStart, lvEnd, m);
+    BVH_LAUNCH(k_node_of, gN, BB, b, lvStart);
+    std::swap(b.order, b.orderNext);
+    {
+      hipError_t e = hipMemcpyAsync(info, b.levelInfo, sizeof info, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      if (e != hipSuccess) return fail(check_hip(e, "ct_bvh_build: level info"));
+    }
+    if (info[2]) return fail(set_error(CT_EDEVICE, "ct_bvh_build: inconsistent partition"));
+    uint32_t nSplit = info[0];
+    lvStart = lvEnd;
+    lvEnd = lvEnd + 2 * nSplit;
+    if (lvEnd > NR) return fail(set_error(CT_EDEVICE, "ct_bvh_build: record overflow"));
+  }
+  uint32_t nrec = lvStart;
+  // the final order must end up in the caller's buffer
+  if (b.order != d_order) BVH_LAUNCH(k_copy_u32, gN, BB, d_order, b.order, n);
+  for (size_t L = levels.size(); L-- > 0;)
+    BVH_LAUNCH(k_sizes, grid_for(levels[L].second - levels[L].first, BB), BB, b,
+               levels[L].first, levels[L].second);
+  BVH_LAUNCH(k_set_u32, 1, 1, b.recIdx, 0u);
+  for (size_t L = 0; L < levels.size(); L++)
+    BVH_LAUNCH(k_dfs, grid_for(levels[L].second - levels[L].first, BB), BB, b,
+               levels[L].first, levels[L].second);
+  BVH_LAUNCH(k_emit, grid_for(nrec, BB), BB, b, nrec, d_nodes);
+#undef BVH_LAUNCH
+  hipError_t e = hipFreeAsync(mem, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return check_hip(e, "ct_bvh_build: finish");
+  *h_nodeCount = nrec;
+  return CT_OK;
+}

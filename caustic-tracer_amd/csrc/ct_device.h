@@ -1,0 +1,385 @@
+// Device-side primitives of the gfx950 path tracer.
+//
+// Arithmetic contract: the scene stays in its packed float32 layout in HBM
+// (pack.ts:1-17, bvh.ts:1-16); every quantity derived from it is computed in
+// binary64, in the operation order of the reference (a JS engine computes in
+// doubles).  CDNA4 issues v_fma/v_mul/v_add_f64 at the same lane rate as
+// unpacked FP32, so the double-precision twin costs registers, not issue
+// slots, and closest-hit IDs, t, u, v come out bit-identical to the
+// reference's.  The library is compiled with -ffp-contract=off so no a*b+c is
+// fused; f64 division and sqrt are the correctly rounded expansions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "caustic_amd.h"
+
+namespace ct {
+
+constexpr int PRIM_STRIDE = 16;  // floats per primitive record (pack.ts:30)
+constexpr int MAT_STRIDE = 16;   // floats per material record (pack.ts:31)
+constexpr int NODE_STRIDE = 8;   // floats per BVH node (bvh.ts:18)
+constexpr double T_MIN = 1e-4;   // intersect.ts:14
+constexpr double PI_D = 3.141592653589793;
+constexpr double TWO_PI = 2 * PI_D;
+constexpr double INV_PI = 1 / PI_D;
+constexpr double SHADOW_SHORTEN = 1 - 1e-3;  // tracer.ts:36
+
+// Read-only view of a device-resident packed scene.
+struct DevScene {
+  const float4* __restrict__ prims;  // 4 float4 per primitive
+  const float4* __restrict__ nodes;  // 2 float4 per node
+  const float4* __restrict__ mats;   // 4 float4 per material
+  const int32_t* __restrict__ lights;
+  const uint32_t* __restrict__ primObject;
+  int primCount;
+  int nodeCount;
+  int lightCount;
+  int skyKind;  // 0 none, 1 gradient
+  double zenith[3], horizon[3], skyStrength;
+};
+
+// ---------------------------------------------------------------------------
+// RNG: PCG RXS-M-XS seeding + xorshift32 stream (rng.ts:10-64)
+
+__device__ __forceinline__ uint32_t pcg_hash(uint32_t input) {
+  uint32_t state = input * 747796405u + 2891336453u;
+  uint32_t shifted = (state >> ((state >> 28) + 4u)) ^ state;
+  uint32_t word = shifted * 277803737u;
+  return (word >> 22) ^ word;
+}
+
+__device__ __forceinline__ uint32_t seed_for(uint32_t pixel, uint32_t frame) {
+  uint32_t a = pcg_hash(pixel);
+  uint32_t b = pcg_hash(frame ^ 0x9e3779b9u);
+  uint32_t s = pcg_hash(a ^ (b * 0x85ebca6bu));
+  return s == 0 ? 0x1234567u : s;
+}
+
+struct Rng {
+  uint32_t s;
+  __device__ __forceinline__ uint32_t u32() {
+    uint32_t x = s;
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    s = x;
+    return x;
+  }
+  // u32 / 2^32 exactly (a power-of-two scale is exact in binary64)
+  __device__ __forceinline__ double next() { return (double)u32() * 2.3283064365386963e-10; }
+  __device__ __forceinline__ int pick(int n) {
+    double f = floor(next() * n);
+    double m = (double)(n - 1);
+    return (int)(m < f ? m : f);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Packed-record access
+
+struct PrimGeom {
+  float4 a;  // a.xyz, type
+  float4 b;  // b.xyz (sphere: radius in x), object
+  float4 c;  // c.xyz, area
+};
+
+__device__ __forceinline__ PrimGeom load_prim(const DevScene& S, int p) {
+  const float4* r = S.prims + (size_t)p * 4;
+  PrimGeom g;
+  g.a = r[0];
+  g.b = r[1];
+  g.c = r[2];
+  return g;
+}
+
+// Math.max(0, x): +0 for x <= 0 (including -0), NaN stays NaN.
+__device__ __forceinline__ double max0(double x) { return x > 0 ? x : (x != x ? x : 0.0); }
+
+// ---------------------------------------------------------------------------
+// Intersection (intersect.ts:20-123), binary64 over float32 inputs
+
+__device__ __forceinline__ double ray_sphere(double ox, double oy, double oz, double dx,
+                                             double dy, double dz, double cx, double cy,
+                                             double cz, double r, double tMin, double tMax) {
+  double lx = ox - cx, ly = oy - cy, lz = oz - cz;
+  double a = dx * dx + dy * dy + dz * dz;
+  double halfB = lx * dx + ly * dy + lz * dz;
+  double c = lx * lx + ly * ly + lz * lz - r * r;
+  double disc = halfB * halfB - a * c;
+  if (disc < 0) return -1;
+  double s = sqrt(disc);
+  double t = (-halfB - s) / a;
+  if (t <= tMin || t >= tMax) {
+    t = (-halfB + s) / a;
+    if (t <= tMin || t >= tMax) return -1;
+  }
+  return t;
+}
+
+// Two-sided Moller-Trumbore; returns true and t,u,v on a hit in (tMin, tMax).
+__device__ __forceinline__ bool ray_triangle(double ox, double oy, double oz, double dx,
+                                             double dy, double dz, double ax, double ay,
+                                             double az, double bx, double by, double bz,
+                                             double cx, double cy, double cz, double tMin,
+                                             double tMax, double& tOut, double& uOut,
+                                             double& vOut) {
+  double e1x = bx - ax, e1y = by - ay, e1z = bz - az;
+  double e2x = cx - ax, e2y = cy - ay, e2z = cz - az;
+  double px = dy * e2z - dz * e2y;
+  double py = dz * e2x - dx * e2z;
+  double pz = dx * e2y - dy * e2x;
+  double det = e1x * px + e1y * py + e1z * pz;
+  if (det > -1e-12 && det < 1e-12) return false;
+  double invDet = 1 / det;
+  double tx = ox - ax, ty = oy - ay, tz = oz - az;
+  double u = (tx * px + ty * py + tz * pz) * invDet;
+  if (u < 0 || u > 1) return false;
+  double qx = ty * e1z - tz * e1y;
+  double qy = tz * e1x - tx * e1z;
+  double qz = tx * e1y - ty * e1x;
+  double v = (dx * qx + dy * qy + dz * qz) * invDet;
+  if (v < 0 || u + v > 1) return false;
+  double t = (e2x * qx + e2y * qy + e2z * qz) * invDet;
+  if (t <= tMin || t >= tMax) return false;
+  tOut = t;
+  uOut = u;
+  vOut = v;
+  return true;
+}
+
+// Slab test; returns +inf on a miss (entry before tMax required), else max(tNear,0).
+__device__ __forceinline__ double ray_aabb(double ox, double oy, double oz, double idx,
+                                           double idy, double idz, float4 lo, float4 hi,
+                                           double tMax) {
+  double t0 = ((double)lo.x - ox) * idx;
+  double t1 = ((double)hi.x - ox) * idx;
+  double tNear = t0 < t1 ? t0 : t1;
+  double tFar = t0 < t1 ? t1 : t0;
+  t0 = ((double)lo.y - oy) * idy;
+  t1 = ((double)hi.y - oy) * idy;
+  if (t0 < t1) {
+    if (t0 > tNear) tNear = t0;
+    if (t1 < tFar) tFar = t1;
+  } else {
+    if (t1 > tNear) tNear = t1;
+    if (t0 < tFar) tFar = t0;
+  }
+  t0 = ((double)lo.z - oz) * idz;
+  t1 = ((double)hi.z - oz) * idz;
+  if (t0 < t1) {
+    if (t0 > tNear) tNear = t0;
+    if (t1 < tFar) tFar = t1;
+  } else {
+    if (t1 > tNear) tNear = t1;
+    if (t0 < tFar) tFar = t0;
+  }
+  if (tNear > tFar || tFar < 0 || tNear > tMax) return __builtin_inf();
+  return tNear < 0 ? 0 : tNear;
+}
+
+__device__ __forceinline__ double safe_inv(double d) { return 1 / (d == 0 ? 1e-20 : d); }
+
+struct Hit {
+  double t, u, v;
+  int prim;
+};
+
+struct TraversalCounts {
+  uint32_t nodes, tris, spheres;
+};
+
+// Leaf primitive loop shared by both closest-hit walks (intersect.ts:155-193,
+// 324-351): strict '<' keeps the first prim tested on exact ties; a sphere hit
+// leaves u, v as the last triangle hit set them.
+template <bool COUNT>
+__device__ __forceinline__ void test_leaf(const DevScene& S, float leafWord, double ox, double oy,
+                                          double oz, double dx, double dy, double dz,
+                                          double tMin, Hit& h, TraversalCounts& cnt) {
+  int li = (int)leafWord;
+  int count = li & 7;
+  int start = li >> 3;
+  for (int p = start; p < start + count; p++) {
+    PrimGeom g = load_prim(S, p);
+    if (g.a.w == 0.0f) {  // PRIM_SPHERE
+      if (COUNT) cnt.spheres++;
+      double t = ray_sphere(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, tMin, h.t);
+      if (t > 0) {
+        h.t = t;
+        h.prim = p;
+      }
+    } else {
+      if (COUNT) cnt.tris++;
+      double t, u, v;
+      if (ray_triangle(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, g.b.y, g.b.z, g.c.x,
+                       g.c.y, g.c.z, tMin, h.t, t, u, v)) {
+        h.t = t;
+        h.prim = p;
+        h.u = u;
+        h.v = v;
+      }
+    }
+  }
+}
+
+// Stackless skip-link closest hit (intersect.ts:290-365): node hit -> node+1,
+// miss -> skip.  h.t enters as tMax and leaves as the closest t (prim -1 = miss).
+template <bool COUNT>
+__device__ __forceinline__ void closest_stackless(const DevScene& S, double ox, double oy,
+                                                  double oz, double dx, double dy, double dz,
+                                                  double tMin, Hit& h, TraversalCounts& cnt) {
+  double idx = safe_inv(dx), idy = safe_inv(dy), idz = safe_inv(dz);
+  int node = 0;
+  const int nodeCount = S.nodeCount;
+  while (node < nodeCount) {
+    float4 lo = S.nodes[2 * (size_t)node];
+    float4 hi = S.nodes[2 * (size_t)node + 1];
+    if (COUNT) cnt.nodes++;
+    double entry = ray_aabb(ox, oy, oz, idx, idy, idz, lo, hi, h.t);
+    if (entry == __builtin_inf()) {
+      node = (int)lo.w;
+      continue;
+    }
+    if (hi.w >= 0.0f) test_leaf<COUNT>(S, hi.w, ox, oy, oz, dx, dy, dz, tMin, h, cnt);
+    node++;
+  }
+}
+
+// Stackless any-hit on (tMin, tMax) (intersect.ts:368-420).
+template <bool COUNT>
+__device__ __forceinline__ bool occluded(const DevScene& S, double ox, double oy, double oz,
+                                         double dx, double dy, double dz, double tMin,
+                                         double tMax, TraversalCounts& cnt) {
+  double idx = safe_inv(dx), idy = safe_inv(dy), idz = safe_inv(dz);
+  int node = 0;
+  const int nodeCount = S.nodeCount;
+  while (node < nodeCount) {
+    float4 lo = S.nodes[2 * (size_t)node];
+    float4 hi = S.nodes[2 * (size_t)node + 1];
+    if (COUNT) cnt.nodes++;
+    double entry = ray_aabb(ox, oy, oz, idx, idy, idz, lo, hi, tMax);
+    if (entry == __builtin_inf()) {
+      node = (int)lo.w;
+      continue;
+    }
+    if (hi.w >= 0.0f) {
+      int li = (int)hi.w;
+      int count = li & 7;
+      int start = li >> 3;
+      for (int p = start; p < start + count; p++) {
+        PrimGeom g = load_prim(S, p);
+        if (g.a.w == 0.0f) {
+          if (COUNT) cnt.spheres++;
+          if (ray_sphere(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, tMin, tMax) > 0)
+            return true;
+        } else {
+          if (COUNT) cnt.tris++;
+          double t, u, v;
+          if (ray_triangle(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, g.b.y, g.b.z,
+                           g.c.x, g.c.y, g.c.z, tMin, tMax, t, u, v))
+            return true;
+        }
+      }
+    }
+    node++;
+  }
+  return false;
+}
+
+// Largest float not above x (stack entries are stored rounded down so that a
+// pop skips a subtree only when the binary64 entry would have been skipped).
+__device__ __forceinline__ float f32_round_down(double x) {
+  float f = (float)x;
+  if ((double)f > x) {  // step one float toward -inf
+    uint32_t b = __float_as_uint(f);
+    if (f > 0.0f) b -= 1;
+    else if (f < 0.0f) b += 1;
+    else b = 0x80000001u;  // -smallest subnormal
+    f = __uint_as_float(b);
+  }
+  return f;
+}
+
+// Ordered front-to-back closest hit (intersect.ts:202-282): both children
+// tested against the shrinking bound, nearer first (tl <= tr -> left), the far
+// child pushed with its entry distance, pops skip entries >= closest.
+// The stack lives in LDS, STACK entries per lane at stride `lanes`
+// (conflict-free); entries are float32 rounded down, so a pop skips a subtree
+// only when the binary64 entry would have been skipped too.  A ray that needs
+// more than STACK deferred subtrees finishes with the stackless walk from the
+// root under its current bound: that walk visits every box entered before the
+// bound and re-testing already-tested primitives cannot change a strict-'<'
+// minimum, so the closest hit is unchanged and the loop always ends.
+template <int STACK, bool COUNT>
+__device__ __forceinline__ void closest_ordered(const DevScene& S, double ox, double oy,
+                                               double oz, double dx, double dy, double dz,
+                                               double tMin, Hit& h, int* __restrict__ sNode,
+                                               float* __restrict__ sEntry, int lanes,
+                                               TraversalCounts& cnt) {
+  double idx = safe_inv(dx), idy = safe_inv(dy), idz = safe_inv(dz);
+  if (COUNT) cnt.nodes++;
+  float4 rlo = S.nodes[0], rhi = S.nodes[1];
+  if (ray_aabb(ox, oy, oz, idx, idy, idz, rlo, rhi, h.t) == __builtin_inf()) return;
+  int sp = 0;
+  int node = 0;
+  float leafWord = rhi.w;
+  const double INF = __builtin_inf();
+  for (;;) {
+    if (leafWord >= 0.0f) {
+      test_leaf<COUNT>(S, leafWord, ox, oy, oz, dx, dy, dz, tMin, h, cnt);
+    } else {
+      int left = node + 1;
+      float4 llo = S.nodes[2 * (size_t)left], lhi = S.nodes[2 * (size_t)left + 1];
+      int right = (int)llo.w;
+      float4 rlo2 = S.nodes[2 * (size_t)right], rhi2 = S.nodes[2 * (size_t)right + 1];
+      if (COUNT) cnt.nodes += 2;
+      double tl = ray_aabb(ox, oy, oz, idx, idy, idz, llo, lhi, h.t);
+      double tr = ray_aabb(ox, oy, oz, idx, idy, idz, rlo2, rhi2, h.t);
+      if (tl != INF) {
+        if (tr != INF) {
+          if (sp == STACK) {
+            closest_stackless<COUNT>(S, ox, oy, oz, dx, dy, dz, tMin, h, cnt);
+            return;
+          }
+          int slot = sp * lanes;
+          if (tl <= tr) {
+            sNode[slot] = right;
+            sEntry[slot] = f32_round_down(tr);
+            node = left;
+            leafWord = lhi.w;
+          } else {
+            sNode[slot] = left;
+            sEntry[slot] = f32_round_down(tl);
+            node = right;
+            leafWord = rhi2.w;
+          }
+          sp++;
+        } else {
+          node = left;
+          leafWord = lhi.w;
+        }
+        continue;
+      }
+      if (tr != INF) {
+        node = right;
+        leafWord = rhi2.w;
+        continue;
+      }
+    }
+    // pop, skipping entries the shrinking bound already excludes
+    bool found = false;
+    while (sp > 0) {
+      sp--;
+      if ((double)sEntry[sp * lanes] < h.t) {
+        node = sNode[sp * lanes];
+        found = true;
+        break;
+      }
+    }
+    if (!found) return;
+    leafWord = S.nodes[2 * (size_t)node + 1].w;
+  }
+}
+
+}  // namespace ct

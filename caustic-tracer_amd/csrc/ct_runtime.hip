@@ -1,0 +1,222 @@
+// Runtime part of the C ABI: errors, device memory, device-resident scenes.
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "ct_internal.h"
+
+namespace ct {
+
+static thread_local std::string g_error;
+
+int set_error(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+int check_hip(hipError_t e, const char* what) {
+  if (e == hipSuccess) return CT_OK;
+  int code = (e == hipErrorOutOfMemory) ? CT_ENOMEM : CT_EDEVICE;
+  return set_error(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+static size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+}  // namespace ct
+
+using namespace ct;
+
+extern "C" {
+
+const char* ct_last_error(void) { return g_error.c_str(); }
+
+int ct_init(int32_t device) {
+  int n = 0;
+  CT_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n)
+    return set_error(CT_EINVAL, "ct_init: device " + std::to_string(device) + " of " +
+                                    std::to_string(n));
+  CT_HIP(hipSetDevice(device));
+  return CT_OK;
+}
+
+int ct_device_alloc(void** p, size_t bytes) {
+  if (!p) return set_error(CT_EINVAL, "ct_device_alloc: null out pointer");
+  CT_HIP(hipMalloc(p, bytes ? bytes : 1));
+  return CT_OK;
+}
+
+int ct_device_free(void* p) {
+  if (p) CT_HIP(hipFree(p));
+  return CT_OK;
+}
+
+int ct_copy_h2d(void* dst, const void* src, size_t bytes) {
+  CT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return CT_OK;
+}
+
+int ct_copy_d2h(void* dst, const void* src, size_t bytes) {
+  CT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return CT_OK;
+}
+
+int ct_stream_sync(void* stream) {
+  CT_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return CT_OK;
+}
+
+// Host-side check that the kernels' loops end and stay in bounds on these
+// arrays: every skip link moves forward and stays <= nodeCount, interior nodes
+// have both children inside the array, leaves reference existing primitives,
+// light and object indices are in range.
+static int validate_scene(const float* nodes, uint32_t nodeCount, uint32_t primCount,
+                          uint32_t matCount, const int32_t* lights, uint32_t lightCount,
+                          const uint32_t* primObject) {
+  for (uint32_t i = 0; i < nodeCount; i++) {
+    const float* nd = nodes + (size_t)i * NODE_STRIDE;
+    double skip = nd[3], leaf = nd[7];
+    if (!(skip > i && skip <= nodeCount))
+      return set_error(CT_EINVAL, "ct_scene_create: node " + std::to_string(i) +
+                                      " has a skip link that does not move forward");
+    if (leaf >= 0) {
+      int li = (int)leaf;
+      uint32_t count = (uint32_t)(li & 7), start = (uint32_t)(li >> 3);
+      if (count && (uint64_t)start + count > primCount)
+        return set_error(CT_EINVAL, "ct_scene_create: leaf " + std::to_string(i) +
+                                        " references primitives beyond primCount");
+    } else {
+      if (i + 1 >= nodeCount)
+        return set_error(CT_EINVAL, "ct_scene_create: interior node without children");
+      double right = nodes[(size_t)(i + 1) * NODE_STRIDE + 3];
+      if (!(right < nodeCount))
+        return set_error(CT_EINVAL, "ct_scene_create: right child beyond nodeCount");
+    }
+  }
+  for (uint32_t k = 0; k < lightCount; k++)
+    if (lights[k] < 0 || (uint32_t)lights[k] >= primCount)
+      return set_error(CT_EINVAL, "ct_scene_create: light index out of range");
+  uint32_t mc = matCount ? matCount : 1;
+  for (uint32_t p = 0; p < primCount; p++)
+    if (primObject[p] >= mc)
+      return set_error(CT_EINVAL, "ct_scene_create: primObject beyond matCount");
+  return CT_OK;
+}
+
+static int scene_create_impl(const float* prims, uint32_t primCount, const float* nodes,
+                             uint32_t nodeCount, const float* mats, uint32_t matCount,
+                             const int32_t* lights, uint32_t lightCount,
+                             const uint32_t* primObject, const ct_sky* sky, hipMemcpyKind kind,
+                             hipStream_t stream, ct_scene** out) {
+  if (!out) return set_error(CT_EINVAL, "ct_scene_create: null out");
+  *out = nullptr;
+  if (nodeCount == 0 || !nodes)
+    return set_error(CT_EINVAL, "ct_scene_create: a packed scene has at least one node");
+  if ((primCount && !prims) || (matCount && !mats) || (lightCount && !lights) ||
+      (primCount && !primObject))
+    return set_error(CT_EINVAL, "ct_scene_create: null array with non-zero count");
+  if (nodeCount > (1u << 24))
+    return set_error(CT_ELIMIT,
+                     "ct_scene_create: skip links are float32 and exact only below 2^24 nodes");
+  if (kind == hipMemcpyHostToDevice) {
+    int v = validate_scene(nodes, nodeCount, primCount, matCount, lights, lightCount, primObject);
+    if (v) return v;
+  } else {
+    std::vector<float> hn((size_t)nodeCount * NODE_STRIDE);
+    std::vector<int32_t> hl(lightCount);
+    std::vector<uint32_t> ho(primCount);
+    CT_HIP(hipMemcpyAsync(hn.data(), nodes, hn.size() * 4, hipMemcpyDeviceToHost, stream));
+    if (lightCount)
+      CT_HIP(hipMemcpyAsync(hl.data(), lights, hl.size() * 4, hipMemcpyDeviceToHost, stream));
+    if (primCount)
+      CT_HIP(hipMemcpyAsync(ho.data(), primObject, ho.size() * 4, hipMemcpyDeviceToHost, stream));
+    CT_HIP(hipStreamSynchronize(stream));
+    int v = validate_scene(hn.data(), nodeCount, primCount, matCount, hl.data(), lightCount,
+                           ho.data());
+    if (v) return v;
+  }
+  // The array lengths PackedScene carries: max(1, n) records each (pack.ts:136-138, 111).
+  size_t pc = primCount ? primCount : 1, mc = matCount ? matCount : 1;
+  size_t szPrims = align_up(pc * PRIM_STRIDE * sizeof(float));
+  size_t szNodes = align_up((size_t)nodeCount * NODE_STRIDE * sizeof(float));
+  size_t szMats = align_up(mc * MAT_STRIDE * sizeof(float));
+  size_t szLights = align_up((lightCount ? lightCount : 1) * sizeof(int32_t));
+  size_t szObj = align_up(pc * sizeof(uint32_t));
+  size_t total = szPrims + szNodes + szMats + szLights + szObj;
+  ct_scene* s = new ct_scene();
+  hipError_t e = hipMalloc(&s->buf, total);
+  if (e != hipSuccess) {
+    delete s;
+    return check_hip(e, "ct_scene_create: hipMalloc");
+  }
+  s->bytes = total;
+  if (hipGetDevice(&s->device) != hipSuccess) s->device = 0;
+  char* base = (char*)s->buf;
+  char* pPrims = base;
+  char* pNodes = pPrims + szPrims;
+  char* pMats = pNodes + szNodes;
+  char* pLights = pMats + szMats;
+  char* pObj = pLights + szLights;
+  e = hipMemsetAsync(base, 0, total, stream);
+  if (e == hipSuccess && primCount)
+    e = hipMemcpyAsync(pPrims, prims, (size_t)primCount * PRIM_STRIDE * sizeof(float), kind, stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(pNodes, nodes, (size_t)nodeCount * NODE_STRIDE * sizeof(float), kind, stream);
+  if (e == hipSuccess && matCount)
+    e = hipMemcpyAsync(pMats, mats, (size_t)matCount * MAT_STRIDE * sizeof(float), kind, stream);
+  if (e == hipSuccess && lightCount)
+    e = hipMemcpyAsync(pLights, lights, (size_t)lightCount * sizeof(int32_t), kind, stream);
+  if (e == hipSuccess && primCount)
+    e = hipMemcpyAsync(pObj, primObject, (size_t)primCount * sizeof(uint32_t), kind, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) {
+    (void)hipFree(s->buf);
+    delete s;
+    return check_hip(e, "ct_scene_create: upload");
+  }
+  DevScene& d = s->dev;
+  d.prims = (const float4*)pPrims;
+  d.nodes = (const float4*)pNodes;
+  d.mats = (const float4*)pMats;
+  d.lights = (const int32_t*)pLights;
+  d.primObject = (const uint32_t*)pObj;
+  d.primCount = (int)primCount;
+  d.nodeCount = (int)nodeCount;
+  d.lightCount = (int)lightCount;
+  d.skyKind = (sky && sky->kind == 1) ? 1 : 0;
+  for (int k = 0; k < 3; k++) {
+    d.zenith[k] = d.skyKind ? sky->zenith[k] : 0;
+    d.horizon[k] = d.skyKind ? sky->horizon[k] : 0;
+  }
+  d.skyStrength = d.skyKind ? sky->strength : 0;
+  *out = s;
+  return CT_OK;
+}
+
+int ct_scene_create(const float* prims, uint32_t primCount, const float* nodes,
+                    uint32_t nodeCount, const float* mats, uint32_t matCount,
+                    const int32_t* lights, uint32_t lightCount, const uint32_t* primObject,
+                    const ct_sky* sky, ct_scene** out) {
+  return scene_create_impl(prims, primCount, nodes, nodeCount, mats, matCount, lights,
+                           lightCount, primObject, sky, hipMemcpyHostToDevice, nullptr, out);
+}
+
+int ct_scene_create_device(const float* prims, uint32_t primCount, const float* nodes,
+                           uint32_t nodeCount, const float* mats, uint32_t matCount,
+                           const int32_t* lights, uint32_t lightCount,
+                           const uint32_t* primObject, const ct_sky* sky, void* stream,
+                           ct_scene** out) {
+  return scene_create_impl(prims, primCount, nodes, nodeCount, mats, matCount, lights,
+                           lightCount, primObject, sky, hipMemcpyDeviceToDevice,
+                           (hipStream_t)stream, out);
+}
+
+int ct_scene_destroy(ct_scene* s) {
+  if (!s) return CT_OK;
+  hipError_t e = hipFree(s->buf);
+  delete s;
+  return check_hip(e, "ct_scene_destroy");
+}
+
+}  // extern "C"

@@ -1,0 +1,145 @@
+// Batched closest-hit and any-hit queries over a device-resident scene.
+//
+// One lane per ray.  Rays are read as three 16-byte loads (6 binary64); the
+// scene is read straight from the packed float32 arrays: 2 x 16 B per node
+// and 3 x 16 B per primitive, the algorithmic minimum of the layout.
+#include "ct_internal.h"
+
+namespace ct {
+
+constexpr int QUERY_BLOCK = 256;
+constexpr int ORDERED_STACK = 16;
+
+__device__ __forceinline__ void load_ray(const double* __restrict__ rays, size_t i, double& ox,
+                                         double& oy, double& oz, double& dx, double& dy,
+                                         double& dz) {
+  const double2* r = (const double2*)(rays + 6 * i);
+  double2 a = r[0], b = r[1], c = r[2];
+  ox = a.x; oy = a.y; oz = b.x; dx = b.y; dy = c.x; dz = c.y;
+}
+
+// Sum a per-lane count over the wave and add it to a 64-bit global counter.
+__device__ __forceinline__ void wave_add(unsigned long long* dst, uint32_t v) {
+  unsigned long long x = v;
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  if ((threadIdx.x & 63) == 0 && x) atomicAdd(dst, x);
+}
+
+template <int WALK, bool COUNT>
+__global__ void __launch_bounds__(QUERY_BLOCK)
+k_closest(DevScene S, const double* __restrict__ rays, uint32_t n, double tMin,
+          const double* __restrict__ tMax, int32_t* __restrict__ prim, double* __restrict__ tOut,
+          double* __restrict__ uOut, double* __restrict__ vOut,
+          unsigned long long* __restrict__ counts) {
+  __shared__ int sNode[WALK == CT_WALK_ORDERED ? ORDERED_STACK * QUERY_BLOCK : 1];
+  __shared__ float sEntry[WALK == CT_WALK_ORDERED ? ORDERED_STACK * QUERY_BLOCK : 1];
+  TraversalCounts cnt = {0, 0, 0};
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double ox, oy, oz, dx, dy, dz;
+    load_ray(rays, i, ox, oy, oz, dx, dy, dz);
+    Hit h;
+    h.t = tMax ? tMax[i] : __builtin_inf();
+    h.u = 0;
+    h.v = 0;
+    h.prim = -1;
+    if (WALK == CT_WALK_ORDERED)
+      closest_ordered<ORDERED_STACK, COUNT>(S, ox, oy, oz, dx, dy, dz, tMin, h,
+                                            sNode + threadIdx.x, sEntry + threadIdx.x,
+                                            QUERY_BLOCK, cnt);
+    else
+      closest_stackless<COUNT>(S, ox, oy, oz, dx, dy, dz, tMin, h, cnt);
+    if (prim) prim[i] = h.prim;
+    if (tOut) tOut[i] = h.prim < 0 ? __builtin_inf() : h.t;
+    if (uOut) uOut[i] = h.u;
+    if (vOut) vOut[i] = h.v;
+  }
+  if (COUNT) {
+    wave_add(counts + 0, cnt.nodes);
+    wave_add(counts + 1, cnt.tris);
+    wave_add(counts + 2, cnt.spheres);
+  }
+}
+
+__global__ void __launch_bounds__(QUERY_BLOCK)
+k_occluded(DevScene S, const double* __restrict__ rays, uint32_t n, double tMin,
+           const double* __restrict__ tMax, uint8_t* __restrict__ out) {
+  TraversalCounts cnt = {0, 0, 0};
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double ox, oy, oz, dx, dy, dz;
+    load_ray(rays, i, ox, oy, oz, dx, dy, dz);
+    double tm = tMax ? tMax[i] : __builtin_inf();
+    out[i] = occluded<false>(S, ox, oy, oz, dx, dy, dz, tMin, tm, cnt) ? 1 : 0;
+  }
+}
+
+template <int WALK, bool COUNT>
+static int launch_closest(const ct_scene* s, const double* rays, uint32_t n, double tMin,
+                          const double* tMax, int32_t* prim, double* t, double* u, double* v,
+                          unsigned long long* counts, hipStream_t stream) {
+  unsigned grid = grid_for(n, QUERY_BLOCK, 256 * 32);
+  hipLaunchKernelGGL((k_closest<WALK, COUNT>), dim3(grid), dim3(QUERY_BLOCK), 0, stream, s->dev,
+                     rays, n, tMin, tMax, prim, t, u, v, counts);
+  CT_LAUNCH_CHECK("k_closest");
+  return CT_OK;
+}
+
+}  // namespace ct
+
+using namespace ct;
+
+extern "C" {
+
+int ct_closest_hit(const ct_scene* s, const double* rays, uint32_t n, double tMin,
+                   const double* tMax, int32_t walk, int32_t* prim, double* t, double* u,
+                   double* v, void* stream) {
+  if (!s) return set_error(CT_EINVAL, "ct_closest_hit: null scene");
+  if (n == 0) return CT_OK;
+  if (!rays) return set_error(CT_EINVAL, "ct_closest_hit: null rays");
+  hipStream_t st = (hipStream_t)stream;
+  if (walk == CT_WALK_ORDERED)
+    return launch_closest<CT_WALK_ORDERED, false>(s, rays, n, tMin, tMax, prim, t, u, v, nullptr, st);
+  if (walk == CT_WALK_STACKLESS)
+    return launch_closest<CT_WALK_STACKLESS, false>(s, rays, n, tMin, tMax, prim, t, u, v, nullptr, st);
+  return set_error(CT_EINVAL, "ct_closest_hit: unknown walk");
+}
+
+int ct_closest_hit_counts(const ct_scene* s, const double* rays, uint32_t n, double tMin,
+                          const double* tMax, int32_t walk, uint64_t counts[3], void* stream) {
+  if (!s || !counts) return set_error(CT_EINVAL, "ct_closest_hit_counts: null argument");
+  counts[0] = counts[1] = counts[2] = 0;
+  if (n == 0) return CT_OK;
+  hipStream_t st = (hipStream_t)stream;
+  unsigned long long* d = nullptr;
+  CT_HIP(hipMallocAsync((void**)&d, 3 * sizeof(unsigned long long), st));
+  unsigned long long zero[3] = {0, 0, 0};
+  CT_HIP(hipMemcpyAsync(d, zero, sizeof zero, hipMemcpyHostToDevice, st));
+  int rc;
+  if (walk == CT_WALK_ORDERED)
+    rc = launch_closest<CT_WALK_ORDERED, true>(s, rays, n, tMin, tMax, nullptr, nullptr, nullptr, nullptr, d, st);
+  else if (walk == CT_WALK_STACKLESS)
+    rc = launch_closest<CT_WALK_STACKLESS, true>(s, rays, n, tMin, tMax, nullptr, nullptr, nullptr, nullptr, d, st);
+  else
+    rc = set_error(CT_EINVAL, "ct_closest_hit_counts: unknown walk");
+  unsigned long long h[3] = {0, 0, 0};
+  if (rc == CT_OK) CT_HIP(hipMemcpyAsync(h, d, sizeof h, hipMemcpyDeviceToHost, st));
+  CT_HIP(hipFreeAsync(d, st));
+  CT_HIP(hipStreamSynchronize(st));
+  for (int k = 0; k < 3; k++) counts[k] = h[k];
+  return rc;
+}
+
+int ct_occluded(const ct_scene* s, const double* rays, uint32_t n, double tMin,
+                const double* tMax, uint8_t* out, void* stream) {
+  if (!s) return set_error(CT_EINVAL, "ct_occluded: null scene");
+  if (n == 0) return CT_OK;
+  if (!rays || !out) return set_error(CT_EINVAL, "ct_occluded: null array");
+  unsigned grid = grid_for(n, QUERY_BLOCK, 256 * 32);
+  hipLaunchKernelGGL(k_occluded, dim3(grid), dim3(QUERY_BLOCK), 0, (hipStream_t)stream, s->dev,
+                     rays, n, tMin, tMax, out);
+  CT_LAUNCH_CHECK("k_occluded");
+  return CT_OK;
+}
+
+}  // extern "C"

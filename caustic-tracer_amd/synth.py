@@ -1,0 +1,92 @@
+"""Synthetic workloads for the tracer, in the reference's scene format.
+
+All coordinates are rounded to float32 before packing, so the FP64 bounds the
+packer derives from the scene equal those derived from the packed records.
+"""
+import math
+
+import numpy as np
+
+
+def _f32(a):
+    return np.asarray(a, np.float64).astype(np.float32).astype(np.float64)
+
+
+def _random_frames(rng, n):
+    """Uniform unit normals on the sphere and a uniform in-plane angle."""
+    z = rng.uniform(-1, 1, n)
+    phi = rng.uniform(0, 2 * math.pi, n)
+    r = np.sqrt(np.maximum(0, 1 - z * z))
+    nrm = np.stack([r * np.cos(phi), r * np.sin(phi), z], 1)
+    helper = np.where(np.abs(nrm[:, :1]) > 0.9, [[0.0, 1.0, 0.0]], [[1.0, 0.0, 0.0]])
+    t = np.cross(helper, nrm)
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    b = np.cross(nrm, t)
+    ang = rng.uniform(0, 2 * math.pi, n)
+    e0 = np.cos(ang)[:, None] * t + np.sin(ang)[:, None] * b
+    e1 = np.cos(ang + 2 * math.pi / 3)[:, None] * t + np.sin(ang + 2 * math.pi / 3)[:, None] * b
+    e2 = np.cos(ang + 4 * math.pi / 3)[:, None] * t + np.sin(ang + 4 * math.pi / 3)[:, None] * b
+    return e0, e1, e2
+
+
+def equilateral_soup(rng, n, extent=10.0, mu=math.log(0.05), sigma=0.8):
+    """n equilateral triangles: centroids uniform in [-extent, extent]^3, edge
+    length lognormal(mu, sigma), orientation uniform. Returns (n, 3, 3)."""
+    c = rng.uniform(-extent, extent, (n, 3))
+    s = rng.lognormal(mu, sigma, n)
+    R = s / math.sqrt(3)  # circumradius
+    e0, e1, e2 = _random_frames(rng, n)
+    tri = np.stack([c + R[:, None] * e0, c + R[:, None] * e1, c + R[:, None] * e2], 1)
+    return _f32(tri)
+
+
+def soup_scene(n_tris=1_000_000, n_lights=64, n_materials=1000, seed=2024, light_edge=0.5,
+               light_radiance=20.0):
+    """Triangle soup: n_tris triangles spread over n_materials mesh objects whose
+    materials are drawn 70% Lambertian / 20% metal / 10% dielectric, plus one
+    emissive object of n_lights triangles, gradient sky, camera outside the cloud."""
+    rng = np.random.default_rng(seed)
+    tris = equilateral_soup(rng, n_tris)
+    which = rng.integers(0, n_materials, n_tris)
+    kinds = rng.choice(3, size=n_materials, p=[0.7, 0.2, 0.1])
+    objects = []
+    order = np.argsort(which, kind="stable")
+    counts = np.bincount(which, minlength=n_materials)
+    offs = np.concatenate([[0], np.cumsum(counts)])
+    for m in range(n_materials):
+        sel = tris[order[offs[m]:offs[m + 1]]]
+        if kinds[m] == 0:
+            mat = {"type": "lambertian",
+                   "albedo": tuple(float(x) for x in _f32(rng.uniform(0.2, 0.9, 3)))}
+        elif kinds[m] == 1:
+            mat = {"type": "metal", "albedo": tuple(float(x) for x in _f32(rng.uniform(0.5, 1.0, 3))),
+                   "roughness": float(_f32(rng.uniform(0, 0.3)))}
+        else:
+            mat = {"type": "dielectric", "ior": 1.5}
+        objects.append({"name": f"mesh{m}", "material": mat,
+                        "shape": {"kind": "mesh", "vertices": sel.reshape(-1, 3),
+                                  "indices": np.arange(3 * len(sel))}})
+    lt = equilateral_soup(rng, n_lights, mu=math.log(light_edge), sigma=0.0)
+    objects.append({"name": "lights", "material": {"type": "emissive", "color": (1.0, 1.0, 1.0),
+                                                   "strength": light_radiance},
+                    "shape": {"kind": "mesh", "vertices": lt.reshape(-1, 3),
+                              "indices": np.arange(3 * n_lights)}})
+    eye = (0.0, 0.0, 24.0)
+    dist = 24.0
+    camera = {"target": (0.0, 0.0, 0.0), "yaw": math.atan2(eye[0], eye[2]),
+              "pitch": math.asin(eye[1] / dist), "distance": dist, "fovDeg": 45.0,
+              "lensRadius": 0.0, "focusDistance": dist}
+    return {"name": f"soup{n_tris}", "objects": objects, "camera": camera,
+            "sky": {"kind": "gradient", "zenith": (0.5, 0.7, 1.0), "horizon": (1.0, 1.0, 1.0),
+                    "strength": 0.5}}
+
+
+def random_rays(rng_stream, n, lo=-14.0, hi=14.0, tlo=-8.0, thi=8.0):
+    """Rays in the shape of bvh.test.ts:94-99 from a stream of uniforms in [0,1):
+    origins in [lo,hi]^3 aimed at points of [tlo,thi]^3, unit binary64 directions."""
+    d = np.asarray(rng_stream, np.float64).reshape(n, 6)
+    o = lo + (hi - lo) * d[:, 0:3]
+    t = tlo + (thi - tlo) * d[:, 3:6]
+    v = t - o
+    ln = np.sqrt(v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1] + v[:, 2] * v[:, 2])
+    return np.concatenate([o, v / ln[:, None]], 1)

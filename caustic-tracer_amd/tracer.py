@@ -1,0 +1,252 @@
+"""Host mirror of the reference's render/intersect API over the HIP library.
+
+Names and argument meaning follow src/tracer/tracer.ts, intersect.ts and camera.ts;
+the work runs on the GPU through the C ABI.  A scene is any object with the
+PackedScene fields of pack.ts:46-59 (prims, primCount, nodes, nodeCount, mats,
+matCount, lights, primObject, sky) -- the arrays the reference's packer emits.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _capi as A
+from ._capi import DeviceBuffer, check, lib
+
+T_MIN = 1e-4  # intersect.ts:14
+WALK_STACKLESS = 0
+WALK_ORDERED = 1
+
+
+class DeviceScene:
+    """Device-resident copy of a PackedScene (replaces handing it to the CPU tracer)."""
+
+    def __init__(self, ps):
+        A.init()
+        sky = A.Sky()
+        sk = getattr(ps, "sky", None) or {"kind": "none"}
+        if sk.get("kind") == "gradient":
+            sky.kind = 1
+            sky.zenith = (C.c_double * 3)(*sk["zenith"])
+            sky.horizon = (C.c_double * 3)(*sk["horizon"])
+            sky.strength = sk["strength"]
+        prims = np.ascontiguousarray(ps.prims, np.float32)
+        nodes = np.ascontiguousarray(ps.nodes, np.float32)
+        mats = np.ascontiguousarray(ps.mats, np.float32)
+        lights = np.ascontiguousarray(ps.lights, np.int32)
+        pobj = np.ascontiguousarray(ps.primObject, np.uint32)
+        h = C.c_void_p()
+        check(lib().ct_scene_create(prims.ctypes.data_as(C.c_void_p), ps.primCount,
+                                    nodes.ctypes.data_as(C.c_void_p), ps.nodeCount,
+                                    mats.ctypes.data_as(C.c_void_p), ps.matCount,
+                                    lights.ctypes.data_as(C.c_void_p), len(lights),
+                                    pobj.ctypes.data_as(C.c_void_p), C.byref(sky), C.byref(h)),
+              "ct_scene_create")
+        self.handle = h
+        self.primCount = ps.primCount
+        self.nodeCount = ps.nodeCount
+
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            lib().ct_scene_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_scene(ps):
+    """The device copy of `ps`, uploaded once and cached on the object."""
+    if isinstance(ps, DeviceScene):
+        return ps
+    ds = getattr(ps, "_device_scene", None)
+    if ds is None:
+        ds = DeviceScene(ps)
+        try:
+            ps._device_scene = ds
+        except AttributeError:
+            pass
+    return ds
+
+
+def _rays_on_device(rays):
+    if isinstance(rays, DeviceBuffer):
+        return rays, rays.shape[0] if len(rays.shape) > 1 else rays.nbytes // 48
+    r = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
+    return DeviceBuffer.from_array(r), len(r)
+
+
+def closest_hit(ps, rays, tmax=None, walk=WALK_ORDERED, tmin=T_MIN, to_host=True):
+    """Batched intersectClosest (walk=WALK_ORDERED, intersect.ts:202) or
+    intersectClosestStackless (WALK_STACKLESS, intersect.ts:290).
+
+    Returns dict prim/t/u/v (numpy when to_host, else DeviceBuffers)."""
+    ds = device_scene(ps)
+    d_rays, n = _rays_on_device(rays)
+    d_tmax = None
+    if tmax is not None:
+        d_tmax = tmax if isinstance(tmax, DeviceBuffer) else DeviceBuffer.from_array(
+            np.ascontiguousarray(np.broadcast_to(np.asarray(tmax, np.float64), (n,))))
+    out = {"prim": DeviceBuffer.empty((n,), np.int32), "t": DeviceBuffer.empty((n,), np.float64),
+           "u": DeviceBuffer.empty((n,), np.float64), "v": DeviceBuffer.empty((n,), np.float64)}
+    check(lib().ct_closest_hit(ds.handle, d_rays.ptr, n, tmin, d_tmax.ptr if d_tmax else None,
+                               walk, out["prim"].ptr, out["t"].ptr, out["u"].ptr, out["v"].ptr,
+                               None), "ct_closest_hit")
+    if to_host:
+        return {k: v.numpy() for k, v in out.items()}
+    return out
+
+
+def closest_hit_counts(ps, rays, tmax=None, walk=WALK_ORDERED, tmin=T_MIN):
+    """(node fetches, triangle tests, sphere tests) of a closest-hit batch."""
+    ds = device_scene(ps)
+    d_rays, n = _rays_on_device(rays)
+    d_tmax = None
+    if tmax is not None:
+        d_tmax = DeviceBuffer.from_array(
+            np.ascontiguousarray(np.broadcast_to(np.asarray(tmax, np.float64), (n,))))
+    c = (C.c_uint64 * 3)()
+    check(lib().ct_closest_hit_counts(ds.handle, d_rays.ptr, n, tmin,
+                                      d_tmax.ptr if d_tmax else None, walk, c, None),
+          "ct_closest_hit_counts")
+    return int(c[0]), int(c[1]), int(c[2])
+
+
+def occluded(ps, rays, tmax, tmin=T_MIN):
+    """Batched occluded (intersect.ts:368): bool per ray."""
+    ds = device_scene(ps)
+    d_rays, n = _rays_on_device(rays)
+    d_tmax = DeviceBuffer.from_array(
+        np.ascontiguousarray(np.broadcast_to(np.asarray(tmax, np.float64), (n,))))
+    out = DeviceBuffer.empty((n,), np.uint8)
+    check(lib().ct_occluded(ds.handle, d_rays.ptr, n, tmin, d_tmax.ptr, out.ptr, None),
+          "ct_occluded")
+    return out.numpy().astype(bool)
+
+
+def intersectClosest(ps, ox, oy, oz, dx, dy, dz, tMin, tMax, hit):
+    """Single-ray form of intersect.ts:202 filling a HitRecord-like dict."""
+    r = closest_hit(ps, np.array([[ox, oy, oz, dx, dy, dz]]), None if tMax == math.inf else tMax,
+                    WALK_ORDERED, tMin)
+    hit["t"], hit["prim"], hit["u"], hit["v"] = float(r["t"][0]), int(r["prim"][0]), \
+        float(r["u"][0]), float(r["v"][0])
+    return hit["prim"] >= 0
+
+
+# ---------------------------------------------------------------------------
+# camera.ts
+
+def _normalize(v):
+    ln = math.sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+    if ln > 0:
+        s = 1 / ln
+        return (v[0] * s, v[1] * s, v[2] * s)
+    return (0.0, 0.0, 0.0)
+
+
+def cameraFrame(cam, aspect):
+    """cameraFrame (camera.ts:33-51) as the 16-double frame the C ABI takes."""
+    t = cam["target"]
+    cp = math.cos(cam["pitch"])
+    d = cam["distance"]
+    o = (t[0] + cp * math.sin(cam["yaw"]) * d, t[1] + math.sin(cam["pitch"]) * d,
+         t[2] + cp * math.cos(cam["yaw"]) * d)
+    f = _normalize((t[0] - o[0], t[1] - o[1], t[2] - o[2]))
+    r = (f[1] * 0 - f[2] * 1, f[2] * 0 - f[0] * 0, f[0] * 1 - f[1] * 0)
+    if r[0] * r[0] + r[1] * r[1] + r[2] * r[2] < 1e-12:
+        r = (1.0, 0.0, 0.0)
+    r = _normalize(r)
+    u = (r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2], r[0] * f[1] - r[1] * f[0])
+    fd = cam["focusDistance"]
+    return np.array([*o, *r, *u, *f, math.tan((cam["fovDeg"] * math.pi) / 360), aspect,
+                     cam["lensRadius"], fd if fd > 1e-3 else 1e-3], np.float64)
+
+
+# ---------------------------------------------------------------------------
+# tracer.ts
+
+class Tracer:
+    """Owns the wavefront path state on the device (one per concurrent caller)."""
+
+    def __init__(self, walk=WALK_ORDERED):
+        A.init()
+        h = C.c_void_p()
+        check(lib().ct_tracer_create(C.byref(h)), "ct_tracer_create")
+        self.handle = h
+        self.set_walk(walk)
+
+    def set_walk(self, walk):
+        check(lib().ct_tracer_set_walk(self.handle, walk), "ct_tracer_set_walk")
+
+    def set_profiling(self, on):
+        check(lib().ct_tracer_set_profiling(self.handle, int(bool(on))), "profiling")
+
+    def set_counting(self, on):
+        check(lib().ct_tracer_set_counting(self.handle, int(bool(on))), "counting")
+
+    def accumulate_pass(self, ps, frame, width, height, frame_index, opts, d_accum,
+                        stream=None):
+        o = A.TraceOpts(int(opts.get("maxBounces", 8)), int(bool(opts.get("russianRoulette", True))),
+                        int(bool(opts.get("nee", True))))
+        fr = np.ascontiguousarray(frame, np.float64)
+        check(lib().ct_accumulate_pass(self.handle, device_scene(ps).handle,
+                                       fr.ctypes.data_as(C.c_void_p), width, height,
+                                       frame_index & 0xFFFFFFFF, C.byref(o), d_accum.ptr, stream),
+              "ct_accumulate_pass")
+
+    def render_image(self, ps, frame, opts, d_out, stream=None):
+        o = A.RenderOpts(opts["width"], opts["height"], opts["spp"], opts.get("seed", 0) & 0xFFFFFFFF,
+                         int(opts.get("maxBounces", 8)), int(bool(opts.get("russianRoulette", True))),
+                         int(bool(opts.get("nee", True))))
+        fr = np.ascontiguousarray(frame, np.float64)
+        check(lib().ct_render_image(self.handle, device_scene(ps).handle,
+                                    fr.ctypes.data_as(C.c_void_p), C.byref(o), d_out.ptr, stream),
+              "ct_render_image")
+
+    def stats(self):
+        s = A.RenderStats()
+        check(lib().ct_tracer_stats(self.handle, C.byref(s)), "ct_tracer_stats")
+        return {k: getattr(s, k) for k, _ in A.RenderStats._fields_}
+
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            lib().ct_tracer_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_tracer = None
+
+
+def _tracer():
+    global _default_tracer
+    if _default_tracer is None:
+        _default_tracer = Tracer()
+    return _default_tracer
+
+
+def accumulatePass(ps, frame, width, height, frameIndex, opts, accum):
+    """accumulatePass (tracer.ts:361-395): adds one sample per pixel into the
+    float32 sum `accum` (host numpy array, updated in place, or a DeviceBuffer)."""
+    if isinstance(accum, DeviceBuffer):
+        _tracer().accumulate_pass(ps, frame, width, height, frameIndex, opts, accum)
+        return
+    d = DeviceBuffer.from_array(np.ascontiguousarray(accum, np.float32))
+    _tracer().accumulate_pass(ps, frame, width, height, frameIndex, opts, d)
+    accum[...] = d.numpy().reshape(accum.shape)
+
+
+def renderImage(ps, frame, opts):
+    """renderImage (tracer.ts:348-355): linear RGB float32 (h*w*3), divided by spp."""
+    n = opts["width"] * opts["height"] * 3
+    d = DeviceBuffer.empty((n,), np.float32)
+    _tracer().render_image(ps, frame, opts, d)
+    return d.numpy()

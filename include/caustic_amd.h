@@ -1,0 +1,168 @@
+/*
+ * caustic_amd.h — C ABI of the MI355X (gfx950) path-tracing hot path.
+ *
+ * Plain pointers and sizes only.  Every array argument uses the packed layout
+ * the reference's packer emits (src/tracer/pack.ts:1-17, src/tracer/bvh.ts:1-16):
+ *   prims       float32, 16 per primitive   [a.xyz,type | b.xyz,obj | c.xyz,area | n.xyz,0]
+ *   nodes       float32,  8 per BVH node    [min.xyz,skip | max.xyz,leaf]
+ *   mats        float32, 16 per object      [rgb,type | param,chkScale,chkOn,0 | rgb2,0 | e.rgb,0]
+ *   lights      int32, packed indices of emitting primitives
+ *   primObject  uint32, packed primitive -> scene object
+ * Rays are binary64 like the JS numbers the reference passes: 6 per ray
+ * (ox,oy,oz,dx,dy,dz).  A camera frame is 16 binary64 values:
+ * origin.xyz, right.xyz, up.xyz, forward.xyz, tanHalfFov, aspect, lensRadius,
+ * focusDistance (camera.ts:8-22).
+ *
+ * Pointers named d_* are device (HBM) pointers; h_* are host pointers.
+ * Every call returns 0 on success or a negative CT_E* code; ct_last_error()
+ * returns the message of the calling thread's last failure.  The reference's
+ * hot path never throws (a miss is prim -1 / t = +inf, intersect.ts:213-226);
+ * errors here are only argument, allocation and launch failures.
+ * Handles are independent: calls on different handles may run concurrently
+ * from different threads; calls sharing one ct_tracer must be serialised.
+ */
+#ifndef CAUSTIC_AMD_H
+#define CAUSTIC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CT_OK 0
+#define CT_EINVAL -1  /* bad argument */
+#define CT_ENOMEM -2  /* device allocation failed */
+#define CT_EDEVICE -3 /* HIP runtime or launch error */
+#define CT_ELIMIT -4  /* input beyond a documented size limit */
+
+typedef struct ct_scene ct_scene;
+typedef struct ct_tracer ct_tracer;
+
+/* Sky (scene.ts:57-59): kind 0 none, 1 gradient. */
+typedef struct {
+  int32_t kind;
+  double zenith[3];
+  double horizon[3];
+  double strength;
+} ct_sky;
+
+/* TraceOptions (tracer.ts:27-31). */
+typedef struct {
+  int32_t maxBounces;
+  int32_t russianRoulette;
+  int32_t nee;
+} ct_trace_opts;
+
+/* RenderOptions (tracer.ts:337-342). */
+typedef struct {
+  uint32_t width;
+  uint32_t height;
+  uint32_t spp;
+  uint32_t seed;
+  int32_t maxBounces;
+  int32_t russianRoulette;
+  int32_t nee;
+} ct_render_opts;
+
+/* Which closest-hit walk a query mirrors. */
+#define CT_WALK_STACKLESS 0 /* intersect.ts:290-365 */
+#define CT_WALK_ORDERED 1   /* intersect.ts:202-282 */
+
+/* Counters of the last render on a tracer (not part of the reference API). */
+typedef struct {
+  uint64_t extensionRays;  /* closest-hit rays traced */
+  uint64_t shadowRays;     /* any-hit rays traced */
+  uint64_t paths;          /* camera paths started */
+  uint32_t extendLaunches; /* closest-hit kernel launches */
+  double extendMs;         /* summed device time of those launches (profiling on) */
+  double shadeMs;          /* summed device time of shade launches (profiling on) */
+  double shadowMs;         /* summed device time of any-hit launches (profiling on) */
+  uint64_t nodeVisits;     /* node fetches of closest-hit walks (counting on) */
+  uint64_t triTests;       /* triangle tests of closest-hit walks (counting on) */
+  uint64_t sphereTests;    /* sphere tests of closest-hit walks (counting on) */
+  uint64_t shadowNodeVisits; /* node fetches of any-hit walks (counting on) */
+  uint64_t shadowTriTests;   /* triangle tests of any-hit walks (counting on) */
+  uint64_t shadowSphereTests;/* sphere tests of any-hit walks (counting on) */
+} ct_render_stats;
+
+/* ---- runtime ------------------------------------------------------------ */
+int ct_init(int32_t device);
+const char *ct_last_error(void);
+/* Device memory for hosts without their own allocator (the N-API addon). */
+int ct_device_alloc(void **d_ptr, size_t bytes);
+int ct_device_free(void *d_ptr);
+int ct_copy_h2d(void *d_dst, const void *h_src, size_t bytes);
+int ct_copy_d2h(void *h_dst, const void *d_src, size_t bytes);
+int ct_stream_sync(void *stream);
+
+/* ---- scene (replaces handing PackedScene to the CPU tracer) --------------
+ * Copies the packed arrays (host pointers) into device-resident buffers owned
+ * by the handle.  Mirrors PackedScene (pack.ts:46-59). */
+int ct_scene_create(const float *h_prims, uint32_t primCount, const float *h_nodes,
+                    uint32_t nodeCount, const float *h_mats, uint32_t matCount,
+                    const int32_t *h_lights, uint32_t lightCount, const uint32_t *h_primObject,
+                    const ct_sky *sky, ct_scene **out);
+/* Same from device-resident arrays (copied on the stream). */
+int ct_scene_create_device(const float *d_prims, uint32_t primCount, const float *d_nodes,
+                           uint32_t nodeCount, const float *d_mats, uint32_t matCount,
+                           const int32_t *d_lights, uint32_t lightCount,
+                           const uint32_t *d_primObject, const ct_sky *sky, void *stream,
+                           ct_scene **out);
+int ct_scene_destroy(ct_scene *scene);
+
+/* ---- closest hit / any hit ----------------------------------------------
+ * Batched intersectClosest / intersectClosestStackless (intersect.ts:202,290):
+ * one query per ray with the given tMin and per-ray tMax (d_tMax NULL = +inf).
+ * Outputs per ray: prim (-1 on a miss), t (+inf on a miss), u, v. */
+int ct_closest_hit(const ct_scene *scene, const double *d_rays, uint32_t n, double tMin,
+                   const double *d_tMax, int32_t walk, int32_t *d_prim, double *d_t, double *d_u,
+                   double *d_v, void *stream);
+/* Batched occluded (intersect.ts:368): out[i] = 1 when (tMin, tMax[i]) is blocked. */
+int ct_occluded(const ct_scene *scene, const double *d_rays, uint32_t n, double tMin,
+                const double *d_tMax, uint8_t *d_out, void *stream);
+/* Traversal work of a closest-hit batch: counts[0] node fetches, [1] triangle
+ * tests, [2] sphere tests (the counters of scripts/debug-aabb.ts:27-55).
+ * Synchronises the stream. */
+int ct_closest_hit_counts(const ct_scene *scene, const double *d_rays, uint32_t n, double tMin,
+                          const double *d_tMax, int32_t walk, uint64_t counts[3], void *stream);
+
+/* ---- path tracing ---------------------------------------------------------
+ * A tracer owns the wavefront path state (SoA queues in HBM) and grows it on
+ * demand. */
+int ct_tracer_create(ct_tracer **out);
+int ct_tracer_destroy(ct_tracer *tracer);
+/* Device-time profiling of each launch (HIP events on the launch stream). */
+int ct_tracer_set_profiling(ct_tracer *tracer, int32_t enable);
+/* Per-ray node/primitive counters in the traversal kernels (slower; for the
+ * roofline's algorithmic byte count, never for timing). */
+int ct_tracer_set_counting(ct_tracer *tracer, int32_t enable);
+/* Closest-hit walk used by the extension rays (CT_WALK_*). */
+int ct_tracer_set_walk(ct_tracer *tracer, int32_t walk);
+/* accumulatePass (tracer.ts:361-395): adds one sample per pixel into the
+ * float32 sum d_accum (width*height*3) with frameIndex as the RNG frame. */
+int ct_accumulate_pass(ct_tracer *tracer, const ct_scene *scene, const double *h_frame16,
+                       uint32_t width, uint32_t height, uint32_t frameIndex,
+                       const ct_trace_opts *opts, float *d_accum, void *stream);
+/* renderImage (tracer.ts:348-355): spp passes with frameIndex = s + 7919*seed,
+ * then d_out *= 1/spp.  d_out is width*height*3 float32 (overwritten). */
+int ct_render_image(ct_tracer *tracer, const ct_scene *scene, const double *h_frame16,
+                    const ct_render_opts *opts, float *d_out, void *stream);
+/* Counters and device times of the last accumulate/render (synchronises). */
+int ct_tracer_stats(ct_tracer *tracer, ct_render_stats *out);
+
+/* ---- BVH build --------------------------------------------------------------
+ * buildBvh (bvh.ts:299-301) on the GPU from device-resident PrimBounds
+ * (bvh.ts:22-28: 3 binary64 per primitive for min, max and centroid).
+ * Writes nodes (room for max(1, 2*count-1)*8 floats), *nodeCount (host) and
+ * order (count uint32) bit-identical to the reference builder.  bins is the
+ * SAH bucket count (12 in the reference, bvh.ts:20). */
+int ct_bvh_build(const double *d_min, const double *d_max, const double *d_centroid,
+                 uint32_t count, int32_t bins, float *d_nodes, uint32_t *h_nodeCount,
+                 uint32_t *d_order, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -19,3 +19,24 @@ def pytest_configure(config):
 def O():
     from oracle import oracle as _O
     return _O
+
+
+def load_package():
+    """Import caustic-tracer_amd/ (a hyphenated directory) as caustic_tracer_amd."""
+    import importlib.util
+
+    name = "caustic_tracer_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    path = os.path.join(ROOT, "caustic-tracer_amd", "__init__.py")
+    spec = importlib.util.spec_from_file_location(
+        name, path, submodule_search_locations=[os.path.dirname(path)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.fixture(scope="session")
+def ct():
+    return load_package()

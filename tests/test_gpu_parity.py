@@ -1,0 +1,254 @@
+"""GPU parity: HIP kernels through the C ABI against the CPU oracle.
+
+Integer/index results (closest-hit prim ids, BVH node arrays, order, occlusion
+flags, traversal counters of the stackless walk) must be bit-exact.  Closest-hit
+t/u/v are binary64 on both sides and compared exactly as well.  Images must
+match per pixel within 1e-4 relative (the tolerance of the task statement);
+the observed difference is printed.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import scenes
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-4
+
+
+def rel_err(g, o):
+    g = np.asarray(g, np.float64)
+    o = np.asarray(o, np.float64)
+    den = np.maximum(np.abs(g), np.abs(o))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        r = np.where(den > 0, np.abs(g - o) / den, 0.0)
+    return r
+
+
+@pytest.fixture(scope="module")
+def bvh_scene(O):
+    return O.pack_scene(scenes.random_scene(O.Rng(1234), 1000, 2000))
+
+
+@pytest.fixture(scope="module")
+def many_rays(O):
+    return scenes.ray_set_bvh_test(O.Rng(99), 20000)
+
+
+@pytest.mark.parametrize("walk", [0, 1], ids=["stackless", "ordered"])
+def test_closest_hit_bit_exact(ct, O, bvh_scene, many_rays, walk):
+    owalk = O.WALK_STACKLESS if walk == ct.WALK_STACKLESS else O.WALK_ORDERED
+    g = ct.closest_hit(bvh_scene, many_rays, walk=walk)
+    o = O.closest_batch(bvh_scene, many_rays, walk=owalk)
+    assert (g["prim"] == o["prim"]).all()
+    assert np.array_equal(g["t"], o["t"])
+    hit = o["prim"] >= 0
+    assert np.array_equal(g["u"][hit], o["u"][hit]) and np.array_equal(g["v"][hit], o["v"][hit])
+    assert hit.sum() > 10000
+
+
+def test_closest_hit_finite_tmax_and_counts(ct, O, bvh_scene):
+    rng = O.Rng(5)
+    n = 5000
+    d = rng.take(7 * n).reshape(n, 7)
+    o = -12 + 24 * d[:, :3]
+    dv = -1 + 2 * d[:, 3:6]
+    dv /= np.linalg.norm(dv, axis=1, keepdims=True)
+    rays = np.concatenate([o, dv], 1)
+    tmax = 0.5 + 19.5 * d[:, 6]
+    for walk, owalk in ((ct.WALK_STACKLESS, O.WALK_STACKLESS), (ct.WALK_ORDERED, O.WALK_ORDERED)):
+        g = ct.closest_hit(bvh_scene, rays, tmax, walk=walk)
+        r = O.closest_batch(bvh_scene, rays, tmax, walk=owalk)
+        assert (g["prim"] == r["prim"]).all() and np.array_equal(g["t"], r["t"])
+    # traversal work: the stackless walk is an exact mirror, so are its counters
+    gc = ct.closest_hit_counts(bvh_scene, rays, tmax, walk=ct.WALK_STACKLESS)
+    oc = O.closest_batch(bvh_scene, rays, tmax, walk=O.WALK_STACKLESS)["counts"]
+    assert gc == (oc[0], oc[1], oc[2])
+    # the ordered walk prunes on float32-rounded stack entries: never fewer nodes
+    gc = ct.closest_hit_counts(bvh_scene, rays, tmax, walk=ct.WALK_ORDERED)
+    oc = O.closest_batch(bvh_scene, rays, tmax, walk=O.WALK_ORDERED)["counts"]
+    assert gc[0] >= oc[0] and gc[0] <= oc[0] * 1.001 + 10
+
+
+def test_occluded_bit_exact(ct, O, bvh_scene):
+    rng = O.Rng(7)
+    n = 5000
+    d = rng.take(7 * n).reshape(n, 7)
+    o = -12 + 24 * d[:, :3]
+    dv = -1 + 2 * d[:, 3:6]
+    dv /= np.linalg.norm(dv, axis=1, keepdims=True)
+    rays = np.concatenate([o, dv], 1)
+    tmax = 1 + 29 * d[:, 6]
+    g = ct.occluded(bvh_scene, rays, tmax)
+    r, _ = O.occluded_batch(bvh_scene, rays, tmax)
+    assert (g == r).all() and 0 < g.sum() < n
+
+
+def test_empty_and_coincident(ct, O):
+    empty = O.pack_scene({"objects": [], "sky": {"kind": "none"}})
+    g = ct.closest_hit(empty, np.array([[0, 0, 0, 0, 0, 1.0]]))
+    assert g["prim"][0] == -1 and g["t"][0] == math.inf
+    co = O.pack_scene(scenes.coincident_spheres(40))
+    g = ct.closest_hit(co, np.array([[1, 2, -5, 0, 0, 1.0]]))
+    r = O.closest_batch(co, np.array([[1, 2, -5, 0, 0, 1.0]]))
+    assert g["prim"][0] == r["prim"][0] and g["t"][0] == r["t"][0]
+    assert abs(g["t"][0] - 7.5) < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# BVH build
+
+def _bounds_inputs(O, scene):
+    kind, geom, obj = O.expand_scene(scene)
+    return O.prim_bounds(kind, geom)
+
+
+BUILD_CASES = {
+    "bvh_test_scene": lambda O: scenes.random_scene(O.Rng(1234), 1000, 2000),
+    "cornell": lambda O: scenes.cornell_box(),
+    "coincident40": lambda O: scenes.coincident_spheres(40),
+    "single": lambda O: scenes.coincident_spheres(1),
+    "five": lambda O: scenes.random_scene(O.Rng(3), 2, 3),
+    "glass_spheres": lambda O: scenes.glass_spheres(O.Rng(20240607)),
+}
+
+
+@pytest.mark.parametrize("case", sorted(BUILD_CASES))
+@pytest.mark.parametrize("bins", [12, 16])
+def test_gpu_bvh_build_bit_exact(ct, O, case, bins):
+    sc = BUILD_CASES[case](O)
+    bmin, bmax, bcen = _bounds_inputs(O, sc)
+    on, onc, oo = O.build_bvh(bmin, bmax, bcen, bins)
+    gn, gnc, go = ct.buildBvh(bmin.reshape(-1, 3), bmax.reshape(-1, 3), bcen.reshape(-1, 3), bins)
+    assert gnc == onc
+    assert np.array_equal(gn.view(np.uint32), on.view(np.uint32))
+    assert np.array_equal(go, oo)
+
+
+def test_gpu_bvh_build_empty(ct):
+    z = np.zeros((0, 3))
+    nodes, nc, order = ct.buildBvh(z, z, z)
+    assert nc == 1 and list(nodes) == [0, 0, 0, 1, 0, 0, 0, 0] and len(order) == 0
+
+
+@pytest.mark.parametrize("n", [20000, 200000])
+def test_gpu_bvh_build_soup_bit_exact(ct, O, n):
+    syn = ct.synth
+    rng = np.random.default_rng(n)
+    tris = syn.equilateral_soup(rng, n)
+    kind = np.ones(n, np.int32)
+    geom = tris.reshape(n, 9)
+    bmin, bmax, bcen = O.prim_bounds(kind, geom)
+    on, onc, oo = O.build_bvh(bmin, bmax, bcen, 12)
+    gn, gnc, go = ct.buildBvh(bmin.reshape(-1, 3), bmax.reshape(-1, 3), bcen.reshape(-1, 3), 12)
+    assert gnc == onc
+    assert np.array_equal(gn.view(np.uint32), on.view(np.uint32))
+    assert np.array_equal(go, oo)
+
+
+def test_pack_scene_matches_oracle(ct, O):
+    sc = scenes.random_scene(O.Rng(1234), 1000, 2000)
+    g = ct.packScene(sc)
+    o = O.pack_scene(sc)
+    assert g.primCount == o.primCount and g.nodeCount == o.nodeCount
+    assert np.array_equal(g.prims.view(np.uint32), o.prims.view(np.uint32))
+    assert np.array_equal(g.nodes.view(np.uint32), o.nodes.view(np.uint32))
+    assert np.array_equal(g.mats.view(np.uint32), o.mats.view(np.uint32))
+    assert np.array_equal(g.lights, o.lights) and np.array_equal(g.primObject, o.primObject)
+    for mk in (scenes.cornell_box, scenes.night_emissives):
+        sc = mk()
+        g, o = ct.packScene(sc), O.pack_scene(sc)
+        assert np.array_equal(g.prims.view(np.uint32), o.prims.view(np.uint32))
+        assert np.array_equal(g.nodes.view(np.uint32), o.nodes.view(np.uint32))
+        assert np.array_equal(g.lights, o.lights)
+
+
+# ---------------------------------------------------------------------------
+# rendering
+
+def _render_pair(ct, O, sc, w, h, spp, seed, **kw):
+    ps = O.pack_scene(sc)
+    fr = O.camera_frame(sc["camera"], w / h)
+    o = O.render_image(ps, fr, w, h, spp, seed=seed, **kw)
+    opts = {"width": w, "height": h, "spp": spp, "seed": seed,
+            "maxBounces": kw.get("max_bounces", 8),
+            "russianRoulette": kw.get("russian_roulette", True), "nee": kw.get("nee", True)}
+    g = ct.renderImage(ps, fr, opts)
+    return g, o
+
+
+def test_render_cornell_matches_oracle_and_golden(ct, O):
+    from test_oracle_golden import GOLDEN, read_ppm
+    g, o = _render_pair(ct, O, scenes.cornell_box(), 96, 64, 16, 1)
+    err = rel_err(g, o)
+    print(f"cornell 96x64x16: max rel err {err.max():.3e}, exact pixels {(g == o).mean():.4f}")
+    assert err.max() <= REL_TOL
+    rgba = O.tonemap_buffer(g, 96, 64, 1, 0.0, "aces").reshape(64, 96, 4)
+    _, _, gold = read_ppm(GOLDEN)
+    diff = np.abs(rgba[:, :, :3].astype(int) - gold.astype(int))
+    assert diff.max() <= 1 and (diff > 0).mean() < 1e-3
+
+
+@pytest.mark.parametrize("walk", [0, 1], ids=["stackless", "ordered"])
+def test_render_c1_cornell_variant(ct, O, walk):
+    sc = scenes.cornell_c1()
+    ps = O.pack_scene(sc, bins=16)
+    w = h = 128
+    fr = O.camera_frame(sc["camera"], 1.0)
+    o = O.render_image(ps, fr, w, h, 4, seed=1, max_bounces=8)
+    tr = ct.Tracer(walk=walk)
+    d = ct.DeviceBuffer.empty((w * h * 3,), np.float32)
+    tr.render_image(ps, fr, {"width": w, "height": h, "spp": 4, "seed": 1, "maxBounces": 8}, d)
+    g = d.numpy()
+    err = rel_err(g, o)
+    print(f"C1 128^2x4 walk={walk}: max rel err {err.max():.3e}")
+    assert err.max() <= REL_TOL
+
+
+def test_accumulate_pass_lens_sky_checker(ct, O):
+    sc = scenes.glass_spheres(O.Rng(20240607))
+    ps = O.pack_scene(sc)
+    w, h = 96, 64
+    fr = O.camera_frame(sc["camera"], w / h)
+    acc_o = np.zeros(w * h * 3, np.float32)
+    acc_g = np.zeros(w * h * 3, np.float32)
+    for f in (11, 12):
+        O.accumulate_pass(ps, fr, w, h, f, acc_o, max_bounces=8)
+        ct.accumulatePass(ps, fr, w, h, f, {"maxBounces": 8}, acc_g)
+    err = rel_err(acc_g, acc_o)
+    print(f"glass spheres 2 passes: max rel err {err.max():.3e}")
+    assert err.max() <= REL_TOL
+
+
+def test_sphere_lights(ct, O):
+    g, o = _render_pair(ct, O, scenes.night_emissives(), 64, 48, 4, 2)
+    err = rel_err(g, o)
+    print(f"night emissives: max rel err {err.max():.3e}")
+    assert err.max() <= REL_TOL
+
+
+@pytest.mark.parametrize("nee,rr,seed,tol_mean,tol_px", [(False, False, 3, 0.05, 0.05),
+                                                         (True, False, 9, 0.03, 0.08),
+                                                         (False, True, 5, 0.03, None)])
+def test_furnace_on_gpu(ct, O, nee, rr, seed, tol_mean, tol_px):
+    sc = scenes.furnace()
+    ps = O.pack_scene(sc)
+    fr = O.camera_frame(sc["camera"], 1.0)
+    w = 8 if not nee and not rr else 4
+    img = ct.renderImage(ps, fr, {"width": w, "height": w, "spp": 4096, "seed": seed,
+                                  "maxBounces": 64, "russianRoulette": rr, "nee": nee})
+    assert abs(img.mean() - 5) / 5 < tol_mean
+    if tol_px is not None:
+        assert np.all(np.abs(img - 5) / 5 < tol_px)
+
+
+def test_render_is_deterministic(ct, O):
+    sc = scenes.cornell_c1()
+    ps = O.pack_scene(sc)
+    fr = O.camera_frame(sc["camera"], 1.0)
+    opts = {"width": 64, "height": 64, "spp": 8, "seed": 4, "maxBounces": 8}
+    a = ct.renderImage(ps, fr, opts)
+    b = ct.renderImage(ps, fr, opts)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
