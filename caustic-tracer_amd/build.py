@@ -38,5 +38,28 @@ def build(verbose=False):
     return LIB
 
 
+NODE_DIR = os.path.join(HERE, "node")
+ADDON = os.path.join(NODE_DIR, "caustic_amd.node")
+NODE_INCLUDE = "/usr/include/node"
+
+
+def build_addon(verbose=False):
+    """N-API addon (plain shared object against node_api.h, linked to the library)."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        return None
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-Wall", "-I", NODE_INCLUDE,
+           os.path.join(NODE_DIR, "caustic_napi.c"), "-o", ADDON + ".tmp", "-L", HERE,
+           "-lcaustic_amd", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("gcc failed building the N-API addon")
+    os.replace(ADDON + ".tmp", ADDON)
+    return ADDON
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_addon(verbose=True))

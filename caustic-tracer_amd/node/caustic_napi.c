@@ -1,0 +1,413 @@
+/*
+ * N-API addon: the Node/TypeScript side of the C ABI (include/caustic_amd.h).
+ *
+ * Each export takes the same typed arrays the reference's packer and tracer
+ * use (pack.ts:46-59, tracer.ts:337-395) and forwards them to libcaustic_amd.so.
+ * Bulk arrays cross as host typed arrays; the addon stages them through
+ * device buffers.  A non-zero status from the library becomes a thrown Error
+ * carrying ct_last_error().
+ *
+ *   createScene(prims, primCount, nodes, nodeCount, mats, matCount, lights,
+ *               primObject, sky?) -> scene handle            (PackedScene upload)
+ *   buildBvh(min, max, centroid, bins) -> {nodes, nodeCount, order}   (bvh.ts:299)
+ *   closestHit(scene, rays, tMin, tMax|null, walk)
+ *       -> {prim: Int32Array, t, u, v: Float64Array}          (intersect.ts:202/290)
+ *   occluded(scene, rays, tMin, tMax) -> Uint8Array            (intersect.ts:368)
+ *   accumulatePass(scene, frame16, width, height, frameIndex, maxBounces,
+ *                  russianRoulette, nee, accum)               (tracer.ts:361)
+ *   renderImage(scene, frame16, width, height, spp, seed, maxBounces,
+ *               russianRoulette, nee) -> Float32Array          (tracer.ts:348)
+ */
+#define NAPI_VERSION 4
+#include <node_api.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/caustic_amd.h"
+
+#define CALL(env, expr)                                               \
+  do {                                                                \
+    if ((expr) != napi_ok) {                                          \
+      napi_throw_error((env), NULL, "caustic_amd: N-API call failed: " #expr); \
+      return NULL;                                                    \
+    }                                                                 \
+  } while (0)
+
+static napi_value throw_ct(napi_env env, int rc, const char *what) {
+  char msg[512];
+  snprintf(msg, sizeof msg, "%s failed (%d): %s", what, rc, ct_last_error());
+  napi_throw_error(env, NULL, msg);
+  return NULL;
+}
+
+static int g_init = 0;
+static int ensure_init(napi_env env) {
+  if (g_init) return 1;
+  int rc = ct_init(0);
+  if (rc) {
+    throw_ct(env, rc, "ct_init");
+    return 0;
+  }
+  g_init = 1;
+  return 1;
+}
+
+/* typed-array view: data pointer and element count (NULL/0 for null/undefined) */
+static int typed(napi_env env, napi_value v, napi_typedarray_type want, void **data,
+                 size_t *len) {
+  napi_valuetype t;
+  *data = NULL;
+  *len = 0;
+  if (napi_typeof(env, v, &t) != napi_ok) return 0;
+  if (t == napi_null || t == napi_undefined) return 1;
+  bool is = false;
+  napi_is_typedarray(env, v, &is);
+  if (!is) {
+    napi_throw_type_error(env, NULL, "caustic_amd: expected a typed array");
+    return 0;
+  }
+  napi_typedarray_type ty;
+  napi_value ab;
+  size_t off;
+  if (napi_get_typedarray_info(env, v, &ty, len, data, &ab, &off) != napi_ok) return 0;
+  if (ty != want) {
+    napi_throw_type_error(env, NULL, "caustic_amd: typed array of the wrong element type");
+    return 0;
+  }
+  return 1;
+}
+
+static double num(napi_env env, napi_value v) {
+  double d = 0;
+  napi_get_value_double(env, v, &d);
+  return d;
+}
+
+static void *to_device(const void *h, size_t bytes, int *rc) {
+  void *d = NULL;
+  *rc = ct_device_alloc(&d, bytes);
+  if (*rc) return NULL;
+  if (bytes) *rc = ct_copy_h2d(d, h, bytes);
+  if (*rc) {
+    ct_device_free(d);
+    return NULL;
+  }
+  return d;
+}
+
+static napi_value new_typed(napi_env env, napi_typedarray_type ty, size_t elems, size_t esize,
+                            void **data) {
+  napi_value ab, arr;
+  if (napi_create_arraybuffer(env, elems * esize, data, &ab) != napi_ok) return NULL;
+  if (napi_create_typedarray(env, ty, elems, ab, 0, &arr) != napi_ok) return NULL;
+  return arr;
+}
+
+/* ---- scene handle ----------------------------------------------------- */
+static void scene_finalize(napi_env env, void *data, void *hint) {
+  (void)env;
+  (void)hint;
+  ct_scene_destroy((ct_scene *)data);
+}
+
+static napi_value create_scene(napi_env env, napi_callback_info info) {
+  size_t argc = 9;
+  napi_value a[9];
+  CALL(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  if (argc < 8) {
+    napi_throw_error(env, NULL, "createScene: 8 arguments required");
+    return NULL;
+  }
+  if (!ensure_init(env)) return NULL;
+  void *prims, *nodes, *mats, *lights, *pobj;
+  size_t np_, nn, nm, nl, no;
+  if (!typed(env, a[0], napi_float32_array, &prims, &np_)) return NULL;
+  uint32_t primCount = (uint32_t)num(env, a[1]);
+  if (!typed(env, a[2], napi_float32_array, &nodes, &nn)) return NULL;
+  uint32_t nodeCount = (uint32_t)num(env, a[3]);
+  if (!typed(env, a[4], napi_float32_array, &mats, &nm)) return NULL;
+  uint32_t matCount = (uint32_t)num(env, a[5]);
+  if (!typed(env, a[6], napi_int32_array, &lights, &nl)) return NULL;
+  if (!typed(env, a[7], napi_uint32_array, &pobj, &no)) return NULL;
+  if (np_ < (size_t)primCount * 16 || nn < (size_t)nodeCount * 8 || nm < (size_t)matCount * 16 ||
+      no < primCount) {
+    napi_throw_range_error(env, NULL, "createScene: array shorter than its count");
+    return NULL;
+  }
+  ct_sky sky;
+  memset(&sky, 0, sizeof sky);
+  if (argc > 8) {
+    napi_valuetype t;
+    napi_typeof(env, a[8], &t);
+    if (t == napi_object) {
+      /* {kind: 'gradient', zenith:{r,g,b}, horizon:{r,g,b}, strength} (scene.ts:57-59) */
+      napi_value kind, z, h, s, c;
+      char buf[16] = {0};
+      size_t l;
+      if (napi_get_named_property(env, a[8], "kind", &kind) == napi_ok &&
+          napi_get_value_string_utf8(env, kind, buf, sizeof buf, &l) == napi_ok &&
+          strcmp(buf, "gradient") == 0) {
+        static const char *ch[3] = {"r", "g", "b"};
+        sky.kind = 1;
+        CALL(env, napi_get_named_property(env, a[8], "zenith", &z));
+        CALL(env, napi_get_named_property(env, a[8], "horizon", &h));
+        CALL(env, napi_get_named_property(env, a[8], "strength", &s));
+        for (int k = 0; k < 3; k++) {
+          CALL(env, napi_get_named_property(env, z, ch[k], &c));
+          sky.zenith[k] = num(env, c);
+          CALL(env, napi_get_named_property(env, h, ch[k], &c));
+          sky.horizon[k] = num(env, c);
+        }
+        sky.strength = num(env, s);
+      }
+    }
+  }
+  ct_scene *scene = NULL;
+  int rc = ct_scene_create((const float *)prims, primCount, (const float *)nodes, nodeCount,
+                           (const float *)mats, matCount, (const int32_t *)lights, (uint32_t)nl,
+                           (const uint32_t *)pobj, &sky, &scene);
+  if (rc) return throw_ct(env, rc, "ct_scene_create");
+  napi_value ext;
+  CALL(env, napi_create_external(env, scene, scene_finalize, NULL, &ext));
+  return ext;
+}
+
+static ct_scene *get_scene(napi_env env, napi_value v) {
+  void *p = NULL;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, NULL, "caustic_amd: not a scene handle");
+    return NULL;
+  }
+  return (ct_scene *)p;
+}
+
+/* ---- buildBvh ----------------------------------------------------------- */
+static napi_value build_bvh(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value a[4];
+  CALL(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  if (!ensure_init(env)) return NULL;
+  void *mn, *mx, *cn;
+  size_t l0, l1, l2;
+  if (!typed(env, a[0], napi_float64_array, &mn, &l0) ||
+      !typed(env, a[1], napi_float64_array, &mx, &l1) ||
+      !typed(env, a[2], napi_float64_array, &cn, &l2))
+    return NULL;
+  if (l0 != l1 || l1 != l2 || l0 % 3) {
+    napi_throw_range_error(env, NULL, "buildBvh: PrimBounds arrays must be 3*count long");
+    return NULL;
+  }
+  int bins = argc > 3 ? (int)num(env, a[3]) : 12;
+  uint32_t n = (uint32_t)(l0 / 3);
+  size_t maxNodes = n ? 2 * (size_t)n - 1 : 1;
+  int rc = 0;
+  void *dmn = to_device(mn, l0 * 8, &rc), *dmx = NULL, *dcn = NULL, *dnodes = NULL, *dord = NULL;
+  if (!rc) dmx = to_device(mx, l1 * 8, &rc);
+  if (!rc) dcn = to_device(cn, l2 * 8, &rc);
+  if (!rc) rc = ct_device_alloc(&dnodes, maxNodes * 32);
+  if (!rc) rc = ct_device_alloc(&dord, (n ? n : 1) * 4);
+  uint32_t nodeCount = 0;
+  if (!rc) rc = ct_bvh_build((const double *)dmn, (const double *)dmx, (const double *)dcn, n,
+                             bins, (float *)dnodes, &nodeCount, (uint32_t *)dord, NULL);
+  napi_value res = NULL;
+  if (!rc) {
+    void *hn, *ho;
+    napi_value nodes = new_typed(env, napi_float32_array, (size_t)nodeCount * 8, 4, &hn);
+    napi_value order = new_typed(env, napi_uint32_array, n, 4, &ho);
+    rc = ct_copy_d2h(hn, dnodes, (size_t)nodeCount * 32);
+    if (!rc && n) rc = ct_copy_d2h(ho, dord, (size_t)n * 4);
+    if (!rc) {
+      napi_value nc;
+      napi_create_object(env, &res);
+      napi_create_uint32(env, nodeCount, &nc);
+      napi_set_named_property(env, res, "nodes", nodes);
+      napi_set_named_property(env, res, "nodeCount", nc);
+      napi_set_named_property(env, res, "order", order);
+    }
+  }
+  ct_device_free(dmn);
+  ct_device_free(dmx);
+  ct_device_free(dcn);
+  ct_device_free(dnodes);
+  ct_device_free(dord);
+  if (rc) return throw_ct(env, rc, "ct_bvh_build");
+  return res;
+}
+
+/* ---- closestHit / occluded --------------------------------------------- */
+static napi_value closest_hit(napi_env env, napi_callback_info info) {
+  size_t argc = 5;
+  napi_value a[5];
+  CALL(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  ct_scene *s = get_scene(env, a[0]);
+  if (!s) return NULL;
+  void *rays, *tmax;
+  size_t nr, nt;
+  if (!typed(env, a[1], napi_float64_array, &rays, &nr)) return NULL;
+  double tMin = num(env, a[2]);
+  if (!typed(env, a[3], napi_float64_array, &tmax, &nt)) return NULL;
+  int walk = argc > 4 ? (int)num(env, a[4]) : CT_WALK_ORDERED;
+  uint32_t n = (uint32_t)(nr / 6);
+  if (tmax && nt < n) {
+    napi_throw_range_error(env, NULL, "closestHit: tMax shorter than the ray count");
+    return NULL;
+  }
+  int rc = 0;
+  void *dr = to_device(rays, (size_t)n * 48, &rc), *dt = NULL, *dp = NULL, *dT = NULL,
+       *dU = NULL, *dV = NULL;
+  if (!rc && tmax) dt = to_device(tmax, (size_t)n * 8, &rc);
+  if (!rc) rc = ct_device_alloc(&dp, (size_t)n * 4);
+  if (!rc) rc = ct_device_alloc(&dT, (size_t)n * 8);
+  if (!rc) rc = ct_device_alloc(&dU, (size_t)n * 8);
+  if (!rc) rc = ct_device_alloc(&dV, (size_t)n * 8);
+  if (!rc) rc = ct_closest_hit(s, (const double *)dr, n, tMin, (const double *)dt, walk,
+                               (int32_t *)dp, (double *)dT, (double *)dU, (double *)dV, NULL);
+  napi_value res = NULL;
+  if (!rc) {
+    void *hp, *hT, *hU, *hV;
+    napi_value P = new_typed(env, napi_int32_array, n, 4, &hp);
+    napi_value T = new_typed(env, napi_float64_array, n, 8, &hT);
+    napi_value U = new_typed(env, napi_float64_array, n, 8, &hU);
+    napi_value V = new_typed(env, napi_float64_array, n, 8, &hV);
+    if (n) {
+      rc = ct_copy_d2h(hp, dp, (size_t)n * 4);
+      if (!rc) rc = ct_copy_d2h(hT, dT, (size_t)n * 8);
+      if (!rc) rc = ct_copy_d2h(hU, dU, (size_t)n * 8);
+      if (!rc) rc = ct_copy_d2h(hV, dV, (size_t)n * 8);
+    }
+    napi_create_object(env, &res);
+    napi_set_named_property(env, res, "prim", P);
+    napi_set_named_property(env, res, "t", T);
+    napi_set_named_property(env, res, "u", U);
+    napi_set_named_property(env, res, "v", V);
+  }
+  ct_device_free(dr);
+  ct_device_free(dt);
+  ct_device_free(dp);
+  ct_device_free(dT);
+  ct_device_free(dU);
+  ct_device_free(dV);
+  if (rc) return throw_ct(env, rc, "ct_closest_hit");
+  return res;
+}
+
+static napi_value occluded(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value a[4];
+  CALL(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  ct_scene *s = get_scene(env, a[0]);
+  if (!s) return NULL;
+  void *rays, *tmax;
+  size_t nr, nt;
+  if (!typed(env, a[1], napi_float64_array, &rays, &nr)) return NULL;
+  double tMin = num(env, a[2]);
+  if (!typed(env, a[3], napi_float64_array, &tmax, &nt)) return NULL;
+  uint32_t n = (uint32_t)(nr / 6);
+  if (!tmax || nt < n) {
+    napi_throw_range_error(env, NULL, "occluded: one tMax per ray required");
+    return NULL;
+  }
+  int rc = 0;
+  void *dr = to_device(rays, (size_t)n * 48, &rc), *dt = NULL, *dout = NULL;
+  if (!rc) dt = to_device(tmax, (size_t)n * 8, &rc);
+  if (!rc) rc = ct_device_alloc(&dout, n ? n : 1);
+  if (!rc) rc = ct_occluded(s, (const double *)dr, n, tMin, (const double *)dt, (uint8_t *)dout, NULL);
+  napi_value res = NULL;
+  if (!rc) {
+    void *h;
+    res = new_typed(env, napi_uint8_array, n, 1, &h);
+    if (n) rc = ct_copy_d2h(h, dout, n);
+  }
+  ct_device_free(dr);
+  ct_device_free(dt);
+  ct_device_free(dout);
+  if (rc) return throw_ct(env, rc, "ct_occluded");
+  return res;
+}
+
+/* ---- accumulatePass / renderImage ---------------------------------------- */
+static ct_tracer *g_tracer = NULL;
+
+static napi_value accumulate_pass(napi_env env, napi_callback_info info) {
+  size_t argc = 9;
+  napi_value a[9];
+  CALL(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  ct_scene *s = get_scene(env, a[0]);
+  if (!s) return NULL;
+  void *frame, *accum;
+  size_t nf, na;
+  if (!typed(env, a[1], napi_float64_array, &frame, &nf)) return NULL;
+  uint32_t w = (uint32_t)num(env, a[2]), h = (uint32_t)num(env, a[3]);
+  uint32_t frameIndex = (uint32_t)(int64_t)num(env, a[4]);
+  ct_trace_opts o = {(int32_t)num(env, a[5]), (int32_t)num(env, a[6]), (int32_t)num(env, a[7])};
+  if (!typed(env, a[8], napi_float32_array, &accum, &na)) return NULL;
+  if (nf < 16 || na < (size_t)w * h * 3) {
+    napi_throw_range_error(env, NULL, "accumulatePass: frame needs 16 values, accum w*h*3");
+    return NULL;
+  }
+  int rc = 0;
+  if (!g_tracer) rc = ct_tracer_create(&g_tracer);
+  void *d = NULL;
+  if (!rc) d = to_device(accum, (size_t)w * h * 12, &rc);
+  if (!rc) rc = ct_accumulate_pass(g_tracer, s, (const double *)frame, w, h, frameIndex, &o,
+                                   (float *)d, NULL);
+  if (!rc) rc = ct_copy_d2h(accum, d, (size_t)w * h * 12);
+  ct_device_free(d);
+  if (rc) return throw_ct(env, rc, "ct_accumulate_pass");
+  return NULL;
+}
+
+static napi_value render_image(napi_env env, napi_callback_info info) {
+  size_t argc = 9;
+  napi_value a[9];
+  CALL(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  ct_scene *s = get_scene(env, a[0]);
+  if (!s) return NULL;
+  void *frame;
+  size_t nf;
+  if (!typed(env, a[1], napi_float64_array, &frame, &nf)) return NULL;
+  if (nf < 16) {
+    napi_throw_range_error(env, NULL, "renderImage: frame needs 16 values");
+    return NULL;
+  }
+  ct_render_opts o;
+  o.width = (uint32_t)num(env, a[2]);
+  o.height = (uint32_t)num(env, a[3]);
+  o.spp = (uint32_t)num(env, a[4]);
+  o.seed = (uint32_t)(int64_t)num(env, a[5]);
+  o.maxBounces = (int32_t)num(env, a[6]);
+  o.russianRoulette = (int32_t)num(env, a[7]);
+  o.nee = (int32_t)num(env, a[8]);
+  size_t n = (size_t)o.width * o.height * 3;
+  int rc = 0;
+  if (!g_tracer) rc = ct_tracer_create(&g_tracer);
+  void *d = NULL;
+  if (!rc) rc = ct_device_alloc(&d, n * 4);
+  if (!rc) rc = ct_render_image(g_tracer, s, (const double *)frame, &o, (float *)d, NULL);
+  napi_value res = NULL;
+  if (!rc) {
+    void *h;
+    res = new_typed(env, napi_float32_array, n, 4, &h);
+    rc = ct_copy_d2h(h, d, n * 4);
+  }
+  ct_device_free(d);
+  if (rc) return throw_ct(env, rc, "ct_render_image");
+  return res;
+}
+
+static napi_value init(napi_env env, napi_value exports) {
+  napi_property_descriptor props[] = {
+      {"createScene", NULL, create_scene, NULL, NULL, NULL, napi_default, NULL},
+      {"buildBvh", NULL, build_bvh, NULL, NULL, NULL, napi_default, NULL},
+      {"closestHit", NULL, closest_hit, NULL, NULL, NULL, napi_default, NULL},
+      {"occluded", NULL, occluded, NULL, NULL, NULL, napi_default, NULL},
+      {"accumulatePass", NULL, accumulate_pass, NULL, NULL, NULL, napi_default, NULL},
+      {"renderImage", NULL, render_image, NULL, NULL, NULL, napi_default, NULL},
+  };
+  CALL(env, napi_define_properties(env, exports, sizeof props / sizeof props[0], props));
+  return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, init)

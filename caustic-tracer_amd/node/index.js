@@ -1,0 +1,97 @@
+'use strict'
+/**
+ * Node mirror of the reference tracer's public API for the accelerated path
+ * (src/tracer/index.ts:1-20): the functions keep the reference's names,
+ * arguments and return values, and run on the MI355X through the N-API addon.
+ * A PackedScene is the object packScene (pack.ts:195) returns; it is uploaded
+ * once and cached per object.  Plain JS (no TypeScript-only syntax), so Node 12
+ * loads it directly.
+ */
+const path = require('path')
+const addon = require(path.join(__dirname, 'caustic_amd.node'))
+
+const WALK_STACKLESS = 0
+const WALK_ORDERED = 1
+const T_MIN = 1e-4
+
+const scenes = new WeakMap()
+function deviceScene(ps) {
+  let h = scenes.get(ps)
+  if (h === undefined) {
+    if (!('primCount' in ps)) throw new TypeError('caustic_amd: expected a PackedScene (packScene output)')
+    h = addon.createScene(ps.prims, ps.primCount, ps.nodes, ps.nodeCount, ps.mats, ps.matCount,
+      ps.lights, ps.primObject, ps.sky)
+    scenes.set(ps, h)
+  }
+  return h
+}
+
+/** CameraFrame (camera.ts:8-22) -> the 16 doubles of the C ABI. */
+function frameArray(f) {
+  if (f instanceof Float64Array) return f
+  return Float64Array.from([f.origin.x, f.origin.y, f.origin.z, f.right.x, f.right.y, f.right.z,
+    f.up.x, f.up.y, f.up.z, f.forward.x, f.forward.y, f.forward.z, f.tanHalfFov, f.aspect,
+    f.lensRadius, f.focusDistance])
+}
+
+/** buildBvh (bvh.ts:299) on the GPU; same {nodes, nodeCount, order}. */
+function buildBvh(pb, bins) {
+  return addon.buildBvh(pb.min, pb.max, pb.centroid, bins === undefined ? 12 : bins)
+}
+
+function single(ps, walk, ox, oy, oz, dx, dy, dz, tMin, tMax, hit) {
+  const r = addon.closestHit(deviceScene(ps), Float64Array.of(ox, oy, oz, dx, dy, dz), tMin,
+    tMax === Infinity ? null : Float64Array.of(tMax), walk)
+  hit.prim = r.prim[0]
+  hit.t = r.t[0]
+  if (hit.prim >= 0) {
+    hit.u = r.u[0]
+    hit.v = r.v[0]
+  }
+  return hit.prim >= 0
+}
+
+/** intersectClosest (intersect.ts:202). */
+function intersectClosest(ps, ox, oy, oz, dx, dy, dz, tMin, tMax, hit) {
+  return single(ps, WALK_ORDERED, ox, oy, oz, dx, dy, dz, tMin, tMax, hit)
+}
+
+/** intersectClosestStackless (intersect.ts:290). */
+function intersectClosestStackless(ps, ox, oy, oz, dx, dy, dz, tMin, tMax, hit) {
+  return single(ps, WALK_STACKLESS, ox, oy, oz, dx, dy, dz, tMin, tMax, hit)
+}
+
+/** occluded (intersect.ts:368). */
+function occluded(ps, ox, oy, oz, dx, dy, dz, tMin, tMax) {
+  return addon.occluded(deviceScene(ps), Float64Array.of(ox, oy, oz, dx, dy, dz), tMin,
+    Float64Array.of(tMax))[0] === 1
+}
+
+/** Batched closest hit: rays = Float64Array(6n); tMax = Float64Array(n) or null. */
+function closestHitBatch(ps, rays, tMin, tMax, walk) {
+  return addon.closestHit(deviceScene(ps), rays, tMin === undefined ? T_MIN : tMin,
+    tMax || null, walk === undefined ? WALK_ORDERED : walk)
+}
+
+/** Batched occluded: Uint8Array of 0/1. */
+function occludedBatch(ps, rays, tMin, tMax) {
+  return addon.occluded(deviceScene(ps), rays, tMin === undefined ? T_MIN : tMin, tMax)
+}
+
+/** accumulatePass (tracer.ts:361): adds one sample per pixel into accum. */
+function accumulatePass(ps, frame, width, height, frameIndex, opts, accum) {
+  addon.accumulatePass(deviceScene(ps), frameArray(frame), width, height, frameIndex,
+    opts.maxBounces, opts.russianRoulette ? 1 : 0, opts.nee ? 1 : 0, accum)
+}
+
+/** renderImage (tracer.ts:348): linear RGB averaged over opts.spp. */
+function renderImage(ps, frame, opts) {
+  const seed = opts.seed === undefined ? 0 : opts.seed
+  return addon.renderImage(deviceScene(ps), frameArray(frame), opts.width, opts.height, opts.spp,
+    seed, opts.maxBounces, opts.russianRoulette ? 1 : 0, opts.nee ? 1 : 0)
+}
+
+module.exports = {
+  WALK_STACKLESS, WALK_ORDERED, T_MIN, buildBvh, intersectClosest, intersectClosestStackless,
+  occluded, closestHitBatch, occludedBatch, accumulatePass, renderImage, frameArray,
+}

@@ -1,0 +1,58 @@
+"""The N-API addon and its Node mirror (caustic-tracer_amd/node): loads in the
+image's Node, exports the reference API names, and renders through the GPU."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import scenes
+from conftest import ROOT
+
+NODE = shutil.which("node")
+INDEX = os.path.join(ROOT, "caustic-tracer_amd", "node", "index.js")
+ADDON = os.path.join(ROOT, "caustic-tracer_amd", "node", "caustic_amd.node")
+
+needs_node = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON),
+                                reason="node or the built addon is absent")
+
+
+@needs_node
+def test_addon_loads_and_exports_reference_names():
+    out = subprocess.run([NODE, "-e", f"console.log(Object.keys(require({INDEX!r})).join(','))"],
+                         capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    names = set(out.stdout.strip().split(","))
+    for n in ("buildBvh", "intersectClosest", "intersectClosestStackless", "occluded",
+              "accumulatePass", "renderImage"):
+        assert n in names
+
+
+@needs_node
+@pytest.mark.gpu
+def test_node_render_matches_oracle(O, tmp_path):
+    sc = scenes.cornell_box()
+    ps = O.pack_scene(sc)
+    w, h, spp = 48, 32, 4
+    fr = O.camera_frame(sc["camera"], w / h)
+    for name, arr in (("prims", ps.prims), ("nodes", ps.nodes), ("mats", ps.mats),
+                      ("lights", ps.lights), ("primObject", ps.primObject), ("frame", fr)):
+        arr.tofile(tmp_path / f"{name}.bin")
+    (tmp_path / "meta.json").write_text(json.dumps(
+        {"primCount": ps.primCount, "nodeCount": ps.nodeCount, "matCount": ps.matCount}))
+    script = os.path.join(ROOT, "tests", "node", "render_cornell.js")
+    r = subprocess.run([NODE, script, str(tmp_path), str(w), str(h), str(spp), "1"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    g = np.fromfile(tmp_path / "out.bin", np.float32)
+    o = O.render_image(ps, fr, w, h, spp, seed=1)
+    den = np.maximum(np.abs(g), np.abs(o)).astype(np.float64)
+    err = np.where(den > 0, np.abs(g - o) / np.where(den > 0, den, 1), 0)
+    assert err.max() <= 1e-4
+    res = json.loads((tmp_path / "hit.json").read_text())
+    ref = O.closest_batch(ps, np.array([[0, 0.5, 1.6, 0, 0, -1.0]]))
+    assert res["r"] and res["hit"]["prim"] == int(ref["prim"][0]) and res["hit"]["t"] == ref["t"][0]
+    occ, _ = O.occluded_batch(ps, np.array([[0, 0.5, 1.6, 0, 0, -1.0]]), 100.0)
+    assert res["blocked"] == bool(occ[0])
