@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--spp", type=int, default=4)
     ap.add_argument("--depth", type=int, default=6)
     ap.add_argument("--walk", choices=["ordered", "stackless"], default="ordered")
+    ap.add_argument("--shadow-walk", choices=["ordered", "stackless"], default="ordered")
+    ap.add_argument("--stack", type=int, default=10, help="LDS stack entries of the ordered walks")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     args = ap.parse_args()
@@ -127,6 +129,8 @@ def main():
     frame = ct.cameraFrame(scene["camera"], w / h)
     walk = ct.WALK_ORDERED if args.walk == "ordered" else ct.WALK_STACKLESS
     tr = ct.Tracer(walk=walk)
+    tr.set_traversal(ct.WALK_ORDERED if args.shadow_walk == "ordered" else ct.WALK_STACKLESS,
+                     args.stack)
     out = ct.DeviceBuffer.empty((w * h * 3,), np.float32)
     opts = {"width": w, "height": h, "spp": spp, "seed": 1, "maxBounces": depth,
             "russianRoulette": True, "nee": True}
@@ -165,11 +169,14 @@ def main():
     achieved = ext_bytes / ext_s / 1e9 if ext_s > 0 else 0.0
     log(f"extend: {launches} launches, {prof['extendMs']:.2f} ms total, "
         f"{cnt['nodeVisits'] / max(1, cnt['extensionRays']):.1f} nodes/ray, "
-        f"{cnt['triTests'] / max(1, cnt['extensionRays']):.1f} tris/ray, {achieved:.0f} GB/s; "
-        f"shade {prof['shadeMs']:.2f} ms, shadow {prof['shadowMs']:.2f} ms")
+        f"{cnt['triTests'] / max(1, cnt['extensionRays']):.1f} tris/ray, {achieved:.0f} GB/s, "
+        f"{cnt['stackFallbacks']} stack fallbacks; shade {prof['shadeMs']:.2f} ms, "
+        f"shadow {prof['shadowMs']:.2f} ms "
+        f"({cnt['shadowNodeVisits'] / max(1, cnt['shadowRays']):.1f} nodes/ray, "
+        f"{cnt['shadowStackFallbacks']} fallbacks)")
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": f"k_extend<{args.walk}>",
+            "kernel": f"k_extend<{args.walk},{args.stack}>",
             "bytes_per_launch": ext_bytes // launches,
             "avg_launch_ms": round(prof["extendMs"] / launches, 4)}
     cpu = None
@@ -183,7 +190,8 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"triangle soup {args.tris} tris + 64 emissive, {w}x{h}, "
-                               f"{spp} spp, maxBounces {depth}, RR+NEE, walk {args.walk}",
+                               f"{spp} spp, maxBounces {depth}, RR+NEE, walk {args.walk}, "
+                               f"shadow walk {args.shadow_walk}, stack {args.stack}",
                    "triangles": args.tris, "width": w, "height": h, "spp": spp,
                    "max_bounces": depth, "parallelism": "single GPU"},
         "spp_mpix_per_s": round(sppmpix, 3),

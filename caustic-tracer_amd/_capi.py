@@ -34,7 +34,8 @@ class RenderStats(C.Structure):
                 ("extendLaunches", C.c_uint32), ("extendMs", C.c_double), ("shadeMs", C.c_double),
                 ("shadowMs", C.c_double), ("nodeVisits", C.c_uint64), ("triTests", C.c_uint64),
                 ("sphereTests", C.c_uint64), ("shadowNodeVisits", C.c_uint64),
-                ("shadowTriTests", C.c_uint64), ("shadowSphereTests", C.c_uint64)]
+                ("shadowTriTests", C.c_uint64), ("shadowSphereTests", C.c_uint64),
+                ("stackFallbacks", C.c_uint64), ("shadowStackFallbacks", C.c_uint64)]
 
 
 # Every function of the C ABI with its argument types (restype is int status
@@ -60,6 +61,7 @@ SIGNATURES = {
     "ct_tracer_set_profiling": [vp, i32],
     "ct_tracer_set_counting": [vp, i32],
     "ct_tracer_set_walk": [vp, i32],
+    "ct_tracer_set_traversal": [vp, i32, i32],
     "ct_accumulate_pass": [vp, vp, vp, u32, u32, u32, C.POINTER(TraceOpts), vp, vp],
     "ct_render_image": [vp, vp, vp, C.POINTER(RenderOpts), vp, vp],
     "ct_tracer_stats": [vp, C.POINTER(RenderStats)],

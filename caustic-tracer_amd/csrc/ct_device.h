@@ -187,7 +187,7 @@ struct Hit {
 };
 
 struct TraversalCounts {
-  uint32_t nodes, tris, spheres;
+  uint32_t nodes, tris, spheres, fallbacks;
 };
 
 // Leaf primitive loop shared by both closest-hit walks (intersect.ts:155-193,
@@ -339,6 +339,7 @@ __device__ __forceinline__ void closest_ordered(const DevScene& S, double ox, do
       if (tl != INF) {
         if (tr != INF) {
           if (sp == STACK) {
+            if (COUNT) cnt.fallbacks++;
             closest_stackless<COUNT>(S, ox, oy, oz, dx, dy, dz, tMin, h, cnt);
             return;
           }
@@ -378,6 +379,87 @@ __device__ __forceinline__ void closest_ordered(const DevScene& S, double ox, do
       }
     }
     if (!found) return;
+    leafWord = S.nodes[2 * (size_t)node + 1].w;
+  }
+}
+
+// Any-hit with the ordered walk: nearer child first, first blocker ends the
+// walk.  Whether (tMin, tMax) is blocked does not depend on the visiting order,
+// so this returns exactly what the stackless any-hit walk of intersect.ts:368-420
+// returns while usually meeting the blocker sooner.  Stack overflow falls back
+// to that stackless walk from the root.
+template <int STACK, bool COUNT>
+__device__ __forceinline__ bool occluded_ordered(const DevScene& S, double ox, double oy,
+                                                 double oz, double dx, double dy, double dz,
+                                                 double tMin, double tMax,
+                                                 int* __restrict__ sNode, int lanes,
+                                                 TraversalCounts& cnt) {
+  double idx = safe_inv(dx), idy = safe_inv(dy), idz = safe_inv(dz);
+  if (COUNT) cnt.nodes++;
+  float4 rlo = S.nodes[0], rhi = S.nodes[1];
+  if (ray_aabb(ox, oy, oz, idx, idy, idz, rlo, rhi, tMax) == __builtin_inf()) return false;
+  int sp = 0;
+  int node = 0;
+  float leafWord = rhi.w;
+  const double INF = __builtin_inf();
+  for (;;) {
+    if (leafWord >= 0.0f) {
+      int li = (int)leafWord;
+      int count = li & 7;
+      int start = li >> 3;
+      for (int p = start; p < start + count; p++) {
+        PrimGeom g = load_prim(S, p);
+        if (g.a.w == 0.0f) {
+          if (COUNT) cnt.spheres++;
+          if (ray_sphere(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, tMin, tMax) > 0)
+            return true;
+        } else {
+          if (COUNT) cnt.tris++;
+          double t, u, v;
+          if (ray_triangle(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, g.b.y, g.b.z,
+                           g.c.x, g.c.y, g.c.z, tMin, tMax, t, u, v))
+            return true;
+        }
+      }
+    } else {
+      int left = node + 1;
+      float4 llo = S.nodes[2 * (size_t)left], lhi = S.nodes[2 * (size_t)left + 1];
+      int right = (int)llo.w;
+      float4 rlo2 = S.nodes[2 * (size_t)right], rhi2 = S.nodes[2 * (size_t)right + 1];
+      if (COUNT) cnt.nodes += 2;
+      double tl = ray_aabb(ox, oy, oz, idx, idy, idz, llo, lhi, tMax);
+      double tr = ray_aabb(ox, oy, oz, idx, idy, idz, rlo2, rhi2, tMax);
+      if (tl != INF) {
+        if (tr != INF) {
+          if (sp == STACK) {
+            if (COUNT) cnt.fallbacks++;
+            return occluded<COUNT>(S, ox, oy, oz, dx, dy, dz, tMin, tMax, cnt);
+          }
+          int far = tl <= tr ? right : left;
+          sNode[sp * lanes] = far;
+          sp++;
+          if (tl <= tr) {
+            node = left;
+            leafWord = lhi.w;
+          } else {
+            node = right;
+            leafWord = rhi2.w;
+          }
+        } else {
+          node = left;
+          leafWord = lhi.w;
+        }
+        continue;
+      }
+      if (tr != INF) {
+        node = right;
+        leafWord = rhi2.w;
+        continue;
+      }
+    }
+    if (sp == 0) return false;
+    sp--;
+    node = sNode[sp * lanes];
     leafWord = S.nodes[2 * (size_t)node + 1].w;
   }
 }

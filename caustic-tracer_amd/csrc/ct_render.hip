@@ -140,13 +140,13 @@ k_generate(PathSoA P, Frame F, uint32_t width, uint32_t height, uint32_t passes,
   if (blockIdx.x == 0 && threadIdx.x == 0) P.ctr[0] = (uint32_t)total;
 }
 
-template <int WALK, bool COUNT>
+template <int WALK, int STACK, bool COUNT>
 __global__ void __launch_bounds__(RBLOCK)
 k_extend(PathSoA P, DevScene S, const uint32_t* __restrict__ q, const uint32_t* __restrict__ count) {
-  __shared__ int sNode[WALK == CT_WALK_ORDERED ? EXT_STACK * RBLOCK : 1];
-  __shared__ float sEntry[WALK == CT_WALK_ORDERED ? EXT_STACK * RBLOCK : 1];
+  __shared__ int sNode[WALK == CT_WALK_ORDERED ? STACK * RBLOCK : 1];
+  __shared__ float sEntry[WALK == CT_WALK_ORDERED ? STACK * RBLOCK : 1];
   const uint32_t n = *count;
-  TraversalCounts cnt = {0, 0, 0};
+  TraversalCounts cnt = {0, 0, 0, 0};
   uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < n; slot += stride) {
     uint32_t pid = q[slot];
@@ -157,8 +157,8 @@ k_extend(PathSoA P, DevScene S, const uint32_t* __restrict__ q, const uint32_t* 
     double ox = P.ox[pid], oy = P.oy[pid], oz = P.oz[pid];
     double dx = P.dx[pid], dy = P.dy[pid], dz = P.dz[pid];
     if (WALK == CT_WALK_ORDERED)
-      closest_ordered<EXT_STACK, COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, h, sNode + threadIdx.x,
-                                        sEntry + threadIdx.x, RBLOCK, cnt);
+      closest_ordered<STACK, COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, h, sNode + threadIdx.x,
+                                    sEntry + threadIdx.x, RBLOCK, cnt);
     else
       closest_stackless<COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
     P.hitPrim[pid] = h.prim;
@@ -168,6 +168,7 @@ k_extend(PathSoA P, DevScene S, const uint32_t* __restrict__ q, const uint32_t* 
     wave_add64(P.ctr64 + 0, cnt.nodes);
     wave_add64(P.ctr64 + 1, cnt.tris);
     wave_add64(P.ctr64 + 2, cnt.spheres);
+    wave_add64(P.ctr64 + 6, cnt.fallbacks);
   }
 }
 
@@ -576,17 +577,24 @@ k_shade(PathSoA P, DevScene S, const uint32_t* __restrict__ qin,
 }
 
 // Any-hit of the NEE ray; adds the pending term when unoccluded (tracer.ts:268-282).
-template <bool COUNT>
+template <int WALK, int STACK, bool COUNT>
 __global__ void __launch_bounds__(RBLOCK)
 k_shadow(PathSoA P, DevScene S, const uint32_t* __restrict__ count) {
+  __shared__ int sNode[WALK == CT_WALK_ORDERED ? STACK * RBLOCK : 1];
   const uint32_t n = *count;
-  TraversalCounts cnt = {0, 0, 0};
+  TraversalCounts cnt = {0, 0, 0, 0};
   uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < n; slot += stride) {
     uint32_t pid = P.sPath[slot];
     double ox = P.ox[pid], oy = P.oy[pid], oz = P.oz[pid];
-    bool blocked = occluded<COUNT>(S, ox, oy, oz, P.sdx[slot], P.sdy[slot], P.sdz[slot], T_MIN,
-                                   P.sTmax[slot], cnt);
+    bool blocked;
+    if (WALK == CT_WALK_ORDERED)
+      blocked = occluded_ordered<STACK, COUNT>(S, ox, oy, oz, P.sdx[slot], P.sdy[slot],
+                                               P.sdz[slot], T_MIN, P.sTmax[slot],
+                                               sNode + threadIdx.x, RBLOCK, cnt);
+    else
+      blocked = occluded<COUNT>(S, ox, oy, oz, P.sdx[slot], P.sdy[slot], P.sdz[slot], T_MIN,
+                                P.sTmax[slot], cnt);
     if (!blocked) {
       P.Lr[pid] += P.sCr[slot];
       P.Lg[pid] += P.sCg[slot];
@@ -597,6 +605,60 @@ k_shadow(PathSoA P, DevScene S, const uint32_t* __restrict__ count) {
     wave_add64(P.ctr64 + 3, cnt.nodes);
     wave_add64(P.ctr64 + 4, cnt.tris);
     wave_add64(P.ctr64 + 5, cnt.spheres);
+    wave_add64(P.ctr64 + 7, cnt.fallbacks);
+  }
+}
+
+template <int WALK, int STACK>
+static void launch_extend(bool counting, unsigned grid, hipStream_t st, const PathSoA& P,
+                          const DevScene& S, const uint32_t* q, const uint32_t* c) {
+  if (counting)
+    hipLaunchKernelGGL((k_extend<WALK, STACK, true>), dim3(grid), dim3(RBLOCK), 0, st, P, S, q, c);
+  else
+    hipLaunchKernelGGL((k_extend<WALK, STACK, false>), dim3(grid), dim3(RBLOCK), 0, st, P, S, q, c);
+}
+
+template <int WALK, int STACK>
+static void launch_shadow(bool counting, unsigned grid, hipStream_t st, const PathSoA& P,
+                          const DevScene& S, const uint32_t* c) {
+  if (counting)
+    hipLaunchKernelGGL((k_shadow<WALK, STACK, true>), dim3(grid), dim3(RBLOCK), 0, st, P, S, c);
+  else
+    hipLaunchKernelGGL((k_shadow<WALK, STACK, false>), dim3(grid), dim3(RBLOCK), 0, st, P, S, c);
+}
+
+// Stack depths the ordered walks are instantiated for (LDS per 256-lane block:
+// 8 B/entry for closest hit, 4 B/entry for any hit).
+static bool stack_supported(int d) { return d == 8 || d == 10 || d == 12 || d == 16 || d == 24; }
+
+static void dispatch_extend(int walk, int stack, bool counting, unsigned grid, hipStream_t st,
+                            const PathSoA& P, const DevScene& S, const uint32_t* q,
+                            const uint32_t* c) {
+  if (walk == CT_WALK_STACKLESS) {
+    launch_extend<CT_WALK_STACKLESS, 1>(counting, grid, st, P, S, q, c);
+    return;
+  }
+  switch (stack) {
+    case 8: launch_extend<CT_WALK_ORDERED, 8>(counting, grid, st, P, S, q, c); break;
+    case 10: launch_extend<CT_WALK_ORDERED, 10>(counting, grid, st, P, S, q, c); break;
+    case 12: launch_extend<CT_WALK_ORDERED, 12>(counting, grid, st, P, S, q, c); break;
+    case 16: launch_extend<CT_WALK_ORDERED, 16>(counting, grid, st, P, S, q, c); break;
+    default: launch_extend<CT_WALK_ORDERED, 24>(counting, grid, st, P, S, q, c); break;
+  }
+}
+
+static void dispatch_shadow(int walk, int stack, bool counting, unsigned grid, hipStream_t st,
+                            const PathSoA& P, const DevScene& S, const uint32_t* c) {
+  if (walk == CT_WALK_STACKLESS) {
+    launch_shadow<CT_WALK_STACKLESS, 1>(counting, grid, st, P, S, c);
+    return;
+  }
+  switch (stack) {
+    case 8: launch_shadow<CT_WALK_ORDERED, 8>(counting, grid, st, P, S, c); break;
+    case 10: launch_shadow<CT_WALK_ORDERED, 10>(counting, grid, st, P, S, c); break;
+    case 12: launch_shadow<CT_WALK_ORDERED, 12>(counting, grid, st, P, S, c); break;
+    case 16: launch_shadow<CT_WALK_ORDERED, 16>(counting, grid, st, P, S, c); break;
+    default: launch_shadow<CT_WALK_ORDERED, 24>(counting, grid, st, P, S, c); break;
   }
 }
 
@@ -644,7 +706,9 @@ struct ct_tracer {
   PathSoA P{};
   bool profiling = false;
   bool counting = false;
-  int walk = CT_WALK_ORDERED;
+  int walk = CT_WALK_ORDERED;        // closest-hit walk of extension rays
+  int shadowWalk = CT_WALK_ORDERED;  // any-hit walk of NEE rays
+  int stack = 10;                    // LDS stack entries of the ordered walks
   // stats of the last run
   int lastBounces = 0;
   uint64_t lastPaths = 0;
@@ -746,17 +810,7 @@ static int run_passes(ct_tracer* T, const ct_scene* s, const Frame& F, uint32_t 
     uint32_t* cout = P.ctr + b + 1;
     uint32_t* scount = P.ctr + MAX_BOUNCE_SLOTS + b;
     mark(T, st, 0, true);
-    if (T->walk == CT_WALK_ORDERED) {
-      if (T->counting)
-        hipLaunchKernelGGL((k_extend<CT_WALK_ORDERED, true>), dim3(gridPaths), dim3(RBLOCK), 0, st, P, s->dev, qin, cin);
-      else
-        hipLaunchKernelGGL((k_extend<CT_WALK_ORDERED, false>), dim3(gridPaths), dim3(RBLOCK), 0, st, P, s->dev, qin, cin);
-    } else {
-      if (T->counting)
-        hipLaunchKernelGGL((k_extend<CT_WALK_STACKLESS, true>), dim3(gridPaths), dim3(RBLOCK), 0, st, P, s->dev, qin, cin);
-      else
-        hipLaunchKernelGGL((k_extend<CT_WALK_STACKLESS, false>), dim3(gridPaths), dim3(RBLOCK), 0, st, P, s->dev, qin, cin);
-    }
+    dispatch_extend(T->walk, T->stack, T->counting, gridPaths, st, P, s->dev, qin, cin);
     CT_LAUNCH_CHECK("k_extend");
     mark(T, st, 0, false);
     mark(T, st, 1, true);
@@ -765,10 +819,7 @@ static int run_passes(ct_tracer* T, const ct_scene* s, const Frame& F, uint32_t 
     CT_LAUNCH_CHECK("k_shade");
     mark(T, st, 1, false);
     mark(T, st, 2, true);
-    if (T->counting)
-      hipLaunchKernelGGL(k_shadow<true>, dim3(gridPaths), dim3(RBLOCK), 0, st, P, s->dev, scount);
-    else
-      hipLaunchKernelGGL(k_shadow<false>, dim3(gridPaths), dim3(RBLOCK), 0, st, P, s->dev, scount);
+    dispatch_shadow(T->shadowWalk, T->stack, T->counting, gridPaths, st, P, s->dev, scount);
     CT_LAUNCH_CHECK("k_shadow");
     mark(T, st, 2, false);
   }
@@ -828,6 +879,17 @@ int ct_tracer_set_profiling(ct_tracer* T, int32_t enable) {
 int ct_tracer_set_counting(ct_tracer* T, int32_t enable) {
   if (!T) return set_error(CT_EINVAL, "null tracer");
   T->counting = enable != 0;
+  return CT_OK;
+}
+
+int ct_tracer_set_traversal(ct_tracer* T, int32_t shadowWalk, int32_t stackDepth) {
+  if (!T) return set_error(CT_EINVAL, "null tracer");
+  if (shadowWalk != CT_WALK_ORDERED && shadowWalk != CT_WALK_STACKLESS)
+    return set_error(CT_EINVAL, "ct_tracer_set_traversal: unknown walk");
+  if (!stack_supported(stackDepth))
+    return set_error(CT_EINVAL, "ct_tracer_set_traversal: stack depth must be 8, 10, 12, 16 or 24");
+  T->shadowWalk = shadowWalk;
+  T->stack = stackDepth;
   return CT_OK;
 }
 
@@ -920,8 +982,10 @@ int ct_tracer_stats(ct_tracer* T, ct_render_stats* out) {
     out->extendLaunches = (uint32_t)T->lastBounces;
   }
   if (T->counting && T->buf) {
-    unsigned long long h[6];
+    unsigned long long h[8];
     CT_HIP(hipMemcpy(h, T->P.ctr64, sizeof h, hipMemcpyDeviceToHost));
+    out->stackFallbacks = h[6];
+    out->shadowStackFallbacks = h[7];
     out->nodeVisits = h[0];
     out->triTests = h[1];
     out->sphereTests = h[2];

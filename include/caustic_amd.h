@@ -85,6 +85,8 @@ typedef struct {
   uint64_t shadowNodeVisits; /* node fetches of any-hit walks (counting on) */
   uint64_t shadowTriTests;   /* triangle tests of any-hit walks (counting on) */
   uint64_t shadowSphereTests;/* sphere tests of any-hit walks (counting on) */
+  uint64_t stackFallbacks;   /* closest-hit rays that outgrew the LDS stack (counting on) */
+  uint64_t shadowStackFallbacks; /* any-hit rays that outgrew the LDS stack (counting on) */
 } ct_render_stats;
 
 /* ---- runtime ------------------------------------------------------------ */
@@ -140,6 +142,10 @@ int ct_tracer_set_profiling(ct_tracer *tracer, int32_t enable);
 int ct_tracer_set_counting(ct_tracer *tracer, int32_t enable);
 /* Closest-hit walk used by the extension rays (CT_WALK_*). */
 int ct_tracer_set_walk(ct_tracer *tracer, int32_t walk);
+/* Walk of the NEE any-hit rays (CT_WALK_*; the result does not depend on it)
+ * and the LDS stack depth of the ordered walks (8, 10, 12, 16 or 24 entries;
+ * deeper rays finish with the stackless walk, same result). */
+int ct_tracer_set_traversal(ct_tracer *tracer, int32_t shadowWalk, int32_t stackDepth);
 /* accumulatePass (tracer.ts:361-395): adds one sample per pixel into the
  * float32 sum d_accum (width*height*3) with frameIndex as the RNG frame. */
 int ct_accumulate_pass(ct_tracer *tracer, const ct_scene *scene, const double *h_frame16,
