@@ -88,7 +88,7 @@ def main():
     ap.add_argument("--depth", type=int, default=6)
     ap.add_argument("--walk", choices=["ordered", "stackless"], default="ordered")
     ap.add_argument("--shadow-walk", choices=["ordered", "stackless"], default="ordered")
-    ap.add_argument("--stack", type=int, default=10, help="LDS stack entries of the ordered walks")
+    ap.add_argument("--stack", type=int, default=12, help="LDS stack entries of the ordered walks")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     args = ap.parse_args()

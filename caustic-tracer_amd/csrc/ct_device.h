@@ -464,4 +464,211 @@ __device__ __forceinline__ bool occluded_ordered(const DevScene& S, double ox, d
   }
 }
 
+// ---------------------------------------------------------------------------
+// Wave-friendly ordered walk.
+//
+// Same slab values as ray_aabb ((b - o) * inv in binary64) reduced with
+// min/max instead of the reference's compare-and-assign chains; the two agree
+// on every hit/miss decision and entry value (they can differ only in the
+// sign of a zero entry, which no comparison distinguishes).
+__device__ __forceinline__ bool box_enter(double ox, double oy, double oz, double ix, double iy,
+                                          double iz, float4 lo, float4 hi, double tMax,
+                                          double& entry) {
+  double tx0 = ((double)lo.x - ox) * ix, tx1 = ((double)hi.x - ox) * ix;
+  double ty0 = ((double)lo.y - oy) * iy, ty1 = ((double)hi.y - oy) * iy;
+  double tz0 = ((double)lo.z - oz) * iz, tz1 = ((double)hi.z - oz) * iz;
+  double tn = fmax(fmax(fmin(tx0, tx1), fmin(ty0, ty1)), fmax(fmin(tz0, tz1), 0.0));
+  double tf = fmin(fmin(fmax(tx0, tx1), fmax(ty0, ty1)), fmin(fmax(tz0, tz1), tMax));
+  entry = tn;
+  return tn <= tf;
+}
+
+// Work item encoding shared by the stack and the current-node register:
+// >= 0 interior node index, <= -2 a leaf (-2 - leafWord), -1 nothing.
+__device__ __forceinline__ int enc_child(int idx, float leafWord) {
+  return leafWord >= 0.0f ? -2 - (int)leafWord : idx;
+}
+
+// Closest hit with the ordered walk of intersect.ts:202-282 (nearer child
+// first, far child deferred with its entry, deferred entries >= closest
+// skipped) organised for 64-wide waves: interior steps run in an inner loop;
+// a lane that reaches a leaf parks it and keeps descending until every lane
+// of the wave holds a leaf (or has nothing left), then all parked leaves are
+// tested together.  Parking a leaf only delays when `closest` shrinks, so the
+// walk visits a superset of what the reference visits and returns the same
+// minimum.  Stack overflow falls back to the stackless walk from the root.
+template <int STACK, bool COUNT>
+__device__ __forceinline__ void closest_wave(const DevScene& S, double ox, double oy, double oz,
+                                            double dx, double dy, double dz, double tMin, Hit& h,
+                                            int* __restrict__ sNode, float* __restrict__ sEntry,
+                                            int lanes, TraversalCounts& cnt) {
+  const double ix = safe_inv(dx), iy = safe_inv(dy), iz = safe_inv(dz);
+  if (COUNT) cnt.nodes++;
+  int cur = -1;
+  {
+    float4 lo = S.nodes[0], hi = S.nodes[1];
+    double e;
+    if (box_enter(ox, oy, oz, ix, iy, iz, lo, hi, h.t, e)) cur = enc_child(0, hi.w);
+  }
+  int sp = 0;
+  int pend = -1;  // parked leaf word
+  bool overflow = false;
+  while (cur != -1 || pend >= 0) {
+    // descend
+    while (cur != -1) {
+      if (cur <= -2) {  // reached a leaf
+        if (pend >= 0) break;
+        pend = -2 - cur;
+        cur = -1;
+      } else {
+        int left = cur + 1;
+        float4 llo = S.nodes[2 * (size_t)left], lhi = S.nodes[2 * (size_t)left + 1];
+        int right = (int)llo.w;
+        float4 rlo = S.nodes[2 * (size_t)right], rhi = S.nodes[2 * (size_t)right + 1];
+        if (COUNT) cnt.nodes += 2;
+        double tl, tr;
+        bool hl = box_enter(ox, oy, oz, ix, iy, iz, llo, lhi, h.t, tl);
+        bool hr = box_enter(ox, oy, oz, ix, iy, iz, rlo, rhi, h.t, tr);
+        int el = enc_child(left, lhi.w), er = enc_child(right, rhi.w);
+        if (hl && hr) {
+          if (sp == STACK) {
+            overflow = true;
+            cur = -1;
+            break;
+          }
+          bool leftFirst = tl <= tr;
+          sNode[sp * lanes] = leftFirst ? er : el;
+          sEntry[sp * lanes] = f32_round_down(leftFirst ? tr : tl);
+          sp++;
+          cur = leftFirst ? el : er;
+        } else if (hl) {
+          cur = el;
+        } else if (hr) {
+          cur = er;
+        } else {
+          cur = -1;
+        }
+      }
+      if (cur == -1) {  // pop, skipping entries the bound already excludes
+        while (sp > 0) {
+          sp--;
+          if ((double)sEntry[sp * lanes] < h.t) {
+            cur = sNode[sp * lanes];
+            break;
+          }
+        }
+      }
+      if (__all(pend >= 0 || cur == -1)) break;
+    }
+    if (overflow) break;
+    // test the parked leaf
+    if (pend >= 0) {
+      test_leaf<COUNT>(S, (float)pend, ox, oy, oz, dx, dy, dz, tMin, h, cnt);
+      pend = -1;
+    }
+  }
+  if (overflow) {
+    if (COUNT) cnt.fallbacks++;
+    if (pend >= 0) test_leaf<COUNT>(S, (float)pend, ox, oy, oz, dx, dy, dz, tMin, h, cnt);
+    closest_stackless<COUNT>(S, ox, oy, oz, dx, dy, dz, tMin, h, cnt);
+  }
+}
+
+// Any-hit counterpart: same wave organisation, first blocker ends the walk.
+template <int STACK, bool COUNT>
+__device__ __forceinline__ bool occluded_wave(const DevScene& S, double ox, double oy, double oz,
+                                              double dx, double dy, double dz, double tMin,
+                                              double tMax, int* __restrict__ sNode, int lanes,
+                                              TraversalCounts& cnt) {
+  const double ix = safe_inv(dx), iy = safe_inv(dy), iz = safe_inv(dz);
+  if (COUNT) cnt.nodes++;
+  int cur = -1;
+  {
+    float4 lo = S.nodes[0], hi = S.nodes[1];
+    double e;
+    if (box_enter(ox, oy, oz, ix, iy, iz, lo, hi, tMax, e)) cur = enc_child(0, hi.w);
+  }
+  int sp = 0;
+  int pend = -1;
+  bool overflow = false, blocked = false;
+  while (cur != -1 || pend >= 0) {
+    while (cur != -1) {
+      if (cur <= -2) {
+        if (pend >= 0) break;
+        pend = -2 - cur;
+        cur = -1;
+      } else {
+        int left = cur + 1;
+        float4 llo = S.nodes[2 * (size_t)left], lhi = S.nodes[2 * (size_t)left + 1];
+        int right = (int)llo.w;
+        float4 rlo = S.nodes[2 * (size_t)right], rhi = S.nodes[2 * (size_t)right + 1];
+        if (COUNT) cnt.nodes += 2;
+        double tl, tr;
+        bool hl = box_enter(ox, oy, oz, ix, iy, iz, llo, lhi, tMax, tl);
+        bool hr = box_enter(ox, oy, oz, ix, iy, iz, rlo, rhi, tMax, tr);
+        int el = enc_child(left, lhi.w), er = enc_child(right, rhi.w);
+        if (hl && hr) {
+          if (sp == STACK) {
+            overflow = true;
+            cur = -1;
+            break;
+          }
+          bool leftFirst = tl <= tr;
+          sNode[sp * lanes] = leftFirst ? er : el;
+          sp++;
+          cur = leftFirst ? el : er;
+        } else if (hl) {
+          cur = el;
+        } else if (hr) {
+          cur = er;
+        } else {
+          cur = -1;
+        }
+      }
+      if (cur == -1 && sp > 0) {
+        sp--;
+        cur = sNode[sp * lanes];
+      }
+      if (__all(pend >= 0 || cur == -1)) break;
+    }
+    if (overflow) break;
+    if (pend >= 0) {
+      int li = pend;
+      int count = li & 7, start = li >> 3;
+      for (int p = start; p < start + count && !blocked; p++) {
+        PrimGeom g = load_prim(S, p);
+        if (g.a.w == 0.0f) {
+          if (COUNT) cnt.spheres++;
+          blocked = ray_sphere(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, tMin, tMax) > 0;
+        } else {
+          if (COUNT) cnt.tris++;
+          double t, u, v;
+          blocked = ray_triangle(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, g.b.y, g.b.z,
+                                 g.c.x, g.c.y, g.c.z, tMin, tMax, t, u, v);
+        }
+      }
+      pend = -1;
+      if (blocked) return true;
+    }
+  }
+  if (overflow) {
+    if (COUNT) cnt.fallbacks++;
+    if (pend >= 0) {
+      int li = pend;
+      int count = li & 7, start = li >> 3;
+      for (int p = start; p < start + count; p++) {
+        PrimGeom g = load_prim(S, p);
+        double t, u, v;
+        if (g.a.w == 0.0f ? ray_sphere(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, tMin,
+                                       tMax) > 0
+                          : ray_triangle(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x,
+                                         g.b.y, g.b.z, g.c.x, g.c.y, g.c.z, tMin, tMax, t, u, v))
+          return true;
+      }
+    }
+    return occluded<COUNT>(S, ox, oy, oz, dx, dy, dz, tMin, tMax, cnt);
+  }
+  return false;
+}
+
 }  // namespace ct

@@ -157,8 +157,8 @@ k_extend(PathSoA P, DevScene S, const uint32_t* __restrict__ q, const uint32_t* 
     double ox = P.ox[pid], oy = P.oy[pid], oz = P.oz[pid];
     double dx = P.dx[pid], dy = P.dy[pid], dz = P.dz[pid];
     if (WALK == CT_WALK_ORDERED)
-      closest_ordered<STACK, COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, h, sNode + threadIdx.x,
-                                    sEntry + threadIdx.x, RBLOCK, cnt);
+      closest_wave<STACK, COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, h, sNode + threadIdx.x,
+                                 sEntry + threadIdx.x, RBLOCK, cnt);
     else
       closest_stackless<COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
     P.hitPrim[pid] = h.prim;
@@ -589,9 +589,9 @@ k_shadow(PathSoA P, DevScene S, const uint32_t* __restrict__ count) {
     double ox = P.ox[pid], oy = P.oy[pid], oz = P.oz[pid];
     bool blocked;
     if (WALK == CT_WALK_ORDERED)
-      blocked = occluded_ordered<STACK, COUNT>(S, ox, oy, oz, P.sdx[slot], P.sdy[slot],
-                                               P.sdz[slot], T_MIN, P.sTmax[slot],
-                                               sNode + threadIdx.x, RBLOCK, cnt);
+      blocked = occluded_wave<STACK, COUNT>(S, ox, oy, oz, P.sdx[slot], P.sdy[slot],
+                                            P.sdz[slot], T_MIN, P.sTmax[slot],
+                                            sNode + threadIdx.x, RBLOCK, cnt);
     else
       blocked = occluded<COUNT>(S, ox, oy, oz, P.sdx[slot], P.sdy[slot], P.sdz[slot], T_MIN,
                                 P.sTmax[slot], cnt);
@@ -708,7 +708,7 @@ struct ct_tracer {
   bool counting = false;
   int walk = CT_WALK_ORDERED;        // closest-hit walk of extension rays
   int shadowWalk = CT_WALK_ORDERED;  // any-hit walk of NEE rays
-  int stack = 10;                    // LDS stack entries of the ordered walks
+  int stack = 12;                    // LDS stack entries of the ordered walks
   // stats of the last run
   int lastBounces = 0;
   uint64_t lastPaths = 0;

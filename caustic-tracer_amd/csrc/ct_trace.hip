@@ -33,7 +33,7 @@ k_closest(DevScene S, const double* __restrict__ rays, uint32_t n, double tMin,
           unsigned long long* __restrict__ counts) {
   __shared__ int sNode[WALK == CT_WALK_ORDERED ? ORDERED_STACK * QUERY_BLOCK : 1];
   __shared__ float sEntry[WALK == CT_WALK_ORDERED ? ORDERED_STACK * QUERY_BLOCK : 1];
-  TraversalCounts cnt = {0, 0, 0};
+  TraversalCounts cnt = {0, 0, 0, 0};
   size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     double ox, oy, oz, dx, dy, dz;
@@ -44,9 +44,9 @@ k_closest(DevScene S, const double* __restrict__ rays, uint32_t n, double tMin,
     h.v = 0;
     h.prim = -1;
     if (WALK == CT_WALK_ORDERED)
-      closest_ordered<ORDERED_STACK, COUNT>(S, ox, oy, oz, dx, dy, dz, tMin, h,
-                                            sNode + threadIdx.x, sEntry + threadIdx.x,
-                                            QUERY_BLOCK, cnt);
+      closest_wave<ORDERED_STACK, COUNT>(S, ox, oy, oz, dx, dy, dz, tMin, h,
+                                         sNode + threadIdx.x, sEntry + threadIdx.x,
+                                         QUERY_BLOCK, cnt);
     else
       closest_stackless<COUNT>(S, ox, oy, oz, dx, dy, dz, tMin, h, cnt);
     if (prim) prim[i] = h.prim;
@@ -64,7 +64,7 @@ k_closest(DevScene S, const double* __restrict__ rays, uint32_t n, double tMin,
 __global__ void __launch_bounds__(QUERY_BLOCK)
 k_occluded(DevScene S, const double* __restrict__ rays, uint32_t n, double tMin,
            const double* __restrict__ tMax, uint8_t* __restrict__ out) {
-  TraversalCounts cnt = {0, 0, 0};
+  TraversalCounts cnt = {0, 0, 0, 0};
   size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     double ox, oy, oz, dx, dy, dz;

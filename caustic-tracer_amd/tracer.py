@@ -181,7 +181,7 @@ class Tracer:
     def set_walk(self, walk):
         check(lib().ct_tracer_set_walk(self.handle, walk), "ct_tracer_set_walk")
 
-    def set_traversal(self, shadow_walk=WALK_ORDERED, stack=10):
+    def set_traversal(self, shadow_walk=WALK_ORDERED, stack=12):
         check(lib().ct_tracer_set_traversal(self.handle, shadow_walk, stack), "set_traversal")
 
     def set_profiling(self, on):

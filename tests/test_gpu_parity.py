@@ -66,10 +66,11 @@ def test_closest_hit_finite_tmax_and_counts(ct, O, bvh_scene):
     gc = ct.closest_hit_counts(bvh_scene, rays, tmax, walk=ct.WALK_STACKLESS)
     oc = O.closest_batch(bvh_scene, rays, tmax, walk=O.WALK_STACKLESS)["counts"]
     assert gc == (oc[0], oc[1], oc[2])
-    # the ordered walk prunes on float32-rounded stack entries: never fewer nodes
+    # the GPU's ordered walk prunes on float32-rounded stack entries and tests
+    # parked leaves late: it visits a superset of the reference walk's nodes
     gc = ct.closest_hit_counts(bvh_scene, rays, tmax, walk=ct.WALK_ORDERED)
     oc = O.closest_batch(bvh_scene, rays, tmax, walk=O.WALK_ORDERED)["counts"]
-    assert gc[0] >= oc[0] and gc[0] <= oc[0] * 1.001 + 10
+    assert gc[0] >= oc[0] and gc[0] <= oc[0] * 1.5
 
 
 def test_occluded_bit_exact(ct, O, bvh_scene):
