@@ -88,7 +88,9 @@ def main():
     ap.add_argument("--depth", type=int, default=6)
     ap.add_argument("--walk", choices=["ordered", "stackless"], default="ordered")
     ap.add_argument("--shadow-walk", choices=["ordered", "stackless"], default="ordered")
-    ap.add_argument("--stack", type=int, default=12, help="LDS stack entries of the ordered walks")
+    ap.add_argument("--stack", type=int, default=16, help="LDS stack entries of the ordered walks")
+    ap.add_argument("--persist-blocks", type=int, default=2048,
+                    help="workgroups of the persistent traversal launches (0: one lane per ray)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     args = ap.parse_args()
@@ -131,6 +133,7 @@ def main():
     tr = ct.Tracer(walk=walk)
     tr.set_traversal(ct.WALK_ORDERED if args.shadow_walk == "ordered" else ct.WALK_STACKLESS,
                      args.stack)
+    tr.set_schedule(args.persist_blocks > 0, max(1, args.persist_blocks))
     out = ct.DeviceBuffer.empty((w * h * 3,), np.float32)
     opts = {"width": w, "height": h, "spp": spp, "seed": 1, "maxBounces": depth,
             "russianRoulette": True, "nee": True}
@@ -191,7 +194,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"triangle soup {args.tris} tris + 64 emissive, {w}x{h}, "
                                f"{spp} spp, maxBounces {depth}, RR+NEE, walk {args.walk}, "
-                               f"shadow walk {args.shadow_walk}, stack {args.stack}",
+                               f"shadow walk {args.shadow_walk}, stack {args.stack}, "
+                               f"persistent blocks {args.persist_blocks}",
                    "triangles": args.tris, "width": w, "height": h, "spp": spp,
                    "max_bounces": depth, "parallelism": "single GPU"},
         "spp_mpix_per_s": round(sppmpix, 3),

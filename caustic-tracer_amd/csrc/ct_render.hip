@@ -43,8 +43,9 @@ struct PathSoA {
   // shadow queue (one slot per live path at most)
   uint32_t* sPath;
   double *sdx, *sdy, *sdz, *sTmax, *sCr, *sCg, *sCb;
-  // counters: live[b] at [b], shadow[b] at [MAX_BOUNCE_SLOTS + b]; 64-bit work
-  // counters at ctr64
+  // counters: live[b] at [b], shadow[b] at [MAX_BOUNCE_SLOTS + b], ray-fetch
+  // cursors of the persistent extend/shadow launches at [2*MAX + b] and
+  // [3*MAX + b]; 64-bit work counters at ctr64
   uint32_t* ctr;
   unsigned long long* ctr64;
 };
@@ -609,9 +610,270 @@ k_shadow(PathSoA P, DevScene S, const uint32_t* __restrict__ count) {
   }
 }
 
+
+// Persistent closest-hit: each lane pulls the next ray from the bounce's queue
+// as soon as its current ray is finished (wave-aggregated fetch), so lanes do
+// not idle until the slowest ray of their wave ends.  The walk itself is the
+// wave-organised ordered walk of closest_wave (same visiting rules, same
+// result), written as a state machine that survives ray changes.
+template <int STACK, bool COUNT>
+__global__ void __launch_bounds__(RBLOCK)
+k_extend_persist(PathSoA P, DevScene S, const uint32_t* __restrict__ q,
+                 const uint32_t* __restrict__ count, uint32_t* __restrict__ fetch) {
+  __shared__ int sNodeAll[STACK * RBLOCK];
+  __shared__ float sEntryAll[STACK * RBLOCK];
+  int* sNode = sNodeAll + threadIdx.x;
+  float* sEntry = sEntryAll + threadIdx.x;
+  const int lanes = RBLOCK;
+  const uint32_t n = *count;
+  TraversalCounts cnt = {0, 0, 0, 0};
+  bool have = false, exhausted = false;
+  uint32_t pid = 0;
+  double ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0;
+  Hit h;
+  h.t = 0; h.u = 0; h.v = 0; h.prim = -1;
+  int cur = -1, sp = 0, pend = -1;
+  const int lane = __lane_id();
+  for (;;) {
+    if (!exhausted) {
+      bool need = !have;
+      unsigned long long m = __ballot(need);
+      if (m) {
+        int leader = __ffsll((long long)__ballot(1)) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(fetch, (uint32_t)__popcll(m));
+        base = __shfl(base, leader, 64);
+        if (base + (uint32_t)__popcll(m) >= n) exhausted = true;
+        if (need) {
+          uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+          if (slot < n) {
+            pid = q[slot];
+            ox = P.ox[pid]; oy = P.oy[pid]; oz = P.oz[pid];
+            dx = P.dx[pid]; dy = P.dy[pid]; dz = P.dz[pid];
+            ix = safe_inv(dx); iy = safe_inv(dy); iz = safe_inv(dz);
+            h.t = __builtin_inf(); h.u = 0; h.v = 0; h.prim = -1;
+            sp = 0; pend = -1; cur = -1;
+            if (COUNT) cnt.nodes++;
+            float4 lo = S.nodes[0], hi = S.nodes[1];
+            double e;
+            if (box_enter(ox, oy, oz, ix, iy, iz, lo, hi, h.t, e)) cur = enc_child(0, hi.w);
+            have = true;
+          }
+        }
+      }
+    }
+    if (!__any(have)) break;
+    if (have) {
+      bool overflow = false;
+      while (cur != -1) {
+        if (cur <= -2) {
+          if (pend >= 0) break;
+          pend = -2 - cur;
+          cur = -1;
+        } else {
+          int left = cur + 1;
+          float4 llo = S.nodes[2 * (size_t)left], lhi = S.nodes[2 * (size_t)left + 1];
+          int right = (int)llo.w;
+          float4 rlo = S.nodes[2 * (size_t)right], rhi = S.nodes[2 * (size_t)right + 1];
+          if (COUNT) cnt.nodes += 2;
+          double tl, tr;
+          bool hl = box_enter(ox, oy, oz, ix, iy, iz, llo, lhi, h.t, tl);
+          bool hr = box_enter(ox, oy, oz, ix, iy, iz, rlo, rhi, h.t, tr);
+          int el = enc_child(left, lhi.w), er = enc_child(right, rhi.w);
+          if (hl && hr) {
+            if (sp == STACK) {
+              overflow = true;
+              cur = -1;
+              break;
+            }
+            bool leftFirst = tl <= tr;
+            sNode[sp * lanes] = leftFirst ? er : el;
+            sEntry[sp * lanes] = f32_round_down(leftFirst ? tr : tl);
+            sp++;
+            cur = leftFirst ? el : er;
+          } else if (hl) {
+            cur = el;
+          } else if (hr) {
+            cur = er;
+          } else {
+            cur = -1;
+          }
+        }
+        if (cur == -1) {
+          while (sp > 0) {
+            sp--;
+            if ((double)sEntry[sp * lanes] < h.t) {
+              cur = sNode[sp * lanes];
+              break;
+            }
+          }
+        }
+        if (__all(pend >= 0 || cur == -1)) break;
+      }
+      if (pend >= 0) {
+        test_leaf<COUNT>(S, (float)pend, ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
+        pend = -1;
+      }
+      if (overflow) {
+        if (COUNT) cnt.fallbacks++;
+        closest_stackless<COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
+        cur = -1;
+        sp = 0;
+      }
+      if (cur == -1 && pend < 0) {
+        P.hitPrim[pid] = h.prim;
+        P.hitT[pid] = h.t;
+        have = false;
+      }
+    }
+  }
+  if (COUNT) {
+    wave_add64(P.ctr64 + 0, cnt.nodes);
+    wave_add64(P.ctr64 + 1, cnt.tris);
+    wave_add64(P.ctr64 + 2, cnt.spheres);
+    wave_add64(P.ctr64 + 6, cnt.fallbacks);
+  }
+}
+
+// Persistent any-hit of the NEE rays (same lane refill as k_extend_persist).
+template <int STACK, bool COUNT>
+__global__ void __launch_bounds__(RBLOCK)
+k_shadow_persist(PathSoA P, DevScene S, const uint32_t* __restrict__ count,
+                 uint32_t* __restrict__ fetch) {
+  __shared__ int sNodeAll[STACK * RBLOCK];
+  int* sNode = sNodeAll + threadIdx.x;
+  const int lanes = RBLOCK;
+  const uint32_t n = *count;
+  TraversalCounts cnt = {0, 0, 0, 0};
+  bool have = false, exhausted = false;
+  uint32_t slotCur = 0, pid = 0;
+  double ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0, tMax = 0;
+  int cur = -1, sp = 0, pend = -1;
+  const int lane = __lane_id();
+  for (;;) {
+    if (!exhausted) {
+      bool need = !have;
+      unsigned long long m = __ballot(need);
+      if (m) {
+        int leader = __ffsll((long long)__ballot(1)) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(fetch, (uint32_t)__popcll(m));
+        base = __shfl(base, leader, 64);
+        if (base + (uint32_t)__popcll(m) >= n) exhausted = true;
+        if (need) {
+          uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+          if (slot < n) {
+            slotCur = slot;
+            pid = P.sPath[slot];
+            ox = P.ox[pid]; oy = P.oy[pid]; oz = P.oz[pid];
+            dx = P.sdx[slot]; dy = P.sdy[slot]; dz = P.sdz[slot];
+            tMax = P.sTmax[slot];
+            ix = safe_inv(dx); iy = safe_inv(dy); iz = safe_inv(dz);
+            sp = 0; pend = -1; cur = -1;
+            if (COUNT) cnt.nodes++;
+            float4 lo = S.nodes[0], hi = S.nodes[1];
+            double e;
+            if (box_enter(ox, oy, oz, ix, iy, iz, lo, hi, tMax, e)) cur = enc_child(0, hi.w);
+            have = true;
+          }
+        }
+      }
+    }
+    if (!__any(have)) break;
+    if (have) {
+      bool overflow = false, blocked = false;
+      while (cur != -1) {
+        if (cur <= -2) {
+          if (pend >= 0) break;
+          pend = -2 - cur;
+          cur = -1;
+        } else {
+          int left = cur + 1;
+          float4 llo = S.nodes[2 * (size_t)left], lhi = S.nodes[2 * (size_t)left + 1];
+          int right = (int)llo.w;
+          float4 rlo = S.nodes[2 * (size_t)right], rhi = S.nodes[2 * (size_t)right + 1];
+          if (COUNT) cnt.nodes += 2;
+          double tl, tr;
+          bool hl = box_enter(ox, oy, oz, ix, iy, iz, llo, lhi, tMax, tl);
+          bool hr = box_enter(ox, oy, oz, ix, iy, iz, rlo, rhi, tMax, tr);
+          int el = enc_child(left, lhi.w), er = enc_child(right, rhi.w);
+          if (hl && hr) {
+            if (sp == STACK) {
+              overflow = true;
+              cur = -1;
+              break;
+            }
+            bool leftFirst = tl <= tr;
+            sNode[sp * lanes] = leftFirst ? er : el;
+            sp++;
+            cur = leftFirst ? el : er;
+          } else if (hl) {
+            cur = el;
+          } else if (hr) {
+            cur = er;
+          } else {
+            cur = -1;
+          }
+        }
+        if (cur == -1 && sp > 0) {
+          sp--;
+          cur = sNode[sp * lanes];
+        }
+        if (__all(pend >= 0 || cur == -1)) break;
+      }
+      if (pend >= 0) {
+        int count = pend & 7, start = pend >> 3;
+        for (int p = start; p < start + count && !blocked; p++) {
+          PrimGeom g = load_prim(S, p);
+          if (g.a.w == 0.0f) {
+            if (COUNT) cnt.spheres++;
+            blocked = ray_sphere(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, T_MIN, tMax) > 0;
+          } else {
+            if (COUNT) cnt.tris++;
+            double t, u, v;
+            blocked = ray_triangle(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, g.b.y,
+                                   g.b.z, g.c.x, g.c.y, g.c.z, T_MIN, tMax, t, u, v);
+          }
+        }
+        pend = -1;
+      }
+      if (overflow && !blocked) {
+        if (COUNT) cnt.fallbacks++;
+        blocked = occluded<COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, tMax, cnt);
+      }
+      if (blocked || overflow) {
+        cur = -1;
+        sp = 0;
+      }
+      if (cur == -1 && pend < 0) {
+        if (!blocked) {
+          P.Lr[pid] += P.sCr[slotCur];
+          P.Lg[pid] += P.sCg[slotCur];
+          P.Lb[pid] += P.sCb[slotCur];
+        }
+        have = false;
+      }
+    }
+  }
+  if (COUNT) {
+    wave_add64(P.ctr64 + 3, cnt.nodes);
+    wave_add64(P.ctr64 + 4, cnt.tris);
+    wave_add64(P.ctr64 + 5, cnt.spheres);
+    wave_add64(P.ctr64 + 7, cnt.fallbacks);
+  }
+}
+
 template <int WALK, int STACK>
 static void launch_extend(bool counting, unsigned grid, hipStream_t st, const PathSoA& P,
-                          const DevScene& S, const uint32_t* q, const uint32_t* c) {
+                          const DevScene& S, const uint32_t* q, const uint32_t* c,
+                          uint32_t* fetch = nullptr) {
+  if (WALK == CT_WALK_ORDERED && fetch) {
+    if (counting)
+      hipLaunchKernelGGL((k_extend_persist<STACK, true>), dim3(grid), dim3(RBLOCK), 0, st, P, S, q, c, fetch);
+    else
+      hipLaunchKernelGGL((k_extend_persist<STACK, false>), dim3(grid), dim3(RBLOCK), 0, st, P, S, q, c, fetch);
+    return;
+  }
   if (counting)
     hipLaunchKernelGGL((k_extend<WALK, STACK, true>), dim3(grid), dim3(RBLOCK), 0, st, P, S, q, c);
   else
@@ -620,7 +882,14 @@ static void launch_extend(bool counting, unsigned grid, hipStream_t st, const Pa
 
 template <int WALK, int STACK>
 static void launch_shadow(bool counting, unsigned grid, hipStream_t st, const PathSoA& P,
-                          const DevScene& S, const uint32_t* c) {
+                          const DevScene& S, const uint32_t* c, uint32_t* fetch = nullptr) {
+  if (WALK == CT_WALK_ORDERED && fetch) {
+    if (counting)
+      hipLaunchKernelGGL((k_shadow_persist<STACK, true>), dim3(grid), dim3(RBLOCK), 0, st, P, S, c, fetch);
+    else
+      hipLaunchKernelGGL((k_shadow_persist<STACK, false>), dim3(grid), dim3(RBLOCK), 0, st, P, S, c, fetch);
+    return;
+  }
   if (counting)
     hipLaunchKernelGGL((k_shadow<WALK, STACK, true>), dim3(grid), dim3(RBLOCK), 0, st, P, S, c);
   else
@@ -633,32 +902,32 @@ static bool stack_supported(int d) { return d == 8 || d == 10 || d == 12 || d ==
 
 static void dispatch_extend(int walk, int stack, bool counting, unsigned grid, hipStream_t st,
                             const PathSoA& P, const DevScene& S, const uint32_t* q,
-                            const uint32_t* c) {
+                            const uint32_t* c, uint32_t* f) {
   if (walk == CT_WALK_STACKLESS) {
     launch_extend<CT_WALK_STACKLESS, 1>(counting, grid, st, P, S, q, c);
     return;
   }
   switch (stack) {
-    case 8: launch_extend<CT_WALK_ORDERED, 8>(counting, grid, st, P, S, q, c); break;
-    case 10: launch_extend<CT_WALK_ORDERED, 10>(counting, grid, st, P, S, q, c); break;
-    case 12: launch_extend<CT_WALK_ORDERED, 12>(counting, grid, st, P, S, q, c); break;
-    case 16: launch_extend<CT_WALK_ORDERED, 16>(counting, grid, st, P, S, q, c); break;
-    default: launch_extend<CT_WALK_ORDERED, 24>(counting, grid, st, P, S, q, c); break;
+    case 8: launch_extend<CT_WALK_ORDERED, 8>(counting, grid, st, P, S, q, c, f); break;
+    case 10: launch_extend<CT_WALK_ORDERED, 10>(counting, grid, st, P, S, q, c, f); break;
+    case 12: launch_extend<CT_WALK_ORDERED, 12>(counting, grid, st, P, S, q, c, f); break;
+    case 16: launch_extend<CT_WALK_ORDERED, 16>(counting, grid, st, P, S, q, c, f); break;
+    default: launch_extend<CT_WALK_ORDERED, 24>(counting, grid, st, P, S, q, c, f); break;
   }
 }
 
 static void dispatch_shadow(int walk, int stack, bool counting, unsigned grid, hipStream_t st,
-                            const PathSoA& P, const DevScene& S, const uint32_t* c) {
+                            const PathSoA& P, const DevScene& S, const uint32_t* c, uint32_t* f) {
   if (walk == CT_WALK_STACKLESS) {
     launch_shadow<CT_WALK_STACKLESS, 1>(counting, grid, st, P, S, c);
     return;
   }
   switch (stack) {
-    case 8: launch_shadow<CT_WALK_ORDERED, 8>(counting, grid, st, P, S, c); break;
-    case 10: launch_shadow<CT_WALK_ORDERED, 10>(counting, grid, st, P, S, c); break;
-    case 12: launch_shadow<CT_WALK_ORDERED, 12>(counting, grid, st, P, S, c); break;
-    case 16: launch_shadow<CT_WALK_ORDERED, 16>(counting, grid, st, P, S, c); break;
-    default: launch_shadow<CT_WALK_ORDERED, 24>(counting, grid, st, P, S, c); break;
+    case 8: launch_shadow<CT_WALK_ORDERED, 8>(counting, grid, st, P, S, c, f); break;
+    case 10: launch_shadow<CT_WALK_ORDERED, 10>(counting, grid, st, P, S, c, f); break;
+    case 12: launch_shadow<CT_WALK_ORDERED, 12>(counting, grid, st, P, S, c, f); break;
+    case 16: launch_shadow<CT_WALK_ORDERED, 16>(counting, grid, st, P, S, c, f); break;
+    default: launch_shadow<CT_WALK_ORDERED, 24>(counting, grid, st, P, S, c, f); break;
   }
 }
 
@@ -708,7 +977,9 @@ struct ct_tracer {
   bool counting = false;
   int walk = CT_WALK_ORDERED;        // closest-hit walk of extension rays
   int shadowWalk = CT_WALK_ORDERED;  // any-hit walk of NEE rays
-  int stack = 12;                    // LDS stack entries of the ordered walks
+  int stack = 16;                    // LDS stack entries of the ordered walks
+  bool persistent = true;            // lanes refill with new rays as they finish
+  unsigned persistBlocks = 256 * 8;  // workgroups of a persistent launch
   // stats of the last run
   int lastBounces = 0;
   uint64_t lastPaths = 0;
@@ -730,7 +1001,7 @@ static int tracer_reserve(ct_tracer* T, size_t paths) {
   }
   size_t n = paths;
   size_t d = al(n * 8), u32 = al(n * 4), u8 = al(n);
-  size_t total = 21 * d + 5 * u32 + u8 + al(2 * MAX_BOUNCE_SLOTS * 4) + al(16 * 8);
+  size_t total = 21 * d + 5 * u32 + u8 + al(4 * MAX_BOUNCE_SLOTS * 4) + al(16 * 8);
   CT_HIP(hipMalloc(&T->buf, total));
   char* p = (char*)T->buf;
   auto takeD = [&]() { double* r = (double*)p; p += d; return r; };
@@ -747,7 +1018,7 @@ static int tracer_reserve(ct_tracer* T, size_t paths) {
   P.hitPrim = (int32_t*)takeU();
   P.q0 = takeU(); P.q1 = takeU(); P.sPath = takeU();
   P.specular = (uint8_t*)p; p += u8;
-  P.ctr = (uint32_t*)p; p += al(2 * MAX_BOUNCE_SLOTS * 4);
+  P.ctr = (uint32_t*)p; p += al(4 * MAX_BOUNCE_SLOTS * 4);
   P.ctr64 = (unsigned long long*)p;
   T->capPaths = paths;
   return CT_OK;
@@ -796,7 +1067,7 @@ static int run_passes(ct_tracer* T, const ct_scene* s, const Frame& F, uint32_t 
   int nb = opts.maxBounces + 1;
   // live/shadow queue counts restart per batch; the 64-bit work counters keep
   // summing over the batches of one render
-  hipLaunchKernelGGL(k_clear, dim3(2), dim3(256), 0, st, P.ctr, 2 * MAX_BOUNCE_SLOTS, P.ctr64,
+  hipLaunchKernelGGL(k_clear, dim3(4), dim3(256), 0, st, P.ctr, 4 * MAX_BOUNCE_SLOTS, P.ctr64,
                      firstBatch ? 16 : 0);
   CT_LAUNCH_CHECK("k_clear");
   unsigned gridPaths = grid_for(paths, RBLOCK, 256 * 16);
@@ -810,7 +1081,10 @@ static int run_passes(ct_tracer* T, const ct_scene* s, const Frame& F, uint32_t 
     uint32_t* cout = P.ctr + b + 1;
     uint32_t* scount = P.ctr + MAX_BOUNCE_SLOTS + b;
     mark(T, st, 0, true);
-    dispatch_extend(T->walk, T->stack, T->counting, gridPaths, st, P, s->dev, qin, cin);
+    uint32_t* fExt = T->persistent ? P.ctr + 2 * MAX_BOUNCE_SLOTS + b : nullptr;
+    uint32_t* fSh = T->persistent ? P.ctr + 3 * MAX_BOUNCE_SLOTS + b : nullptr;
+    unsigned gridTrace = T->persistent ? grid_for(paths, RBLOCK, T->persistBlocks) : gridPaths;
+    dispatch_extend(T->walk, T->stack, T->counting, gridTrace, st, P, s->dev, qin, cin, fExt);
     CT_LAUNCH_CHECK("k_extend");
     mark(T, st, 0, false);
     mark(T, st, 1, true);
@@ -819,7 +1093,7 @@ static int run_passes(ct_tracer* T, const ct_scene* s, const Frame& F, uint32_t 
     CT_LAUNCH_CHECK("k_shade");
     mark(T, st, 1, false);
     mark(T, st, 2, true);
-    dispatch_shadow(T->shadowWalk, T->stack, T->counting, gridPaths, st, P, s->dev, scount);
+    dispatch_shadow(T->shadowWalk, T->stack, T->counting, gridTrace, st, P, s->dev, scount, fSh);
     CT_LAUNCH_CHECK("k_shadow");
     mark(T, st, 2, false);
   }
@@ -890,6 +1164,15 @@ int ct_tracer_set_traversal(ct_tracer* T, int32_t shadowWalk, int32_t stackDepth
     return set_error(CT_EINVAL, "ct_tracer_set_traversal: stack depth must be 8, 10, 12, 16 or 24");
   T->shadowWalk = shadowWalk;
   T->stack = stackDepth;
+  return CT_OK;
+}
+
+int ct_tracer_set_schedule(ct_tracer* T, int32_t persistent, int32_t blocks) {
+  if (!T) return set_error(CT_EINVAL, "null tracer");
+  if (persistent && (blocks < 1 || blocks > 65535))
+    return set_error(CT_EINVAL, "ct_tracer_set_schedule: blocks must be in [1, 65535]");
+  T->persistent = persistent != 0;
+  if (persistent) T->persistBlocks = (unsigned)blocks;
   return CT_OK;
 }
 

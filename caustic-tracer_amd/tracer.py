@@ -181,8 +181,11 @@ class Tracer:
     def set_walk(self, walk):
         check(lib().ct_tracer_set_walk(self.handle, walk), "ct_tracer_set_walk")
 
-    def set_traversal(self, shadow_walk=WALK_ORDERED, stack=12):
+    def set_traversal(self, shadow_walk=WALK_ORDERED, stack=16):
         check(lib().ct_tracer_set_traversal(self.handle, shadow_walk, stack), "set_traversal")
+
+    def set_schedule(self, persistent=True, blocks=2048):
+        check(lib().ct_tracer_set_schedule(self.handle, int(bool(persistent)), blocks), "schedule")
 
     def set_profiling(self, on):
         check(lib().ct_tracer_set_profiling(self.handle, int(bool(on))), "profiling")

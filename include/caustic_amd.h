@@ -146,6 +146,10 @@ int ct_tracer_set_walk(ct_tracer *tracer, int32_t walk);
  * and the LDS stack depth of the ordered walks (8, 10, 12, 16 or 24 entries;
  * deeper rays finish with the stackless walk, same result). */
 int ct_tracer_set_traversal(ct_tracer *tracer, int32_t shadowWalk, int32_t stackDepth);
+/* Launch schedule of the traversal kernels: persistent = 1 runs `blocks`
+ * workgroups of 256 lanes that pull rays from the queue as lanes free up;
+ * 0 launches one lane per queue slot.  Results do not depend on it. */
+int ct_tracer_set_schedule(ct_tracer *tracer, int32_t persistent, int32_t blocks);
 /* accumulatePass (tracer.ts:361-395): adds one sample per pixel into the
  * float32 sum d_accum (width*height*3) with frameIndex as the RNG frame. */
 int ct_accumulate_pass(ct_tracer *tracer, const ct_scene *scene, const double *h_frame16,
