@@ -119,6 +119,12 @@ __device__ __forceinline__ double ray_sphere(double ox, double oy, double oz, do
 }
 
 // Two-sided Moller-Trumbore; returns true and t,u,v on a hit in (tMin, tMax).
+// The arithmetic of every value returned is the reference's.  Before the
+// division the barycentric numerators are checked against |det|: a triangle
+// is dropped early only when u = num_u/det is certainly < 0 or > 1 (margins
+// of 1e-14 relative and 1e-250 absolute keep rounding and underflow out of
+// the decision), or likewise for v and u + v, i.e. only when the reference's
+// own tests would reject it; otherwise the reference's computation runs.
 __device__ __forceinline__ bool ray_triangle(double ox, double oy, double oz, double dx,
                                              double dy, double dz, double ax, double ay,
                                              double az, double bx, double by, double bz,
@@ -132,14 +138,22 @@ __device__ __forceinline__ bool ray_triangle(double ox, double oy, double oz, do
   double pz = dx * e2y - dy * e2x;
   double det = e1x * px + e1y * py + e1z * pz;
   if (det > -1e-12 && det < 1e-12) return false;
-  double invDet = 1 / det;
   double tx = ox - ax, ty = oy - ay, tz = oz - az;
-  double u = (tx * px + ty * py + tz * pz) * invDet;
-  if (u < 0 || u > 1) return false;
+  double un = tx * px + ty * py + tz * pz;
   double qx = ty * e1z - tz * e1y;
   double qy = tz * e1x - tx * e1z;
   double qz = tx * e1y - ty * e1x;
-  double v = (dx * qx + dy * qy + dz * qz) * invDet;
+  double vn = dx * qx + dy * qy + dz * qz;
+  {
+    double ad = fabs(det);
+    double su = det > 0 ? un : -un, sv = det > 0 ? vn : -vn;
+    double lim = ad * (1 + 1e-14);
+    if (su < -1e-250 || su > lim || sv < -1e-250 || su + sv > lim * (1 + 1e-14)) return false;
+  }
+  double invDet = 1 / det;
+  double u = un * invDet;
+  if (u < 0 || u > 1) return false;
+  double v = vn * invDet;
   if (v < 0 || u + v > 1) return false;
   double t = (e2x * qx + e2y * qy + e2z * qz) * invDet;
   if (t <= tMin || t >= tMax) return false;
@@ -477,6 +491,25 @@ __device__ __forceinline__ bool box_enter(double ox, double oy, double oz, doubl
   double tx0 = ((double)lo.x - ox) * ix, tx1 = ((double)hi.x - ox) * ix;
   double ty0 = ((double)lo.y - oy) * iy, ty1 = ((double)hi.y - oy) * iy;
   double tz0 = ((double)lo.z - oz) * iz, tz1 = ((double)hi.z - oz) * iz;
+  double tn = fmax(fmax(fmin(tx0, tx1), fmin(ty0, ty1)), fmax(fmin(tz0, tz1), 0.0));
+  double tf = fmin(fmin(fmax(tx0, tx1), fmax(ty0, ty1)), fmin(fmax(tz0, tz1), tMax));
+  entry = tn;
+  return tn <= tf;
+}
+
+// Slab test with one fused multiply-add per plane: t = b*inv - (o*inv).  Its
+// values differ from (b - o)*inv by about 1e-16 relative (a few 1e-15 scene
+// units), while every box is padded by at least 1e-6 beyond the primitives it
+// holds (pack.ts:153,177, bvh.ts:140-147).  A ray that hits a primitive inside
+// a box therefore crosses that box with a margin no rounding difference can
+// remove: boxes holding a closer hit are accepted exactly when the reference
+// accepts them, and the closest hit is unchanged.  oi = o * inv per ray.
+__device__ __forceinline__ bool box_enter_fma(double ix, double iy, double iz, double oix,
+                                              double oiy, double oiz, float4 lo, float4 hi,
+                                              double tMax, double& entry) {
+  double tx0 = fma((double)lo.x, ix, -oix), tx1 = fma((double)hi.x, ix, -oix);
+  double ty0 = fma((double)lo.y, iy, -oiy), ty1 = fma((double)hi.y, iy, -oiy);
+  double tz0 = fma((double)lo.z, iz, -oiz), tz1 = fma((double)hi.z, iz, -oiz);
   double tn = fmax(fmax(fmin(tx0, tx1), fmin(ty0, ty1)), fmax(fmin(tz0, tz1), 0.0));
   double tf = fmin(fmin(fmax(tx0, tx1), fmax(ty0, ty1)), fmin(fmax(tz0, tz1), tMax));
   entry = tn;

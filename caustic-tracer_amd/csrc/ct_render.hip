@@ -630,6 +630,7 @@ k_extend_persist(PathSoA P, DevScene S, const uint32_t* __restrict__ q,
   bool have = false, exhausted = false;
   uint32_t pid = 0;
   double ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0;
+  double oix = 0, oiy = 0, oiz = 0;
   Hit h;
   h.t = 0; h.u = 0; h.v = 0; h.prim = -1;
   int cur = -1, sp = 0, pend = -1;
@@ -651,12 +652,13 @@ k_extend_persist(PathSoA P, DevScene S, const uint32_t* __restrict__ q,
             ox = P.ox[pid]; oy = P.oy[pid]; oz = P.oz[pid];
             dx = P.dx[pid]; dy = P.dy[pid]; dz = P.dz[pid];
             ix = safe_inv(dx); iy = safe_inv(dy); iz = safe_inv(dz);
+            oix = ox * ix; oiy = oy * iy; oiz = oz * iz;
             h.t = __builtin_inf(); h.u = 0; h.v = 0; h.prim = -1;
             sp = 0; pend = -1; cur = -1;
             if (COUNT) cnt.nodes++;
             float4 lo = S.nodes[0], hi = S.nodes[1];
             double e;
-            if (box_enter(ox, oy, oz, ix, iy, iz, lo, hi, h.t, e)) cur = enc_child(0, hi.w);
+            if (box_enter_fma(ix, iy, iz, oix, oiy, oiz, lo, hi, h.t, e)) cur = enc_child(0, hi.w);
             have = true;
           }
         }
@@ -677,8 +679,8 @@ k_extend_persist(PathSoA P, DevScene S, const uint32_t* __restrict__ q,
           float4 rlo = S.nodes[2 * (size_t)right], rhi = S.nodes[2 * (size_t)right + 1];
           if (COUNT) cnt.nodes += 2;
           double tl, tr;
-          bool hl = box_enter(ox, oy, oz, ix, iy, iz, llo, lhi, h.t, tl);
-          bool hr = box_enter(ox, oy, oz, ix, iy, iz, rlo, rhi, h.t, tr);
+          bool hl = box_enter_fma(ix, iy, iz, oix, oiy, oiz, llo, lhi, h.t, tl);
+          bool hr = box_enter_fma(ix, iy, iz, oix, oiy, oiz, rlo, rhi, h.t, tr);
           int el = enc_child(left, lhi.w), er = enc_child(right, rhi.w);
           if (hl && hr) {
             if (sp == STACK) {
@@ -748,6 +750,7 @@ k_shadow_persist(PathSoA P, DevScene S, const uint32_t* __restrict__ count,
   bool have = false, exhausted = false;
   uint32_t slotCur = 0, pid = 0;
   double ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0, tMax = 0;
+  double oix = 0, oiy = 0, oiz = 0;
   int cur = -1, sp = 0, pend = -1;
   const int lane = __lane_id();
   for (;;) {
@@ -769,11 +772,12 @@ k_shadow_persist(PathSoA P, DevScene S, const uint32_t* __restrict__ count,
             dx = P.sdx[slot]; dy = P.sdy[slot]; dz = P.sdz[slot];
             tMax = P.sTmax[slot];
             ix = safe_inv(dx); iy = safe_inv(dy); iz = safe_inv(dz);
+            oix = ox * ix; oiy = oy * iy; oiz = oz * iz;
             sp = 0; pend = -1; cur = -1;
             if (COUNT) cnt.nodes++;
             float4 lo = S.nodes[0], hi = S.nodes[1];
             double e;
-            if (box_enter(ox, oy, oz, ix, iy, iz, lo, hi, tMax, e)) cur = enc_child(0, hi.w);
+            if (box_enter_fma(ix, iy, iz, oix, oiy, oiz, lo, hi, tMax, e)) cur = enc_child(0, hi.w);
             have = true;
           }
         }
@@ -794,8 +798,8 @@ k_shadow_persist(PathSoA P, DevScene S, const uint32_t* __restrict__ count,
           float4 rlo = S.nodes[2 * (size_t)right], rhi = S.nodes[2 * (size_t)right + 1];
           if (COUNT) cnt.nodes += 2;
           double tl, tr;
-          bool hl = box_enter(ox, oy, oz, ix, iy, iz, llo, lhi, tMax, tl);
-          bool hr = box_enter(ox, oy, oz, ix, iy, iz, rlo, rhi, tMax, tr);
+          bool hl = box_enter_fma(ix, iy, iz, oix, oiy, oiz, llo, lhi, tMax, tl);
+          bool hr = box_enter_fma(ix, iy, iz, oix, oiy, oiz, rlo, rhi, tMax, tr);
           int el = enc_child(left, lhi.w), er = enc_child(right, rhi.w);
           if (hl && hr) {
             if (sp == STACK) {
