@@ -86,9 +86,9 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--spp", type=int, default=4)
     ap.add_argument("--depth", type=int, default=6)
-    ap.add_argument("--walk", choices=["ordered", "stackless"], default="ordered")
-    ap.add_argument("--shadow-walk", choices=["ordered", "stackless"], default="ordered")
-    ap.add_argument("--stack", type=int, default=16, help="LDS stack entries of the ordered walks")
+    ap.add_argument("--walk", choices=["ordered", "stackless", "wide"], default="wide")
+    ap.add_argument("--shadow-walk", choices=["ordered", "stackless", "wide"], default="wide")
+    ap.add_argument("--stack", type=int, default=24, help="LDS stack entries of the ordered walks")
     ap.add_argument("--persist-blocks", type=int, default=2048,
                     help="workgroups of the persistent traversal launches (0: one lane per ray)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -129,10 +129,10 @@ def main():
 
     ds = ct.DeviceScene(ps)
     frame = ct.cameraFrame(scene["camera"], w / h)
-    walk = ct.WALK_ORDERED if args.walk == "ordered" else ct.WALK_STACKLESS
+    walks = {"ordered": ct.WALK_ORDERED, "stackless": ct.WALK_STACKLESS, "wide": ct.WALK_WIDE}
+    walk = walks[args.walk]
     tr = ct.Tracer(walk=walk)
-    tr.set_traversal(ct.WALK_ORDERED if args.shadow_walk == "ordered" else ct.WALK_STACKLESS,
-                     args.stack)
+    tr.set_traversal(walks[args.shadow_walk], args.stack)
     tr.set_schedule(args.persist_blocks > 0, max(1, args.persist_blocks))
     out = ct.DeviceBuffer.empty((w * h * 3,), np.float32)
     opts = {"width": w, "height": h, "spp": spp, "seed": 1, "maxBounces": depth,

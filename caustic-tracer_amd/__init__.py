@@ -10,13 +10,13 @@ call that needs the library fails loudly when it is missing.
 from . import synth
 from ._capi import CTError, DeviceBuffer, init, sync
 from .pack import PackedScene, buildBvh, packScene, primBounds
-from .tracer import (T_MIN, WALK_ORDERED, WALK_STACKLESS, DeviceScene, Tracer, accumulatePass,
+from .tracer import (T_MIN, WALK_ORDERED, WALK_STACKLESS, WALK_WIDE, DeviceScene, Tracer, accumulatePass,
                      cameraFrame, closest_hit, closest_hit_counts, intersectClosest, occluded,
                      renderImage)
 
 __all__ = [
     "synth", "CTError", "DeviceBuffer", "init", "sync", "PackedScene", "buildBvh", "packScene",
-    "primBounds", "T_MIN", "WALK_ORDERED", "WALK_STACKLESS", "DeviceScene", "Tracer",
+    "primBounds", "T_MIN", "WALK_ORDERED", "WALK_STACKLESS", "WALK_WIDE", "DeviceScene", "Tracer",
     "accumulatePass", "cameraFrame", "closest_hit", "closest_hit_counts", "intersectClosest",
     "occluded", "renderImage",
 ]

@@ -613,17 +613,17 @@ k_shadow(PathSoA P, DevScene S, const uint32_t* __restrict__ count) {
 
 // Persistent closest-hit: each lane pulls the next ray from the bounce's queue
 // as soon as its current ray is finished (wave-aggregated fetch), so lanes do
-// not idle until the slowest ray of their wave ends.  The walk itself is the
-// wave-organised ordered walk of closest_wave (same visiting rules, same
-// result), written as a state machine that survives ray changes.
-template <int STACK, bool COUNT>
+// not idle until the slowest ray of their wave ends.  The walk is the
+// wave-organised ordered walk of closest_wave (leaf parking, bf16 entries),
+// over the reference's binary nodes or the derived 4-wide nodes (WIDE).
+template <bool WIDE, int STACK, bool COUNT>
 __global__ void __launch_bounds__(RBLOCK)
 k_extend_persist(PathSoA P, DevScene S, const uint32_t* __restrict__ q,
                  const uint32_t* __restrict__ count, uint32_t* __restrict__ fetch) {
   __shared__ int sNodeAll[STACK * RBLOCK];
-  __shared__ float sEntryAll[STACK * RBLOCK];
+  __shared__ uint16_t sEntryAll[STACK * RBLOCK];
   int* sNode = sNodeAll + threadIdx.x;
-  float* sEntry = sEntryAll + threadIdx.x;
+  uint16_t* sEntry = sEntryAll + threadIdx.x;
   const int lanes = RBLOCK;
   const uint32_t n = *count;
   TraversalCounts cnt = {0, 0, 0, 0};
@@ -654,11 +654,8 @@ k_extend_persist(PathSoA P, DevScene S, const uint32_t* __restrict__ q,
             ix = safe_inv(dx); iy = safe_inv(dy); iz = safe_inv(dz);
             oix = ox * ix; oiy = oy * iy; oiz = oz * iz;
             h.t = __builtin_inf(); h.u = 0; h.v = 0; h.prim = -1;
-            sp = 0; pend = -1; cur = -1;
-            if (COUNT) cnt.nodes++;
-            float4 lo = S.nodes[0], hi = S.nodes[1];
-            double e;
-            if (box_enter_fma(ix, iy, iz, oix, oiy, oiz, lo, hi, h.t, e)) cur = enc_child(0, hi.w);
+            sp = 0; pend = -1;
+            cur = walk_root<WIDE>(S, ix, iy, iz, oix, oiy, oiz, h.t, cnt, COUNT);
             have = true;
           }
         }
@@ -672,39 +669,16 @@ k_extend_persist(PathSoA P, DevScene S, const uint32_t* __restrict__ q,
           if (pend >= 0) break;
           pend = -2 - cur;
           cur = -1;
-        } else {
-          int left = cur + 1;
-          float4 llo = S.nodes[2 * (size_t)left], lhi = S.nodes[2 * (size_t)left + 1];
-          int right = (int)llo.w;
-          float4 rlo = S.nodes[2 * (size_t)right], rhi = S.nodes[2 * (size_t)right + 1];
-          if (COUNT) cnt.nodes += 2;
-          double tl, tr;
-          bool hl = box_enter_fma(ix, iy, iz, oix, oiy, oiz, llo, lhi, h.t, tl);
-          bool hr = box_enter_fma(ix, iy, iz, oix, oiy, oiz, rlo, rhi, h.t, tr);
-          int el = enc_child(left, lhi.w), er = enc_child(right, rhi.w);
-          if (hl && hr) {
-            if (sp == STACK) {
-              overflow = true;
-              cur = -1;
-              break;
-            }
-            bool leftFirst = tl <= tr;
-            sNode[sp * lanes] = leftFirst ? er : el;
-            sEntry[sp * lanes] = f32_round_down(leftFirst ? tr : tl);
-            sp++;
-            cur = leftFirst ? el : er;
-          } else if (hl) {
-            cur = el;
-          } else if (hr) {
-            cur = er;
-          } else {
-            cur = -1;
-          }
+        } else if (!interior_step<WIDE, true, STACK, COUNT>(S, ix, iy, iz, oix, oiy, oiz, h.t,
+                                                            cur, sp, sNode, sEntry, lanes, cnt)) {
+          overflow = true;
+          cur = -1;
+          break;
         }
         if (cur == -1) {
           while (sp > 0) {
             sp--;
-            if ((double)sEntry[sp * lanes] < h.t) {
+            if (entry_from_bf16(sEntry[sp * lanes]) < h.t) {
               cur = sNode[sp * lanes];
               break;
             }
@@ -737,8 +711,8 @@ k_extend_persist(PathSoA P, DevScene S, const uint32_t* __restrict__ q,
   }
 }
 
-// Persistent any-hit of the NEE rays (same lane refill as k_extend_persist).
-template <int STACK, bool COUNT>
+// Persistent any-hit of the NEE rays (same lane refill and walk organisation).
+template <bool WIDE, int STACK, bool COUNT>
 __global__ void __launch_bounds__(RBLOCK)
 k_shadow_persist(PathSoA P, DevScene S, const uint32_t* __restrict__ count,
                  uint32_t* __restrict__ fetch) {
@@ -773,11 +747,8 @@ k_shadow_persist(PathSoA P, DevScene S, const uint32_t* __restrict__ count,
             tMax = P.sTmax[slot];
             ix = safe_inv(dx); iy = safe_inv(dy); iz = safe_inv(dz);
             oix = ox * ix; oiy = oy * iy; oiz = oz * iz;
-            sp = 0; pend = -1; cur = -1;
-            if (COUNT) cnt.nodes++;
-            float4 lo = S.nodes[0], hi = S.nodes[1];
-            double e;
-            if (box_enter_fma(ix, iy, iz, oix, oiy, oiz, lo, hi, tMax, e)) cur = enc_child(0, hi.w);
+            sp = 0; pend = -1;
+            cur = walk_root<WIDE>(S, ix, iy, iz, oix, oiy, oiz, tMax, cnt, COUNT);
             have = true;
           }
         }
@@ -791,33 +762,12 @@ k_shadow_persist(PathSoA P, DevScene S, const uint32_t* __restrict__ count,
           if (pend >= 0) break;
           pend = -2 - cur;
           cur = -1;
-        } else {
-          int left = cur + 1;
-          float4 llo = S.nodes[2 * (size_t)left], lhi = S.nodes[2 * (size_t)left + 1];
-          int right = (int)llo.w;
-          float4 rlo = S.nodes[2 * (size_t)right], rhi = S.nodes[2 * (size_t)right + 1];
-          if (COUNT) cnt.nodes += 2;
-          double tl, tr;
-          bool hl = box_enter_fma(ix, iy, iz, oix, oiy, oiz, llo, lhi, tMax, tl);
-          bool hr = box_enter_fma(ix, iy, iz, oix, oiy, oiz, rlo, rhi, tMax, tr);
-          int el = enc_child(left, lhi.w), er = enc_child(right, rhi.w);
-          if (hl && hr) {
-            if (sp == STACK) {
-              overflow = true;
-              cur = -1;
-              break;
-            }
-            bool leftFirst = tl <= tr;
-            sNode[sp * lanes] = leftFirst ? er : el;
-            sp++;
-            cur = leftFirst ? el : er;
-          } else if (hl) {
-            cur = el;
-          } else if (hr) {
-            cur = er;
-          } else {
-            cur = -1;
-          }
+        } else if (!interior_step<WIDE, false, STACK, COUNT>(S, ix, iy, iz, oix, oiy, oiz, tMax,
+                                                             cur, sp, sNode, nullptr, lanes,
+                                                             cnt)) {
+          overflow = true;
+          cur = -1;
+          break;
         }
         if (cur == -1 && sp > 0) {
           sp--;
@@ -871,68 +821,89 @@ template <int WALK, int STACK>
 static void launch_extend(bool counting, unsigned grid, hipStream_t st, const PathSoA& P,
                           const DevScene& S, const uint32_t* q, const uint32_t* c,
                           uint32_t* fetch = nullptr) {
-  if (WALK == CT_WALK_ORDERED && fetch) {
+  if (WALK != CT_WALK_STACKLESS && fetch) {
+    constexpr bool W = WALK == CT_WALK_WIDE;
     if (counting)
-      hipLaunchKernelGGL((k_extend_persist<STACK, true>), dim3(grid), dim3(RBLOCK), 0, st, P, S, q, c, fetch);
+      hipLaunchKernelGGL((k_extend_persist<W, STACK, true>), dim3(grid), dim3(RBLOCK), 0, st, P, S, q, c, fetch);
     else
-      hipLaunchKernelGGL((k_extend_persist<STACK, false>), dim3(grid), dim3(RBLOCK), 0, st, P, S, q, c, fetch);
+      hipLaunchKernelGGL((k_extend_persist<W, STACK, false>), dim3(grid), dim3(RBLOCK), 0, st, P, S, q, c, fetch);
     return;
   }
-  if (counting)
-    hipLaunchKernelGGL((k_extend<WALK, STACK, true>), dim3(grid), dim3(RBLOCK), 0, st, P, S, q, c);
-  else
-    hipLaunchKernelGGL((k_extend<WALK, STACK, false>), dim3(grid), dim3(RBLOCK), 0, st, P, S, q, c);
+  if constexpr (WALK != CT_WALK_WIDE) {
+    if (counting)
+      hipLaunchKernelGGL((k_extend<WALK, STACK, true>), dim3(grid), dim3(RBLOCK), 0, st, P, S, q, c);
+    else
+      hipLaunchKernelGGL((k_extend<WALK, STACK, false>), dim3(grid), dim3(RBLOCK), 0, st, P, S, q, c);
+  }
 }
 
 template <int WALK, int STACK>
 static void launch_shadow(bool counting, unsigned grid, hipStream_t st, const PathSoA& P,
                           const DevScene& S, const uint32_t* c, uint32_t* fetch = nullptr) {
-  if (WALK == CT_WALK_ORDERED && fetch) {
+  if (WALK != CT_WALK_STACKLESS && fetch) {
+    constexpr bool W = WALK == CT_WALK_WIDE;
     if (counting)
-      hipLaunchKernelGGL((k_shadow_persist<STACK, true>), dim3(grid), dim3(RBLOCK), 0, st, P, S, c, fetch);
+      hipLaunchKernelGGL((k_shadow_persist<W, STACK, true>), dim3(grid), dim3(RBLOCK), 0, st, P, S, c, fetch);
     else
-      hipLaunchKernelGGL((k_shadow_persist<STACK, false>), dim3(grid), dim3(RBLOCK), 0, st, P, S, c, fetch);
+      hipLaunchKernelGGL((k_shadow_persist<W, STACK, false>), dim3(grid), dim3(RBLOCK), 0, st, P, S, c, fetch);
     return;
   }
-  if (counting)
-    hipLaunchKernelGGL((k_shadow<WALK, STACK, true>), dim3(grid), dim3(RBLOCK), 0, st, P, S, c);
-  else
-    hipLaunchKernelGGL((k_shadow<WALK, STACK, false>), dim3(grid), dim3(RBLOCK), 0, st, P, S, c);
+  if constexpr (WALK != CT_WALK_WIDE) {
+    if (counting)
+      hipLaunchKernelGGL((k_shadow<WALK, STACK, true>), dim3(grid), dim3(RBLOCK), 0, st, P, S, c);
+    else
+      hipLaunchKernelGGL((k_shadow<WALK, STACK, false>), dim3(grid), dim3(RBLOCK), 0, st, P, S, c);
+  }
 }
 
 // Stack depths the ordered walks are instantiated for (LDS per 256-lane block:
-// 8 B/entry for closest hit, 4 B/entry for any hit).
-static bool stack_supported(int d) { return d == 8 || d == 10 || d == 12 || d == 16 || d == 24; }
+// 6 B/entry for closest hit, 4 B/entry for any hit).
+static bool stack_supported(int d) { return d == 8 || d == 12 || d == 16 || d == 24; }
 
+template <int WALK>
+static void dispatch_extend_w(int stack, bool counting, unsigned grid, hipStream_t st,
+                              const PathSoA& P, const DevScene& S, const uint32_t* q,
+                              const uint32_t* c, uint32_t* f) {
+  switch (stack) {
+    case 8: launch_extend<WALK, 8>(counting, grid, st, P, S, q, c, f); break;
+    case 12: launch_extend<WALK, 12>(counting, grid, st, P, S, q, c, f); break;
+    case 16: launch_extend<WALK, 16>(counting, grid, st, P, S, q, c, f); break;
+    default: launch_extend<WALK, 24>(counting, grid, st, P, S, q, c, f); break;
+  }
+}
+
+template <int WALK>
+static void dispatch_shadow_w(int stack, bool counting, unsigned grid, hipStream_t st,
+                              const PathSoA& P, const DevScene& S, const uint32_t* c,
+                              uint32_t* f) {
+  switch (stack) {
+    case 8: launch_shadow<WALK, 8>(counting, grid, st, P, S, c, f); break;
+    case 12: launch_shadow<WALK, 12>(counting, grid, st, P, S, c, f); break;
+    case 16: launch_shadow<WALK, 16>(counting, grid, st, P, S, c, f); break;
+    default: launch_shadow<WALK, 24>(counting, grid, st, P, S, c, f); break;
+  }
+}
+
+// The wide walk always runs persistent (f must be the launch's fetch cursor).
 static void dispatch_extend(int walk, int stack, bool counting, unsigned grid, hipStream_t st,
                             const PathSoA& P, const DevScene& S, const uint32_t* q,
                             const uint32_t* c, uint32_t* f) {
-  if (walk == CT_WALK_STACKLESS) {
+  if (walk == CT_WALK_STACKLESS)
     launch_extend<CT_WALK_STACKLESS, 1>(counting, grid, st, P, S, q, c);
-    return;
-  }
-  switch (stack) {
-    case 8: launch_extend<CT_WALK_ORDERED, 8>(counting, grid, st, P, S, q, c, f); break;
-    case 10: launch_extend<CT_WALK_ORDERED, 10>(counting, grid, st, P, S, q, c, f); break;
-    case 12: launch_extend<CT_WALK_ORDERED, 12>(counting, grid, st, P, S, q, c, f); break;
-    case 16: launch_extend<CT_WALK_ORDERED, 16>(counting, grid, st, P, S, q, c, f); break;
-    default: launch_extend<CT_WALK_ORDERED, 24>(counting, grid, st, P, S, q, c, f); break;
-  }
+  else if (walk == CT_WALK_WIDE)
+    dispatch_extend_w<CT_WALK_WIDE>(stack, counting, grid, st, P, S, q, c, f);
+  else
+    dispatch_extend_w<CT_WALK_ORDERED>(stack, counting, grid, st, P, S, q, c, f);
 }
 
 static void dispatch_shadow(int walk, int stack, bool counting, unsigned grid, hipStream_t st,
                             const PathSoA& P, const DevScene& S, const uint32_t* c, uint32_t* f) {
-  if (walk == CT_WALK_STACKLESS) {
+  if (walk == CT_WALK_STACKLESS)
     launch_shadow<CT_WALK_STACKLESS, 1>(counting, grid, st, P, S, c);
-    return;
-  }
-  switch (stack) {
-    case 8: launch_shadow<CT_WALK_ORDERED, 8>(counting, grid, st, P, S, c, f); break;
-    case 10: launch_shadow<CT_WALK_ORDERED, 10>(counting, grid, st, P, S, c, f); break;
-    case 12: launch_shadow<CT_WALK_ORDERED, 12>(counting, grid, st, P, S, c, f); break;
-    case 16: launch_shadow<CT_WALK_ORDERED, 16>(counting, grid, st, P, S, c, f); break;
-    default: launch_shadow<CT_WALK_ORDERED, 24>(counting, grid, st, P, S, c, f); break;
-  }
+  else if (walk == CT_WALK_WIDE)
+    dispatch_shadow_w<CT_WALK_WIDE>(stack, counting, grid, st, P, S, c, f);
+  else
+    dispatch_shadow_w<CT_WALK_ORDERED>(stack, counting, grid, st, P, S, c, f);
 }
 
 // Adds each pass's sample in pass order, dropping non-finite samples
@@ -979,9 +950,9 @@ struct ct_tracer {
   PathSoA P{};
   bool profiling = false;
   bool counting = false;
-  int walk = CT_WALK_ORDERED;        // closest-hit walk of extension rays
-  int shadowWalk = CT_WALK_ORDERED;  // any-hit walk of NEE rays
-  int stack = 16;                    // LDS stack entries of the ordered walks
+  int walk = CT_WALK_WIDE;           // closest-hit walk of extension rays
+  int shadowWalk = CT_WALK_WIDE;     // any-hit walk of NEE rays
+  int stack = 24;                    // LDS stack entries of the ordered walks
   bool persistent = true;            // lanes refill with new rays as they finish
   unsigned persistBlocks = 256 * 8;  // workgroups of a persistent launch
   // stats of the last run
@@ -1085,9 +1056,10 @@ static int run_passes(ct_tracer* T, const ct_scene* s, const Frame& F, uint32_t 
     uint32_t* cout = P.ctr + b + 1;
     uint32_t* scount = P.ctr + MAX_BOUNCE_SLOTS + b;
     mark(T, st, 0, true);
-    uint32_t* fExt = T->persistent ? P.ctr + 2 * MAX_BOUNCE_SLOTS + b : nullptr;
-    uint32_t* fSh = T->persistent ? P.ctr + 3 * MAX_BOUNCE_SLOTS + b : nullptr;
-    unsigned gridTrace = T->persistent ? grid_for(paths, RBLOCK, T->persistBlocks) : gridPaths;
+    bool persist = T->persistent || T->walk == CT_WALK_WIDE || T->shadowWalk == CT_WALK_WIDE;
+    uint32_t* fExt = persist ? P.ctr + 2 * MAX_BOUNCE_SLOTS + b : nullptr;
+    uint32_t* fSh = persist ? P.ctr + 3 * MAX_BOUNCE_SLOTS + b : nullptr;
+    unsigned gridTrace = persist ? grid_for(paths, RBLOCK, T->persistBlocks) : gridPaths;
     dispatch_extend(T->walk, T->stack, T->counting, gridTrace, st, P, s->dev, qin, cin, fExt);
     CT_LAUNCH_CHECK("k_extend");
     mark(T, st, 0, false);
@@ -1162,10 +1134,11 @@ int ct_tracer_set_counting(ct_tracer* T, int32_t enable) {
 
 int ct_tracer_set_traversal(ct_tracer* T, int32_t shadowWalk, int32_t stackDepth) {
   if (!T) return set_error(CT_EINVAL, "null tracer");
-  if (shadowWalk != CT_WALK_ORDERED && shadowWalk != CT_WALK_STACKLESS)
+  if (shadowWalk != CT_WALK_ORDERED && shadowWalk != CT_WALK_STACKLESS &&
+      shadowWalk != CT_WALK_WIDE)
     return set_error(CT_EINVAL, "ct_tracer_set_traversal: unknown walk");
   if (!stack_supported(stackDepth))
-    return set_error(CT_EINVAL, "ct_tracer_set_traversal: stack depth must be 8, 10, 12, 16 or 24");
+    return set_error(CT_EINVAL, "ct_tracer_set_traversal: stack depth must be 8, 12, 16 or 24");
   T->shadowWalk = shadowWalk;
   T->stack = stackDepth;
   return CT_OK;
@@ -1182,7 +1155,7 @@ int ct_tracer_set_schedule(ct_tracer* T, int32_t persistent, int32_t blocks) {
 
 int ct_tracer_set_walk(ct_tracer* T, int32_t walk) {
   if (!T) return set_error(CT_EINVAL, "null tracer");
-  if (walk != CT_WALK_ORDERED && walk != CT_WALK_STACKLESS)
+  if (walk != CT_WALK_ORDERED && walk != CT_WALK_STACKLESS && walk != CT_WALK_WIDE)
     return set_error(CT_EINVAL, "ct_tracer_set_walk: unknown walk");
   T->walk = walk;
   return CT_OK;

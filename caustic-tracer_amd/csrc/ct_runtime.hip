@@ -104,6 +104,74 @@ static int validate_scene(const float* nodes, uint32_t nodeCount, uint32_t primC
   return CT_OK;
 }
 
+// 4-wide nodes from the binary tree: every interior node at even depth becomes
+// a wide node whose children are its grandchildren (or its children where
+// those are leaves).  Boxes are the reference nodes' float32 boxes, copied
+// bit for bit; only the intermediate boxes are skipped, which widens the set
+// of subtrees a walk enters and never narrows it.
+static void build_wide(const float* nodes, uint32_t nodeCount, std::vector<float>& out) {
+  out.clear();
+  auto leafWord = [&](uint32_t i) { return nodes[(size_t)i * NODE_STRIDE + 7]; };
+  auto isLeaf = [&](uint32_t i) { return leafWord(i) >= 0.0f; };
+  auto code = [&](uint32_t i) { return -2 - (int32_t)leafWord(i); };
+  struct Item { uint32_t n2; uint32_t slot; };  // binary node to expand into wide node `slot`
+  std::vector<Item> work;
+  auto alloc = [&]() {
+    uint32_t slot = (uint32_t)(out.size() / 32);
+    out.resize(out.size() + 32);
+    float* r = &out[(size_t)slot * 32];
+    for (int k = 0; k < 4; k++) {
+      // an empty slot: a box at +inf on every axis, which no ray enters
+      r[k] = r[4 + k] = r[8 + k] = __builtin_inff();
+      r[12 + k] = r[16 + k] = r[20 + k] = __builtin_inff();
+      int32_t empty = -1;
+      memcpy(&r[24 + k], &empty, 4);
+    }
+    return slot;
+  };
+  auto put = [&](uint32_t slot, int k, uint32_t n2, int32_t c) {
+    float* r = &out[(size_t)slot * 32];
+    const float* b = nodes + (size_t)n2 * NODE_STRIDE;
+    r[k] = b[0]; r[4 + k] = b[1]; r[8 + k] = b[2];
+    r[12 + k] = b[4]; r[16 + k] = b[5]; r[20 + k] = b[6];
+    memcpy(&r[24 + k], &c, 4);
+  };
+  uint32_t root = alloc();
+  if (isLeaf(0)) {
+    put(root, 0, 0, code(0));
+    return;
+  }
+  work.push_back({0, root});
+  std::vector<uint32_t> kids;
+  while (!work.empty()) {
+    Item it = work.back();
+    work.pop_back();
+    kids.clear();
+    uint32_t L = it.n2 + 1;
+    uint32_t R = (uint32_t)nodes[(size_t)L * NODE_STRIDE + 3];
+    for (uint32_t c : {L, R}) {
+      if (isLeaf(c)) {
+        kids.push_back(c);
+      } else {
+        uint32_t cl = c + 1;
+        kids.push_back(cl);
+        kids.push_back((uint32_t)nodes[(size_t)cl * NODE_STRIDE + 3]);
+      }
+    }
+    for (size_t k = 0; k < kids.size(); k++) {
+      uint32_t g = kids[k];
+      if (isLeaf(g)) {
+        put(it.slot, (int)k, g, code(g));
+      } else {
+        uint32_t ws = alloc();
+        put(it.slot, (int)k, g, (int32_t)ws);
+        work.push_back({g, ws});
+      }
+    }
+  }
+  (void)nodeCount;
+}
+
 static int scene_create_impl(const float* prims, uint32_t primCount, const float* nodes,
                              uint32_t nodeCount, const float* mats, uint32_t matCount,
                              const int32_t* lights, uint32_t lightCount,
@@ -119,9 +187,11 @@ static int scene_create_impl(const float* prims, uint32_t primCount, const float
   if (nodeCount > (1u << 24))
     return set_error(CT_ELIMIT,
                      "ct_scene_create: skip links are float32 and exact only below 2^24 nodes");
+  std::vector<float> wide;
   if (kind == hipMemcpyHostToDevice) {
     int v = validate_scene(nodes, nodeCount, primCount, matCount, lights, lightCount, primObject);
     if (v) return v;
+    build_wide(nodes, nodeCount, wide);
   } else {
     std::vector<float> hn((size_t)nodeCount * NODE_STRIDE);
     std::vector<int32_t> hl(lightCount);
@@ -135,6 +205,7 @@ static int scene_create_impl(const float* prims, uint32_t primCount, const float
     int v = validate_scene(hn.data(), nodeCount, primCount, matCount, hl.data(), lightCount,
                            ho.data());
     if (v) return v;
+    build_wide(hn.data(), nodeCount, wide);
   }
   // The array lengths PackedScene carries: max(1, n) records each (pack.ts:136-138, 111).
   size_t pc = primCount ? primCount : 1, mc = matCount ? matCount : 1;
@@ -143,7 +214,8 @@ static int scene_create_impl(const float* prims, uint32_t primCount, const float
   size_t szMats = align_up(mc * MAT_STRIDE * sizeof(float));
   size_t szLights = align_up((lightCount ? lightCount : 1) * sizeof(int32_t));
   size_t szObj = align_up(pc * sizeof(uint32_t));
-  size_t total = szPrims + szNodes + szMats + szLights + szObj;
+  size_t szWide = align_up(wide.size() * sizeof(float));
+  size_t total = szPrims + szNodes + szMats + szLights + szObj + szWide;
   ct_scene* s = new ct_scene();
   hipError_t e = hipMalloc(&s->buf, total);
   if (e != hipSuccess) {
@@ -158,6 +230,7 @@ static int scene_create_impl(const float* prims, uint32_t primCount, const float
   char* pMats = pNodes + szNodes;
   char* pLights = pMats + szMats;
   char* pObj = pLights + szLights;
+  char* pWide = pObj + szObj;
   e = hipMemsetAsync(base, 0, total, stream);
   if (e == hipSuccess && primCount)
     e = hipMemcpyAsync(pPrims, prims, (size_t)primCount * PRIM_STRIDE * sizeof(float), kind, stream);
@@ -169,6 +242,9 @@ static int scene_create_impl(const float* prims, uint32_t primCount, const float
     e = hipMemcpyAsync(pLights, lights, (size_t)lightCount * sizeof(int32_t), kind, stream);
   if (e == hipSuccess && primCount)
     e = hipMemcpyAsync(pObj, primObject, (size_t)primCount * sizeof(uint32_t), kind, stream);
+  if (e == hipSuccess && !wide.empty())
+    e = hipMemcpyAsync(pWide, wide.data(), wide.size() * sizeof(float), hipMemcpyHostToDevice,
+                       stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e != hipSuccess) {
     (void)hipFree(s->buf);
@@ -181,6 +257,8 @@ static int scene_create_impl(const float* prims, uint32_t primCount, const float
   d.mats = (const float4*)pMats;
   d.lights = (const int32_t*)pLights;
   d.primObject = (const uint32_t*)pObj;
+  d.wide = (const float4*)pWide;
+  d.wideCount = (int)(wide.size() / 32);
   d.primCount = (int)primCount;
   d.nodeCount = (int)nodeCount;
   d.lightCount = (int)lightCount;

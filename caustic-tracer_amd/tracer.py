@@ -16,6 +16,7 @@ from ._capi import DeviceBuffer, check, lib
 T_MIN = 1e-4  # intersect.ts:14
 WALK_STACKLESS = 0
 WALK_ORDERED = 1
+WALK_WIDE = 2
 
 
 class DeviceScene:
@@ -171,7 +172,7 @@ def cameraFrame(cam, aspect):
 class Tracer:
     """Owns the wavefront path state on the device (one per concurrent caller)."""
 
-    def __init__(self, walk=WALK_ORDERED):
+    def __init__(self, walk=WALK_WIDE):
         A.init()
         h = C.c_void_p()
         check(lib().ct_tracer_create(C.byref(h)), "ct_tracer_create")
@@ -181,7 +182,7 @@ class Tracer:
     def set_walk(self, walk):
         check(lib().ct_tracer_set_walk(self.handle, walk), "ct_tracer_set_walk")
 
-    def set_traversal(self, shadow_walk=WALK_ORDERED, stack=16):
+    def set_traversal(self, shadow_walk=WALK_WIDE, stack=24):
         check(lib().ct_tracer_set_traversal(self.handle, shadow_walk, stack), "set_traversal")
 
     def set_schedule(self, persistent=True, blocks=2048):

@@ -69,6 +69,9 @@ typedef struct {
 /* Which closest-hit walk a query mirrors. */
 #define CT_WALK_STACKLESS 0 /* intersect.ts:290-365 */
 #define CT_WALK_ORDERED 1   /* intersect.ts:202-282 */
+/* Ordered walk over 4-wide nodes derived from the reference BVH at upload
+ * (same closest hit; path tracing only). */
+#define CT_WALK_WIDE 2
 
 /* Counters of the last render on a tracer (not part of the reference API). */
 typedef struct {
@@ -143,7 +146,7 @@ int ct_tracer_set_counting(ct_tracer *tracer, int32_t enable);
 /* Closest-hit walk used by the extension rays (CT_WALK_*). */
 int ct_tracer_set_walk(ct_tracer *tracer, int32_t walk);
 /* Walk of the NEE any-hit rays (CT_WALK_*; the result does not depend on it)
- * and the LDS stack depth of the ordered walks (8, 10, 12, 16 or 24 entries;
+ * and the LDS stack depth of the ordered walks (8, 12, 16 or 24 entries;
  * deeper rays finish with the stackless walk, same result). */
 int ct_tracer_set_traversal(ct_tracer *tracer, int32_t shadowWalk, int32_t stackDepth);
 /* Launch schedule of the traversal kernels: persistent = 1 runs `blocks`

@@ -192,7 +192,7 @@ def test_render_cornell_matches_oracle_and_golden(ct, O):
     assert diff.max() <= 1 and (diff > 0).mean() < 1e-3
 
 
-@pytest.mark.parametrize("walk", [0, 1], ids=["stackless", "ordered"])
+@pytest.mark.parametrize("walk", [0, 1, 2], ids=["stackless", "ordered", "wide"])
 def test_render_c1_cornell_variant(ct, O, walk):
     sc = scenes.cornell_c1()
     ps = O.pack_scene(sc, bins=16)
@@ -200,6 +200,7 @@ def test_render_c1_cornell_variant(ct, O, walk):
     fr = O.camera_frame(sc["camera"], 1.0)
     o = O.render_image(ps, fr, w, h, 4, seed=1, max_bounces=8)
     tr = ct.Tracer(walk=walk)
+    tr.set_traversal(walk, 16)
     d = ct.DeviceBuffer.empty((w * h * 3,), np.float32)
     tr.render_image(ps, fr, {"width": w, "height": h, "spp": 4, "seed": 1, "maxBounces": 8}, d)
     g = d.numpy()
@@ -253,3 +254,26 @@ def test_render_is_deterministic(ct, O):
     a = ct.renderImage(ps, fr, opts)
     b = ct.renderImage(ps, fr, opts)
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("walk,stack", [(2, 16), (2, 8), (1, 8)], ids=["wide16", "wide8", "ordered8"])
+def test_render_soup_all_walks(ct, O, walk, stack):
+    """20k-triangle soup (materials of all kinds, sky, 64 lights): the wide and
+    binary ordered walks, with stacks small enough to force the stackless
+    fallback, against the oracle."""
+    sc = ct.synth.soup_scene(20000, n_lights=64, n_materials=50, seed=5)
+    ps = O.pack_scene(sc)
+    w, h = 96, 64
+    fr = O.camera_frame(sc["camera"], w / h)
+    o = O.render_image(ps, fr, w, h, 2, seed=3, max_bounces=6)
+    tr = ct.Tracer(walk=walk)
+    tr.set_traversal(walk, stack)
+    tr.set_counting(True)
+    d = ct.DeviceBuffer.empty((w * h * 3,), np.float32)
+    tr.render_image(ps, fr, {"width": w, "height": h, "spp": 2, "seed": 3, "maxBounces": 6}, d)
+    g = d.numpy()
+    st = tr.stats()
+    err = rel_err(g, o)
+    print(f"soup 20k walk={walk} stack={stack}: max rel err {err.max():.3e}, "
+          f"fallbacks {st['stackFallbacks']}/{st['shadowStackFallbacks']}")
+    assert err.max() <= REL_TOL
