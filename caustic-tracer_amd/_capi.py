@@ -35,7 +35,22 @@ class RenderStats(C.Structure):
                 ("shadowMs", C.c_double), ("nodeVisits", C.c_uint64), ("triTests", C.c_uint64),
                 ("sphereTests", C.c_uint64), ("shadowNodeVisits", C.c_uint64),
                 ("shadowTriTests", C.c_uint64), ("shadowSphereTests", C.c_uint64),
-                ("stackFallbacks", C.c_uint64), ("shadowStackFallbacks", C.c_uint64)]
+                ("stackFallbacks", C.c_uint64), ("shadowStackFallbacks", C.c_uint64),
+                ("rounds", C.c_uint32)]
+
+
+class Job(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("frame", C.c_double * 16), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("spp", C.c_uint32), ("seed", C.c_uint32),
+                ("maxBounces", C.c_int32), ("russianRoulette", C.c_int32), ("nee", C.c_int32),
+                ("d_out", C.c_void_p)]
+
+
+class PassJob(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("frame", C.c_double * 16), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("frameIndex", C.c_uint32), ("passes", C.c_uint32),
+                ("maxBounces", C.c_int32), ("russianRoulette", C.c_int32), ("nee", C.c_int32),
+                ("d_accum", C.c_void_p)]
 
 
 # Every function of the C ABI with its argument types (restype is int status
@@ -66,6 +81,10 @@ SIGNATURES = {
     "ct_accumulate_pass": [vp, vp, vp, u32, u32, u32, C.POINTER(TraceOpts), vp, vp],
     "ct_render_image": [vp, vp, vp, C.POINTER(RenderOpts), vp, vp],
     "ct_tracer_stats": [vp, C.POINTER(RenderStats)],
+    "ct_render_batch": [vp, vp, u32, vp],
+    "ct_accumulate_batch": [vp, vp, u32, vp],
+    "ct_tracer_set_capacity": [vp, C.c_uint64],
+    "ct_tracer_set_graph": [vp, i32],
     "ct_bvh_build": [vp, vp, vp, u32, i32, vp, C.POINTER(u32), vp, vp],
 }
 _RESTYPES = {"ct_last_error": C.c_char_p}

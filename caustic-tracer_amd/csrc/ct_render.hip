@@ -8,14 +8,20 @@
 //   shadow     any-hit of the NEE ray, adds its pending contribution
 //   finalize   per pixel, samples added in pass order     tracer.ts:386-392
 //
-// A path is (pass, pixel); all passes of a batch run together and the
-// finalize kernel adds their radiance in pass order, so the float32 sums are
-// the ones accumulatePass produces pass after pass.  The NEE term of bounce b
-// is added by the shadow kernel before bounce b+1 is shaded, keeping the
-// reference's order of additions into L.  Live paths are compacted with a
-// wave64 ballot + one atomic per wave.
+// A launch round traces one or more jobs at once.  A job is (scene, camera,
+// image size, trace options, frame indices of its passes, float32 sum); a
+// path is (job, pass, pixel) and carries its job id, so rays of different
+// scenes, sizes and depths share every launch (MULTI instantiation).  A
+// single-job round reads its job through uniform (scalar) loads.  Each job's
+// finalize adds its passes in pass order, so every float32 sum is the one
+// accumulatePass produces pass after pass, whatever else shared the launches.
+// The NEE term of bounce b is added by the shadow kernel before bounce b+1 is
+// shaded, keeping the reference's order of additions into L.  Live paths are
+// compacted with a wave64 ballot + one atomic per wave; traversal kernels are
+// persistent and refill lanes from the queue as rays finish.
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "ct_internal.h"
@@ -24,11 +30,29 @@ namespace ct {
 
 constexpr int RBLOCK = 256;
 constexpr int MAX_BOUNCE_SLOTS = 256;  // counters per kind (bounces + 1 <= this)
-constexpr int EXT_STACK = 16;
 
 struct Frame {
   double o[3], r[3], u[3], f[3];
   double tanHalfFov, aspect, lensRadius, focusDistance;
+};
+
+struct TraceOpts {
+  int maxBounces, rr, nee;
+};
+
+// One job of a launch round (device copy).
+struct JobDev {
+  DevScene S;
+  Frame F;
+  TraceOpts opts;
+  uint32_t width, height, npix;
+  uint32_t passes;     // passes of this job in the round
+  uint32_t frameBase;  // frame index of its first pass (uint32 arithmetic)
+  uint32_t pathBase;   // first path id
+  uint32_t pixelBase;  // first finalize index
+  int doScale;         // multiply the sum by `scale` after adding (renderImage)
+  double scale;
+  float* accum;        // width*height*3 float32 sum
 };
 
 struct PathSoA {
@@ -38,14 +62,15 @@ struct PathSoA {
   double *prevPdf, *hitT;
   uint32_t* rng;
   int32_t* hitPrim;
+  uint32_t* jobOf;
   uint8_t* specular;
   uint32_t *q0, *q1;
   // shadow queue (one slot per live path at most)
   uint32_t* sPath;
   double *sdx, *sdy, *sdz, *sTmax, *sCr, *sCg, *sCb;
   // counters: live[b] at [b], shadow[b] at [MAX_BOUNCE_SLOTS + b], ray-fetch
-  // cursors of the persistent extend/shadow launches at [2*MAX + b] and
-  // [3*MAX + b]; 64-bit work counters at ctr64
+  // cursors of the extend/shadow launches at [2*MAX + b] and [3*MAX + b];
+  // 64-bit work counters at ctr64
   uint32_t* ctr;
   unsigned long long* ctr64;
 };
@@ -92,19 +117,36 @@ __global__ void k_clear(uint32_t* ctr, int n, unsigned long long* ctr64, int n64
   if (i < n64) ctr64[i] = 0;
 }
 
+// Job of a path / finalize index: the last job whose base is <= i.
+template <bool MULTI, bool PIXEL>
+__device__ __forceinline__ int job_of(const JobDev* __restrict__ jobs, int njobs, size_t i) {
+  if (!MULTI) return 0;
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    size_t base = PIXEL ? jobs[mid].pixelBase : jobs[mid].pathBase;
+    if (base <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // camera.ts:62-72 generateRay with the pixel/lens draws of tracer.ts:373-384.
+template <bool MULTI>
 __global__ void __launch_bounds__(RBLOCK)
-k_generate(PathSoA P, Frame F, uint32_t width, uint32_t height, uint32_t passes,
-           uint32_t frameBase) {
-  const uint32_t npix = width * height;
-  const size_t total = (size_t)npix * passes;
+k_generate(PathSoA P, const JobDev* __restrict__ jobs, int njobs, size_t total) {
   size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t pid = (size_t)blockIdx.x * blockDim.x + threadIdx.x; pid < total; pid += stride) {
-    uint32_t pass = (uint32_t)(pid / npix);
-    uint32_t pixel = (uint32_t)(pid - (size_t)pass * npix);
+    int j = job_of<MULTI, false>(jobs, njobs, pid);
+    const JobDev& J = jobs[j];
+    const Frame& F = J.F;
+    const uint32_t width = J.width, height = J.height, npix = J.npix;
+    size_t local = pid - J.pathBase;
+    uint32_t pass = (uint32_t)(local / npix);
+    uint32_t pixel = (uint32_t)(local - (size_t)pass * npix);
     uint32_t y = pixel / width, x = pixel - y * width;
     Rng rng;
-    rng.s = seed_for(pixel, frameBase + pass);
+    rng.s = seed_for(pixel, J.frameBase + pass);
     double sx = (x + rng.next()) / width;
     double sy = 1 - (y + rng.next()) / height;
     double lx = 0, ly = 0;
@@ -136,41 +178,10 @@ k_generate(PathSoA P, Frame F, uint32_t width, uint32_t height, uint32_t passes,
     P.prevPdf[pid] = 0;
     P.specular[pid] = 1;
     P.rng[pid] = rng.s;
+    if (MULTI) P.jobOf[pid] = (uint32_t)j;
     P.q0[pid] = (uint32_t)pid;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) P.ctr[0] = (uint32_t)total;
-}
-
-template <int WALK, int STACK, bool COUNT>
-__global__ void __launch_bounds__(RBLOCK)
-k_extend(PathSoA P, DevScene S, const uint32_t* __restrict__ q, const uint32_t* __restrict__ count) {
-  __shared__ int sNode[WALK == CT_WALK_ORDERED ? STACK * RBLOCK : 1];
-  __shared__ float sEntry[WALK == CT_WALK_ORDERED ? STACK * RBLOCK : 1];
-  const uint32_t n = *count;
-  TraversalCounts cnt = {0, 0, 0, 0};
-  uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < n; slot += stride) {
-    uint32_t pid = q[slot];
-    Hit h;
-    h.t = __builtin_inf();
-    h.u = h.v = 0;
-    h.prim = -1;
-    double ox = P.ox[pid], oy = P.oy[pid], oz = P.oz[pid];
-    double dx = P.dx[pid], dy = P.dy[pid], dz = P.dz[pid];
-    if (WALK == CT_WALK_ORDERED)
-      closest_wave<STACK, COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, h, sNode + threadIdx.x,
-                                 sEntry + threadIdx.x, RBLOCK, cnt);
-    else
-      closest_stackless<COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
-    P.hitPrim[pid] = h.prim;
-    P.hitT[pid] = h.t;
-  }
-  if (COUNT) {
-    wave_add64(P.ctr64 + 0, cnt.nodes);
-    wave_add64(P.ctr64 + 1, cnt.tris);
-    wave_add64(P.ctr64 + 2, cnt.spheres);
-    wave_add64(P.ctr64 + 6, cnt.fallbacks);
-  }
 }
 
 // lightPdfScalar (tracer.ts:61-90)
@@ -339,18 +350,13 @@ __device__ __forceinline__ double dielectric_reflectance(double cosI, double eta
   return schlick(cosI, etaI, etaT);
 }
 
-struct TraceOpts {
-  int maxBounces, rr, nee;
-};
-
 // One bounce of radianceInto for every live path.
+template <bool MULTI>
 __global__ void __launch_bounds__(RBLOCK)
-k_shade(PathSoA P, DevScene S, const uint32_t* __restrict__ qin,
+k_shade(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__ qin,
         const uint32_t* __restrict__ countIn, uint32_t* __restrict__ qout,
-        uint32_t* __restrict__ countOut, uint32_t* __restrict__ shadowCount, int bounce,
-        TraceOpts opts) {
+        uint32_t* __restrict__ countOut, uint32_t* __restrict__ shadowCount, int bounce) {
   const uint32_t n = *countIn;
-  const bool useNee = opts.nee && S.lightCount > 0;
   uint32_t stride = gridDim.x * blockDim.x;
   // Every lane of a wave runs the same number of iterations so the wave-wide
   // ballots below see the whole wave.
@@ -362,6 +368,10 @@ k_shade(PathSoA P, DevScene S, const uint32_t* __restrict__ qin,
     bool alive = false, shadow = false;
     double wdir[3] = {0, 0, 0}, sTmax = 0, sc[3] = {0, 0, 0};
     if (valid) {
+      const JobDev& J = jobs[MULTI ? P.jobOf[pid] : 0];
+      const DevScene& S = J.S;
+      const TraceOpts opts = J.opts;
+      const bool useNee = opts.nee && S.lightCount > 0;
       double ox = P.ox[pid], oy = P.oy[pid], oz = P.oz[pid];
       double dx = P.dx[pid], dy = P.dy[pid], dz = P.dz[pid];
       double br = P.br[pid], bg = P.bg[pid], bb = P.bb[pid];
@@ -577,51 +587,36 @@ k_shade(PathSoA P, DevScene S, const uint32_t* __restrict__ qin,
   }
 }
 
-// Any-hit of the NEE ray; adds the pending term when unoccluded (tracer.ts:268-282).
-template <int WALK, int STACK, bool COUNT>
-__global__ void __launch_bounds__(RBLOCK)
-k_shadow(PathSoA P, DevScene S, const uint32_t* __restrict__ count) {
-  __shared__ int sNode[WALK == CT_WALK_ORDERED ? STACK * RBLOCK : 1];
-  const uint32_t n = *count;
-  TraversalCounts cnt = {0, 0, 0, 0};
-  uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < n; slot += stride) {
-    uint32_t pid = P.sPath[slot];
-    double ox = P.ox[pid], oy = P.oy[pid], oz = P.oz[pid];
-    bool blocked;
-    if (WALK == CT_WALK_ORDERED)
-      blocked = occluded_wave<STACK, COUNT>(S, ox, oy, oz, P.sdx[slot], P.sdy[slot],
-                                            P.sdz[slot], T_MIN, P.sTmax[slot],
-                                            sNode + threadIdx.x, RBLOCK, cnt);
-    else
-      blocked = occluded<COUNT>(S, ox, oy, oz, P.sdx[slot], P.sdy[slot], P.sdz[slot], T_MIN,
-                                P.sTmax[slot], cnt);
-    if (!blocked) {
-      P.Lr[pid] += P.sCr[slot];
-      P.Lg[pid] += P.sCg[slot];
-      P.Lb[pid] += P.sCb[slot];
-    }
-  }
-  if (COUNT) {
-    wave_add64(P.ctr64 + 3, cnt.nodes);
-    wave_add64(P.ctr64 + 4, cnt.tris);
-    wave_add64(P.ctr64 + 5, cnt.spheres);
-    wave_add64(P.ctr64 + 7, cnt.fallbacks);
-  }
+// Lane refill shared by the persistent traversal kernels: lanes without a ray
+// take the next queue slots (one atomic per wave).  Returns the slot for this
+// lane or UINT32_MAX.
+__device__ __forceinline__ uint32_t refill(bool need, uint32_t n, uint32_t* fetch,
+                                           bool& exhausted) {
+  unsigned long long m = __ballot(need);
+  if (!m) return 0xffffffffu;
+  const int lane = __lane_id();
+  int leader = __ffsll((long long)__ballot(1)) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(fetch, (uint32_t)__popcll(m));
+  base = __shfl(base, leader, 64);
+  if (base + (uint32_t)__popcll(m) >= n) exhausted = true;
+  if (!need) return 0xffffffffu;
+  uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+  return slot < n ? slot : 0xffffffffu;
 }
 
-
-// Persistent closest-hit: each lane pulls the next ray from the bounce's queue
-// as soon as its current ray is finished (wave-aggregated fetch), so lanes do
-// not idle until the slowest ray of their wave ends.  The walk is the
-// wave-organised ordered walk of closest_wave (leaf parking, bf16 entries),
-// over the reference's binary nodes or the derived 4-wide nodes (WIDE).
-template <bool WIDE, int STACK, bool COUNT>
+// Persistent closest hit.  WALK: CT_WALK_STACKLESS (the reference's skip-link
+// loop, exact mirror), CT_WALK_ORDERED (binary nodes) or CT_WALK_WIDE
+// (derived 4-wide nodes); the ordered forms park leaves until the whole wave
+// holds one (closest_wave) and keep bf16 lower bounds of deferred entries.
+template <int WALK, int STACK, bool COUNT, bool MULTI>
 __global__ void __launch_bounds__(RBLOCK)
-k_extend_persist(PathSoA P, DevScene S, const uint32_t* __restrict__ q,
-                 const uint32_t* __restrict__ count, uint32_t* __restrict__ fetch) {
-  __shared__ int sNodeAll[STACK * RBLOCK];
-  __shared__ uint16_t sEntryAll[STACK * RBLOCK];
+k_extend(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__ q,
+         const uint32_t* __restrict__ count, uint32_t* __restrict__ fetch) {
+  constexpr bool WIDE = WALK == CT_WALK_WIDE;
+  constexpr int SD = WALK == CT_WALK_STACKLESS ? 1 : STACK;
+  __shared__ int sNodeAll[SD * RBLOCK];
+  __shared__ uint16_t sEntryAll[SD * RBLOCK];
   int* sNode = sNodeAll + threadIdx.x;
   uint16_t* sEntry = sEntryAll + threadIdx.x;
   const int lanes = RBLOCK;
@@ -629,72 +624,71 @@ k_extend_persist(PathSoA P, DevScene S, const uint32_t* __restrict__ q,
   TraversalCounts cnt = {0, 0, 0, 0};
   bool have = false, exhausted = false;
   uint32_t pid = 0;
+  DevScene S = jobs[0].S;
   double ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0;
   double oix = 0, oiy = 0, oiz = 0;
   Hit h;
   h.t = 0; h.u = 0; h.v = 0; h.prim = -1;
   int cur = -1, sp = 0, pend = -1;
-  const int lane = __lane_id();
   for (;;) {
     if (!exhausted) {
-      bool need = !have;
-      unsigned long long m = __ballot(need);
-      if (m) {
-        int leader = __ffsll((long long)__ballot(1)) - 1;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(fetch, (uint32_t)__popcll(m));
-        base = __shfl(base, leader, 64);
-        if (base + (uint32_t)__popcll(m) >= n) exhausted = true;
-        if (need) {
-          uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-          if (slot < n) {
-            pid = q[slot];
-            ox = P.ox[pid]; oy = P.oy[pid]; oz = P.oz[pid];
-            dx = P.dx[pid]; dy = P.dy[pid]; dz = P.dz[pid];
-            ix = safe_inv(dx); iy = safe_inv(dy); iz = safe_inv(dz);
-            oix = ox * ix; oiy = oy * iy; oiz = oz * iz;
-            h.t = __builtin_inf(); h.u = 0; h.v = 0; h.prim = -1;
-            sp = 0; pend = -1;
-            cur = walk_root<WIDE>(S, ix, iy, iz, oix, oiy, oiz, h.t, cnt, COUNT);
-            have = true;
-          }
+      uint32_t slot = refill(!have, n, fetch, exhausted);
+      if (slot != 0xffffffffu) {
+        pid = q[slot];
+        if (MULTI) S = jobs[P.jobOf[pid]].S;
+        ox = P.ox[pid]; oy = P.oy[pid]; oz = P.oz[pid];
+        dx = P.dx[pid]; dy = P.dy[pid]; dz = P.dz[pid];
+        h.t = __builtin_inf(); h.u = 0; h.v = 0; h.prim = -1;
+        have = true;
+        if (WALK == CT_WALK_STACKLESS) {
+          closest_stackless<COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
+          cur = -1;
+          pend = -1;
+        } else {
+          ix = safe_inv(dx); iy = safe_inv(dy); iz = safe_inv(dz);
+          oix = ox * ix; oiy = oy * iy; oiz = oz * iz;
+          sp = 0; pend = -1;
+          cur = walk_root<WIDE>(S, ix, iy, iz, oix, oiy, oiz, h.t, cnt, COUNT);
         }
       }
     }
     if (!__any(have)) break;
     if (have) {
       bool overflow = false;
-      while (cur != -1) {
-        if (cur <= -2) {
-          if (pend >= 0) break;
-          pend = -2 - cur;
-          cur = -1;
-        } else if (!interior_step<WIDE, true, STACK, COUNT>(S, ix, iy, iz, oix, oiy, oiz, h.t,
-                                                            cur, sp, sNode, sEntry, lanes, cnt)) {
-          overflow = true;
-          cur = -1;
-          break;
-        }
-        if (cur == -1) {
-          while (sp > 0) {
-            sp--;
-            if (entry_from_bf16(sEntry[sp * lanes]) < h.t) {
-              cur = sNode[sp * lanes];
-              break;
+      if (WALK != CT_WALK_STACKLESS) {
+        while (cur != -1) {
+          if (cur <= -2) {
+            if (pend >= 0) break;
+            pend = -2 - cur;
+            cur = -1;
+          } else if (!interior_step<WIDE, true, SD, COUNT>(S, ix, iy, iz, oix, oiy, oiz, h.t,
+                                                           cur, sp, sNode, sEntry, lanes,
+                                                           cnt)) {
+            overflow = true;
+            cur = -1;
+            break;
+          }
+          if (cur == -1) {
+            while (sp > 0) {
+              sp--;
+              if (entry_from_bf16(sEntry[sp * lanes]) < h.t) {
+                cur = sNode[sp * lanes];
+                break;
+              }
             }
           }
+          if (__all(pend >= 0 || cur == -1)) break;
         }
-        if (__all(pend >= 0 || cur == -1)) break;
-      }
-      if (pend >= 0) {
-        test_leaf<COUNT>(S, (float)pend, ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
-        pend = -1;
-      }
-      if (overflow) {
-        if (COUNT) cnt.fallbacks++;
-        closest_stackless<COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
-        cur = -1;
-        sp = 0;
+        if (pend >= 0) {
+          test_leaf<COUNT>(S, (float)pend, ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
+          pend = -1;
+        }
+        if (overflow) {
+          if (COUNT) cnt.fallbacks++;
+          closest_stackless<COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
+          cur = -1;
+          sp = 0;
+        }
       }
       if (cur == -1 && pend < 0) {
         P.hitPrim[pid] = h.prim;
@@ -711,93 +705,105 @@ k_extend_persist(PathSoA P, DevScene S, const uint32_t* __restrict__ q,
   }
 }
 
-// Persistent any-hit of the NEE rays (same lane refill and walk organisation).
-template <bool WIDE, int STACK, bool COUNT>
+__device__ __forceinline__ bool leaf_blocks(const DevScene& S, int leafWord, double ox, double oy,
+                                            double oz, double dx, double dy, double dz,
+                                            double tMax, TraversalCounts& cnt, bool count) {
+  int c = leafWord & 7, start = leafWord >> 3;
+  for (int p = start; p < start + c; p++) {
+    PrimGeom g = load_prim(S, p);
+    if (g.a.w == 0.0f) {
+      if (count) cnt.spheres++;
+      if (ray_sphere(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, T_MIN, tMax) > 0)
+        return true;
+    } else {
+      if (count) cnt.tris++;
+      double t, u, v;
+      if (ray_triangle(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, g.b.y, g.b.z, g.c.x,
+                       g.c.y, g.c.z, T_MIN, tMax, t, u, v))
+        return true;
+    }
+  }
+  return false;
+}
+
+// Persistent any hit of the NEE rays; adds the pending term when unoccluded
+// (tracer.ts:268-282).  Whether (tMin, tMax) is blocked does not depend on
+// the visiting order, so every walk returns the reference's answer.
+template <int WALK, int STACK, bool COUNT, bool MULTI>
 __global__ void __launch_bounds__(RBLOCK)
-k_shadow_persist(PathSoA P, DevScene S, const uint32_t* __restrict__ count,
-                 uint32_t* __restrict__ fetch) {
-  __shared__ int sNodeAll[STACK * RBLOCK];
+k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__ count,
+         uint32_t* __restrict__ fetch) {
+  constexpr bool WIDE = WALK == CT_WALK_WIDE;
+  constexpr int SD = WALK == CT_WALK_STACKLESS ? 1 : STACK;
+  __shared__ int sNodeAll[SD * RBLOCK];
   int* sNode = sNodeAll + threadIdx.x;
   const int lanes = RBLOCK;
   const uint32_t n = *count;
   TraversalCounts cnt = {0, 0, 0, 0};
-  bool have = false, exhausted = false;
+  bool have = false, exhausted = false, blocked = false;
   uint32_t slotCur = 0, pid = 0;
+  DevScene S = jobs[0].S;
   double ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0, tMax = 0;
   double oix = 0, oiy = 0, oiz = 0;
   int cur = -1, sp = 0, pend = -1;
-  const int lane = __lane_id();
   for (;;) {
     if (!exhausted) {
-      bool need = !have;
-      unsigned long long m = __ballot(need);
-      if (m) {
-        int leader = __ffsll((long long)__ballot(1)) - 1;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(fetch, (uint32_t)__popcll(m));
-        base = __shfl(base, leader, 64);
-        if (base + (uint32_t)__popcll(m) >= n) exhausted = true;
-        if (need) {
-          uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-          if (slot < n) {
-            slotCur = slot;
-            pid = P.sPath[slot];
-            ox = P.ox[pid]; oy = P.oy[pid]; oz = P.oz[pid];
-            dx = P.sdx[slot]; dy = P.sdy[slot]; dz = P.sdz[slot];
-            tMax = P.sTmax[slot];
-            ix = safe_inv(dx); iy = safe_inv(dy); iz = safe_inv(dz);
-            oix = ox * ix; oiy = oy * iy; oiz = oz * iz;
-            sp = 0; pend = -1;
-            cur = walk_root<WIDE>(S, ix, iy, iz, oix, oiy, oiz, tMax, cnt, COUNT);
-            have = true;
-          }
+      uint32_t slot = refill(!have, n, fetch, exhausted);
+      if (slot != 0xffffffffu) {
+        slotCur = slot;
+        pid = P.sPath[slot];
+        if (MULTI) S = jobs[P.jobOf[pid]].S;
+        ox = P.ox[pid]; oy = P.oy[pid]; oz = P.oz[pid];
+        dx = P.sdx[slot]; dy = P.sdy[slot]; dz = P.sdz[slot];
+        tMax = P.sTmax[slot];
+        have = true;
+        blocked = false;
+        if (WALK == CT_WALK_STACKLESS) {
+          blocked = occluded<COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, tMax, cnt);
+          cur = -1;
+          pend = -1;
+        } else {
+          ix = safe_inv(dx); iy = safe_inv(dy); iz = safe_inv(dz);
+          oix = ox * ix; oiy = oy * iy; oiz = oz * iz;
+          sp = 0; pend = -1;
+          cur = walk_root<WIDE>(S, ix, iy, iz, oix, oiy, oiz, tMax, cnt, COUNT);
         }
       }
     }
     if (!__any(have)) break;
     if (have) {
-      bool overflow = false, blocked = false;
-      while (cur != -1) {
-        if (cur <= -2) {
-          if (pend >= 0) break;
-          pend = -2 - cur;
-          cur = -1;
-        } else if (!interior_step<WIDE, false, STACK, COUNT>(S, ix, iy, iz, oix, oiy, oiz, tMax,
-                                                             cur, sp, sNode, nullptr, lanes,
-                                                             cnt)) {
-          overflow = true;
-          cur = -1;
-          break;
-        }
-        if (cur == -1 && sp > 0) {
-          sp--;
-          cur = sNode[sp * lanes];
-        }
-        if (__all(pend >= 0 || cur == -1)) break;
-      }
-      if (pend >= 0) {
-        int count = pend & 7, start = pend >> 3;
-        for (int p = start; p < start + count && !blocked; p++) {
-          PrimGeom g = load_prim(S, p);
-          if (g.a.w == 0.0f) {
-            if (COUNT) cnt.spheres++;
-            blocked = ray_sphere(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, T_MIN, tMax) > 0;
-          } else {
-            if (COUNT) cnt.tris++;
-            double t, u, v;
-            blocked = ray_triangle(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, g.b.y,
-                                   g.b.z, g.c.x, g.c.y, g.c.z, T_MIN, tMax, t, u, v);
+      if (WALK != CT_WALK_STACKLESS) {
+        bool overflow = false;
+        while (cur != -1) {
+          if (cur <= -2) {
+            if (pend >= 0) break;
+            pend = -2 - cur;
+            cur = -1;
+          } else if (!interior_step<WIDE, false, SD, COUNT>(S, ix, iy, iz, oix, oiy, oiz, tMax,
+                                                            cur, sp, sNode, nullptr, lanes,
+                                                            cnt)) {
+            overflow = true;
+            cur = -1;
+            break;
           }
+          if (cur == -1 && sp > 0) {
+            sp--;
+            cur = sNode[sp * lanes];
+          }
+          if (__all(pend >= 0 || cur == -1)) break;
         }
-        pend = -1;
-      }
-      if (overflow && !blocked) {
-        if (COUNT) cnt.fallbacks++;
-        blocked = occluded<COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, tMax, cnt);
-      }
-      if (blocked || overflow) {
-        cur = -1;
-        sp = 0;
+        if (pend >= 0) {
+          blocked = leaf_blocks(S, pend, ox, oy, oz, dx, dy, dz, tMax, cnt, COUNT);
+          pend = -1;
+        }
+        if (overflow && !blocked) {
+          if (COUNT) cnt.fallbacks++;
+          blocked = occluded<COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, tMax, cnt);
+        }
+        if (blocked || overflow) {
+          cur = -1;
+          sp = 0;
+        }
       }
       if (cur == -1 && pend < 0) {
         if (!blocked) {
@@ -817,106 +823,19 @@ k_shadow_persist(PathSoA P, DevScene S, const uint32_t* __restrict__ count,
   }
 }
 
-template <int WALK, int STACK>
-static void launch_extend(bool counting, unsigned grid, hipStream_t st, const PathSoA& P,
-                          const DevScene& S, const uint32_t* q, const uint32_t* c,
-                          uint32_t* fetch = nullptr) {
-  if (WALK != CT_WALK_STACKLESS && fetch) {
-    constexpr bool W = WALK == CT_WALK_WIDE;
-    if (counting)
-      hipLaunchKernelGGL((k_extend_persist<W, STACK, true>), dim3(grid), dim3(RBLOCK), 0, st, P, S, q, c, fetch);
-    else
-      hipLaunchKernelGGL((k_extend_persist<W, STACK, false>), dim3(grid), dim3(RBLOCK), 0, st, P, S, q, c, fetch);
-    return;
-  }
-  if constexpr (WALK != CT_WALK_WIDE) {
-    if (counting)
-      hipLaunchKernelGGL((k_extend<WALK, STACK, true>), dim3(grid), dim3(RBLOCK), 0, st, P, S, q, c);
-    else
-      hipLaunchKernelGGL((k_extend<WALK, STACK, false>), dim3(grid), dim3(RBLOCK), 0, st, P, S, q, c);
-  }
-}
-
-template <int WALK, int STACK>
-static void launch_shadow(bool counting, unsigned grid, hipStream_t st, const PathSoA& P,
-                          const DevScene& S, const uint32_t* c, uint32_t* fetch = nullptr) {
-  if (WALK != CT_WALK_STACKLESS && fetch) {
-    constexpr bool W = WALK == CT_WALK_WIDE;
-    if (counting)
-      hipLaunchKernelGGL((k_shadow_persist<W, STACK, true>), dim3(grid), dim3(RBLOCK), 0, st, P, S, c, fetch);
-    else
-      hipLaunchKernelGGL((k_shadow_persist<W, STACK, false>), dim3(grid), dim3(RBLOCK), 0, st, P, S, c, fetch);
-    return;
-  }
-  if constexpr (WALK != CT_WALK_WIDE) {
-    if (counting)
-      hipLaunchKernelGGL((k_shadow<WALK, STACK, true>), dim3(grid), dim3(RBLOCK), 0, st, P, S, c);
-    else
-      hipLaunchKernelGGL((k_shadow<WALK, STACK, false>), dim3(grid), dim3(RBLOCK), 0, st, P, S, c);
-  }
-}
-
-// Stack depths the ordered walks are instantiated for (LDS per 256-lane block:
-// 6 B/entry for closest hit, 4 B/entry for any hit).
-static bool stack_supported(int d) { return d == 8 || d == 12 || d == 16 || d == 24; }
-
-template <int WALK>
-static void dispatch_extend_w(int stack, bool counting, unsigned grid, hipStream_t st,
-                              const PathSoA& P, const DevScene& S, const uint32_t* q,
-                              const uint32_t* c, uint32_t* f) {
-  switch (stack) {
-    case 8: launch_extend<WALK, 8>(counting, grid, st, P, S, q, c, f); break;
-    case 12: launch_extend<WALK, 12>(counting, grid, st, P, S, q, c, f); break;
-    case 16: launch_extend<WALK, 16>(counting, grid, st, P, S, q, c, f); break;
-    default: launch_extend<WALK, 24>(counting, grid, st, P, S, q, c, f); break;
-  }
-}
-
-template <int WALK>
-static void dispatch_shadow_w(int stack, bool counting, unsigned grid, hipStream_t st,
-                              const PathSoA& P, const DevScene& S, const uint32_t* c,
-                              uint32_t* f) {
-  switch (stack) {
-    case 8: launch_shadow<WALK, 8>(counting, grid, st, P, S, c, f); break;
-    case 12: launch_shadow<WALK, 12>(counting, grid, st, P, S, c, f); break;
-    case 16: launch_shadow<WALK, 16>(counting, grid, st, P, S, c, f); break;
-    default: launch_shadow<WALK, 24>(counting, grid, st, P, S, c, f); break;
-  }
-}
-
-// The wide walk always runs persistent (f must be the launch's fetch cursor).
-static void dispatch_extend(int walk, int stack, bool counting, unsigned grid, hipStream_t st,
-                            const PathSoA& P, const DevScene& S, const uint32_t* q,
-                            const uint32_t* c, uint32_t* f) {
-  if (walk == CT_WALK_STACKLESS)
-    launch_extend<CT_WALK_STACKLESS, 1>(counting, grid, st, P, S, q, c);
-  else if (walk == CT_WALK_WIDE)
-    dispatch_extend_w<CT_WALK_WIDE>(stack, counting, grid, st, P, S, q, c, f);
-  else
-    dispatch_extend_w<CT_WALK_ORDERED>(stack, counting, grid, st, P, S, q, c, f);
-}
-
-static void dispatch_shadow(int walk, int stack, bool counting, unsigned grid, hipStream_t st,
-                            const PathSoA& P, const DevScene& S, const uint32_t* c, uint32_t* f) {
-  if (walk == CT_WALK_STACKLESS)
-    launch_shadow<CT_WALK_STACKLESS, 1>(counting, grid, st, P, S, c);
-  else if (walk == CT_WALK_WIDE)
-    dispatch_shadow_w<CT_WALK_WIDE>(stack, counting, grid, st, P, S, c, f);
-  else
-    dispatch_shadow_w<CT_WALK_ORDERED>(stack, counting, grid, st, P, S, c, f);
-}
-
 // Adds each pass's sample in pass order, dropping non-finite samples
 // (tracer.ts:386-392); optional final scale (tracer.ts:352-353).
+template <bool MULTI>
 __global__ void __launch_bounds__(RBLOCK)
-k_finalize(PathSoA P, uint32_t npix, uint32_t passes, float* __restrict__ accum, int doScale,
-           double scale) {
-  uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < npix; pix += stride) {
-    float a0 = accum[3 * (size_t)pix], a1 = accum[3 * (size_t)pix + 1],
-          a2 = accum[3 * (size_t)pix + 2];
-    for (uint32_t s = 0; s < passes; s++) {
-      size_t pid = (size_t)s * npix + pix;
+k_finalize(PathSoA P, const JobDev* __restrict__ jobs, int njobs, size_t totalPix) {
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t gi = (size_t)blockIdx.x * blockDim.x + threadIdx.x; gi < totalPix; gi += stride) {
+    const JobDev& J = jobs[job_of<MULTI, true>(jobs, njobs, gi)];
+    uint32_t pix = (uint32_t)(gi - J.pixelBase);
+    float* acc = J.accum + 3 * (size_t)pix;
+    float a0 = acc[0], a1 = acc[1], a2 = acc[2];
+    for (uint32_t s = 0; s < J.passes; s++) {
+      size_t pid = J.pathBase + (size_t)s * J.npix + pix;
       double cr = P.Lr[pid], cg = P.Lg[pid], cb = P.Lb[pid];
       if (isfinite(cr) && isfinite(cg) && isfinite(cb)) {
         a0 = (float)((double)a0 + cr);
@@ -924,14 +843,14 @@ k_finalize(PathSoA P, uint32_t npix, uint32_t passes, float* __restrict__ accum,
         a2 = (float)((double)a2 + cb);
       }
     }
-    if (doScale) {
-      a0 = (float)((double)a0 * scale);
-      a1 = (float)((double)a1 * scale);
-      a2 = (float)((double)a2 * scale);
+    if (J.doScale) {
+      a0 = (float)((double)a0 * J.scale);
+      a1 = (float)((double)a1 * J.scale);
+      a2 = (float)((double)a2 * J.scale);
     }
-    accum[3 * (size_t)pix] = a0;
-    accum[3 * (size_t)pix + 1] = a1;
-    accum[3 * (size_t)pix + 2] = a2;
+    acc[0] = a0;
+    acc[1] = a1;
+    acc[2] = a2;
   }
 }
 
@@ -948,19 +867,27 @@ struct ct_tracer {
   void* buf = nullptr;
   size_t capPaths = 0;
   PathSoA P{};
+  JobDev* jobsDev = nullptr;  // device job table
+  JobDev* jobsHost = nullptr; // pinned staging for the table
+  size_t capJobs = 0;
   bool profiling = false;
   bool counting = false;
   int walk = CT_WALK_WIDE;           // closest-hit walk of extension rays
   int shadowWalk = CT_WALK_WIDE;     // any-hit walk of NEE rays
   int stack = 24;                    // LDS stack entries of the ordered walks
-  bool persistent = true;            // lanes refill with new rays as they finish
-  unsigned persistBlocks = 256 * 8;  // workgroups of a persistent launch
-  // stats of the last run
+  unsigned persistBlocks = 256 * 8;  // workgroups of the traversal launches (0: one lane per ray)
+  size_t maxPaths = size_t(1) << 25; // paths in flight per round (~190 B of state each)
+  bool graph = false;                // capture rounds into a hipGraph and replay them
+  hipStream_t ownStream = nullptr;   // capture stream
+  hipGraphExec_t graphExec = nullptr;
+  std::vector<char> graphKey;
+  // stats of the last call
   int lastBounces = 0;
   uint64_t lastPaths = 0;
   uint64_t lastExtension = 0;
   uint64_t lastShadow = 0;
-  std::vector<hipEvent_t> events;  // pairs per launch when profiling
+  uint32_t lastRounds = 0;
+  std::vector<hipEvent_t> events;  // (begin, end) pairs per launch when profiling
   std::vector<int> eventKind;      // 0 extend, 1 shade, 2 shadow
   hipStream_t lastStream = nullptr;
 };
@@ -976,7 +903,7 @@ static int tracer_reserve(ct_tracer* T, size_t paths) {
   }
   size_t n = paths;
   size_t d = al(n * 8), u32 = al(n * 4), u8 = al(n);
-  size_t total = 21 * d + 5 * u32 + u8 + al(4 * MAX_BOUNCE_SLOTS * 4) + al(16 * 8);
+  size_t total = 21 * d + 6 * u32 + u8 + al(4 * MAX_BOUNCE_SLOTS * 4) + al(16 * 8);
   CT_HIP(hipMalloc(&T->buf, total));
   char* p = (char*)T->buf;
   auto takeD = [&]() { double* r = (double*)p; p += d; return r; };
@@ -991,11 +918,36 @@ static int tracer_reserve(ct_tracer* T, size_t paths) {
   P.sCr = takeD(); P.sCg = takeD(); P.sCb = takeD();
   P.rng = takeU();
   P.hitPrim = (int32_t*)takeU();
+  P.jobOf = takeU();
   P.q0 = takeU(); P.q1 = takeU(); P.sPath = takeU();
   P.specular = (uint8_t*)p; p += u8;
   P.ctr = (uint32_t*)p; p += al(4 * MAX_BOUNCE_SLOTS * 4);
   P.ctr64 = (unsigned long long*)p;
   T->capPaths = paths;
+  // path buffers moved: a captured graph refers to the old ones
+  if (T->graphExec) {
+    (void)hipGraphExecDestroy(T->graphExec);
+    T->graphExec = nullptr;
+    T->graphKey.clear();
+  }
+  return CT_OK;
+}
+
+static int jobs_reserve(ct_tracer* T, size_t n) {
+  if (n <= T->capJobs) return CT_OK;
+  if (T->jobsDev) CT_HIP(hipFree(T->jobsDev));
+  if (T->jobsHost) CT_HIP(hipHostFree(T->jobsHost));
+  T->jobsDev = nullptr;
+  T->jobsHost = nullptr;
+  size_t cap = std::max<size_t>(n, 64);
+  CT_HIP(hipMalloc((void**)&T->jobsDev, cap * sizeof(JobDev)));
+  CT_HIP(hipHostMalloc((void**)&T->jobsHost, cap * sizeof(JobDev), hipHostMallocDefault));
+  T->capJobs = cap;
+  if (T->graphExec) {
+    (void)hipGraphExecDestroy(T->graphExec);
+    T->graphExec = nullptr;
+    T->graphKey.clear();
+  }
   return CT_OK;
 }
 
@@ -1015,84 +967,206 @@ static Frame make_frame(const double* f) {
 }
 
 static void clear_events(ct_tracer* T) {
-  for (hipEvent_t e : T->events) hipEventDestroy(e);
+  for (hipEvent_t e : T->events) (void)hipEventDestroy(e);
   T->events.clear();
   T->eventKind.clear();
 }
 
 static void mark(ct_tracer* T, hipStream_t st, int kind, bool begin) {
-  if (!T->profiling) return;
+  if (!T->profiling || T->graph) return;
   hipEvent_t e;
-  hipEventCreate(&e);
-  hipEventRecord(e, st);
+  if (hipEventCreate(&e) != hipSuccess) return;
+  (void)hipEventRecord(e, st);
   T->events.push_back(e);
   if (begin) T->eventKind.push_back(kind);
 }
 
-// Trace `passes` passes of width x height (frame indices frameBase..+passes-1)
-// and add them into accum (optionally scaled at the end).
-static int run_passes(ct_tracer* T, const ct_scene* s, const Frame& F, uint32_t width,
-                      uint32_t height, uint32_t passes, uint32_t frameBase, TraceOpts opts,
-                      float* accum, bool doScale, double scale, hipStream_t st, bool firstBatch) {
-  size_t npix = (size_t)width * height;
-  size_t paths = npix * passes;
-  int rc = tracer_reserve(T, paths);
-  if (rc) return rc;
+// Instantiated stack depths of the ordered walks (LDS per 256-lane block:
+// 6 B/entry for closest hit, 4 B/entry for any hit).
+static bool stack_supported(int d) { return d == 8 || d == 16 || d == 24; }
+
+template <int WALK, int STACK, bool MULTI>
+static void launch_trace(bool counting, unsigned grid, hipStream_t st, const PathSoA& P,
+                         const JobDev* jobs, const uint32_t* q, const uint32_t* c, uint32_t* f,
+                         const uint32_t* sc, uint32_t* sf, bool shadowPhase) {
+  if (!shadowPhase) {
+    if (counting)
+      hipLaunchKernelGGL((k_extend<WALK, STACK, true, MULTI>), dim3(grid), dim3(RBLOCK), 0, st, P, jobs, q, c, f);
+    else
+      hipLaunchKernelGGL((k_extend<WALK, STACK, false, MULTI>), dim3(grid), dim3(RBLOCK), 0, st, P, jobs, q, c, f);
+  } else {
+    if (counting)
+      hipLaunchKernelGGL((k_shadow<WALK, STACK, true, MULTI>), dim3(grid), dim3(RBLOCK), 0, st, P, jobs, sc, sf);
+    else
+      hipLaunchKernelGGL((k_shadow<WALK, STACK, false, MULTI>), dim3(grid), dim3(RBLOCK), 0, st, P, jobs, sc, sf);
+  }
+}
+
+template <bool MULTI>
+static void dispatch_trace(int walk, int stack, bool counting, unsigned grid, hipStream_t st,
+                           const PathSoA& P, const JobDev* jobs, const uint32_t* q,
+                           const uint32_t* c, uint32_t* f, const uint32_t* sc, uint32_t* sf,
+                           bool shadowPhase) {
+  if (walk == CT_WALK_STACKLESS) {
+    launch_trace<CT_WALK_STACKLESS, 1, MULTI>(counting, grid, st, P, jobs, q, c, f, sc, sf, shadowPhase);
+  } else if (walk == CT_WALK_WIDE) {
+    if (stack == 8)
+      launch_trace<CT_WALK_WIDE, 8, MULTI>(counting, grid, st, P, jobs, q, c, f, sc, sf, shadowPhase);
+    else if (stack == 16)
+      launch_trace<CT_WALK_WIDE, 16, MULTI>(counting, grid, st, P, jobs, q, c, f, sc, sf, shadowPhase);
+    else
+      launch_trace<CT_WALK_WIDE, 24, MULTI>(counting, grid, st, P, jobs, q, c, f, sc, sf, shadowPhase);
+  } else {
+    if (stack == 8)
+      launch_trace<CT_WALK_ORDERED, 8, MULTI>(counting, grid, st, P, jobs, q, c, f, sc, sf, shadowPhase);
+    else if (stack == 16)
+      launch_trace<CT_WALK_ORDERED, 16, MULTI>(counting, grid, st, P, jobs, q, c, f, sc, sf, shadowPhase);
+    else
+      launch_trace<CT_WALK_ORDERED, 24, MULTI>(counting, grid, st, P, jobs, q, c, f, sc, sf, shadowPhase);
+  }
+}
+
+// One job's share of a launch round.
+struct RoundJob {
+  const ct_scene* scene;
+  Frame F;
+  TraceOpts opts;
+  uint32_t width, height;
+  uint32_t frameBase;
+  uint32_t passes;
+  float* accum;
+  int doScale;
+  double scale;
+};
+
+// Kernel launches of one round (all on `st`; graph-capturable).
+template <bool MULTI>
+static int launch_round(ct_tracer* T, int njobs, size_t paths, size_t pix, int nb,
+                        hipStream_t st) {
   PathSoA P = T->P;
-  int nb = opts.maxBounces + 1;
-  // live/shadow queue counts restart per batch; the 64-bit work counters keep
-  // summing over the batches of one render
-  hipLaunchKernelGGL(k_clear, dim3(4), dim3(256), 0, st, P.ctr, 4 * MAX_BOUNCE_SLOTS, P.ctr64,
-                     firstBatch ? 16 : 0);
+  const JobDev* jobs = T->jobsDev;
+  hipLaunchKernelGGL(k_clear, dim3(4), dim3(256), 0, st, P.ctr, 4 * MAX_BOUNCE_SLOTS, P.ctr64, 0);
   CT_LAUNCH_CHECK("k_clear");
   unsigned gridPaths = grid_for(paths, RBLOCK, 256 * 16);
-  hipLaunchKernelGGL(k_generate, dim3(gridPaths), dim3(RBLOCK), 0, st, P, F, width, height,
-                     passes, frameBase);
+  hipLaunchKernelGGL((k_generate<MULTI>), dim3(gridPaths), dim3(RBLOCK), 0, st, P, jobs, njobs,
+                     paths);
   CT_LAUNCH_CHECK("k_generate");
+  unsigned gridTrace =
+      T->persistBlocks ? grid_for(paths, RBLOCK, T->persistBlocks) : grid_for(paths, RBLOCK, 1u << 30);
   for (int b = 0; b < nb; b++) {
     uint32_t* qin = (b & 1) ? P.q1 : P.q0;
     uint32_t* qout = (b & 1) ? P.q0 : P.q1;
     uint32_t* cin = P.ctr + b;
     uint32_t* cout = P.ctr + b + 1;
     uint32_t* scount = P.ctr + MAX_BOUNCE_SLOTS + b;
+    uint32_t* fExt = P.ctr + 2 * MAX_BOUNCE_SLOTS + b;
+    uint32_t* fSh = P.ctr + 3 * MAX_BOUNCE_SLOTS + b;
     mark(T, st, 0, true);
-    bool persist = T->persistent || T->walk == CT_WALK_WIDE || T->shadowWalk == CT_WALK_WIDE;
-    uint32_t* fExt = persist ? P.ctr + 2 * MAX_BOUNCE_SLOTS + b : nullptr;
-    uint32_t* fSh = persist ? P.ctr + 3 * MAX_BOUNCE_SLOTS + b : nullptr;
-    unsigned gridTrace = persist ? grid_for(paths, RBLOCK, T->persistBlocks) : gridPaths;
-    dispatch_extend(T->walk, T->stack, T->counting, gridTrace, st, P, s->dev, qin, cin, fExt);
+    dispatch_trace<MULTI>(T->walk, T->stack, T->counting, gridTrace, st, P, jobs, qin, cin, fExt,
+                          scount, fSh, false);
     CT_LAUNCH_CHECK("k_extend");
     mark(T, st, 0, false);
     mark(T, st, 1, true);
-    hipLaunchKernelGGL(k_shade, dim3(gridPaths), dim3(RBLOCK), 0, st, P, s->dev, qin, cin, qout,
-                       cout, scount, b, opts);
+    hipLaunchKernelGGL((k_shade<MULTI>), dim3(gridPaths), dim3(RBLOCK), 0, st, P, jobs, qin,
+                       cin, qout, cout, scount, b);
     CT_LAUNCH_CHECK("k_shade");
     mark(T, st, 1, false);
     mark(T, st, 2, true);
-    dispatch_shadow(T->shadowWalk, T->stack, T->counting, gridTrace, st, P, s->dev, scount, fSh);
+    dispatch_trace<MULTI>(T->shadowWalk, T->stack, T->counting, gridTrace, st, P, jobs, qin, cin,
+                          fExt, scount, fSh, true);
     CT_LAUNCH_CHECK("k_shadow");
     mark(T, st, 2, false);
   }
-  unsigned gridPix = grid_for(npix, RBLOCK, 256 * 16);
-  hipLaunchKernelGGL(k_finalize, dim3(gridPix), dim3(RBLOCK), 0, st, P, (uint32_t)npix, passes,
-                     accum, doScale ? 1 : 0, scale);
+  unsigned gridPix = grid_for(pix, RBLOCK, 256 * 16);
+  hipLaunchKernelGGL((k_finalize<MULTI>), dim3(gridPix), dim3(RBLOCK), 0, st, P, jobs, njobs,
+                     pix);
   CT_LAUNCH_CHECK("k_finalize");
   return CT_OK;
-}
-
-// Largest number of passes per batch so path state stays within a fixed budget.
-static uint32_t passes_per_batch(size_t npix, uint32_t passes) {
-  const size_t maxPaths = size_t(1) << 25;  // 33.5M paths ~ 6.4 GB of path state
-  size_t per = npix ? maxPaths / npix : passes;
-  if (per < 1) per = 1;
-  return (uint32_t)(per < passes ? per : passes);
 }
 
 struct BatchTotals {
   uint64_t ext = 0, shadow = 0;
 };
 
-static int read_batch_counts(ct_tracer* T, hipStream_t st, int nb, BatchTotals& tot) {
+// Traces one round: every job's passes, all jobs' paths sharing the launches.
+static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t st,
+                     BatchTotals& tot) {
+  size_t paths = 0, pix = 0;
+  int maxB = 0;
+  for (const RoundJob& j : R) {
+    paths += (size_t)j.width * j.height * j.passes;
+    pix += (size_t)j.width * j.height;
+    maxB = std::max(maxB, j.opts.maxBounces);
+  }
+  if (paths == 0) return CT_OK;
+  if (paths > 0xffffffffu) return set_error(CT_ELIMIT, "render: more than 2^32 paths in a round");
+  int rc = tracer_reserve(T, paths);
+  if (rc) return rc;
+  rc = jobs_reserve(T, R.size());
+  if (rc) return rc;
+  // the previous round's table copy must be done before the staging is reused
+  CT_HIP(hipStreamSynchronize(st));
+  size_t pb = 0, xb = 0;
+  for (size_t k = 0; k < R.size(); k++) {
+    const RoundJob& j = R[k];
+    JobDev& d = T->jobsHost[k];
+    d.S = j.scene->dev;
+    d.F = j.F;
+    d.opts = j.opts;
+    d.width = j.width;
+    d.height = j.height;
+    d.npix = j.width * j.height;
+    d.passes = j.passes;
+    d.frameBase = j.frameBase;
+    d.pathBase = (uint32_t)pb;
+    d.pixelBase = (uint32_t)xb;
+    d.doScale = j.doScale;
+    d.scale = j.scale;
+    d.accum = j.accum;
+    pb += (size_t)d.npix * j.passes;
+    xb += d.npix;
+  }
+  CT_HIP(hipMemcpyAsync(T->jobsDev, T->jobsHost, R.size() * sizeof(JobDev),
+                        hipMemcpyHostToDevice, st));
+  int nb = maxB + 1;
+  bool multi = R.size() > 1;
+  if (T->graph) {
+    // the captured launches depend only on the round's shape and the tracer's
+    // settings; the job table contents are read from device memory at replay
+    std::vector<char> key(sizeof(size_t) * 2 + sizeof(int) * 8);
+    size_t kk[2] = {paths, pix};
+    int ki[8] = {nb, (int)R.size(), T->walk, T->shadowWalk, T->stack, (int)T->persistBlocks,
+                 T->counting, multi};
+    memcpy(key.data(), kk, sizeof kk);
+    memcpy(key.data() + sizeof kk, ki, sizeof ki);
+    if (!T->graphExec || key != T->graphKey) {
+      if (T->graphExec) {
+        (void)hipGraphExecDestroy(T->graphExec);
+        T->graphExec = nullptr;
+      }
+      if (!T->ownStream) CT_HIP(hipStreamCreateWithFlags(&T->ownStream, hipStreamNonBlocking));
+      CT_HIP(hipStreamSynchronize(st));
+      CT_HIP(hipStreamBeginCapture(T->ownStream, hipStreamCaptureModeThreadLocal));
+      rc = multi ? launch_round<true>(T, (int)R.size(), paths, pix, nb, T->ownStream)
+                 : launch_round<false>(T, (int)R.size(), paths, pix, nb, T->ownStream);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(T->ownStream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return check_hip(e, "hipStreamEndCapture");
+      e = hipGraphInstantiate(&T->graphExec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return check_hip(e, "hipGraphInstantiate");
+      T->graphKey = key;
+    }
+    CT_HIP(hipGraphLaunch(T->graphExec, st));
+  } else {
+    rc = multi ? launch_round<true>(T, (int)R.size(), paths, pix, nb, st)
+               : launch_round<false>(T, (int)R.size(), paths, pix, nb, st);
+    if (rc) return rc;
+  }
   std::vector<uint32_t> h(2 * MAX_BOUNCE_SLOTS);
   CT_HIP(hipMemcpyAsync(h.data(), T->P.ctr, h.size() * 4, hipMemcpyDeviceToHost, st));
   CT_HIP(hipStreamSynchronize(st));
@@ -1100,6 +1174,90 @@ static int read_batch_counts(ct_tracer* T, hipStream_t st, int nb, BatchTotals& 
     tot.ext += h[b];
     tot.shadow += h[MAX_BOUNCE_SLOTS + b];
   }
+  T->lastBounces = std::max(T->lastBounces, nb);
+  T->lastRounds++;
+  return CT_OK;
+}
+
+// A job to render from `first` to `first + count` passes; `finish` scales by
+// 1/total when its last pass is added (renderImage).
+struct PlanJob {
+  const ct_scene* scene;
+  Frame F;
+  TraceOpts opts;
+  uint32_t width, height;
+  uint32_t frameBase;  // frame index of pass 0
+  uint32_t first, count;
+  float* accum;
+  bool finish;
+  double scale;
+};
+
+// Packs jobs into rounds of at most maxPaths paths (whole passes per job) and
+// traces them; returns the totals.
+static int run_plan(ct_tracer* T, std::vector<PlanJob> jobs, hipStream_t st) {
+  clear_events(T);
+  T->lastStream = st;
+  T->lastBounces = 0;
+  T->lastRounds = 0;
+  BatchTotals tot;
+  uint64_t paths = 0;
+  for (const PlanJob& j : jobs) {
+    size_t npix = (size_t)j.width * j.height;
+    if (j.count && npix > T->maxPaths)
+      return set_error(CT_ELIMIT, "render: one pass of a job exceeds the tracer's path capacity");
+    paths += (uint64_t)npix * j.count;
+  }
+  if (T->counting && paths) {
+    // work counters sum over the rounds of one call
+    int rc = tracer_reserve(T, std::min<uint64_t>(paths, T->maxPaths));
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, T->P.ctr, 0, T->P.ctr64, 16);
+    CT_LAUNCH_CHECK("k_clear");
+  }
+  std::vector<uint32_t> done(jobs.size(), 0);
+  for (;;) {
+    std::vector<RoundJob> R;
+    size_t budget = T->maxPaths;
+    for (size_t k = 0; k < jobs.size(); k++) {
+      const PlanJob& j = jobs[k];
+      uint32_t left = j.count - done[k];
+      size_t npix = (size_t)j.width * j.height;
+      if (left == 0 || npix == 0) continue;
+      size_t fit = budget / npix;
+      if (fit == 0) continue;
+      uint32_t p = (uint32_t)std::min<size_t>(left, fit);
+      RoundJob r;
+      r.scene = j.scene;
+      r.F = j.F;
+      r.opts = j.opts;
+      r.width = j.width;
+      r.height = j.height;
+      r.frameBase = j.frameBase + j.first + done[k];
+      r.passes = p;
+      r.accum = j.accum;
+      r.doScale = (j.finish && done[k] + p == j.count) ? 1 : 0;
+      r.scale = j.scale;
+      R.push_back(r);
+      done[k] += p;
+      budget -= p * npix;
+    }
+    if (R.empty()) break;
+    int rc = run_round(T, R, st, tot);
+    if (rc) return rc;
+  }
+  T->lastExtension = tot.ext;
+  T->lastShadow = tot.shadow;
+  T->lastPaths = paths;
+  return CT_OK;
+}
+
+static int check_job(const ct_scene* s, const double* frame, uint32_t w, uint32_t h,
+                     int32_t maxBounces, const void* out) {
+  if (!s || !frame || !out) return set_error(CT_EINVAL, "render: null argument");
+  if (maxBounces < 0 || maxBounces + 2 > MAX_BOUNCE_SLOTS)
+    return set_error(CT_ELIMIT, "render: maxBounces must be in [0, 254]");
+  if ((size_t)w * h > (size_t)1 << 26) return set_error(CT_ELIMIT, "render: more than 2^26 pixels");
   return CT_OK;
 }
 
@@ -1115,7 +1273,11 @@ int ct_tracer_destroy(ct_tracer* T) {
   if (!T) return CT_OK;
   clear_events(T);
   hipError_t e = hipSuccess;
+  if (T->graphExec) (void)hipGraphExecDestroy(T->graphExec);
   if (T->buf) e = hipFree(T->buf);
+  if (T->jobsDev) (void)hipFree(T->jobsDev);
+  if (T->jobsHost) (void)hipHostFree(T->jobsHost);
+  if (T->ownStream) (void)hipStreamDestroy(T->ownStream);
   delete T;
   return check_hip(e, "ct_tracer_destroy");
 }
@@ -1138,7 +1300,7 @@ int ct_tracer_set_traversal(ct_tracer* T, int32_t shadowWalk, int32_t stackDepth
       shadowWalk != CT_WALK_WIDE)
     return set_error(CT_EINVAL, "ct_tracer_set_traversal: unknown walk");
   if (!stack_supported(stackDepth))
-    return set_error(CT_EINVAL, "ct_tracer_set_traversal: stack depth must be 8, 12, 16 or 24");
+    return set_error(CT_EINVAL, "ct_tracer_set_traversal: stack depth must be 8, 16 or 24");
   T->shadowWalk = shadowWalk;
   T->stack = stackDepth;
   return CT_OK;
@@ -1148,8 +1310,26 @@ int ct_tracer_set_schedule(ct_tracer* T, int32_t persistent, int32_t blocks) {
   if (!T) return set_error(CT_EINVAL, "null tracer");
   if (persistent && (blocks < 1 || blocks > 65535))
     return set_error(CT_EINVAL, "ct_tracer_set_schedule: blocks must be in [1, 65535]");
-  T->persistent = persistent != 0;
-  if (persistent) T->persistBlocks = (unsigned)blocks;
+  T->persistBlocks = persistent ? (unsigned)blocks : 0u;
+  return CT_OK;
+}
+
+int ct_tracer_set_capacity(ct_tracer* T, uint64_t maxPaths) {
+  if (!T) return set_error(CT_EINVAL, "null tracer");
+  if (maxPaths < 1024 || maxPaths > ((uint64_t)1 << 31))
+    return set_error(CT_EINVAL, "ct_tracer_set_capacity: maxPaths must be in [1024, 2^31]");
+  T->maxPaths = (size_t)maxPaths;
+  return CT_OK;
+}
+
+int ct_tracer_set_graph(ct_tracer* T, int32_t enable) {
+  if (!T) return set_error(CT_EINVAL, "null tracer");
+  T->graph = enable != 0;
+  if (!T->graph && T->graphExec) {
+    (void)hipGraphExecDestroy(T->graphExec);
+    T->graphExec = nullptr;
+    T->graphKey.clear();
+  }
   return CT_OK;
 }
 
@@ -1161,59 +1341,67 @@ int ct_tracer_set_walk(ct_tracer* T, int32_t walk) {
   return CT_OK;
 }
 
-static int render_common(ct_tracer* T, const ct_scene* s, const double* frame16, uint32_t width,
-                         uint32_t height, uint32_t spp, uint32_t frameBase, TraceOpts opts,
-                         float* out, bool average, hipStream_t st) {
-  if (!T || !s || !frame16 || !out) return set_error(CT_EINVAL, "render: null argument");
-  if (opts.maxBounces < 0 || opts.maxBounces + 2 > MAX_BOUNCE_SLOTS)
-    return set_error(CT_ELIMIT, "render: maxBounces must be in [0, 254]");
-  if ((size_t)width * height > (size_t)1 << 26)
-    return set_error(CT_ELIMIT, "render: more than 2^26 pixels");
-  clear_events(T);
-  T->lastStream = st;
-  size_t npix = (size_t)width * height;
-  if (npix == 0 || spp == 0) return CT_OK;
-  Frame F = make_frame(frame16);
-  int nb = opts.maxBounces + 1;
-  uint32_t per = passes_per_batch(npix, spp);
-  BatchTotals tot;
-  if (average) {
-    unsigned g = grid_for(npix * 3, RBLOCK, 256 * 16);
-    hipLaunchKernelGGL(k_zero, dim3(g), dim3(RBLOCK), 0, st, out, npix * 3);
-    CT_LAUNCH_CHECK("k_zero");
-  }
-  for (uint32_t s0 = 0; s0 < spp; s0 += per) {
-    uint32_t passes = (spp - s0) < per ? (spp - s0) : per;
-    bool last = s0 + passes >= spp;
-    int rc = run_passes(T, s, F, width, height, passes, frameBase + s0, opts, out,
-                        average && last, average ? 1.0 / spp : 1.0, st, s0 == 0);
-    if (rc) return rc;
-    rc = read_batch_counts(T, st, nb, tot);
-    if (rc) return rc;
-  }
-  T->lastExtension = tot.ext;
-  T->lastShadow = tot.shadow;
-  T->lastBounces = nb;
-  T->lastPaths = (uint64_t)npix * spp;
-  return CT_OK;
-}
-
 int ct_accumulate_pass(ct_tracer* T, const ct_scene* s, const double* frame16, uint32_t width,
                        uint32_t height, uint32_t frameIndex, const ct_trace_opts* opts,
                        float* accum, void* stream) {
-  if (!opts) return set_error(CT_EINVAL, "ct_accumulate_pass: null opts");
-  TraceOpts o{opts->maxBounces, opts->russianRoulette != 0, opts->nee != 0};
-  return render_common(T, s, frame16, width, height, 1, frameIndex, o, accum, false,
-                       (hipStream_t)stream);
+  if (!T || !opts) return set_error(CT_EINVAL, "ct_accumulate_pass: null argument");
+  int rc = check_job(s, frame16, width, height, opts->maxBounces, accum);
+  if (rc) return rc;
+  PlanJob j{s, make_frame(frame16), {opts->maxBounces, opts->russianRoulette != 0, opts->nee != 0},
+            width, height, frameIndex, 0, 1, accum, false, 1.0};
+  return run_plan(T, {j}, (hipStream_t)stream);
 }
 
 int ct_render_image(ct_tracer* T, const ct_scene* s, const double* frame16,
                     const ct_render_opts* opts, float* out, void* stream) {
-  if (!opts) return set_error(CT_EINVAL, "ct_render_image: null opts");
-  TraceOpts o{opts->maxBounces, opts->russianRoulette != 0, opts->nee != 0};
-  uint32_t base = opts->seed * 7919u;  // frameIndex = s + seed*7919 (uint32 wrap)
-  return render_common(T, s, frame16, opts->width, opts->height, opts->spp, base, o, out, true,
-                       (hipStream_t)stream);
+  if (!T || !opts) return set_error(CT_EINVAL, "ct_render_image: null argument");
+  int rc = check_job(s, frame16, opts->width, opts->height, opts->maxBounces, out);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  size_t n = (size_t)opts->width * opts->height * 3;
+  if (n) {
+    hipLaunchKernelGGL(k_zero, dim3(grid_for(n, RBLOCK, 256 * 16)), dim3(RBLOCK), 0, st, out, n);
+    CT_LAUNCH_CHECK("k_zero");
+  }
+  PlanJob j{s, make_frame(frame16),
+            {opts->maxBounces, opts->russianRoulette != 0, opts->nee != 0},
+            opts->width, opts->height, opts->seed * 7919u, 0, opts->spp, out, true,
+            opts->spp ? 1.0 / opts->spp : 1.0};
+  return run_plan(T, {j}, st);
+}
+
+int ct_render_batch(ct_tracer* T, const ct_job* jobs, uint32_t n, void* stream) {
+  if (!T || (n && !jobs)) return set_error(CT_EINVAL, "ct_render_batch: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<PlanJob> plan;
+  for (uint32_t k = 0; k < n; k++) {
+    const ct_job& J = jobs[k];
+    int rc = check_job(J.scene, J.frame, J.width, J.height, J.maxBounces, J.d_out);
+    if (rc) return rc;
+    size_t m = (size_t)J.width * J.height * 3;
+    if (m) {
+      hipLaunchKernelGGL(k_zero, dim3(grid_for(m, RBLOCK, 256 * 16)), dim3(RBLOCK), 0, st, J.d_out, m);
+      CT_LAUNCH_CHECK("k_zero");
+    }
+    plan.push_back({J.scene, make_frame(J.frame),
+                    {J.maxBounces, J.russianRoulette != 0, J.nee != 0}, J.width, J.height,
+                    J.seed * 7919u, 0, J.spp, J.d_out, true, J.spp ? 1.0 / J.spp : 1.0});
+  }
+  return run_plan(T, plan, st);
+}
+
+int ct_accumulate_batch(ct_tracer* T, const ct_pass_job* jobs, uint32_t n, void* stream) {
+  if (!T || (n && !jobs)) return set_error(CT_EINVAL, "ct_accumulate_batch: null argument");
+  std::vector<PlanJob> plan;
+  for (uint32_t k = 0; k < n; k++) {
+    const ct_pass_job& J = jobs[k];
+    int rc = check_job(J.scene, J.frame, J.width, J.height, J.maxBounces, J.d_accum);
+    if (rc) return rc;
+    plan.push_back({J.scene, make_frame(J.frame),
+                    {J.maxBounces, J.russianRoulette != 0, J.nee != 0}, J.width, J.height,
+                    J.frameIndex, 0, J.passes, J.d_accum, false, 1.0});
+  }
+  return run_plan(T, plan, (hipStream_t)stream);
 }
 
 int ct_tracer_stats(ct_tracer* T, ct_render_stats* out) {
@@ -1223,8 +1411,8 @@ int ct_tracer_stats(ct_tracer* T, ct_render_stats* out) {
   out->extensionRays = T->lastExtension;
   out->shadowRays = T->lastShadow;
   out->paths = T->lastPaths;
-  if (T->profiling) {
-    // events come in (begin, end) pairs in launch order
+  out->rounds = T->lastRounds;
+  if (T->profiling && !T->graph) {
     for (size_t i = 0; i + 1 < T->events.size(); i += 2) {
       float ms = 0;
       CT_HIP(hipEventElapsedTime(&ms, T->events[i], T->events[i + 1]));
@@ -1239,19 +1427,19 @@ int ct_tracer_stats(ct_tracer* T, ct_render_stats* out) {
       }
     }
   } else {
-    out->extendLaunches = (uint32_t)T->lastBounces;
+    out->extendLaunches = (uint32_t)T->lastBounces * T->lastRounds;
   }
   if (T->counting && T->buf) {
     unsigned long long h[8];
     CT_HIP(hipMemcpy(h, T->P.ctr64, sizeof h, hipMemcpyDeviceToHost));
-    out->stackFallbacks = h[6];
-    out->shadowStackFallbacks = h[7];
     out->nodeVisits = h[0];
     out->triTests = h[1];
     out->sphereTests = h[2];
     out->shadowNodeVisits = h[3];
     out->shadowTriTests = h[4];
     out->shadowSphereTests = h[5];
+    out->stackFallbacks = h[6];
+    out->shadowStackFallbacks = h[7];
   }
   return CT_OK;
 }

@@ -213,6 +213,50 @@ class Tracer:
                                     fr.ctypes.data_as(C.c_void_p), C.byref(o), d_out.ptr, stream),
               "ct_render_image")
 
+    def set_capacity(self, max_paths):
+        check(lib().ct_tracer_set_capacity(self.handle, int(max_paths)), "set_capacity")
+
+    def set_graph(self, on):
+        check(lib().ct_tracer_set_graph(self.handle, int(bool(on))), "set_graph")
+
+    @staticmethod
+    def _frame16(frame):
+        return (C.c_double * 16)(*[float(x) for x in frame])
+
+    def render_batch(self, jobs, stream=None):
+        """jobs: list of dicts {scene (PackedScene/DeviceScene), frame, width, height, spp,
+        seed, maxBounces, russianRoulette, nee, out (DeviceBuffer)} -- one renderImage each,
+        all traced in shared launches."""
+        arr = (A.Job * len(jobs))()
+        for k, j in enumerate(jobs):
+            arr[k].scene = device_scene(j["scene"]).handle.value
+            arr[k].frame = self._frame16(j["frame"])
+            arr[k].width, arr[k].height = j["width"], j["height"]
+            arr[k].spp, arr[k].seed = j["spp"], j.get("seed", 0) & 0xFFFFFFFF
+            arr[k].maxBounces = int(j.get("maxBounces", 8))
+            arr[k].russianRoulette = int(bool(j.get("russianRoulette", True)))
+            arr[k].nee = int(bool(j.get("nee", True)))
+            arr[k].d_out = j["out"].ptr.value
+        check(lib().ct_render_batch(self.handle, C.cast(arr, C.c_void_p), len(jobs), stream),
+              "ct_render_batch")
+
+    def accumulate_batch(self, jobs, stream=None):
+        """jobs: list of dicts {scene, frame, width, height, frameIndex, passes, maxBounces,
+        russianRoulette, nee, accum (DeviceBuffer)} -- progressive passes, shared launches."""
+        arr = (A.PassJob * len(jobs))()
+        for k, j in enumerate(jobs):
+            arr[k].scene = device_scene(j["scene"]).handle.value
+            arr[k].frame = self._frame16(j["frame"])
+            arr[k].width, arr[k].height = j["width"], j["height"]
+            arr[k].frameIndex = j["frameIndex"] & 0xFFFFFFFF
+            arr[k].passes = j["passes"]
+            arr[k].maxBounces = int(j.get("maxBounces", 8))
+            arr[k].russianRoulette = int(bool(j.get("russianRoulette", True)))
+            arr[k].nee = int(bool(j.get("nee", True)))
+            arr[k].d_accum = j["accum"].ptr.value
+        check(lib().ct_accumulate_batch(self.handle, C.cast(arr, C.c_void_p), len(jobs), stream),
+              "ct_accumulate_batch")
+
     def stats(self):
         s = A.RenderStats()
         check(lib().ct_tracer_stats(self.handle, C.byref(s)), "ct_tracer_stats")

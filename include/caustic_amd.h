@@ -90,7 +90,31 @@ typedef struct {
   uint64_t shadowSphereTests;/* sphere tests of any-hit walks (counting on) */
   uint64_t stackFallbacks;   /* closest-hit rays that outgrew the LDS stack (counting on) */
   uint64_t shadowStackFallbacks; /* any-hit rays that outgrew the LDS stack (counting on) */
+  uint32_t rounds;           /* launch rounds the call was split into */
 } ct_render_stats;
+
+/* One image of a batch render: renderImage (tracer.ts:348-355) of `scene`
+ * seen through `frame` into d_out (width*height*3 float32, overwritten). */
+typedef struct {
+  const ct_scene *scene;
+  double frame[16];
+  uint32_t width, height, spp, seed;
+  int32_t maxBounces, russianRoulette, nee;
+  float *d_out;
+} ct_job;
+
+/* Passes of one progressive image: `passes` calls of accumulatePass
+ * (tracer.ts:361-395) with frame indices frameIndex, frameIndex+1, ...
+ * adding into the float32 sum d_accum (width*height*3). */
+typedef struct {
+  const ct_scene *scene;
+  double frame[16];
+  uint32_t width, height;
+  uint32_t frameIndex;
+  uint32_t passes;
+  int32_t maxBounces, russianRoulette, nee;
+  float *d_accum;
+} ct_pass_job;
 
 /* ---- runtime ------------------------------------------------------------ */
 int ct_init(int32_t device);
@@ -146,12 +170,12 @@ int ct_tracer_set_counting(ct_tracer *tracer, int32_t enable);
 /* Closest-hit walk used by the extension rays (CT_WALK_*). */
 int ct_tracer_set_walk(ct_tracer *tracer, int32_t walk);
 /* Walk of the NEE any-hit rays (CT_WALK_*; the result does not depend on it)
- * and the LDS stack depth of the ordered walks (8, 12, 16 or 24 entries;
+ * and the LDS stack depth of the ordered walks (8, 16 or 24 entries;
  * deeper rays finish with the stackless walk, same result). */
 int ct_tracer_set_traversal(ct_tracer *tracer, int32_t shadowWalk, int32_t stackDepth);
 /* Launch schedule of the traversal kernels: persistent = 1 runs `blocks`
  * workgroups of 256 lanes that pull rays from the queue as lanes free up;
- * 0 launches one lane per queue slot.  Results do not depend on it. */
+ * 0 launches one lane per path.  Results do not depend on it. */
 int ct_tracer_set_schedule(ct_tracer *tracer, int32_t persistent, int32_t blocks);
 /* accumulatePass (tracer.ts:361-395): adds one sample per pixel into the
  * float32 sum d_accum (width*height*3) with frameIndex as the RNG frame. */
@@ -162,6 +186,18 @@ int ct_accumulate_pass(ct_tracer *tracer, const ct_scene *scene, const double *h
  * then d_out *= 1/spp.  d_out is width*height*3 float32 (overwritten). */
 int ct_render_image(ct_tracer *tracer, const ct_scene *scene, const double *h_frame16,
                     const ct_render_opts *opts, float *d_out, void *stream);
+/* Several scenes / cameras / sizes / depths traced in shared launches: the
+ * paths of all jobs share every extend, shade and shadow launch (rounds of at
+ * most the tracer's path capacity).  Each output equals what the job alone
+ * produces (each pixel's samples are added in pass order per job).  Not part of
+ * the reference API: it serves the batching and serving forms. */
+int ct_render_batch(ct_tracer *tracer, const ct_job *jobs, uint32_t n, void *stream);
+int ct_accumulate_batch(ct_tracer *tracer, const ct_pass_job *jobs, uint32_t n, void *stream);
+/* Paths in flight per round (path state is about 190 B per path; default 2^25). */
+int ct_tracer_set_capacity(ct_tracer *tracer, uint64_t maxPaths);
+/* Capture each round's launches into a hipGraph and replay it while the
+ * round's shape (paths, pixels, jobs, bounces, settings) repeats. */
+int ct_tracer_set_graph(ct_tracer *tracer, int32_t enable);
 /* Counters and device times of the last accumulate/render (synchronises). */
 int ct_tracer_stats(ct_tracer *tracer, ct_render_stats *out);
 

@@ -1,0 +1,111 @@
+"""Multi-job launches (batching and serving forms) on the GPU.
+
+Every job of a shared launch must produce exactly (bit for bit) what it
+produces alone, and what the oracle's renderImage / accumulatePass produce
+within the 1e-4 per-pixel tolerance; graph replays must equal eager runs."""
+import numpy as np
+import pytest
+
+import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _jobs(ct, O):
+    sc1 = scenes.cornell_box()
+    sc2 = scenes.night_emissives()
+    sc3 = ct.synth.soup_scene(20000, n_lights=16, n_materials=20, seed=9)
+    out = []
+    for sc, (w, h, spp, seed, depth) in ((sc1, (64, 48, 4, 1, 8)), (sc2, (40, 30, 2, 7, 5)),
+                                         (sc3, (48, 32, 3, 2, 6))):
+        ps = O.pack_scene(sc)
+        out.append({"scene": ps, "frame": O.camera_frame(sc["camera"], w / h), "width": w,
+                    "height": h, "spp": spp, "seed": seed, "maxBounces": depth})
+    return out
+
+
+def _alloc(ct, j):
+    return ct.DeviceBuffer.empty((j["width"] * j["height"] * 3,), np.float32)
+
+
+def rel_err(g, o):
+    g = np.asarray(g, np.float64)
+    o = np.asarray(o, np.float64)
+    den = np.maximum(np.abs(g), np.abs(o))
+    return np.where(den > 0, np.abs(g - o) / np.where(den > 0, den, 1), 0.0)
+
+
+def test_batch_equals_alone_and_oracle(ct, O):
+    jobs = _jobs(ct, O)
+    tr = ct.Tracer()
+    for j in jobs:
+        j["out"] = _alloc(ct, j)
+    tr.render_batch(jobs)
+    st = tr.stats()
+    mixed = [j["out"].numpy() for j in jobs]
+    for j, m in zip(jobs, mixed):
+        alone = ct.renderImage(j["scene"], j["frame"], j)
+        assert np.array_equal(alone.view(np.uint32), m.view(np.uint32))
+        o = O.render_image(j["scene"], j["frame"], j["width"], j["height"], j["spp"],
+                           seed=j["seed"], max_bounces=j["maxBounces"])
+        assert rel_err(m, o).max() <= 1e-4
+    assert st["rounds"] == 1
+
+
+def test_batch_split_into_rounds(ct, O):
+    jobs = _jobs(ct, O)
+    tr = ct.Tracer()
+    tr.set_capacity(4096)  # forces many rounds, one pass of each job at most per round
+    for j in jobs:
+        j["out"] = _alloc(ct, j)
+    tr.render_batch(jobs)
+    assert tr.stats()["rounds"] > 3
+    for j in jobs:
+        alone = ct.renderImage(j["scene"], j["frame"], j)
+        assert np.array_equal(alone.view(np.uint32), j["out"].numpy().view(np.uint32))
+
+
+def test_progressive_passes_resume(ct, O):
+    """(accum, passes done) resumes a render: 1 + 2 passes equal 3 in one call,
+    and equal the oracle's accumulatePass sums."""
+    jobs = _jobs(ct, O)
+    tr = ct.Tracer()
+    a = [_alloc(ct, j) for j in jobs]
+    b = [_alloc(ct, j) for j in jobs]
+    for buf in a + b:
+        buf.upload(np.zeros(buf.shape, np.float32))
+
+    def pj(j, acc, first, passes):
+        return {**j, "frameIndex": first, "passes": passes, "accum": acc}
+
+    tr.accumulate_batch([pj(j, x, 100, 1) for j, x in zip(jobs, a)])
+    tr.accumulate_batch([pj(j, x, 101, 2) for j, x in zip(jobs, a)])
+    tr.accumulate_batch([pj(j, x, 100, 3) for j, x in zip(jobs, b)])
+    for j, x, y in zip(jobs, a, b):
+        assert np.array_equal(x.numpy().view(np.uint32), y.numpy().view(np.uint32))
+        acc = np.zeros(j["width"] * j["height"] * 3, np.float32)
+        for f in (100, 101, 102):
+            O.accumulate_pass(j["scene"], j["frame"], j["width"], j["height"], f, acc,
+                              max_bounces=j["maxBounces"])
+        assert rel_err(x.numpy(), acc).max() <= 1e-4
+
+
+def test_graph_replays_equal_eager(ct, O):
+    jobs = _jobs(ct, O)
+    eager = ct.Tracer()
+    for j in jobs:
+        j["out"] = _alloc(ct, j)
+    eager.render_batch(jobs)
+    ref = [j["out"].numpy() for j in jobs]
+    tr = ct.Tracer()
+    tr.set_graph(True)
+    for rep in range(3):
+        for j in jobs:
+            j["out"].upload(np.full(j["out"].shape, np.nan, np.float32))
+        tr.render_batch(jobs)
+        ct.sync()
+        got = [j["out"].numpy() for j in jobs]
+        for r, g in zip(ref, got):
+            assert np.array_equal(r.view(np.uint32), g.view(np.uint32)), rep
+        # eager work between replays
+        eager.render_batch(jobs[:1])
