@@ -12,6 +12,15 @@ Prints one JSON line: Mrays/s (extension + shadow rays per second) as `value`,
 spp*Mpix/s beside it, the roofline of the dominant kernel (closest-hit
 extend) and a CPU baseline: the oracle's restatement of the reference tracer
 on a bounded row sample of the same frame on the host cores.
+
+--form batch: 64 soup scenes of lognormal size (1k-500k triangles, median
+20k), resolutions from {128,256,512,768}^2, 1-32 spp, maxBounces 8, traced
+together after the launch settings are autotuned, replayed as a hipGraph;
+beside it the same jobs rendered one after another with renderImage.
+--form serve: 128 resident scenes (1k-1M triangles) and a Poisson request
+stream (random camera, 128^2-1024^2, 1-64 spp, maxBounces 4-12) through the
+serving loop; p50/p99 time-to-complete and aggregate throughput, beside the
+same requests rendered one at a time.
 """
 import argparse
 import importlib.util
@@ -78,6 +87,7 @@ def cpu_baseline(ct, ps, frame, w, h, spp, depth, budget_s=12.0):
 
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--form", choices=["plain", "batch", "serve"], default="plain")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
@@ -93,16 +103,46 @@ def main():
                     help="workgroups of the persistent traversal launches (0: one lane per ray)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--scenes", type=int, default=0, help="batch/serve: scene count (0: 64/128)")
+    ap.add_argument("--tune-trials", type=int, default=16, help="batch: autotuner trials")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve: requests per second")
+    ap.add_argument("--duration", type=float, default=2.0, help="serve: seconds of arrivals")
+    ap.add_argument("--max-paths", type=int, default=1 << 24, help="serve: paths per round")
+    ap.add_argument("--policy", choices=["fifo", "srpt"], default="fifo")
+    ap.add_argument("--hbm-budget-gib", type=float, default=8.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one process, one "
               "MI355X); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
-
     ct = load_package()
     ct.init(0)
+    if args.form == "batch":
+        res = batch_form(ct, args)
+    elif args.form == "serve":
+        res = serve_form(ct, args)
+    else:
+        res = plain_form(ct, args)
+    print(json.dumps(res), flush=True)
+
+
+def extend_roofline(cnt, prof, kernel):
+    """Roofline of the closest-hit launches: algorithmic bytes from the counting
+    run over the summed HIP-event time of the same launches."""
+    launches = max(1, prof["extendLaunches"])
+    ext_bytes = (32 * cnt["nodeVisits"] + 48 * cnt["triTests"] + 32 * cnt["sphereTests"] +
+                 (24 + 16) * cnt["extensionRays"])
+    ext_s = prof["extendMs"] / 1e3
+    achieved = ext_bytes / ext_s / 1e9 if ext_s > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
+            "bytes_per_launch": ext_bytes // launches,
+            "avg_launch_ms": round(prof["extendMs"] / launches, 4)}
+
+
+def plain_form(ct, args):
+    import numpy as np
     w, h, spp, depth = args.width, args.height, args.spp, args.depth
     t = time.perf_counter()
     scene = ct.synth.soup_scene(args.tris)
@@ -165,23 +205,14 @@ def main():
     tr.render_image(ds, frame, opts, out)
     cnt = tr.stats()
     tr.set_counting(False)
-    launches = max(1, prof["extendLaunches"])
-    ext_bytes = (32 * cnt["nodeVisits"] + 48 * cnt["triTests"] + 32 * cnt["sphereTests"] +
-                 (24 + 16) * cnt["extensionRays"])
-    ext_s = prof["extendMs"] / 1e3
-    achieved = ext_bytes / ext_s / 1e9 if ext_s > 0 else 0.0
-    log(f"extend: {launches} launches, {prof['extendMs']:.2f} ms total, "
+    roof = extend_roofline(cnt, prof, f"k_extend<{args.walk},{args.stack}>")
+    log(f"extend: {prof['extendLaunches']} launches, {prof['extendMs']:.2f} ms total, "
         f"{cnt['nodeVisits'] / max(1, cnt['extensionRays']):.1f} nodes/ray, "
-        f"{cnt['triTests'] / max(1, cnt['extensionRays']):.1f} tris/ray, {achieved:.0f} GB/s, "
+        f"{cnt['triTests'] / max(1, cnt['extensionRays']):.1f} tris/ray, {roof['achieved']:.0f} GB/s, "
         f"{cnt['stackFallbacks']} stack fallbacks; shade {prof['shadeMs']:.2f} ms, "
         f"shadow {prof['shadowMs']:.2f} ms "
         f"({cnt['shadowNodeVisits'] / max(1, cnt['shadowRays']):.1f} nodes/ray, "
         f"{cnt['shadowStackFallbacks']} fallbacks)")
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": f"k_extend<{args.walk},{args.stack}>",
-            "bytes_per_launch": ext_bytes // launches,
-            "avg_launch_ms": round(prof["extendMs"] / launches, 4)}
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (oracle on host cores)...")
@@ -202,7 +233,190 @@ def main():
         "rays_per_step": rays, "bvh_build_ms": round(build_ms, 2),
         "roofline": roof, "cpu_baseline": cpu,
     }
-    print(json.dumps(res), flush=True)
+    return res
+
+
+def _digest(a):
+    import hashlib
+    return hashlib.sha1(a.tobytes()).hexdigest()
+
+
+def batch_form(ct, args):
+    import numpy as np
+
+    n = args.scenes or 64
+    t = time.perf_counter()
+    scenes, specs = ct.synth.batch_workload(n)
+    tris = [sum(len(o["shape"]["indices"]) // 3 for o in sc["objects"]) for sc in scenes]
+    log(f"{n} scenes generated ({sum(tris)} triangles, {min(tris)}..{max(tris)}) in "
+        f"{time.perf_counter() - t:.1f} s")
+    t = time.perf_counter()
+    packed = [ct.packScene(sc) for sc in scenes]
+    dss = [ct.DeviceScene(ps) for ps in packed]
+    log(f"packed and uploaded in {time.perf_counter() - t:.1f} s")
+    jobs = []
+    for sp in specs:
+        sc = scenes[sp["sceneIndex"]]
+        j = dict(sp, scene=dss[sp["sceneIndex"]],
+                 frame=ct.cameraFrame(sc["camera"], sp["width"] / sp["height"]))
+        j["out"] = ct.DeviceBuffer.empty((j["width"] * j["height"] * 3,), np.float32)
+        jobs.append(j)
+    samples = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
+    log(f"{len(jobs)} jobs, {samples / 1e6:.1f} M paths")
+
+    # plain: the same jobs one after another through renderImage
+    plain = ct.Tracer()
+    for j in jobs:
+        plain.render_image(j["scene"], j["frame"], j, j["out"])
+    ct.sync()
+    t = time.perf_counter()
+    rays_plain = 0
+    for j in jobs:
+        plain.render_image(j["scene"], j["frame"], j, j["out"])
+        st = plain.stats()
+        rays_plain += st["extensionRays"] + st["shadowRays"]
+    ct.sync()
+    plain_s = time.perf_counter() - t
+    ref = [_digest(j["out"].numpy()) for j in jobs]
+    log(f"plain sequential: {plain_s * 1e3:.1f} ms, {samples / plain_s / 1e6:.1f} spp*Mpix/s")
+
+    tr = ct.Tracer()
+    tune = tr.autotune(jobs, max_trials=args.tune_trials)
+    log(f"autotune: {tune['trials']} trials, {tune['initialMs']:.1f} -> {tune['ms']:.1f} ms, "
+        f"config {tune['config']}")
+    tr.set_graph(True)
+    for _ in range(args.warmup):
+        tr.render_batch(jobs)
+    ct.sync()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        tr.render_batch(jobs)
+    ct.sync()
+    total = time.perf_counter() - t
+    st = tr.stats()
+    rays = st["extensionRays"] + st["shadowRays"]
+    identical = [_digest(j["out"].numpy()) for j in jobs] == ref
+    ms_step = total / args.steps * 1e3
+    mrays = rays * args.steps / total / 1e6
+    sppmpix = samples * args.steps / total / 1e6
+    log(f"batched: {ms_step:.1f} ms/step, {mrays:.0f} Mrays/s, {sppmpix:.1f} spp*Mpix/s, "
+        f"{st['rounds']} round(s), identical to plain: {identical}")
+    tr.set_graph(False)
+    tr.set_profiling(True)
+    tr.render_batch(jobs)
+    prof = tr.stats()
+    tr.set_profiling(False)
+    tr.set_counting(True)
+    tr.render_batch(jobs)
+    cnt = tr.stats()
+    tr.set_counting(False)
+    cfg = tune["config"]
+    roof = extend_roofline(cnt, prof, f"k_extend<walk {cfg['walk']},{cfg['stackDepth']}>")
+    cpu = None
+    if not args.no_cpu:
+        j = max(jobs, key=lambda x: x["width"])
+        ps = packed[j["sceneIndex"]]
+        cpu = cpu_baseline(ct, ps, j["frame"], j["width"], j["height"], j["spp"], j["maxBounces"],
+                           args.cpu_budget)
+    return {
+        "metric": METRIC, "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"batching form: {n} soup scenes ({sum(tris)} triangles, "
+                               f"{min(tris)}-{max(tris)} each), 128^2-768^2, 1-32 spp, "
+                               f"maxBounces 8, one render_batch per step (autotuned, hipGraph)",
+                   "scenes": n, "paths": samples, "parallelism": "single GPU"},
+        "spp_mpix_per_s": round(sppmpix, 3), "rounds": st["rounds"],
+        "plain_sequential": {"ms": round(plain_s * 1e3, 3),
+                             "spp_mpix_per_s": round(samples / plain_s / 1e6, 3),
+                             "mrays_per_s": round(rays_plain / plain_s / 1e6, 2)},
+        "speedup_vs_plain": round(plain_s * 1e3 / ms_step, 3),
+        "identical_to_plain": identical, "autotune": tune,
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+
+
+def serve_form(ct, args):
+    import numpy as np
+
+    n = args.scenes or 128
+    t = time.perf_counter()
+    counts = ct.synth.triangle_counts(n, 5, 1_000, 1_000_000, sigma=1.4)
+    scenes = [ct.synth.family_scene(i, c) for i, c in enumerate(counts)]
+    packed = [ct.packScene(sc) for sc in scenes]
+    dss = [ct.DeviceScene(ps) for ps in packed]
+    log(f"{n} resident scenes ({int(counts.sum())} triangles, {counts.min()}..{counts.max()}) "
+        f"in {time.perf_counter() - t:.1f} s")
+    reqs = ct.synth.request_stream(n, rate=args.rate, duration=args.duration)
+    frames = [ct.cameraFrame(r["camera"], 1.0) for r in reqs]
+    samples = sum(r["width"] * r["height"] * r["spp"] for r in reqs)
+    log(f"{len(reqs)} requests over {args.duration} s at {args.rate}/s: {samples / 1e9:.2f} G "
+        f"samples offered ({samples / args.duration / 1e6:.0f} spp*Mpix/s)")
+    keep = set(range(0, len(reqs), max(1, len(reqs) // 8)))
+
+    # plain: the same requests one at a time through renderImage, in arrival order
+    plain = ct.Tracer()
+    out = ct.DeviceBuffer.empty((1024 * 1024 * 3,), np.float32)
+    warm = reqs[0]
+    plain.render_image(dss[warm["sceneIndex"]], frames[0], dict(warm, spp=1), out)
+    ct.sync()
+    ref = {}
+    t = time.perf_counter()
+    rays_plain = 0
+    for k, r in enumerate(reqs):
+        plain.render_image(dss[r["sceneIndex"]], frames[k], r, out)
+        st = plain.stats()
+        rays_plain += st["extensionRays"] + st["shadowRays"]
+        if k in keep:
+            ref[k] = _digest(out.numpy()[:r["width"] * r["height"] * 3])
+    ct.sync()
+    plain_s = time.perf_counter() - t
+    del plain, out
+    log(f"plain one-at-a-time: {plain_s:.2f} s, {samples / plain_s / 1e6:.1f} spp*Mpix/s")
+
+    srv = ct.Server(hbm_budget=int(args.hbm_budget_gib * (1 << 30)), max_paths=args.max_paths,
+                    policy=args.policy)
+    wid = srv.submit(dss[0], frames[0], 64, 64, 1)  # loads the kernels
+    srv.wait(wid)
+    srv.release(wid)
+    base = srv.stats()
+    res = ct.serve.replay(srv, dss, frames, reqs, keep=keep, log=log)
+    st = res["stats"]
+    wall = res["wall"]
+    srv.close()
+    lat = res["latency"]
+    same = all(_digest(res["images"][k]) == ref[k] for k in keep)
+    pix = st["pixelSamples"] - base["pixelSamples"]
+    rays = (st["extensionRays"] + st["shadowRays"]) - (base["extensionRays"] + base["shadowRays"])
+    p50 = ct.serve.percentile(lat, 50)
+    p99 = ct.serve.percentile(lat, 99)
+    sppmpix = pix / wall / 1e6
+    mrays = rays / wall / 1e6
+    log(f"served: {wall:.2f} s wall, {sppmpix:.1f} spp*Mpix/s, {mrays:.0f} Mrays/s, p50 "
+        f"{p50 * 1e3:.0f} ms, p99 {p99 * 1e3:.0f} ms, GPU busy {st['busySec'] / wall:.2f}, "
+        f"{st['rounds'] - base['rounds']} rounds, same as alone: {same}")
+    return {
+        "metric": METRIC, "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": round(wall * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"serving form: {n} resident soup scenes ({int(counts.sum())} "
+                               f"triangles), Poisson {args.rate}/s for {args.duration} s "
+                               f"({len(reqs)} requests), 128^2-1024^2, 1-64 spp, maxBounces "
+                               f"4-12, policy {args.policy}, {args.max_paths} paths per round",
+                   "requests": len(reqs), "parallelism": "single GPU"},
+        "spp_mpix_per_s": round(sppmpix, 3),
+        "latency_ms": {"p50": round(p50 * 1e3, 2), "p99": round(p99 * 1e3, 2),
+                       "max": round(max(lat) * 1e3, 2)},
+        "offered_spp_mpix_per_s": round(samples / args.duration / 1e6, 3),
+        "gpu_busy_frac": round(st["busySec"] / wall, 4),
+        "plain_one_at_a_time": {"s": round(plain_s, 3),
+                                "spp_mpix_per_s": round(samples / plain_s / 1e6, 3),
+                                "mrays_per_s": round(rays_plain / plain_s / 1e6, 2)},
+        "speedup_vs_plain": round(sppmpix / (samples / plain_s / 1e6), 3),
+        "same_as_alone": same, "roofline": None, "cpu_baseline": None,
+    }
 
 
 if __name__ == "__main__":

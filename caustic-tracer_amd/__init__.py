@@ -7,15 +7,16 @@ intersectClosest / occluded (intersect.ts), accumulatePass / renderImage
 HIP kernels through libcaustic_amd.so; importing works without a GPU, and any
 call that needs the library fails loudly when it is missing.
 """
-from . import synth
+from . import serve, synth
 from ._capi import CTError, DeviceBuffer, init, sync
 from .pack import PackedScene, buildBvh, packScene, primBounds
 from .tracer import (T_MIN, WALK_ORDERED, WALK_STACKLESS, WALK_WIDE, DeviceScene, Tracer, accumulatePass,
                      cameraFrame, closest_hit, closest_hit_counts, intersectClosest, occluded,
                      renderImage)
+from .serve import Server
 
 __all__ = [
-    "synth", "CTError", "DeviceBuffer", "init", "sync", "PackedScene", "buildBvh", "packScene",
+    "serve", "Server", "synth", "CTError", "DeviceBuffer", "init", "sync", "PackedScene", "buildBvh", "packScene",
     "primBounds", "T_MIN", "WALK_ORDERED", "WALK_STACKLESS", "WALK_WIDE", "DeviceScene", "Tracer",
     "accumulatePass", "cameraFrame", "closest_hit", "closest_hit_counts", "intersectClosest",
     "occluded", "renderImage",

@@ -53,6 +53,35 @@ class PassJob(C.Structure):
                 ("d_accum", C.c_void_p)]
 
 
+class TracerConfig(C.Structure):
+    _fields_ = [("maxPaths", C.c_uint64), ("walk", C.c_int32), ("shadowWalk", C.c_int32),
+                ("stackDepth", C.c_int32), ("persistent", C.c_int32), ("blocks", C.c_int32),
+                ("workgroup", C.c_int32)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("config", TracerConfig), ("ms", C.c_double), ("initialMs", C.c_double),
+                ("trials", C.c_uint32), ("fromCache", C.c_int32)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("hbmBudget", C.c_uint64), ("maxPaths", C.c_uint64), ("maxActive", C.c_uint32),
+                ("slicePasses", C.c_uint32), ("policy", C.c_int32), ("graph", C.c_int32)]
+
+
+class JobStatus(C.Structure):
+    _fields_ = [("state", C.c_int32), ("passesDone", C.c_uint32), ("spp", C.c_uint32),
+                ("submitSec", C.c_double), ("admitSec", C.c_double), ("doneSec", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("queued", C.c_uint64), ("active", C.c_uint64), ("rounds", C.c_uint64),
+                ("paths", C.c_uint64), ("extensionRays", C.c_uint64), ("shadowRays", C.c_uint64),
+                ("pixelSamples", C.c_uint64), ("hbmInUse", C.c_uint64), ("busySec", C.c_double),
+                ("nowSec", C.c_double)]
+
+
 # Every function of the C ABI with its argument types (restype is int status
 # unless listed in _RESTYPES).
 vp, u32, i32, f64, sz = C.c_void_p, C.c_uint32, C.c_int32, C.c_double, C.c_size_t
@@ -86,6 +115,18 @@ SIGNATURES = {
     "ct_tracer_set_capacity": [vp, C.c_uint64],
     "ct_tracer_set_graph": [vp, i32],
     "ct_bvh_build": [vp, vp, vp, u32, i32, vp, C.POINTER(u32), vp, vp],
+    "ct_tracer_set_workgroup": [vp, i32],
+    "ct_tracer_get_config": [vp, C.POINTER(TracerConfig)],
+    "ct_tracer_set_config": [vp, C.POINTER(TracerConfig)],
+    "ct_tracer_autotune": [vp, vp, u32, u32, C.c_char_p, C.POINTER(TuneResult), vp],
+    "ct_server_create": [C.POINTER(ServerOpts), C.POINTER(vp)],
+    "ct_server_destroy": [vp],
+    "ct_job_submit": [vp, vp, C.POINTER(C.c_uint64)],
+    "ct_job_poll": [vp, C.c_uint64, C.POINTER(JobStatus)],
+    "ct_job_wait": [vp, C.c_uint64, f64, C.POINTER(JobStatus)],
+    "ct_job_read": [vp, C.c_uint64, vp, i32],
+    "ct_job_release": [vp, C.c_uint64],
+    "ct_server_stats_get": [vp, C.POINTER(ServerStats)],
 }
 _RESTYPES = {"ct_last_error": C.c_char_p}
 

@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcaustic_amd.so")
-SOURCES = ["ct_runtime.hip", "ct_trace.hip", "ct_render.hip", "ct_bvh.hip"]
+SOURCES = ["ct_runtime.hip", "ct_trace.hip", "ct_render.hip", "ct_bvh.hip", "ct_tune.hip",
+           "ct_serve.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -1,0 +1,48 @@
+// The render engine's host interface shared by the translation units that
+// drive it (the C-ABI entry points, the autotuner and the serving loop).
+#pragma once
+
+#include <vector>
+
+#include "ct_internal.h"
+
+namespace ct {
+
+// Camera frame of camera.ts:8-22 (binary64).
+struct Frame {
+  double o[3], r[3], u[3], f[3];
+  double tanHalfFov, aspect, lensRadius, focusDistance;
+};
+
+struct TraceOpts {
+  int maxBounces, rr, nee;
+};
+
+// A job to render from pass `first` to `first + count`; `finish` multiplies
+// the sum by `scale` when the job's last pass of this plan is added
+// (renderImage's final 1/spp).
+struct PlanJob {
+  const ct_scene* scene;
+  Frame F;
+  TraceOpts opts;
+  uint32_t width, height;
+  uint32_t frameBase;  // frame index of pass 0
+  uint32_t first, count;
+  float* accum;
+  bool finish;
+  double scale;
+};
+
+Frame make_frame(const double* frame16);
+
+// Packs the jobs' passes into rounds of at most the tracer's path capacity
+// and traces them on `st`; returns when the last round is done.
+int run_plan(ct_tracer* T, std::vector<PlanJob> jobs, hipStream_t st);
+
+// Sets n floats to zero with a kernel (safe under graph capture).
+int zero_floats(float* p, size_t n, hipStream_t st);
+
+// Path capacity of a tracer (paths in flight per round).
+size_t tracer_capacity(const ct_tracer* T);
+
+}  // namespace ct

@@ -24,21 +24,12 @@
 #include <algorithm>
 #include <vector>
 
-#include "ct_internal.h"
+#include "ct_engine.h"
 
 namespace ct {
 
 constexpr int RBLOCK = 256;
 constexpr int MAX_BOUNCE_SLOTS = 256;  // counters per kind (bounces + 1 <= this)
-
-struct Frame {
-  double o[3], r[3], u[3], f[3];
-  double tanHalfFov, aspect, lensRadius, focusDistance;
-};
-
-struct TraceOpts {
-  int maxBounces, rr, nee;
-};
 
 // One job of a launch round (device copy).
 struct JobDev {
@@ -609,17 +600,17 @@ __device__ __forceinline__ uint32_t refill(bool need, uint32_t n, uint32_t* fetc
 // loop, exact mirror), CT_WALK_ORDERED (binary nodes) or CT_WALK_WIDE
 // (derived 4-wide nodes); the ordered forms park leaves until the whole wave
 // holds one (closest_wave) and keep bf16 lower bounds of deferred entries.
-template <int WALK, int STACK, bool COUNT, bool MULTI>
-__global__ void __launch_bounds__(RBLOCK)
+template <int WALK, int STACK, bool COUNT, bool MULTI, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
 k_extend(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__ q,
          const uint32_t* __restrict__ count, uint32_t* __restrict__ fetch) {
   constexpr bool WIDE = WALK == CT_WALK_WIDE;
   constexpr int SD = WALK == CT_WALK_STACKLESS ? 1 : STACK;
-  __shared__ int sNodeAll[SD * RBLOCK];
-  __shared__ uint16_t sEntryAll[SD * RBLOCK];
+  __shared__ int sNodeAll[SD * BLOCK];
+  __shared__ uint16_t sEntryAll[SD * BLOCK];
   int* sNode = sNodeAll + threadIdx.x;
   uint16_t* sEntry = sEntryAll + threadIdx.x;
-  const int lanes = RBLOCK;
+  const int lanes = BLOCK;
   const uint32_t n = *count;
   TraversalCounts cnt = {0, 0, 0, 0};
   bool have = false, exhausted = false;
@@ -729,15 +720,15 @@ __device__ __forceinline__ bool leaf_blocks(const DevScene& S, int leafWord, dou
 // Persistent any hit of the NEE rays; adds the pending term when unoccluded
 // (tracer.ts:268-282).  Whether (tMin, tMax) is blocked does not depend on
 // the visiting order, so every walk returns the reference's answer.
-template <int WALK, int STACK, bool COUNT, bool MULTI>
-__global__ void __launch_bounds__(RBLOCK)
+template <int WALK, int STACK, bool COUNT, bool MULTI, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
 k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__ count,
          uint32_t* __restrict__ fetch) {
   constexpr bool WIDE = WALK == CT_WALK_WIDE;
   constexpr int SD = WALK == CT_WALK_STACKLESS ? 1 : STACK;
-  __shared__ int sNodeAll[SD * RBLOCK];
+  __shared__ int sNodeAll[SD * BLOCK];
   int* sNode = sNodeAll + threadIdx.x;
-  const int lanes = RBLOCK;
+  const int lanes = BLOCK;
   const uint32_t n = *count;
   TraversalCounts cnt = {0, 0, 0, 0};
   bool have = false, exhausted = false, blocked = false;
@@ -876,6 +867,7 @@ struct ct_tracer {
   int shadowWalk = CT_WALK_WIDE;     // any-hit walk of NEE rays
   int stack = 24;                    // LDS stack entries of the ordered walks
   unsigned persistBlocks = 256 * 8;  // workgroups of the traversal launches (0: one lane per ray)
+  int block = 256;                   // lanes per workgroup of the traversal launches
   size_t maxPaths = size_t(1) << 25; // paths in flight per round (~190 B of state each)
   bool graph = false;                // capture rounds into a hipGraph and replay them
   hipStream_t ownStream = nullptr;   // capture stream
@@ -951,7 +943,7 @@ static int jobs_reserve(ct_tracer* T, size_t n) {
   return CT_OK;
 }
 
-static Frame make_frame(const double* f) {
+Frame ct::make_frame(const double* f) {
   Frame F;
   for (int k = 0; k < 3; k++) {
     F.o[k] = f[k];
@@ -985,45 +977,55 @@ static void mark(ct_tracer* T, hipStream_t st, int kind, bool begin) {
 // 6 B/entry for closest hit, 4 B/entry for any hit).
 static bool stack_supported(int d) { return d == 8 || d == 16 || d == 24; }
 
+template <int WALK, int STACK, bool COUNT, bool MULTI, int BLOCK>
+static void launch_one(unsigned blocks, size_t paths, hipStream_t st, const PathSoA& P,
+                       const JobDev* jobs, const uint32_t* q, const uint32_t* c, uint32_t* f,
+                       const uint32_t* sc, uint32_t* sf, bool shadowPhase) {
+  unsigned grid = grid_for(paths, BLOCK, blocks ? blocks : 1u << 30);
+  if (!shadowPhase)
+    hipLaunchKernelGGL((k_extend<WALK, STACK, COUNT, MULTI, BLOCK>), dim3(grid), dim3(BLOCK), 0,
+                       st, P, jobs, q, c, f);
+  else
+    hipLaunchKernelGGL((k_shadow<WALK, STACK, COUNT, MULTI, BLOCK>), dim3(grid), dim3(BLOCK), 0,
+                       st, P, jobs, sc, sf);
+}
+
+// Work-counting launches always use 256-lane workgroups (the counts do not
+// depend on the workgroup shape).
 template <int WALK, int STACK, bool MULTI>
-static void launch_trace(bool counting, unsigned grid, hipStream_t st, const PathSoA& P,
-                         const JobDev* jobs, const uint32_t* q, const uint32_t* c, uint32_t* f,
-                         const uint32_t* sc, uint32_t* sf, bool shadowPhase) {
-  if (!shadowPhase) {
-    if (counting)
-      hipLaunchKernelGGL((k_extend<WALK, STACK, true, MULTI>), dim3(grid), dim3(RBLOCK), 0, st, P, jobs, q, c, f);
-    else
-      hipLaunchKernelGGL((k_extend<WALK, STACK, false, MULTI>), dim3(grid), dim3(RBLOCK), 0, st, P, jobs, q, c, f);
-  } else {
-    if (counting)
-      hipLaunchKernelGGL((k_shadow<WALK, STACK, true, MULTI>), dim3(grid), dim3(RBLOCK), 0, st, P, jobs, sc, sf);
-    else
-      hipLaunchKernelGGL((k_shadow<WALK, STACK, false, MULTI>), dim3(grid), dim3(RBLOCK), 0, st, P, jobs, sc, sf);
-  }
+static void launch_trace(bool counting, int block, unsigned blocks, size_t paths, hipStream_t st,
+                         const PathSoA& P, const JobDev* jobs, const uint32_t* q,
+                         const uint32_t* c, uint32_t* f, const uint32_t* sc, uint32_t* sf,
+                         bool shadowPhase) {
+  if (counting)
+    launch_one<WALK, STACK, true, MULTI, 256>(blocks, paths, st, P, jobs, q, c, f, sc, sf, shadowPhase);
+  else if (block == 64)
+    launch_one<WALK, STACK, false, MULTI, 64>(blocks, paths, st, P, jobs, q, c, f, sc, sf, shadowPhase);
+  else if (block == 128)
+    launch_one<WALK, STACK, false, MULTI, 128>(blocks, paths, st, P, jobs, q, c, f, sc, sf, shadowPhase);
+  else
+    launch_one<WALK, STACK, false, MULTI, 256>(blocks, paths, st, P, jobs, q, c, f, sc, sf, shadowPhase);
 }
 
 template <bool MULTI>
-static void dispatch_trace(int walk, int stack, bool counting, unsigned grid, hipStream_t st,
-                           const PathSoA& P, const JobDev* jobs, const uint32_t* q,
-                           const uint32_t* c, uint32_t* f, const uint32_t* sc, uint32_t* sf,
-                           bool shadowPhase) {
+static void dispatch_trace(int walk, int stack, bool counting, int block, unsigned blocks,
+                           size_t paths, hipStream_t st, const PathSoA& P, const JobDev* jobs,
+                           const uint32_t* q, const uint32_t* c, uint32_t* f, const uint32_t* sc,
+                           uint32_t* sf, bool shadowPhase) {
+#define CT_TRACE(W, D) \
+  launch_trace<W, D, MULTI>(counting, block, blocks, paths, st, P, jobs, q, c, f, sc, sf, shadowPhase)
   if (walk == CT_WALK_STACKLESS) {
-    launch_trace<CT_WALK_STACKLESS, 1, MULTI>(counting, grid, st, P, jobs, q, c, f, sc, sf, shadowPhase);
+    CT_TRACE(CT_WALK_STACKLESS, 1);
   } else if (walk == CT_WALK_WIDE) {
-    if (stack == 8)
-      launch_trace<CT_WALK_WIDE, 8, MULTI>(counting, grid, st, P, jobs, q, c, f, sc, sf, shadowPhase);
-    else if (stack == 16)
-      launch_trace<CT_WALK_WIDE, 16, MULTI>(counting, grid, st, P, jobs, q, c, f, sc, sf, shadowPhase);
-    else
-      launch_trace<CT_WALK_WIDE, 24, MULTI>(counting, grid, st, P, jobs, q, c, f, sc, sf, shadowPhase);
+    if (stack == 8) CT_TRACE(CT_WALK_WIDE, 8);
+    else if (stack == 16) CT_TRACE(CT_WALK_WIDE, 16);
+    else CT_TRACE(CT_WALK_WIDE, 24);
   } else {
-    if (stack == 8)
-      launch_trace<CT_WALK_ORDERED, 8, MULTI>(counting, grid, st, P, jobs, q, c, f, sc, sf, shadowPhase);
-    else if (stack == 16)
-      launch_trace<CT_WALK_ORDERED, 16, MULTI>(counting, grid, st, P, jobs, q, c, f, sc, sf, shadowPhase);
-    else
-      launch_trace<CT_WALK_ORDERED, 24, MULTI>(counting, grid, st, P, jobs, q, c, f, sc, sf, shadowPhase);
+    if (stack == 8) CT_TRACE(CT_WALK_ORDERED, 8);
+    else if (stack == 16) CT_TRACE(CT_WALK_ORDERED, 16);
+    else CT_TRACE(CT_WALK_ORDERED, 24);
   }
+#undef CT_TRACE
 }
 
 // One job's share of a launch round.
@@ -1051,8 +1053,6 @@ static int launch_round(ct_tracer* T, int njobs, size_t paths, size_t pix, int n
   hipLaunchKernelGGL((k_generate<MULTI>), dim3(gridPaths), dim3(RBLOCK), 0, st, P, jobs, njobs,
                      paths);
   CT_LAUNCH_CHECK("k_generate");
-  unsigned gridTrace =
-      T->persistBlocks ? grid_for(paths, RBLOCK, T->persistBlocks) : grid_for(paths, RBLOCK, 1u << 30);
   for (int b = 0; b < nb; b++) {
     uint32_t* qin = (b & 1) ? P.q1 : P.q0;
     uint32_t* qout = (b & 1) ? P.q0 : P.q1;
@@ -1062,8 +1062,8 @@ static int launch_round(ct_tracer* T, int njobs, size_t paths, size_t pix, int n
     uint32_t* fExt = P.ctr + 2 * MAX_BOUNCE_SLOTS + b;
     uint32_t* fSh = P.ctr + 3 * MAX_BOUNCE_SLOTS + b;
     mark(T, st, 0, true);
-    dispatch_trace<MULTI>(T->walk, T->stack, T->counting, gridTrace, st, P, jobs, qin, cin, fExt,
-                          scount, fSh, false);
+    dispatch_trace<MULTI>(T->walk, T->stack, T->counting, T->block, T->persistBlocks, paths, st,
+                          P, jobs, qin, cin, fExt, scount, fSh, false);
     CT_LAUNCH_CHECK("k_extend");
     mark(T, st, 0, false);
     mark(T, st, 1, true);
@@ -1072,8 +1072,8 @@ static int launch_round(ct_tracer* T, int njobs, size_t paths, size_t pix, int n
     CT_LAUNCH_CHECK("k_shade");
     mark(T, st, 1, false);
     mark(T, st, 2, true);
-    dispatch_trace<MULTI>(T->shadowWalk, T->stack, T->counting, gridTrace, st, P, jobs, qin, cin,
-                          fExt, scount, fSh, true);
+    dispatch_trace<MULTI>(T->shadowWalk, T->stack, T->counting, T->block, T->persistBlocks,
+                          paths, st, P, jobs, qin, cin, fExt, scount, fSh, true);
     CT_LAUNCH_CHECK("k_shadow");
     mark(T, st, 2, false);
   }
@@ -1133,10 +1133,10 @@ static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t s
   if (T->graph) {
     // the captured launches depend only on the round's shape and the tracer's
     // settings; the job table contents are read from device memory at replay
-    std::vector<char> key(sizeof(size_t) * 2 + sizeof(int) * 8);
+    std::vector<char> key(sizeof(size_t) * 2 + sizeof(int) * 9);
     size_t kk[2] = {paths, pix};
-    int ki[8] = {nb, (int)R.size(), T->walk, T->shadowWalk, T->stack, (int)T->persistBlocks,
-                 T->counting, multi};
+    int ki[9] = {nb, (int)R.size(), T->walk, T->shadowWalk, T->stack, (int)T->persistBlocks,
+                 T->counting, multi, T->block};
     memcpy(key.data(), kk, sizeof kk);
     memcpy(key.data() + sizeof kk, ki, sizeof ki);
     if (!T->graphExec || key != T->graphKey) {
@@ -1179,23 +1179,9 @@ static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t s
   return CT_OK;
 }
 
-// A job to render from `first` to `first + count` passes; `finish` scales by
-// 1/total when its last pass is added (renderImage).
-struct PlanJob {
-  const ct_scene* scene;
-  Frame F;
-  TraceOpts opts;
-  uint32_t width, height;
-  uint32_t frameBase;  // frame index of pass 0
-  uint32_t first, count;
-  float* accum;
-  bool finish;
-  double scale;
-};
-
 // Packs jobs into rounds of at most maxPaths paths (whole passes per job) and
 // traces them; returns the totals.
-static int run_plan(ct_tracer* T, std::vector<PlanJob> jobs, hipStream_t st) {
+int ct::run_plan(ct_tracer* T, std::vector<PlanJob> jobs, hipStream_t st) {
   clear_events(T);
   T->lastStream = st;
   T->lastBounces = 0;
@@ -1251,6 +1237,15 @@ static int run_plan(ct_tracer* T, std::vector<PlanJob> jobs, hipStream_t st) {
   T->lastPaths = paths;
   return CT_OK;
 }
+
+int ct::zero_floats(float* p, size_t n, hipStream_t st) {
+  if (n == 0) return CT_OK;
+  hipLaunchKernelGGL(k_zero, dim3(grid_for(n, RBLOCK, 256 * 16)), dim3(RBLOCK), 0, st, p, n);
+  CT_LAUNCH_CHECK("k_zero");
+  return CT_OK;
+}
+
+size_t ct::tracer_capacity(const ct_tracer* T) { return T->maxPaths; }
 
 static int check_job(const ct_scene* s, const double* frame, uint32_t w, uint32_t h,
                      int32_t maxBounces, const void* out) {
@@ -1311,6 +1306,48 @@ int ct_tracer_set_schedule(ct_tracer* T, int32_t persistent, int32_t blocks) {
   if (persistent && (blocks < 1 || blocks > 65535))
     return set_error(CT_EINVAL, "ct_tracer_set_schedule: blocks must be in [1, 65535]");
   T->persistBlocks = persistent ? (unsigned)blocks : 0u;
+  return CT_OK;
+}
+
+int ct_tracer_set_workgroup(ct_tracer* T, int32_t lanes) {
+  if (!T) return set_error(CT_EINVAL, "null tracer");
+  if (lanes != 64 && lanes != 128 && lanes != 256)
+    return set_error(CT_EINVAL, "ct_tracer_set_workgroup: lanes must be 64, 128 or 256");
+  T->block = lanes;
+  return CT_OK;
+}
+
+int ct_tracer_get_config(const ct_tracer* T, ct_tracer_config* out) {
+  if (!T || !out) return set_error(CT_EINVAL, "ct_tracer_get_config: null argument");
+  out->maxPaths = T->maxPaths;
+  out->walk = T->walk;
+  out->shadowWalk = T->shadowWalk;
+  out->stackDepth = T->stack;
+  out->persistent = T->persistBlocks ? 1 : 0;
+  out->blocks = T->persistBlocks ? (int32_t)T->persistBlocks : 2048;
+  out->workgroup = T->block;
+  return CT_OK;
+}
+
+int ct_tracer_set_config(ct_tracer* T, const ct_tracer_config* c) {
+  if (!T || !c) return set_error(CT_EINVAL, "ct_tracer_set_config: null argument");
+  ct_tracer_config old;
+  ct_tracer_get_config(T, &old);
+  int rc = ct_tracer_set_walk(T, c->walk);
+  if (!rc) rc = ct_tracer_set_traversal(T, c->shadowWalk, c->stackDepth);
+  if (!rc) rc = ct_tracer_set_schedule(T, c->persistent, c->blocks);
+  if (!rc) rc = ct_tracer_set_workgroup(T, c->workgroup);
+  if (!rc) rc = ct_tracer_set_capacity(T, c->maxPaths);
+  if (rc) {  // all or nothing
+    std::string msg = ct_last_error();
+    T->walk = old.walk;
+    T->shadowWalk = old.shadowWalk;
+    T->stack = old.stackDepth;
+    T->persistBlocks = old.persistent ? (unsigned)old.blocks : 0u;
+    T->block = old.workgroup;
+    T->maxPaths = old.maxPaths;
+    return set_error(rc, msg);
+  }
   return CT_OK;
 }
 

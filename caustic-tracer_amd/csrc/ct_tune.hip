@@ -1,0 +1,227 @@
+// Launch-setting autotuner for batches of render jobs.
+//
+// The settings it searches change how the work is scheduled, never what is
+// computed: every walk returns the same closest hit and the same occlusion
+// answer, and each pixel's samples are added in pass order whatever the
+// round boundaries are.  So the tuner times ct_render_batch on the caller's
+// own batch and keeps the fastest setting; the outputs it leaves behind are
+// the batch's images.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "ct_internal.h"
+
+namespace ct {
+namespace {
+
+// Upper bound of the path state the tracer keeps per path in flight (bytes).
+constexpr double PATH_BYTES = 200.0;
+
+int ceil_log2(uint64_t x) {
+  int b = 0;
+  while (b < 63 && (uint64_t(1) << b) < x) b++;
+  return b;
+}
+
+std::string signature(const ct_job* jobs, uint32_t n) {
+  int dev = 0;
+  hipDeviceProp_t prop;
+  memset(&prop, 0, sizeof prop);
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    strcpy(prop.gcnArchName, "unknown");
+  uint64_t paths = 0, prims = 0, maxPrims = 0;
+  int maxB = 0;
+  for (uint32_t k = 0; k < n; k++) {
+    paths += (uint64_t)jobs[k].width * jobs[k].height * jobs[k].spp;
+    uint64_t p = (uint64_t)jobs[k].scene->dev.primCount;
+    prims += p;
+    if (p > maxPrims) maxPrims = p;
+    if (jobs[k].maxBounces > maxB) maxB = jobs[k].maxBounces;
+  }
+  std::string arch = prop.gcnArchName;
+  for (char& c : arch)
+    if (c == ' ' || c == '|') c = '_';
+  char buf[256];
+  snprintf(buf, sizeof buf, "%s|cu%d|jobs%u|paths2^%d|prims2^%d|maxprims2^%d|depth%d",
+           arch.c_str(), prop.multiProcessorCount, n, ceil_log2(paths), ceil_log2(prims),
+           ceil_log2(maxPrims), maxB);
+  return buf;
+}
+
+// Cache line: "<signature> maxPaths walk shadowWalk stack persistent blocks workgroup ms".
+bool cache_lookup(const char* path, const std::string& sig, ct_tracer_config& c, double& ms) {
+  std::ifstream f(path);
+  if (!f) return false;
+  bool found = false;
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream is(line);
+    std::string s;
+    ct_tracer_config r;
+    unsigned long long mp = 0;
+    double m = 0;
+    if (!(is >> s >> mp >> r.walk >> r.shadowWalk >> r.stackDepth >> r.persistent >> r.blocks >>
+          r.workgroup >> m))
+      continue;
+    if (s != sig) continue;
+    r.maxPaths = mp;
+    c = r;
+    ms = m;
+    found = true;  // the last matching line wins
+  }
+  return found;
+}
+
+void cache_append(const char* path, const std::string& sig, const ct_tracer_config& c, double ms) {
+  FILE* f = fopen(path, "a");
+  if (!f) return;
+  fprintf(f, "%s %llu %d %d %d %d %d %d %.4f\n", sig.c_str(), (unsigned long long)c.maxPaths,
+          c.walk, c.shadowWalk, c.stackDepth, c.persistent, c.blocks, c.workgroup, ms);
+  fclose(f);
+}
+
+int time_batch(ct_tracer* T, const ct_job* jobs, uint32_t n, hipStream_t st, double& ms) {
+  CT_HIP(hipStreamSynchronize(st));
+  auto t0 = std::chrono::steady_clock::now();
+  int rc = ct_render_batch(T, jobs, n, st);
+  if (rc) return rc;
+  CT_HIP(hipStreamSynchronize(st));
+  ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return CT_OK;
+}
+
+bool same(const ct_tracer_config& a, const ct_tracer_config& b) {
+  return a.maxPaths == b.maxPaths && a.walk == b.walk && a.shadowWalk == b.shadowWalk &&
+         a.stackDepth == b.stackDepth && a.persistent == b.persistent && a.blocks == b.blocks &&
+         a.workgroup == b.workgroup;
+}
+
+}  // namespace
+}  // namespace ct
+
+using namespace ct;
+
+extern "C" int ct_tracer_autotune(ct_tracer* T, const ct_job* jobs, uint32_t n, uint32_t maxTrials,
+                                  const char* cachePath, ct_tune_result* out, void* stream) {
+  if (!T || !out || (n && !jobs)) return set_error(CT_EINVAL, "ct_tracer_autotune: null argument");
+  memset(out, 0, sizeof *out);
+  hipStream_t st = (hipStream_t)stream;
+  for (uint32_t k = 0; k < n; k++)
+    if (!jobs[k].scene) return set_error(CT_EINVAL, "ct_tracer_autotune: job without a scene");
+  ct_tracer_config init;
+  int rc = ct_tracer_get_config(T, &init);
+  if (rc) return rc;
+  const std::string sig = signature(jobs, n);
+
+  ct_tracer_config cached;
+  double cachedMs = 0;
+  if (cachePath && cache_lookup(cachePath, sig, cached, cachedMs)) {
+    rc = ct_tracer_set_config(T, &cached);
+    if (rc) return rc;
+    rc = ct_render_batch(T, jobs, n, st);  // outputs hold the images, as after a search
+    if (rc) return rc;
+    out->config = cached;
+    out->ms = cachedMs;
+    out->initialMs = NAN;
+    out->fromCache = 1;
+    return CT_OK;
+  }
+
+  uint64_t totalPaths = 0, maxPass = 1;
+  for (uint32_t k = 0; k < n; k++) {
+    uint64_t npix = (uint64_t)jobs[k].width * jobs[k].height;
+    totalPaths += npix * jobs[k].spp;
+    if (npix > maxPass) maxPass = npix;
+  }
+  size_t freeB = 0, totalB = 0;
+  CT_HIP(hipMemGetInfo(&freeB, &totalB));
+
+  // Candidate values per dimension.
+  std::vector<uint64_t> caps;
+  for (int e = 22; e <= 28; e += 2) {
+    uint64_t c = uint64_t(1) << e;
+    if (c < maxPass) continue;
+    bool oneRound = c >= totalPaths;
+    if (oneRound) c = std::max<uint64_t>(maxPass, (totalPaths + 1023) / 1024 * 1024);
+    c = std::max<uint64_t>(c, 1024);
+    bool fits = c <= init.maxPaths || c * PATH_BYTES <= 0.7 * (double)freeB;
+    if (fits && (caps.empty() || caps.back() != c)) caps.push_back(c);
+    if (oneRound) break;
+  }
+  struct Policy {
+    int walk, shadowWalk, stack;
+  };
+  const Policy policies[] = {{CT_WALK_WIDE, CT_WALK_WIDE, 24},       {CT_WALK_WIDE, CT_WALK_WIDE, 16},
+                             {CT_WALK_WIDE, CT_WALK_WIDE, 8},        {CT_WALK_ORDERED, CT_WALK_ORDERED, 16},
+                             {CT_WALK_ORDERED, CT_WALK_ORDERED, 24}, {CT_WALK_WIDE, CT_WALK_STACKLESS, 24},
+                             {CT_WALK_STACKLESS, CT_WALK_STACKLESS, 24}};
+  const int workgroups[] = {64, 128, 256};
+  const int lanesInFlight[] = {1 << 18, 1 << 19, 1 << 20};
+
+  // Time the entry setting twice: the first call also allocates the path
+  // state and loads the kernels.
+  ct_tracer_config cur = init;
+  double ms = 0, best = 0;
+  if (cur.maxPaths < maxPass) cur.maxPaths = caps.empty() ? maxPass : caps.front();
+  rc = ct_tracer_set_config(T, &cur);
+  if (rc) return rc;
+  rc = time_batch(T, jobs, n, st, ms);
+  if (rc) return rc;
+  rc = time_batch(T, jobs, n, st, best);
+  if (rc) return rc;
+  uint32_t trials = 1;
+  out->initialMs = best;
+
+  auto attempt = [&](ct_tracer_config cand) -> int {
+    if (same(cand, cur) || trials >= maxTrials) return CT_OK;
+    if (ct_tracer_set_config(T, &cand) != CT_OK) return CT_OK;  // not a valid setting: skip
+    double m = 0;
+    int r = time_batch(T, jobs, n, st, m);
+    if (r) return r;
+    trials++;
+    if (m < best * 0.99) {  // 1% margin against timing noise
+      best = m;
+      cur = cand;
+    }
+    return CT_OK;
+  };
+  for (uint64_t cap : caps) {
+    ct_tracer_config c = cur;
+    c.maxPaths = cap;
+    if ((rc = attempt(c))) return rc;
+  }
+  for (const Policy& p : policies) {
+    ct_tracer_config c = cur;
+    c.walk = p.walk;
+    c.shadowWalk = p.shadowWalk;
+    c.stackDepth = p.stack;
+    if ((rc = attempt(c))) return rc;
+  }
+  const int lanesNow = cur.persistent ? cur.blocks * cur.workgroup : 1 << 19;
+  for (int wg : workgroups) {
+    ct_tracer_config c = cur;
+    c.workgroup = wg;
+    c.persistent = 1;
+    c.blocks = std::max(1, lanesNow / wg);
+    if ((rc = attempt(c))) return rc;
+  }
+  for (int lanes : lanesInFlight) {
+    ct_tracer_config c = cur;
+    c.persistent = 1;
+    c.blocks = std::max(1, lanes / c.workgroup);
+    if ((rc = attempt(c))) return rc;
+  }
+  rc = ct_tracer_set_config(T, &cur);
+  if (rc) return rc;
+  out->config = cur;
+  out->ms = best;
+  out->trials = trials;
+  if (cachePath) cache_append(cachePath, sig, cur, best);
+  return CT_OK;
+}

@@ -81,6 +81,69 @@ def soup_scene(n_tris=1_000_000, n_lights=64, n_materials=1000, seed=2024, light
                     "strength": 0.5}}
 
 
+def triangle_counts(n, seed, lo, hi, median=20_000, sigma=1.0):
+    """n triangle counts, lognormal around `median` (log-sd `sigma`), clipped to [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    return np.clip(np.round(rng.lognormal(math.log(median), sigma, n)), lo, hi).astype(np.int64)
+
+
+def family_scene(i, n_tris):
+    """Scene i of a family: the soup generator with seed i, materials scaled to
+    the triangle count (about 100 triangles per mesh object, at most 1000)."""
+    return soup_scene(int(n_tris), n_lights=64, n_materials=int(min(1000, max(8, n_tris // 100))),
+                      seed=int(i))
+
+
+def orbit_camera(rng, dist_lo=20.0, dist_hi=30.0):
+    """Camera on a sphere around the soup's centre, looking at it."""
+    yaw = float(rng.uniform(-math.pi, math.pi))
+    pitch = float(rng.uniform(-0.6, 0.6))
+    dist = float(rng.uniform(dist_lo, dist_hi))
+    return {"target": (0.0, 0.0, 0.0), "yaw": yaw, "pitch": pitch, "distance": dist,
+            "fovDeg": 45.0, "lensRadius": 0.0, "focusDistance": dist}
+
+
+def batch_workload(n_scenes=64, seed=4, lo=1_000, hi=500_000):
+    """Batching form: n_scenes scenes with lognormal triangle counts in [lo, hi]
+    (median 20k), scene i generated from seed i; per scene one job with a
+    resolution drawn from {128, 256, 512, 768}^2, spp uniform in 1..32 and
+    maxBounces 8.  Returns (scenes, jobs) with jobs as dicts of width, height,
+    spp, seed, maxBounces and the index of their scene."""
+    counts = triangle_counts(n_scenes, seed, lo, hi)
+    rng = np.random.default_rng(seed + 1)
+    scenes, jobs = [], []
+    for i, n in enumerate(counts):
+        scenes.append(family_scene(i, n))
+        side = int(rng.choice([128, 256, 512, 768]))
+        jobs.append({"sceneIndex": i, "width": side, "height": side,
+                     "spp": int(rng.integers(1, 33)), "seed": i + 1, "maxBounces": 8,
+                     "russianRoulette": True, "nee": True})
+    return scenes, jobs
+
+
+def request_stream(n_scenes=128, rate=200.0, duration=60.0, seed=5, res_lo=128, res_hi=1024,
+                   spp_hi=64, depth_lo=4, depth_hi=12):
+    """Serving form: Poisson arrivals at `rate` per second for `duration` seconds;
+    each request names one of n_scenes resident scenes, a random orbit camera,
+    a square resolution that is a multiple of 128 in [res_lo, res_hi], spp
+    uniform in 1..spp_hi and maxBounces uniform in [depth_lo, depth_hi].
+    Returns a list of dicts with the arrival time `at` in seconds."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    sides = np.arange(res_lo, res_hi + 1, 128)
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t >= duration:
+            break
+        out.append({"at": t, "sceneIndex": int(rng.integers(0, n_scenes)),
+                    "camera": orbit_camera(rng), "width": int(rng.choice(sides)),
+                    "spp": int(rng.integers(1, spp_hi + 1)),
+                    "maxBounces": int(rng.integers(depth_lo, depth_hi + 1)),
+                    "seed": int(rng.integers(1, 1 << 30)), "russianRoulette": True, "nee": True})
+        out[-1]["height"] = out[-1]["width"]
+    return out
+
+
 def random_rays(rng_stream, n, lo=-14.0, hi=14.0, tlo=-8.0, thi=8.0):
     """Rays in the shape of bvh.test.ts:94-99 from a stream of uniforms in [0,1):
     origins in [lo,hi]^3 aimed at points of [tlo,thi]^3, unit binary64 directions."""

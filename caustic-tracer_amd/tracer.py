@@ -7,6 +7,7 @@ matCount, lights, primObject, sky) -- the arrays the reference's packer emits.
 """
 import ctypes as C
 import math
+import os
 
 import numpy as np
 
@@ -219,6 +220,32 @@ class Tracer:
     def set_graph(self, on):
         check(lib().ct_tracer_set_graph(self.handle, int(bool(on))), "set_graph")
 
+    def set_workgroup(self, lanes):
+        check(lib().ct_tracer_set_workgroup(self.handle, int(lanes)), "set_workgroup")
+
+    def get_config(self):
+        c = A.TracerConfig()
+        check(lib().ct_tracer_get_config(self.handle, C.byref(c)), "ct_tracer_get_config")
+        return {k: getattr(c, k) for k, _ in A.TracerConfig._fields_}
+
+    def set_config(self, cfg):
+        c = A.TracerConfig(**{k: int(v) for k, v in cfg.items()})
+        check(lib().ct_tracer_set_config(self.handle, C.byref(c)), "ct_tracer_set_config")
+
+    def autotune(self, jobs, max_trials=24, cache_path=None, stream=None):
+        """Times render_batch(jobs) under candidate launch settings, applies the
+        fastest and returns {config, ms, initialMs, trials, fromCache}; the jobs'
+        outputs hold their images afterwards."""
+        arr = self._jobs(jobs)
+        r = A.TuneResult()
+        path = None if cache_path is None else os.fsencode(cache_path)
+        check(lib().ct_tracer_autotune(self.handle, C.cast(arr, C.c_void_p), len(jobs),
+                                       int(max_trials), path, C.byref(r), stream),
+              "ct_tracer_autotune")
+        cfg = {k: getattr(r.config, k) for k, _ in A.TracerConfig._fields_}
+        return {"config": cfg, "ms": r.ms, "initialMs": r.initialMs, "trials": r.trials,
+                "fromCache": bool(r.fromCache)}
+
     @staticmethod
     def _frame16(frame):
         return (C.c_double * 16)(*[float(x) for x in frame])
@@ -227,6 +254,11 @@ class Tracer:
         """jobs: list of dicts {scene (PackedScene/DeviceScene), frame, width, height, spp,
         seed, maxBounces, russianRoulette, nee, out (DeviceBuffer)} -- one renderImage each,
         all traced in shared launches."""
+        arr = self._jobs(jobs)
+        check(lib().ct_render_batch(self.handle, C.cast(arr, C.c_void_p), len(jobs), stream),
+              "ct_render_batch")
+
+    def _jobs(self, jobs):
         arr = (A.Job * len(jobs))()
         for k, j in enumerate(jobs):
             arr[k].scene = device_scene(j["scene"]).handle.value
@@ -237,8 +269,7 @@ class Tracer:
             arr[k].russianRoulette = int(bool(j.get("russianRoulette", True)))
             arr[k].nee = int(bool(j.get("nee", True)))
             arr[k].d_out = j["out"].ptr.value
-        check(lib().ct_render_batch(self.handle, C.cast(arr, C.c_void_p), len(jobs), stream),
-              "ct_render_batch")
+        return arr
 
     def accumulate_batch(self, jobs, stream=None):
         """jobs: list of dicts {scene, frame, width, height, frameIndex, passes, maxBounces,

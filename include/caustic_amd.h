@@ -36,9 +36,11 @@ extern "C" {
 #define CT_ENOMEM -2  /* device allocation failed */
 #define CT_EDEVICE -3 /* HIP runtime or launch error */
 #define CT_ELIMIT -4  /* input beyond a documented size limit */
+#define CT_ETIMEOUT -5 /* a wait ended before the job did */
 
 typedef struct ct_scene ct_scene;
 typedef struct ct_tracer ct_tracer;
+typedef struct ct_server ct_server;
 
 /* Sky (scene.ts:57-59): kind 0 none, 1 gradient. */
 typedef struct {
@@ -174,7 +176,7 @@ int ct_tracer_set_walk(ct_tracer *tracer, int32_t walk);
  * deeper rays finish with the stackless walk, same result). */
 int ct_tracer_set_traversal(ct_tracer *tracer, int32_t shadowWalk, int32_t stackDepth);
 /* Launch schedule of the traversal kernels: persistent = 1 runs `blocks`
- * workgroups of 256 lanes that pull rays from the queue as lanes free up;
+ * workgroups that pull rays from the queue as lanes free up;
  * 0 launches one lane per path.  Results do not depend on it. */
 int ct_tracer_set_schedule(ct_tracer *tracer, int32_t persistent, int32_t blocks);
 /* accumulatePass (tracer.ts:361-395): adds one sample per pixel into the
@@ -198,8 +200,112 @@ int ct_tracer_set_capacity(ct_tracer *tracer, uint64_t maxPaths);
 /* Capture each round's launches into a hipGraph and replay it while the
  * round's shape (paths, pixels, jobs, bounces, settings) repeats. */
 int ct_tracer_set_graph(ct_tracer *tracer, int32_t enable);
+/* Lanes per workgroup of the traversal launches: 64, 128 or 256 (default). */
+int ct_tracer_set_workgroup(ct_tracer *tracer, int32_t lanes);
 /* Counters and device times of the last accumulate/render (synchronises). */
 int ct_tracer_stats(ct_tracer *tracer, ct_render_stats *out);
+
+/* Every launch setting of a tracer in one record (none changes any result). */
+typedef struct ct_tracer_config {
+  uint64_t maxPaths;  /* paths in flight per round */
+  int32_t walk;       /* closest-hit walk of extension rays (CT_WALK_*) */
+  int32_t shadowWalk; /* any-hit walk of NEE rays (CT_WALK_*) */
+  int32_t stackDepth; /* LDS stack entries of the ordered walks: 8, 16, 24 */
+  int32_t persistent; /* 1: persistent traversal launches pulling from the queue */
+  int32_t blocks;     /* workgroups of the persistent launches */
+  int32_t workgroup;  /* lanes per traversal workgroup: 64, 128, 256 */
+} ct_tracer_config;
+int ct_tracer_get_config(const ct_tracer *tracer, ct_tracer_config *out);
+int ct_tracer_set_config(ct_tracer *tracer, const ct_tracer_config *config);
+
+/* ---- autotuner --------------------------------------------------------------
+ * Picks the tracer's launch settings for a batch of jobs by timing
+ * ct_render_batch on the batch itself: coordinate descent over the queue
+ * capacity (paths per round), the traversal policy (ordered walk with an LDS
+ * stack of 8/16/24 entries that restarts a deep ray with the stackless walk,
+ * or the stackless walk alone; binary or 4-wide nodes), the workgroup size
+ * (64/128/256 lanes) and the lanes kept in flight by the persistent launches.
+ * Every trial renders the jobs, so on return the outputs hold the batch's
+ * images (identical under every setting).  The best setting is applied to
+ * the tracer.  With cachePath set, a file of earlier results keyed by the
+ * batch's signature (device, job count, path and primitive counts, depth) is
+ * read first; a hit applies it without trials, a miss appends the result.
+ * No counterpart in the reference (it has one configuration). */
+typedef struct ct_tune_result {
+  ct_tracer_config config; /* chosen setting (applied) */
+  double ms;               /* its time for one ct_render_batch of the jobs */
+  double initialMs;        /* time of the setting the tracer had on entry */
+  uint32_t trials;         /* settings timed (0 when taken from the cache) */
+  int32_t fromCache;
+} ct_tune_result;
+int ct_tracer_autotune(ct_tracer *tracer, const ct_job *jobs, uint32_t n, uint32_t maxTrials,
+                       const char *cachePath, ct_tune_result *out, void *stream);
+
+/* ---- serving -------------------------------------------------------------
+ * A request loop over resident scenes: render jobs (ct_job, d_out ignored)
+ * are submitted as they arrive and traced by the server's own thread with
+ * continuous batching -- every launch round takes passes from all admitted
+ * jobs, and jobs that arrive meanwhile join at the next round once a slot and
+ * HBM for their progressive buffer are free.  The progressive buffer (the
+ * job's float32 sum, width*height*3) stays on the device until released;
+ * when the last pass is added it holds renderImage's result (the sum times
+ * 1/spp), bit-identical to ct_render_image of the job alone.  Scenes must
+ * outlive the jobs that use them.  All functions may be called from any
+ * thread.  The reference has no serving layer; its boundary (renderImage)
+ * is what each job computes. */
+#define CT_JOB_QUEUED 0  /* waiting for a slot or HBM */
+#define CT_JOB_RUNNING 1 /* admitted; passes in progress */
+#define CT_JOB_DONE 2    /* buffer holds the final image */
+#define CT_JOB_FAILED 3  /* device error (see ct_last_error after ct_job_poll) */
+
+#define CT_SERVE_FIFO 0 /* rounds filled in admission order */
+#define CT_SERVE_SRPT 1 /* rounds filled shortest remaining work first */
+
+typedef struct ct_server_opts {
+  uint64_t hbmBudget;   /* bytes of progressive buffers resident at once (0: 4 GiB) */
+  uint64_t maxPaths;    /* paths per launch round (0: the tracer default, 2^25) */
+  uint32_t maxActive;   /* admitted jobs at most (0: 1024) */
+  uint32_t slicePasses; /* passes one job adds per round at most (0: as many as fit) */
+  int32_t policy;       /* CT_SERVE_FIFO or CT_SERVE_SRPT */
+  int32_t graph;        /* replay rounds of a repeated shape as hipGraphs */
+} ct_server_opts;
+
+typedef struct ct_job_status {
+  int32_t state;        /* CT_JOB_* */
+  uint32_t passesDone;  /* passes added to the buffer */
+  uint32_t spp;
+  double submitSec;     /* server clock (seconds since ct_server_create) */
+  double admitSec;      /* -1 while queued */
+  double doneSec;       /* -1 until done */
+} ct_job_status;
+
+typedef struct ct_server_stats {
+  uint64_t submitted, completed, failed;
+  uint64_t queued, active;       /* now */
+  uint64_t rounds;               /* launch rounds traced */
+  uint64_t paths;                /* camera paths traced */
+  uint64_t extensionRays, shadowRays;
+  uint64_t pixelSamples;         /* width*height*passes summed over rounds */
+  uint64_t hbmInUse;             /* bytes of progressive buffers held */
+  double busySec;                /* time spent inside launch rounds */
+  double nowSec;                 /* server clock */
+} ct_server_stats;
+
+int ct_server_create(const ct_server_opts *opts, ct_server **out);
+/* Stops the loop after the round in flight; unfinished jobs are dropped. */
+int ct_server_destroy(ct_server *server);
+int ct_job_submit(ct_server *server, const ct_job *job, uint64_t *id);
+int ct_job_poll(ct_server *server, uint64_t id, ct_job_status *status);
+/* Blocks until the job is done or failed, or timeoutSec passes (CT_ETIMEOUT). */
+int ct_job_wait(ct_server *server, uint64_t id, double timeoutSec, ct_job_status *status);
+/* Copies the job's buffer (width*height*3 float32): the final image once
+ * done, the running sum of passesDone passes before.  dstOnDevice selects a
+ * device or host destination. */
+int ct_job_read(ct_server *server, uint64_t id, float *dst, int32_t dstOnDevice);
+/* Frees the job's buffer and forgets it; a queued job is cancelled.  A
+ * running job is released after its round. */
+int ct_job_release(ct_server *server, uint64_t id);
+int ct_server_stats_get(ct_server *server, ct_server_stats *out);
 
 /* ---- BVH build --------------------------------------------------------------
  * buildBvh (bvh.ts:299-301) on the GPU from device-resident PrimBounds

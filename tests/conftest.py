@@ -40,3 +40,9 @@ def load_package():
 @pytest.fixture(scope="session")
 def ct():
     return load_package()
+
+
+def oracle_packed(O, ps):
+    """The oracle's view of a product PackedScene (same arrays)."""
+    return O.Packed(ps.prims, ps.nodes, ps.nodeCount, ps.mats, ps.lights, ps.primObject, ps.order,
+                    ps.sky)

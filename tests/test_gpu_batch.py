@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import scenes
+from conftest import oracle_packed
 
 pytestmark = pytest.mark.gpu
 
@@ -109,3 +110,79 @@ def test_graph_replays_equal_eager(ct, O):
             assert np.array_equal(r.view(np.uint32), g.view(np.uint32)), rep
         # eager work between replays
         eager.render_batch(jobs[:1])
+
+
+def _family_jobs(ct):
+    """Five small scenes of the batching family (soups from seeds 0..4) with
+    uneven sizes, resolutions and sample counts."""
+    out = []
+    for i, (n, side, spp) in enumerate(((1000, 32, 1), (1500, 48, 2), (3000, 24, 3),
+                                        (1200, 40, 2), (2500, 16, 4))):
+        sc = ct.synth.family_scene(i, n)
+        out.append({"scene": ct.packScene(sc), "frame": ct.cameraFrame(sc["camera"], 1.0),
+                    "width": side, "height": side, "spp": spp, "seed": i + 1, "maxBounces": 8})
+    for j in out:
+        j["out"] = _alloc(ct, j)
+    return out
+
+
+def _images(jobs):
+    return [j["out"].numpy().view(np.uint32) for j in jobs]
+
+
+@pytest.mark.parametrize("setting", [
+    {"workgroup": 64}, {"workgroup": 128}, {"walk": 1, "shadowWalk": 1, "stackDepth": 8},
+    {"walk": 0, "shadowWalk": 0}, {"walk": 2, "shadowWalk": 0, "stackDepth": 16},
+    {"persistent": 0}, {"maxPaths": 4096}, {"blocks": 64, "workgroup": 64}])
+def test_launch_settings_do_not_change_images(ct, O, setting):
+    jobs = _family_jobs(ct)
+    ref_tr = ct.Tracer()
+    ref_tr.render_batch(jobs)
+    ref = _images(jobs)
+    tr = ct.Tracer()
+    cfg = tr.get_config()
+    cfg.update(setting)
+    tr.set_config(cfg)
+    assert tr.get_config() == cfg
+    for j in jobs:
+        j["out"].upload(np.full(j["out"].shape, np.nan, np.float32))
+    tr.render_batch(jobs)
+    for r, g in zip(ref, _images(jobs)):
+        assert np.array_equal(r, g)
+    # and the oracle agrees with one of them
+    j = jobs[2]
+    o = O.render_image(oracle_packed(O, j["scene"]), j["frame"], j["width"], j["height"], j["spp"], seed=j["seed"],
+                       max_bounces=j["maxBounces"])
+    assert rel_err(j["out"].numpy(), o).max() <= 1e-4
+
+
+def test_set_config_is_all_or_nothing(ct):
+    tr = ct.Tracer()
+    before = tr.get_config()
+    bad = dict(before, workgroup=96, walk=0)
+    with pytest.raises(ct.CTError):
+        tr.set_config(bad)
+    assert tr.get_config() == before
+
+
+def test_autotune_picks_a_setting_and_caches_it(ct, tmp_path):
+    jobs = _family_jobs(ct)
+    ct.Tracer().render_batch(jobs)
+    ref = _images(jobs)
+    cache = tmp_path / "tune.txt"
+    tr = ct.Tracer()
+    res = tr.autotune(jobs, max_trials=6, cache_path=str(cache))
+    assert not res["fromCache"] and 1 <= res["trials"] <= 6
+    assert res["ms"] > 0 and res["ms"] <= res["initialMs"]
+    assert tr.get_config() == res["config"]
+    for r, g in zip(ref, _images(jobs)):
+        assert np.array_equal(r, g)
+    assert len(cache.read_text().splitlines()) == 1
+    for j in jobs:
+        j["out"].upload(np.zeros(j["out"].shape, np.float32))
+    tr2 = ct.Tracer()
+    res2 = tr2.autotune(jobs, max_trials=6, cache_path=str(cache))
+    assert res2["fromCache"] and res2["trials"] == 0 and res2["config"] == res["config"]
+    assert tr2.get_config() == res["config"]
+    for r, g in zip(ref, _images(jobs)):
+        assert np.array_equal(r, g)
