@@ -88,6 +88,9 @@ def cpu_baseline(ct, ps, frame, w, h, spp, depth, budget_s=12.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--form", choices=["plain", "batch", "serve"], default="plain")
+    ap.add_argument("--precision", choices=["f64", "f32"], default="f64",
+                    help="f64: the reference's binary64 arithmetic; f32: the precision form "
+                         "(quantized nodes, FP32 intersection and shading)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
@@ -101,6 +104,8 @@ def main():
     ap.add_argument("--stack", type=int, default=24, help="LDS stack entries of the ordered walks")
     ap.add_argument("--persist-blocks", type=int, default=2048,
                     help="workgroups of the persistent traversal launches (0: one lane per ray)")
+    ap.add_argument("--workgroup", type=int, default=256, choices=[64, 128, 256],
+                    help="lanes per workgroup of the traversal launches")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--scenes", type=int, default=0, help="batch/serve: scene count (0: 64/128)")
@@ -127,12 +132,17 @@ def main():
     print(json.dumps(res), flush=True)
 
 
-def extend_roofline(cnt, prof, kernel):
+def extend_roofline(cnt, prof, kernel, precision="f64"):
     """Roofline of the closest-hit launches: algorithmic bytes from the counting
-    run over the summed HIP-event time of the same launches."""
+    run over the summed HIP-event time of the same launches.  Per counted node
+    slot: 32 B (binary64 form: a float32 box of the 4-wide node, 128 B per
+    node) or 16 B (precision form: 64 B quantized node); 48 B per triangle and
+    32 B per sphere test; per ray the ray in and the hit out (binary64: 24 + 16 B,
+    FP32: 24 + 8 B)."""
     launches = max(1, prof["extendLaunches"])
-    ext_bytes = (32 * cnt["nodeVisits"] + 48 * cnt["triTests"] + 32 * cnt["sphereTests"] +
-                 (24 + 16) * cnt["extensionRays"])
+    node_b, ray_b = (16, 32) if precision == "f32" else (32, 40)
+    ext_bytes = (node_b * cnt["nodeVisits"] + 48 * cnt["triTests"] + 32 * cnt["sphereTests"] +
+                 ray_b * cnt["extensionRays"])
     ext_s = prof["extendMs"] / 1e3
     achieved = ext_bytes / ext_s / 1e9 if ext_s > 0 else 0.0
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -174,6 +184,8 @@ def plain_form(ct, args):
     tr = ct.Tracer(walk=walk)
     tr.set_traversal(walks[args.shadow_walk], args.stack)
     tr.set_schedule(args.persist_blocks > 0, max(1, args.persist_blocks))
+    tr.set_precision(ct.PRECISION_F32 if args.precision == "f32" else ct.PRECISION_F64)
+    tr.set_workgroup(args.workgroup)
     out = ct.DeviceBuffer.empty((w * h * 3,), np.float32)
     opts = {"width": w, "height": h, "spp": spp, "seed": 1, "maxBounces": depth,
             "russianRoulette": True, "nee": True}
@@ -205,7 +217,9 @@ def plain_form(ct, args):
     tr.render_image(ds, frame, opts, out)
     cnt = tr.stats()
     tr.set_counting(False)
-    roof = extend_roofline(cnt, prof, f"k_extend<{args.walk},{args.stack}>")
+    kname = (f"k_extend32<{args.stack}>" if args.precision == "f32"
+             else f"k_extend<{args.walk},{args.stack}>")
+    roof = extend_roofline(cnt, prof, kname, args.precision)
     log(f"extend: {prof['extendLaunches']} launches, {prof['extendMs']:.2f} ms total, "
         f"{cnt['nodeVisits'] / max(1, cnt['extensionRays']):.1f} nodes/ray, "
         f"{cnt['triTests'] / max(1, cnt['extensionRays']):.1f} tris/ray, {roof['achieved']:.0f} GB/s, "
@@ -221,10 +235,11 @@ def plain_form(ct, args):
     res = {
         "metric": METRIC, "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.precision, "data": "synthetic",
         "config": {"workload": f"triangle soup {args.tris} tris + 64 emissive, {w}x{h}, "
                                f"{spp} spp, maxBounces {depth}, RR+NEE, walk {args.walk}, "
+                               f"precision {args.precision}, "
                                f"shadow walk {args.shadow_walk}, stack {args.stack}, "
                                f"persistent blocks {args.persist_blocks}",
                    "triangles": args.tris, "width": w, "height": h, "spp": spp,

@@ -10,14 +10,17 @@ call that needs the library fails loudly when it is missing.
 from . import serve, synth
 from ._capi import CTError, DeviceBuffer, init, sync
 from .pack import PackedScene, buildBvh, packScene, primBounds
-from .tracer import (T_MIN, WALK_ORDERED, WALK_STACKLESS, WALK_WIDE, DeviceScene, Tracer, accumulatePass,
-                     cameraFrame, closest_hit, closest_hit_counts, intersectClosest, occluded,
-                     renderImage)
+from .tracer import (PRECISION_F32, PRECISION_F64, T_MIN, WALK_BRUTE, WALK_ORDERED,
+                     WALK_STACKLESS, WALK_WIDE, DeviceScene, Tracer, accumulatePass, cameraFrame,
+                     closest_hit, closest_hit32, closest_hit32_counts, closest_hit_counts,
+                     image_sum_f64, intersectClosest, occluded, occluded32, renderImage)
 from .serve import Server
 
 __all__ = [
     "serve", "Server", "synth", "CTError", "DeviceBuffer", "init", "sync", "PackedScene", "buildBvh", "packScene",
-    "primBounds", "T_MIN", "WALK_ORDERED", "WALK_STACKLESS", "WALK_WIDE", "DeviceScene", "Tracer",
+    "primBounds", "T_MIN", "WALK_ORDERED", "WALK_STACKLESS", "WALK_WIDE", "WALK_BRUTE",
+    "PRECISION_F64", "PRECISION_F32", "closest_hit32", "closest_hit32_counts", "occluded32",
+    "image_sum_f64", "DeviceScene", "Tracer",
     "accumulatePass", "cameraFrame", "closest_hit", "closest_hit_counts", "intersectClosest",
     "occluded", "renderImage",
 ]

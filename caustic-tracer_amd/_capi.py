@@ -66,7 +66,8 @@ class TuneResult(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("hbmBudget", C.c_uint64), ("maxPaths", C.c_uint64), ("maxActive", C.c_uint32),
-                ("slicePasses", C.c_uint32), ("policy", C.c_int32), ("graph", C.c_int32)]
+                ("slicePasses", C.c_uint32), ("policy", C.c_int32), ("graph", C.c_int32),
+                ("precision", C.c_int32)]
 
 
 class JobStatus(C.Structure):
@@ -96,6 +97,8 @@ SIGNATURES = {
     "ct_scene_create": [vp, u32, vp, u32, vp, u32, vp, u32, vp, C.POINTER(Sky), C.POINTER(vp)],
     "ct_scene_create_device": [vp, u32, vp, u32, vp, u32, vp, u32, vp, C.POINTER(Sky), vp,
                                C.POINTER(vp)],
+    "ct_scene_create_ex": [vp, u32, vp, u32, vp, u32, vp, u32, vp, C.POINTER(Sky), vp,
+                           C.POINTER(vp)],
     "ct_scene_destroy": [vp],
     "ct_closest_hit": [vp, vp, u32, f64, vp, i32, vp, vp, vp, vp, vp],
     "ct_occluded": [vp, vp, u32, f64, vp, vp, vp],
@@ -115,7 +118,13 @@ SIGNATURES = {
     "ct_tracer_set_capacity": [vp, C.c_uint64],
     "ct_tracer_set_graph": [vp, i32],
     "ct_bvh_build": [vp, vp, vp, u32, i32, vp, C.POINTER(u32), vp, vp],
+    "ct_bvh_build_ex": [vp, vp, vp, u32, i32, vp, C.POINTER(u32), vp, vp, vp],
     "ct_tracer_set_workgroup": [vp, i32],
+    "ct_tracer_set_precision": [vp, i32],
+    "ct_closest_hit32": [vp, vp, u32, C.c_float, vp, i32, vp, vp, vp, vp, vp],
+    "ct_occluded32": [vp, vp, u32, C.c_float, vp, i32, vp, vp],
+    "ct_closest_hit32_counts": [vp, vp, u32, C.c_float, vp, vp, vp],
+    "ct_image_sum_f64": [vp, C.c_uint64, vp, vp],
     "ct_tracer_get_config": [vp, C.POINTER(TracerConfig)],
     "ct_tracer_set_config": [vp, C.POINTER(TracerConfig)],
     "ct_tracer_autotune": [vp, vp, u32, u32, C.c_char_p, C.POINTER(TuneResult), vp],

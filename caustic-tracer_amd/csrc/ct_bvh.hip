@@ -721,7 +721,8 @@ __global__ void k_dfs(Build b, uint32_t r0, uint32_t r1) {
 }
 
 // One 8-float node per record (bvh.ts:140-164).
-__global__ void k_emit(Build b, uint32_t nrec, float* __restrict__ nodes) {
+__global__ void k_emit(Build b, uint32_t nrec, float* __restrict__ nodes,
+                       uint32_t* __restrict__ words) {
   uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nrec; r += stride) {
     const double* bx = b.recBox + 6 * (size_t)r;
@@ -737,8 +738,10 @@ __global__ void k_emit(Build b, uint32_t nrec, float* __restrict__ nodes) {
     if (b.recLeft[r] < 0) {
       uint32_t start = b.recStart[r], count = b.recEnd[r] - start;
       nd[7] = (float)((double)start * 8 + count);  // encodeLeaf (bvh.ts:60-62)
+      if (words) words[idx] = start * 8 + count;
     } else {
       nd[7] = -1.0f;
+      if (words) words[idx] = 0xffffffffu;
     }
   }
 }
@@ -759,6 +762,13 @@ using namespace ct;
 extern "C" int ct_bvh_build(const double* d_min, const double* d_max, const double* d_cen,
                             uint32_t n, int32_t bins, float* d_nodes, uint32_t* h_nodeCount,
                             uint32_t* d_order, void* stream) {
+  return ct_bvh_build_ex(d_min, d_max, d_cen, n, bins, d_nodes, h_nodeCount, d_order, nullptr,
+                         stream);
+}
+
+extern "C" int ct_bvh_build_ex(const double* d_min, const double* d_max, const double* d_cen,
+                               uint32_t n, int32_t bins, float* d_nodes, uint32_t* h_nodeCount,
+                               uint32_t* d_order, uint32_t* d_leafWords, void* stream) {
   if (!d_nodes || !h_nodeCount || (n && (!d_min || !d_max || !d_cen || !d_order)))
     return set_error(CT_EINVAL, "ct_bvh_build: null argument");
   if (bins < 2 || bins > MAX_BINS)
@@ -768,7 +778,10 @@ extern "C" int ct_bvh_build(const double* d_min, const double* d_max, const doub
   if (n == 0) {
     // a single empty leaf (bvh.ts:93-100)
     float node[8] = {0, 0, 0, 1, 0, 0, 0, 0};
+    uint32_t word = 0;
     CT_HIP(hipMemcpyAsync(d_nodes, node, sizeof node, hipMemcpyHostToDevice, st));
+    if (d_leafWords)
+      CT_HIP(hipMemcpyAsync(d_leafWords, &word, sizeof word, hipMemcpyHostToDevice, st));
     CT_HIP(hipStreamSynchronize(st));
     *h_nodeCount = 1;
     return CT_OK;
@@ -921,7 +934,7 @@ extern "C" int ct_bvh_build(const double* d_min, const double* d_max, const doub
   for (size_t L = 0; L < levels.size(); L++)
     BVH_LAUNCH(k_dfs, grid_for(levels[L].second - levels[L].first, BB), BB, b,
                levels[L].first, levels[L].second);
-  BVH_LAUNCH(k_emit, grid_for(nrec, BB), BB, b, nrec, d_nodes);
+  BVH_LAUNCH(k_emit, grid_for(nrec, BB), BB, b, nrec, d_nodes, d_leafWords);
 #undef BVH_LAUNCH
   hipError_t e = hipFreeAsync(mem, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);

@@ -38,7 +38,17 @@ struct DevScene {
   //  child code[4] | unused]; child code >= 0 another wide node, <= -2 a leaf
   // (-2 - leaf word of the reference node), -1 an empty slot.
   const float4* __restrict__ wide;
+  // Exact leaf words (start*8 + count; 0xffffffff on interior nodes), one per
+  // reference node: the float32 words of the reference layout are exact only
+  // below 2^21 primitives.
+  const uint32_t* __restrict__ leafWords;
+  // Quantized 4-wide nodes of the FP32 precision form, 4 float4 (64 B) each:
+  // [origin.xyz, scale.x | scale.y, scale.z, qlo.x, qlo.y | qlo.z, qhi.x,
+  //  qhi.y, qhi.z | child code[4]]; q* hold one byte per child; a child box
+  // is origin + q*scale (power-of-two scales), rounded outward at upload.
+  const float4* __restrict__ qwide;
   int wideCount;
+  int binaryExact;  // every float32 leaf word equals its exact word
   int primCount;
   int nodeCount;
   int lightCount;
@@ -214,10 +224,10 @@ struct TraversalCounts {
 // 324-351): strict '<' keeps the first prim tested on exact ties; a sphere hit
 // leaves u, v as the last triangle hit set them.
 template <bool COUNT>
-__device__ __forceinline__ void test_leaf(const DevScene& S, float leafWord, double ox, double oy,
+__device__ __forceinline__ void test_leaf(const DevScene& S, int leafWord, double ox, double oy,
                                           double oz, double dx, double dy, double dz,
                                           double tMin, Hit& h, TraversalCounts& cnt) {
-  int li = (int)leafWord;
+  int li = leafWord;
   int count = li & 7;
   int start = li >> 3;
   for (int p = start; p < start + count; p++) {
@@ -261,7 +271,7 @@ __device__ __forceinline__ void closest_stackless(const DevScene& S, double ox, 
       node = (int)lo.w;
       continue;
     }
-    if (hi.w >= 0.0f) test_leaf<COUNT>(S, hi.w, ox, oy, oz, dx, dy, dz, tMin, h, cnt);
+    if (hi.w >= 0.0f) test_leaf<COUNT>(S, (int)S.leafWords[node], ox, oy, oz, dx, dy, dz, tMin, h, cnt);
     node++;
   }
 }
@@ -284,7 +294,7 @@ __device__ __forceinline__ bool occluded(const DevScene& S, double ox, double oy
       continue;
     }
     if (hi.w >= 0.0f) {
-      int li = (int)hi.w;
+      int li = (int)S.leafWords[node];
       int count = li & 7;
       int start = li >> 3;
       for (int p = start; p < start + count; p++) {
@@ -319,169 +329,6 @@ __device__ __forceinline__ float f32_round_down(double x) {
     f = __uint_as_float(b);
   }
   return f;
-}
-
-// Ordered front-to-back closest hit (intersect.ts:202-282): both children
-// tested against the shrinking bound, nearer first (tl <= tr -> left), the far
-// child pushed with its entry distance, pops skip entries >= closest.
-// The stack lives in LDS, STACK entries per lane at stride `lanes`
-// (conflict-free); entries are float32 rounded down, so a pop skips a subtree
-// only when the binary64 entry would have been skipped too.  A ray that needs
-// more than STACK deferred subtrees finishes with the stackless walk from the
-// root under its current bound: that walk visits every box entered before the
-// bound and re-testing already-tested primitives cannot change a strict-'<'
-// minimum, so the closest hit is unchanged and the loop always ends.
-template <int STACK, bool COUNT>
-__device__ __forceinline__ void closest_ordered(const DevScene& S, double ox, double oy,
-                                               double oz, double dx, double dy, double dz,
-                                               double tMin, Hit& h, int* __restrict__ sNode,
-                                               float* __restrict__ sEntry, int lanes,
-                                               TraversalCounts& cnt) {
-  double idx = safe_inv(dx), idy = safe_inv(dy), idz = safe_inv(dz);
-  if (COUNT) cnt.nodes++;
-  float4 rlo = S.nodes[0], rhi = S.nodes[1];
-  if (ray_aabb(ox, oy, oz, idx, idy, idz, rlo, rhi, h.t) == __builtin_inf()) return;
-  int sp = 0;
-  int node = 0;
-  float leafWord = rhi.w;
-  const double INF = __builtin_inf();
-  for (;;) {
-    if (leafWord >= 0.0f) {
-      test_leaf<COUNT>(S, leafWord, ox, oy, oz, dx, dy, dz, tMin, h, cnt);
-    } else {
-      int left = node + 1;
-      float4 llo = S.nodes[2 * (size_t)left], lhi = S.nodes[2 * (size_t)left + 1];
-      int right = (int)llo.w;
-      float4 rlo2 = S.nodes[2 * (size_t)right], rhi2 = S.nodes[2 * (size_t)right + 1];
-      if (COUNT) cnt.nodes += 2;
-      double tl = ray_aabb(ox, oy, oz, idx, idy, idz, llo, lhi, h.t);
-      double tr = ray_aabb(ox, oy, oz, idx, idy, idz, rlo2, rhi2, h.t);
-      if (tl != INF) {
-        if (tr != INF) {
-          if (sp == STACK) {
-            if (COUNT) cnt.fallbacks++;
-            closest_stackless<COUNT>(S, ox, oy, oz, dx, dy, dz, tMin, h, cnt);
-            return;
-          }
-          int slot = sp * lanes;
-          if (tl <= tr) {
-            sNode[slot] = right;
-            sEntry[slot] = f32_round_down(tr);
-            node = left;
-            leafWord = lhi.w;
-          } else {
-            sNode[slot] = left;
-            sEntry[slot] = f32_round_down(tl);
-            node = right;
-            leafWord = rhi2.w;
-          }
-          sp++;
-        } else {
-          node = left;
-          leafWord = lhi.w;
-        }
-        continue;
-      }
-      if (tr != INF) {
-        node = right;
-        leafWord = rhi2.w;
-        continue;
-      }
-    }
-    // pop, skipping entries the shrinking bound already excludes
-    bool found = false;
-    while (sp > 0) {
-      sp--;
-      if ((double)sEntry[sp * lanes] < h.t) {
-        node = sNode[sp * lanes];
-        found = true;
-        break;
-      }
-    }
-    if (!found) return;
-    leafWord = S.nodes[2 * (size_t)node + 1].w;
-  }
-}
-
-// Any-hit with the ordered walk: nearer child first, first blocker ends the
-// walk.  Whether (tMin, tMax) is blocked does not depend on the visiting order,
-// so this returns exactly what the stackless any-hit walk of intersect.ts:368-420
-// returns while usually meeting the blocker sooner.  Stack overflow falls back
-// to that stackless walk from the root.
-template <int STACK, bool COUNT>
-__device__ __forceinline__ bool occluded_ordered(const DevScene& S, double ox, double oy,
-                                                 double oz, double dx, double dy, double dz,
-                                                 double tMin, double tMax,
-                                                 int* __restrict__ sNode, int lanes,
-                                                 TraversalCounts& cnt) {
-  double idx = safe_inv(dx), idy = safe_inv(dy), idz = safe_inv(dz);
-  if (COUNT) cnt.nodes++;
-  float4 rlo = S.nodes[0], rhi = S.nodes[1];
-  if (ray_aabb(ox, oy, oz, idx, idy, idz, rlo, rhi, tMax) == __builtin_inf()) return false;
-  int sp = 0;
-  int node = 0;
-  float leafWord = rhi.w;
-  const double INF = __builtin_inf();
-  for (;;) {
-    if (leafWord >= 0.0f) {
-      int li = (int)leafWord;
-      int count = li & 7;
-      int start = li >> 3;
-      for (int p = start; p < start + count; p++) {
-        PrimGeom g = load_prim(S, p);
-        if (g.a.w == 0.0f) {
-          if (COUNT) cnt.spheres++;
-          if (ray_sphere(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, tMin, tMax) > 0)
-            return true;
-        } else {
-          if (COUNT) cnt.tris++;
-          double t, u, v;
-          if (ray_triangle(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, g.b.y, g.b.z,
-                           g.c.x, g.c.y, g.c.z, tMin, tMax, t, u, v))
-            return true;
-        }
-      }
-    } else {
-      int left = node + 1;
-      float4 llo = S.nodes[2 * (size_t)left], lhi = S.nodes[2 * (size_t)left + 1];
-      int right = (int)llo.w;
-      float4 rlo2 = S.nodes[2 * (size_t)right], rhi2 = S.nodes[2 * (size_t)right + 1];
-      if (COUNT) cnt.nodes += 2;
-      double tl = ray_aabb(ox, oy, oz, idx, idy, idz, llo, lhi, tMax);
-      double tr = ray_aabb(ox, oy, oz, idx, idy, idz, rlo2, rhi2, tMax);
-      if (tl != INF) {
-        if (tr != INF) {
-          if (sp == STACK) {
-            if (COUNT) cnt.fallbacks++;
-            return occluded<COUNT>(S, ox, oy, oz, dx, dy, dz, tMin, tMax, cnt);
-          }
-          int far = tl <= tr ? right : left;
-          sNode[sp * lanes] = far;
-          sp++;
-          if (tl <= tr) {
-            node = left;
-            leafWord = lhi.w;
-          } else {
-            node = right;
-            leafWord = rhi2.w;
-          }
-        } else {
-          node = left;
-          leafWord = lhi.w;
-        }
-        continue;
-      }
-      if (tr != INF) {
-        node = right;
-        leafWord = rhi2.w;
-        continue;
-      }
-    }
-    if (sp == 0) return false;
-    sp--;
-    node = sNode[sp * lanes];
-    leafWord = S.nodes[2 * (size_t)node + 1].w;
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -534,8 +381,8 @@ __device__ __forceinline__ double entry_from_bf16(uint16_t b) {
 
 // Work item encoding shared by the stack and the current-node register:
 // >= 0 interior node index, <= -2 a leaf (-2 - leafWord), -1 nothing.
-__device__ __forceinline__ int enc_child(int idx, float leafWord) {
-  return leafWord >= 0.0f ? -2 - (int)leafWord : idx;
+__device__ __forceinline__ int enc_child(const DevScene& S, int idx, float leafWord) {
+  return leafWord >= 0.0f ? -2 - (int)S.leafWords[idx] : idx;
 }
 
 // Closest hit with the ordered walk of intersect.ts:202-282 (nearer child
@@ -557,7 +404,7 @@ __device__ __forceinline__ void closest_wave(const DevScene& S, double ox, doubl
   {
     float4 lo = S.nodes[0], hi = S.nodes[1];
     double e;
-    if (box_enter(ox, oy, oz, ix, iy, iz, lo, hi, h.t, e)) cur = enc_child(0, hi.w);
+    if (box_enter(ox, oy, oz, ix, iy, iz, lo, hi, h.t, e)) cur = enc_child(S, 0, hi.w);
   }
   int sp = 0;
   int pend = -1;  // parked leaf word
@@ -578,7 +425,7 @@ __device__ __forceinline__ void closest_wave(const DevScene& S, double ox, doubl
         double tl, tr;
         bool hl = box_enter(ox, oy, oz, ix, iy, iz, llo, lhi, h.t, tl);
         bool hr = box_enter(ox, oy, oz, ix, iy, iz, rlo, rhi, h.t, tr);
-        int el = enc_child(left, lhi.w), er = enc_child(right, rhi.w);
+        int el = enc_child(S, left, lhi.w), er = enc_child(S, right, rhi.w);
         if (hl && hr) {
           if (sp == STACK) {
             overflow = true;
@@ -612,112 +459,15 @@ __device__ __forceinline__ void closest_wave(const DevScene& S, double ox, doubl
     if (overflow) break;
     // test the parked leaf
     if (pend >= 0) {
-      test_leaf<COUNT>(S, (float)pend, ox, oy, oz, dx, dy, dz, tMin, h, cnt);
+      test_leaf<COUNT>(S, pend, ox, oy, oz, dx, dy, dz, tMin, h, cnt);
       pend = -1;
     }
   }
   if (overflow) {
     if (COUNT) cnt.fallbacks++;
-    if (pend >= 0) test_leaf<COUNT>(S, (float)pend, ox, oy, oz, dx, dy, dz, tMin, h, cnt);
+    if (pend >= 0) test_leaf<COUNT>(S, pend, ox, oy, oz, dx, dy, dz, tMin, h, cnt);
     closest_stackless<COUNT>(S, ox, oy, oz, dx, dy, dz, tMin, h, cnt);
   }
-}
-
-// Any-hit counterpart: same wave organisation, first blocker ends the walk.
-template <int STACK, bool COUNT>
-__device__ __forceinline__ bool occluded_wave(const DevScene& S, double ox, double oy, double oz,
-                                              double dx, double dy, double dz, double tMin,
-                                              double tMax, int* __restrict__ sNode, int lanes,
-                                              TraversalCounts& cnt) {
-  const double ix = safe_inv(dx), iy = safe_inv(dy), iz = safe_inv(dz);
-  if (COUNT) cnt.nodes++;
-  int cur = -1;
-  {
-    float4 lo = S.nodes[0], hi = S.nodes[1];
-    double e;
-    if (box_enter(ox, oy, oz, ix, iy, iz, lo, hi, tMax, e)) cur = enc_child(0, hi.w);
-  }
-  int sp = 0;
-  int pend = -1;
-  bool overflow = false, blocked = false;
-  while (cur != -1 || pend >= 0) {
-    while (cur != -1) {
-      if (cur <= -2) {
-        if (pend >= 0) break;
-        pend = -2 - cur;
-        cur = -1;
-      } else {
-        int left = cur + 1;
-        float4 llo = S.nodes[2 * (size_t)left], lhi = S.nodes[2 * (size_t)left + 1];
-        int right = (int)llo.w;
-        float4 rlo = S.nodes[2 * (size_t)right], rhi = S.nodes[2 * (size_t)right + 1];
-        if (COUNT) cnt.nodes += 2;
-        double tl, tr;
-        bool hl = box_enter(ox, oy, oz, ix, iy, iz, llo, lhi, tMax, tl);
-        bool hr = box_enter(ox, oy, oz, ix, iy, iz, rlo, rhi, tMax, tr);
-        int el = enc_child(left, lhi.w), er = enc_child(right, rhi.w);
-        if (hl && hr) {
-          if (sp == STACK) {
-            overflow = true;
-            cur = -1;
-            break;
-          }
-          bool leftFirst = tl <= tr;
-          sNode[sp * lanes] = leftFirst ? er : el;
-          sp++;
-          cur = leftFirst ? el : er;
-        } else if (hl) {
-          cur = el;
-        } else if (hr) {
-          cur = er;
-        } else {
-          cur = -1;
-        }
-      }
-      if (cur == -1 && sp > 0) {
-        sp--;
-        cur = sNode[sp * lanes];
-      }
-      if (__all(pend >= 0 || cur == -1)) break;
-    }
-    if (overflow) break;
-    if (pend >= 0) {
-      int li = pend;
-      int count = li & 7, start = li >> 3;
-      for (int p = start; p < start + count && !blocked; p++) {
-        PrimGeom g = load_prim(S, p);
-        if (g.a.w == 0.0f) {
-          if (COUNT) cnt.spheres++;
-          blocked = ray_sphere(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, tMin, tMax) > 0;
-        } else {
-          if (COUNT) cnt.tris++;
-          double t, u, v;
-          blocked = ray_triangle(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, g.b.y, g.b.z,
-                                 g.c.x, g.c.y, g.c.z, tMin, tMax, t, u, v);
-        }
-      }
-      pend = -1;
-      if (blocked) return true;
-    }
-  }
-  if (overflow) {
-    if (COUNT) cnt.fallbacks++;
-    if (pend >= 0) {
-      int li = pend;
-      int count = li & 7, start = li >> 3;
-      for (int p = start; p < start + count; p++) {
-        PrimGeom g = load_prim(S, p);
-        double t, u, v;
-        if (g.a.w == 0.0f ? ray_sphere(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, tMin,
-                                       tMax) > 0
-                          : ray_triangle(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x,
-                                         g.b.y, g.b.z, g.c.x, g.c.y, g.c.z, tMin, tMax, t, u, v))
-          return true;
-      }
-    }
-    return occluded<COUNT>(S, ox, oy, oz, dx, dy, dz, tMin, tMax, cnt);
-  }
-  return false;
 }
 
 // Slab test of child k of a wide node (see box_enter_fma).
@@ -799,7 +549,7 @@ __device__ __forceinline__ bool interior_step(const DevScene& S, double ix, doub
     double tl, tr;
     bool hl = box_enter_fma(ix, iy, iz, oix, oiy, oiz, llo, lhi, tMax, tl);
     bool hr = box_enter_fma(ix, iy, iz, oix, oiy, oiz, rlo, rhi, tMax, tr);
-    int el = enc_child(left, lhi.w), er = enc_child(right, rhi.w);
+    int el = enc_child(S, left, lhi.w), er = enc_child(S, right, rhi.w);
     if (hl && hr) {
       if (sp == STACK) return false;
       bool leftFirst = tl <= tr;
@@ -830,7 +580,7 @@ __device__ __forceinline__ int walk_root(const DevScene& S, double ix, double iy
     if (count) cnt.nodes++;
     float4 lo = S.nodes[0], hi = S.nodes[1];
     double e;
-    return box_enter_fma(ix, iy, iz, oix, oiy, oiz, lo, hi, tMax, e) ? enc_child(0, hi.w) : -1;
+    return box_enter_fma(ix, iy, iz, oix, oiy, oiz, lo, hi, tMax, e) ? enc_child(S, 0, hi.w) : -1;
   }
 }
 

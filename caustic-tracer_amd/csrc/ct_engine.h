@@ -31,6 +31,10 @@ struct PlanJob {
   float* accum;
   bool finish;
   double scale;
+  // FP32 precision form: persistent compensation terms (width*height*3) kept
+  // by the caller across calls, or null for per-call terms folded into
+  // `accum` when the call's last pass is added
+  float* comp = nullptr;
 };
 
 Frame make_frame(const double* frame16);

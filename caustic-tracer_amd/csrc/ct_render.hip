@@ -25,101 +25,14 @@
 #include <vector>
 
 #include "ct_engine.h"
+#include "ct_wavefront.h"
 
 namespace ct {
-
-constexpr int RBLOCK = 256;
-constexpr int MAX_BOUNCE_SLOTS = 256;  // counters per kind (bounces + 1 <= this)
-
-// One job of a launch round (device copy).
-struct JobDev {
-  DevScene S;
-  Frame F;
-  TraceOpts opts;
-  uint32_t width, height, npix;
-  uint32_t passes;     // passes of this job in the round
-  uint32_t frameBase;  // frame index of its first pass (uint32 arithmetic)
-  uint32_t pathBase;   // first path id
-  uint32_t pixelBase;  // first finalize index
-  int doScale;         // multiply the sum by `scale` after adding (renderImage)
-  double scale;
-  float* accum;        // width*height*3 float32 sum
-};
-
-struct PathSoA {
-  double *ox, *oy, *oz, *dx, *dy, *dz;
-  double *br, *bg, *bb;
-  double *Lr, *Lg, *Lb;
-  double *prevPdf, *hitT;
-  uint32_t* rng;
-  int32_t* hitPrim;
-  uint32_t* jobOf;
-  uint8_t* specular;
-  uint32_t *q0, *q1;
-  // shadow queue (one slot per live path at most)
-  uint32_t* sPath;
-  double *sdx, *sdy, *sdz, *sTmax, *sCr, *sCg, *sCb;
-  // counters: live[b] at [b], shadow[b] at [MAX_BOUNCE_SLOTS + b], ray-fetch
-  // cursors of the extend/shadow launches at [2*MAX + b] and [3*MAX + b];
-  // 64-bit work counters at ctr64
-  uint32_t* ctr;
-  unsigned long long* ctr64;
-};
-
-__device__ __forceinline__ int lane_id() { return __lane_id(); }
-
-// Append `pid` to a queue for every lane with `take`; one atomic per wave.
-__device__ __forceinline__ void wave_push(bool take, uint32_t pid, uint32_t* q, uint32_t* count) {
-  unsigned long long active = __ballot(1);
-  unsigned long long m = __ballot(take);
-  if (m == 0) return;
-  int lane = lane_id();
-  int leader = __ffsll((long long)active) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
-  base = __shfl(base, leader, 64);
-  if (take) {
-    unsigned long long below = m & ((1ull << lane) - 1ull);
-    q[base + (uint32_t)__popcll(below)] = pid;
-  }
-}
-
-__device__ __forceinline__ uint32_t wave_reserve(bool take, uint32_t* count) {
-  unsigned long long active = __ballot(1);
-  unsigned long long m = __ballot(take);
-  int lane = lane_id();
-  int leader = __ffsll((long long)active) - 1;
-  uint32_t base = 0;
-  if (m && lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
-  base = __shfl(base, leader, 64);
-  unsigned long long below = m & ((1ull << lane) - 1ull);
-  return base + (uint32_t)__popcll(below);
-}
-
-__device__ __forceinline__ void wave_add64(unsigned long long* dst, uint32_t v) {
-  unsigned long long x = v;
-  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-  if ((threadIdx.x & 63) == 0 && x) atomicAdd(dst, x);
-}
 
 __global__ void k_clear(uint32_t* ctr, int n, unsigned long long* ctr64, int n64) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) ctr[i] = 0;
   if (i < n64) ctr64[i] = 0;
-}
-
-// Job of a path / finalize index: the last job whose base is <= i.
-template <bool MULTI, bool PIXEL>
-__device__ __forceinline__ int job_of(const JobDev* __restrict__ jobs, int njobs, size_t i) {
-  if (!MULTI) return 0;
-  int lo = 0, hi = njobs - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    size_t base = PIXEL ? jobs[mid].pixelBase : jobs[mid].pathBase;
-    if (base <= i) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
 }
 
 // camera.ts:62-72 generateRay with the pixel/lens draws of tracer.ts:373-384.
@@ -578,24 +491,6 @@ k_shade(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__
   }
 }
 
-// Lane refill shared by the persistent traversal kernels: lanes without a ray
-// take the next queue slots (one atomic per wave).  Returns the slot for this
-// lane or UINT32_MAX.
-__device__ __forceinline__ uint32_t refill(bool need, uint32_t n, uint32_t* fetch,
-                                           bool& exhausted) {
-  unsigned long long m = __ballot(need);
-  if (!m) return 0xffffffffu;
-  const int lane = __lane_id();
-  int leader = __ffsll((long long)__ballot(1)) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(fetch, (uint32_t)__popcll(m));
-  base = __shfl(base, leader, 64);
-  if (base + (uint32_t)__popcll(m) >= n) exhausted = true;
-  if (!need) return 0xffffffffu;
-  uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-  return slot < n ? slot : 0xffffffffu;
-}
-
 // Persistent closest hit.  WALK: CT_WALK_STACKLESS (the reference's skip-link
 // loop, exact mirror), CT_WALK_ORDERED (binary nodes) or CT_WALK_WIDE
 // (derived 4-wide nodes); the ordered forms park leaves until the whole wave
@@ -671,7 +566,7 @@ k_extend(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
           if (__all(pend >= 0 || cur == -1)) break;
         }
         if (pend >= 0) {
-          test_leaf<COUNT>(S, (float)pend, ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
+          test_leaf<COUNT>(S, pend, ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
           pend = -1;
         }
         if (overflow) {
@@ -858,6 +753,10 @@ struct ct_tracer {
   void* buf = nullptr;
   size_t capPaths = 0;
   PathSoA P{};
+  PathSoA32 P32{};            // FP32 view of the same allocation (precision form)
+  int precision = CT_PRECISION_F64;
+  float* compBuf = nullptr;   // per-call compensation terms (precision form)
+  size_t capComp = 0;
   JobDev* jobsDev = nullptr;  // device job table
   JobDev* jobsHost = nullptr; // pinned staging for the table
   size_t capJobs = 0;
@@ -915,6 +814,30 @@ static int tracer_reserve(ct_tracer* T, size_t paths) {
   P.specular = (uint8_t*)p; p += u8;
   P.ctr = (uint32_t*)p; p += al(4 * MAX_BOUNCE_SLOTS * 4);
   P.ctr64 = (unsigned long long*)p;
+  // the FP32 state aliases the start of the same allocation (one precision
+  // per round): 21 float arrays, 6 uint32 arrays and the flags fit in the
+  // 21 binary64 arrays
+  {
+    char* f = (char*)T->buf;
+    size_t f4 = al(n * 4);
+    auto takeF = [&]() { float* r = (float*)f; f += f4; return r; };
+    auto takeU32 = [&]() { uint32_t* r = (uint32_t*)f; f += f4; return r; };
+    PathSoA32& Q = T->P32;
+    Q.ox = takeF(); Q.oy = takeF(); Q.oz = takeF();
+    Q.dx = takeF(); Q.dy = takeF(); Q.dz = takeF();
+    Q.br = takeF(); Q.bg = takeF(); Q.bb = takeF();
+    Q.Lr = takeF(); Q.Lg = takeF(); Q.Lb = takeF();
+    Q.prevPdf = takeF(); Q.hitT = takeF();
+    Q.sdx = takeF(); Q.sdy = takeF(); Q.sdz = takeF(); Q.sTmax = takeF();
+    Q.sCr = takeF(); Q.sCg = takeF(); Q.sCb = takeF();
+    Q.rng = takeU32();
+    Q.hitPrim = (int32_t*)takeU32();
+    Q.jobOf = takeU32();
+    Q.q0 = takeU32(); Q.q1 = takeU32(); Q.sPath = takeU32();
+    Q.specular = (uint8_t*)f; f += u8;
+    Q.ctr = P.ctr;
+    Q.ctr64 = P.ctr64;
+  }
   T->capPaths = paths;
   // path buffers moved: a captured graph refers to the old ones
   if (T->graphExec) {
@@ -1039,12 +962,23 @@ struct RoundJob {
   float* accum;
   int doScale;
   double scale;
+  float* comp = nullptr;  // FP32 compensation terms
+  int fold = 0;
 };
+
+static void mark_cb(void* ctx, hipStream_t st, int kind, bool begin) {
+  mark((ct_tracer*)ctx, st, kind, begin);
+}
 
 // Kernel launches of one round (all on `st`; graph-capturable).
 template <bool MULTI>
 static int launch_round(ct_tracer* T, int njobs, size_t paths, size_t pix, int nb,
                         hipStream_t st) {
+  if (T->precision == CT_PRECISION_F32) {
+    RoundLaunch R{T->P32, T->jobsDev, njobs, paths, pix, nb, T->stack, T->block,
+                  T->persistBlocks, T->counting, mark_cb, T};
+    return launch_round32(R, MULTI, st);
+  }
   PathSoA P = T->P;
   const JobDev* jobs = T->jobsDev;
   hipLaunchKernelGGL(k_clear, dim3(4), dim3(256), 0, st, P.ctr, 4 * MAX_BOUNCE_SLOTS, P.ctr64, 0);
@@ -1123,6 +1057,8 @@ static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t s
     d.doScale = j.doScale;
     d.scale = j.scale;
     d.accum = j.accum;
+    d.comp = j.comp;
+    d.fold = j.fold;
     pb += (size_t)d.npix * j.passes;
     xb += d.npix;
   }
@@ -1133,10 +1069,10 @@ static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t s
   if (T->graph) {
     // the captured launches depend only on the round's shape and the tracer's
     // settings; the job table contents are read from device memory at replay
-    std::vector<char> key(sizeof(size_t) * 2 + sizeof(int) * 9);
+    std::vector<char> key(sizeof(size_t) * 2 + sizeof(int) * 10);
     size_t kk[2] = {paths, pix};
-    int ki[9] = {nb, (int)R.size(), T->walk, T->shadowWalk, T->stack, (int)T->persistBlocks,
-                 T->counting, multi, T->block};
+    int ki[10] = {nb, (int)R.size(), T->walk, T->shadowWalk, T->stack, (int)T->persistBlocks,
+                  T->counting, multi, T->block, T->precision};
     memcpy(key.data(), kk, sizeof kk);
     memcpy(key.data() + sizeof kk, ki, sizeof ki);
     if (!T->graphExec || key != T->graphKey) {
@@ -1201,6 +1137,32 @@ int ct::run_plan(ct_tracer* T, std::vector<PlanJob> jobs, hipStream_t st) {
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, T->P.ctr, 0, T->P.ctr64, 16);
     CT_LAUNCH_CHECK("k_clear");
   }
+  // FP32 precision form: compensation terms of jobs without their own live in
+  // a per-call scratch buffer, zeroed here and folded into the sums at the end
+  std::vector<float*> comp(jobs.size(), nullptr);
+  if (T->precision == CT_PRECISION_F32) {
+    size_t need = 0;
+    for (const PlanJob& j : jobs)
+      if (!j.comp) need += (size_t)j.width * j.height * 3;
+    if (need > T->capComp) {
+      if (T->compBuf) CT_HIP(hipFree(T->compBuf));
+      T->compBuf = nullptr;
+      T->capComp = 0;
+      CT_HIP(hipMalloc((void**)&T->compBuf, need * sizeof(float)));
+      T->capComp = need;
+    }
+    int rc = zero_floats(T->compBuf, need, st);
+    if (rc) return rc;
+    size_t off = 0;
+    for (size_t k = 0; k < jobs.size(); k++) {
+      if (jobs[k].comp) {
+        comp[k] = jobs[k].comp;
+      } else {
+        comp[k] = T->compBuf + off;
+        off += (size_t)jobs[k].width * jobs[k].height * 3;
+      }
+    }
+  }
   std::vector<uint32_t> done(jobs.size(), 0);
   for (;;) {
     std::vector<RoundJob> R;
@@ -1224,6 +1186,8 @@ int ct::run_plan(ct_tracer* T, std::vector<PlanJob> jobs, hipStream_t st) {
       r.accum = j.accum;
       r.doScale = (j.finish && done[k] + p == j.count) ? 1 : 0;
       r.scale = j.scale;
+      r.comp = comp[k];
+      r.fold = (comp[k] && !j.comp && done[k] + p == j.count) ? 1 : 0;
       R.push_back(r);
       done[k] += p;
       budget -= p * npix;
@@ -1272,6 +1236,7 @@ int ct_tracer_destroy(ct_tracer* T) {
   if (T->buf) e = hipFree(T->buf);
   if (T->jobsDev) (void)hipFree(T->jobsDev);
   if (T->jobsHost) (void)hipHostFree(T->jobsHost);
+  if (T->compBuf) (void)hipFree(T->compBuf);
   if (T->ownStream) (void)hipStreamDestroy(T->ownStream);
   delete T;
   return check_hip(e, "ct_tracer_destroy");
@@ -1306,6 +1271,14 @@ int ct_tracer_set_schedule(ct_tracer* T, int32_t persistent, int32_t blocks) {
   if (persistent && (blocks < 1 || blocks > 65535))
     return set_error(CT_EINVAL, "ct_tracer_set_schedule: blocks must be in [1, 65535]");
   T->persistBlocks = persistent ? (unsigned)blocks : 0u;
+  return CT_OK;
+}
+
+int ct_tracer_set_precision(ct_tracer* T, int32_t precision) {
+  if (!T) return set_error(CT_EINVAL, "null tracer");
+  if (precision != CT_PRECISION_F64 && precision != CT_PRECISION_F32)
+    return set_error(CT_EINVAL, "ct_tracer_set_precision: unknown precision");
+  T->precision = precision;
   return CT_OK;
 }
 

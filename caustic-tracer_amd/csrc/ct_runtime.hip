@@ -1,6 +1,9 @@
 // Runtime part of the C ABI: errors, device memory, device-resident scenes.
 #include <string.h>
 
+#include <algorithm>
+#include <cmath>
+
 #include <string>
 #include <vector>
 
@@ -73,16 +76,18 @@ int ct_stream_sync(void* stream) {
 // light and object indices are in range.
 static int validate_scene(const float* nodes, uint32_t nodeCount, uint32_t primCount,
                           uint32_t matCount, const int32_t* lights, uint32_t lightCount,
-                          const uint32_t* primObject) {
+                          const uint32_t* primObject, const uint32_t* leafWords) {
   for (uint32_t i = 0; i < nodeCount; i++) {
     const float* nd = nodes + (size_t)i * NODE_STRIDE;
     double skip = nd[3], leaf = nd[7];
     if (!(skip > i && skip <= nodeCount))
       return set_error(CT_EINVAL, "ct_scene_create: node " + std::to_string(i) +
                                       " has a skip link that does not move forward");
+    if ((leaf >= 0) != (leafWords[i] != 0xffffffffu))
+      return set_error(CT_EINVAL, "ct_scene_create: leaf words disagree with the nodes");
     if (leaf >= 0) {
-      int li = (int)leaf;
-      uint32_t count = (uint32_t)(li & 7), start = (uint32_t)(li >> 3);
+      uint32_t w = leafWords[i];
+      uint32_t count = w & 7, start = w >> 3;
       if (count && (uint64_t)start + count > primCount)
         return set_error(CT_EINVAL, "ct_scene_create: leaf " + std::to_string(i) +
                                         " references primitives beyond primCount");
@@ -104,16 +109,35 @@ static int validate_scene(const float* nodes, uint32_t nodeCount, uint32_t primC
   return CT_OK;
 }
 
+// Exact leaf words: the caller's (layout extension for scenes of 2^21 or more
+// primitives), or read from the float32 words of the reference layout.
+// Returns whether every float32 word equals the exact one.
+static bool leaf_words(const float* nodes, uint32_t nodeCount, const uint32_t* given,
+                       std::vector<uint32_t>& out) {
+  out.resize(nodeCount);
+  bool exact = true;
+  for (uint32_t i = 0; i < nodeCount; i++) {
+    float f = nodes[(size_t)i * NODE_STRIDE + 7];
+    if (given) {
+      out[i] = given[i];
+      if (f >= 0 && (double)f != (double)given[i]) exact = false;
+    } else {
+      out[i] = f >= 0 ? (uint32_t)f : 0xffffffffu;
+    }
+  }
+  return exact;
+}
+
 // 4-wide nodes from the binary tree: every interior node at even depth becomes
 // a wide node whose children are its grandchildren (or its children where
 // those are leaves).  Boxes are the reference nodes' float32 boxes, copied
 // bit for bit; only the intermediate boxes are skipped, which widens the set
 // of subtrees a walk enters and never narrows it.
-static void build_wide(const float* nodes, uint32_t nodeCount, std::vector<float>& out) {
+static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* words,
+                       std::vector<float>& out) {
   out.clear();
-  auto leafWord = [&](uint32_t i) { return nodes[(size_t)i * NODE_STRIDE + 7]; };
-  auto isLeaf = [&](uint32_t i) { return leafWord(i) >= 0.0f; };
-  auto code = [&](uint32_t i) { return -2 - (int32_t)leafWord(i); };
+  auto isLeaf = [&](uint32_t i) { return words[i] != 0xffffffffu; };
+  auto code = [&](uint32_t i) { return -2 - (int32_t)words[i]; };
   struct Item { uint32_t n2; uint32_t slot; };  // binary node to expand into wide node `slot`
   std::vector<Item> work;
   auto alloc = [&]() {
@@ -172,10 +196,90 @@ static void build_wide(const float* nodes, uint32_t nodeCount, std::vector<float
   (void)nodeCount;
 }
 
+// Exact difference of two floats in binary64, when it is representable.
+static bool exact_diff(float a, float b, double& d) {
+  double x = a, y = -(double)b;
+  d = x + y;
+  double bb = d - x;
+  double err = (x - (d - bb)) + (y - bb);
+  return err == 0.0;
+}
+
+// 8-bit parent-relative quantization of the 4-wide nodes (precision form).
+// Per node and axis the origin is the children's minimum and the scale the
+// smallest power of two with 255*scale covering the children's extent; child
+// planes are rounded outward (floor below, ceil above) and then checked in
+// exact arithmetic, so every decoded box origin + q*scale contains the float32
+// box it replaces and a walk over them enters every subtree the float32 boxes
+// would (the kernels bound their own rounding separately).
+static void quantize_wide(const std::vector<float>& wide, std::vector<float>& out) {
+  size_t nw = wide.size() / 32;
+  out.assign(nw * 16, 0.0f);
+  for (size_t n = 0; n < nw; n++) {
+    const float* w = &wide[n * 32];
+    float* q = &out[n * 16];
+    int32_t codes[4];
+    memcpy(codes, w + 24, 16);
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    bool any = false;
+    for (int k = 0; k < 4; k++) {
+      if (codes[k] == -1) continue;
+      any = true;
+      for (int a = 0; a < 3; a++) {
+        lo[a] = std::min(lo[a], w[4 * a + k]);
+        hi[a] = std::max(hi[a], w[12 + 4 * a + k]);
+      }
+    }
+    float origin[3] = {0, 0, 0}, scale[3] = {1, 1, 1};
+    uint32_t qlo[3] = {0, 0, 0}, qhi[3] = {0, 0, 0};
+    if (any) {
+      for (int a = 0; a < 3; a++) {
+        origin[a] = lo[a];
+        double ext;
+        bool ok = exact_diff(hi[a], lo[a], ext);
+        int e = -126;
+        while (e < 127 && 255.0 * std::ldexp(1.0, e) < ext) e++;
+        if (!ok && e < 127) e++;  // inexact extent: one more doubling of margin
+        scale[a] = std::ldexp(1.0f, e);
+      }
+      for (int k = 0; k < 4; k++) {
+        if (codes[k] == -1) continue;
+        for (int a = 0; a < 3; a++) {
+          const double s = scale[a];
+          double dl, dh;
+          uint32_t ql = 0, qh = 255;
+          if (exact_diff(w[4 * a + k], origin[a], dl)) {
+            double f = std::floor(dl / s);
+            ql = (uint32_t)std::min(255.0, std::max(0.0, f));
+            while (ql > 0 && ql * s > dl) ql--;  // decoded plane <= box plane
+          }
+          if (exact_diff(w[12 + 4 * a + k], origin[a], dh)) {
+            double c = std::ceil(dh / s);
+            qh = (uint32_t)std::min(255.0, std::max(0.0, c));
+            while (qh < 255 && qh * s < dh) qh++;  // decoded plane >= box plane
+          }
+          qlo[a] |= ql << (8 * k);
+          qhi[a] |= qh << (8 * k);
+        }
+      }
+    }
+    q[0] = origin[0]; q[1] = origin[1]; q[2] = origin[2]; q[3] = scale[0];
+    q[4] = scale[1]; q[5] = scale[2];
+    memcpy(&q[6], &qlo[0], 4);
+    memcpy(&q[7], &qlo[1], 4);
+    memcpy(&q[8], &qlo[2], 4);
+    memcpy(&q[9], &qhi[0], 4);
+    memcpy(&q[10], &qhi[1], 4);
+    memcpy(&q[11], &qhi[2], 4);
+    memcpy(&q[12], codes, 16);
+  }
+}
+
 static int scene_create_impl(const float* prims, uint32_t primCount, const float* nodes,
                              uint32_t nodeCount, const float* mats, uint32_t matCount,
                              const int32_t* lights, uint32_t lightCount,
-                             const uint32_t* primObject, const ct_sky* sky, hipMemcpyKind kind,
+                             const uint32_t* primObject, const ct_sky* sky,
+                             const uint32_t* leafWordsIn, hipMemcpyKind kind,
                              hipStream_t stream, ct_scene** out) {
   if (!out) return set_error(CT_EINVAL, "ct_scene_create: null out");
   *out = nullptr;
@@ -187,11 +291,17 @@ static int scene_create_impl(const float* prims, uint32_t primCount, const float
   if (nodeCount > (1u << 24))
     return set_error(CT_ELIMIT,
                      "ct_scene_create: skip links are float32 and exact only below 2^24 nodes");
-  std::vector<float> wide;
+  if (primCount >= (1u << 28))
+    return set_error(CT_ELIMIT, "ct_scene_create: more than 2^28 primitives");
+  std::vector<float> wide, qwide;
+  std::vector<uint32_t> words;
+  bool binaryExact = true;
   if (kind == hipMemcpyHostToDevice) {
-    int v = validate_scene(nodes, nodeCount, primCount, matCount, lights, lightCount, primObject);
+    binaryExact = leaf_words(nodes, nodeCount, leafWordsIn, words);
+    int v = validate_scene(nodes, nodeCount, primCount, matCount, lights, lightCount, primObject,
+                           words.data());
     if (v) return v;
-    build_wide(nodes, nodeCount, wide);
+    build_wide(nodes, nodeCount, words.data(), wide);
   } else {
     std::vector<float> hn((size_t)nodeCount * NODE_STRIDE);
     std::vector<int32_t> hl(lightCount);
@@ -202,11 +312,13 @@ static int scene_create_impl(const float* prims, uint32_t primCount, const float
     if (primCount)
       CT_HIP(hipMemcpyAsync(ho.data(), primObject, ho.size() * 4, hipMemcpyDeviceToHost, stream));
     CT_HIP(hipStreamSynchronize(stream));
+    binaryExact = leaf_words(hn.data(), nodeCount, leafWordsIn, words);
     int v = validate_scene(hn.data(), nodeCount, primCount, matCount, hl.data(), lightCount,
-                           ho.data());
+                           ho.data(), words.data());
     if (v) return v;
-    build_wide(hn.data(), nodeCount, wide);
+    build_wide(hn.data(), nodeCount, words.data(), wide);
   }
+  quantize_wide(wide, qwide);
   // The array lengths PackedScene carries: max(1, n) records each (pack.ts:136-138, 111).
   size_t pc = primCount ? primCount : 1, mc = matCount ? matCount : 1;
   size_t szPrims = align_up(pc * PRIM_STRIDE * sizeof(float));
@@ -215,7 +327,9 @@ static int scene_create_impl(const float* prims, uint32_t primCount, const float
   size_t szLights = align_up((lightCount ? lightCount : 1) * sizeof(int32_t));
   size_t szObj = align_up(pc * sizeof(uint32_t));
   size_t szWide = align_up(wide.size() * sizeof(float));
-  size_t total = szPrims + szNodes + szMats + szLights + szObj + szWide;
+  size_t szQwide = align_up(qwide.size() * sizeof(float));
+  size_t szWords = align_up(words.size() * sizeof(uint32_t));
+  size_t total = szPrims + szNodes + szMats + szLights + szObj + szWide + szQwide + szWords;
   ct_scene* s = new ct_scene();
   hipError_t e = hipMalloc(&s->buf, total);
   if (e != hipSuccess) {
@@ -231,6 +345,8 @@ static int scene_create_impl(const float* prims, uint32_t primCount, const float
   char* pLights = pMats + szMats;
   char* pObj = pLights + szLights;
   char* pWide = pObj + szObj;
+  char* pQwide = pWide + szWide;
+  char* pWords = pQwide + szQwide;
   e = hipMemsetAsync(base, 0, total, stream);
   if (e == hipSuccess && primCount)
     e = hipMemcpyAsync(pPrims, prims, (size_t)primCount * PRIM_STRIDE * sizeof(float), kind, stream);
@@ -245,6 +361,12 @@ static int scene_create_impl(const float* prims, uint32_t primCount, const float
   if (e == hipSuccess && !wide.empty())
     e = hipMemcpyAsync(pWide, wide.data(), wide.size() * sizeof(float), hipMemcpyHostToDevice,
                        stream);
+  if (e == hipSuccess && !qwide.empty())
+    e = hipMemcpyAsync(pQwide, qwide.data(), qwide.size() * sizeof(float), hipMemcpyHostToDevice,
+                       stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(pWords, words.data(), words.size() * sizeof(uint32_t),
+                       hipMemcpyHostToDevice, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e != hipSuccess) {
     (void)hipFree(s->buf);
@@ -259,6 +381,9 @@ static int scene_create_impl(const float* prims, uint32_t primCount, const float
   d.primObject = (const uint32_t*)pObj;
   d.wide = (const float4*)pWide;
   d.wideCount = (int)(wide.size() / 32);
+  d.qwide = (const float4*)pQwide;
+  d.leafWords = (const uint32_t*)pWords;
+  d.binaryExact = binaryExact ? 1 : 0;
   d.primCount = (int)primCount;
   d.nodeCount = (int)nodeCount;
   d.lightCount = (int)lightCount;
@@ -277,7 +402,18 @@ int ct_scene_create(const float* prims, uint32_t primCount, const float* nodes,
                     const int32_t* lights, uint32_t lightCount, const uint32_t* primObject,
                     const ct_sky* sky, ct_scene** out) {
   return scene_create_impl(prims, primCount, nodes, nodeCount, mats, matCount, lights,
-                           lightCount, primObject, sky, hipMemcpyHostToDevice, nullptr, out);
+                           lightCount, primObject, sky, nullptr, hipMemcpyHostToDevice, nullptr,
+                           out);
+}
+
+int ct_scene_create_ex(const float* prims, uint32_t primCount, const float* nodes,
+                       uint32_t nodeCount, const float* mats, uint32_t matCount,
+                       const int32_t* lights, uint32_t lightCount, const uint32_t* primObject,
+                       const ct_sky* sky, const uint32_t* leafWords, ct_scene** out) {
+  if (!leafWords) return set_error(CT_EINVAL, "ct_scene_create_ex: null leafWords");
+  return scene_create_impl(prims, primCount, nodes, nodeCount, mats, matCount, lights,
+                           lightCount, primObject, sky, leafWords, hipMemcpyHostToDevice,
+                           nullptr, out);
 }
 
 int ct_scene_create_device(const float* prims, uint32_t primCount, const float* nodes,
@@ -286,7 +422,7 @@ int ct_scene_create_device(const float* prims, uint32_t primCount, const float* 
                            const uint32_t* primObject, const ct_sky* sky, void* stream,
                            ct_scene** out) {
   return scene_create_impl(prims, primCount, nodes, nodeCount, mats, matCount, lights,
-                           lightCount, primObject, sky, hipMemcpyDeviceToDevice,
+                           lightCount, primObject, sky, nullptr, hipMemcpyDeviceToDevice,
                            (hipStream_t)stream, out);
 }
 

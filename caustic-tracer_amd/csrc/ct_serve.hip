@@ -31,6 +31,7 @@ struct SJob {
   ct_job desc{};
   size_t bytes = 0;
   float* accum = nullptr;
+  float* comp = nullptr;  // compensation terms (FP32 precision form)
   uint32_t done = 0;      // passes added
   uint32_t inFlight = 0;  // passes in the round being traced
   int state = CT_JOB_QUEUED;
@@ -74,7 +75,7 @@ namespace {
 // Worker thread only (or after it has stopped): device frees never overlap
 // a graph capture of the worker's rounds.
 void free_job(ct_server* S, SJob* j) {
-  if (j->accum) (void)hipFree(j->accum);
+  if (j->accum) (void)hipFree(j->accum);  // one allocation holds the sum and the terms
   if (j->charged) S->st.hbmInUse -= j->bytes;
   delete j;
 }
@@ -141,7 +142,7 @@ void serve_loop(ct_server* S) {
       if (S->opts.slicePasses) p = std::min(p, S->opts.slicePasses);
       PlanJob pj{d.scene, make_frame(d.frame), {d.maxBounces, d.russianRoulette != 0, d.nee != 0},
                  d.width, d.height, d.seed * 7919u, j->done, p, nullptr,
-                 j->done + p == d.spp, 1.0 / d.spp};
+                 j->done + p == d.spp, 1.0 / d.spp, nullptr};
       plan.push_back(pj);
       picked.push_back(j);
       j->inFlight = p;
@@ -159,11 +160,14 @@ void serve_loop(ct_server* S) {
       }
       if (zero_floats(j->accum, j->bytes / sizeof(float), S->stream) != CT_OK)
         j->error = ct_last_error();
+      if (S->opts.precision == CT_PRECISION_F32)
+        j->comp = j->accum + (size_t)j->desc.width * j->desc.height * 3;
     }
     std::vector<PlanJob> run;
     for (size_t k = 0; k < picked.size(); k++) {
       if (!picked[k]->accum || !picked[k]->error.empty()) continue;
       plan[k].accum = picked[k]->accum;
+      plan[k].comp = picked[k]->comp;
       run.push_back(plan[k]);
     }
     Clock::time_point t = Clock::now();
@@ -240,6 +244,8 @@ int ct_server_create(const ct_server_opts* opts, ct_server** out) {
   if (!o.maxActive) o.maxActive = 1024;
   if (o.policy != CT_SERVE_FIFO && o.policy != CT_SERVE_SRPT)
     return set_error(CT_EINVAL, "ct_server_create: unknown policy");
+  if (o.precision != CT_PRECISION_F64 && o.precision != CT_PRECISION_F32)
+    return set_error(CT_EINVAL, "ct_server_create: unknown precision");
   ct_server* S = new ct_server();
   S->opts = o;
   hipError_t e = hipGetDevice(&S->device);
@@ -250,6 +256,7 @@ int ct_server_create(const ct_server_opts* opts, ct_server** out) {
   int rc = ct_tracer_create(&S->T);
   if (!rc && o.maxPaths) rc = ct_tracer_set_capacity(S->T, o.maxPaths);
   if (!rc && o.graph) rc = ct_tracer_set_graph(S->T, 1);
+  if (!rc) rc = ct_tracer_set_precision(S->T, o.precision);
   if (!rc) {
     e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->readStream, hipStreamNonBlocking);
@@ -296,7 +303,7 @@ int ct_job_submit(ct_server* S, const ct_job* job, uint64_t* id) {
     return set_error(CT_ELIMIT, "ct_job_submit: maxBounces must be in [0, 254]");
   if (npix > tracer_capacity(S->T))
     return set_error(CT_ELIMIT, "ct_job_submit: one pass exceeds the round's path capacity");
-  size_t bytes = npix * 3 * sizeof(float);
+  size_t bytes = npix * 3 * sizeof(float) * (S->opts.precision == CT_PRECISION_F32 ? 2 : 1);
   if (bytes > S->opts.hbmBudget)
     return set_error(CT_ELIMIT, "ct_job_submit: the progressive buffer exceeds the HBM budget");
   SJob* j = new SJob();
@@ -361,7 +368,7 @@ int ct_job_read(ct_server* S, uint64_t id, float* dst, int32_t dstOnDevice) {
   j = find(S, id);
   if (!j) return set_error(CT_EINVAL, "ct_job_read: job released while reading");
   if (!j->accum) return set_error(CT_EINVAL, "ct_job_read: the job has no buffer yet");
-  CT_HIP(hipMemcpyAsync(dst, j->accum, j->bytes,
+  CT_HIP(hipMemcpyAsync(dst, j->accum, (size_t)j->desc.width * j->desc.height * 3 * sizeof(float),
                         dstOnDevice ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                         S->readStream));
   CT_HIP(hipStreamSynchronize(S->readStream));

@@ -1,0 +1,151 @@
+// Device-side structures of the wavefront engine shared by the FP64 kernels
+// (reference arithmetic) and the FP32 precision kernels: the job table, the
+// SoA path state and the wave64 queue helpers.
+#pragma once
+
+#include "ct_engine.h"
+
+namespace ct {
+
+constexpr int RBLOCK = 256;
+constexpr int MAX_BOUNCE_SLOTS = 256;  // counters per kind (bounces + 1 <= this)
+
+// One job of a launch round (device copy).
+struct JobDev {
+  DevScene S;
+  Frame F;
+  TraceOpts opts;
+  uint32_t width, height, npix;
+  uint32_t passes;     // passes of this job in the round
+  uint32_t frameBase;  // frame index of its first pass (uint32 arithmetic)
+  uint32_t pathBase;   // first path id
+  uint32_t pixelBase;  // first finalize index
+  int doScale;         // multiply the sum by `scale` after adding (renderImage)
+  double scale;
+  float* accum;        // width*height*3 float32 sum
+  float* comp;         // width*height*3 compensation terms (FP32 precision form)
+  int fold;            // add the compensation into the sum after this round (FP32)
+};
+
+struct PathSoA {
+  double *ox, *oy, *oz, *dx, *dy, *dz;
+  double *br, *bg, *bb;
+  double *Lr, *Lg, *Lb;
+  double *prevPdf, *hitT;
+  uint32_t* rng;
+  int32_t* hitPrim;
+  uint32_t* jobOf;
+  uint8_t* specular;
+  uint32_t *q0, *q1;
+  // shadow queue (one slot per live path at most)
+  uint32_t* sPath;
+  double *sdx, *sdy, *sdz, *sTmax, *sCr, *sCg, *sCb;
+  // counters: live[b] at [b], shadow[b] at [MAX_BOUNCE_SLOTS + b], ray-fetch
+  // cursors of the extend/shadow launches at [2*MAX + b] and [3*MAX + b];
+  // 64-bit work counters at ctr64
+  uint32_t* ctr;
+  unsigned long long* ctr64;
+};
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// Append `pid` to a queue for every lane with `take`; one atomic per wave.
+__device__ __forceinline__ void wave_push(bool take, uint32_t pid, uint32_t* q, uint32_t* count) {
+  unsigned long long active = __ballot(1);
+  unsigned long long m = __ballot(take);
+  if (m == 0) return;
+  int lane = lane_id();
+  int leader = __ffsll((long long)active) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+  base = __shfl(base, leader, 64);
+  if (take) {
+    unsigned long long below = m & ((1ull << lane) - 1ull);
+    q[base + (uint32_t)__popcll(below)] = pid;
+  }
+}
+
+__device__ __forceinline__ uint32_t wave_reserve(bool take, uint32_t* count) {
+  unsigned long long active = __ballot(1);
+  unsigned long long m = __ballot(take);
+  int lane = lane_id();
+  int leader = __ffsll((long long)active) - 1;
+  uint32_t base = 0;
+  if (m && lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+  base = __shfl(base, leader, 64);
+  unsigned long long below = m & ((1ull << lane) - 1ull);
+  return base + (uint32_t)__popcll(below);
+}
+
+__device__ __forceinline__ void wave_add64(unsigned long long* dst, uint32_t v) {
+  unsigned long long x = v;
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  if ((threadIdx.x & 63) == 0 && x) atomicAdd(dst, x);
+}
+
+// FP32 path state of the precision form (same queues and counters).
+struct PathSoA32 {
+  float *ox, *oy, *oz, *dx, *dy, *dz;
+  float *br, *bg, *bb;
+  float *Lr, *Lg, *Lb;
+  float *prevPdf, *hitT;
+  uint32_t* rng;
+  int32_t* hitPrim;
+  uint32_t* jobOf;
+  uint8_t* specular;
+  uint32_t *q0, *q1;
+  uint32_t* sPath;
+  float *sdx, *sdy, *sdz, *sTmax, *sCr, *sCg, *sCb;
+  uint32_t* ctr;
+  unsigned long long* ctr64;
+};
+
+// Job of a path / finalize index: the last job whose base is <= i.
+template <bool MULTI, bool PIXEL>
+__device__ __forceinline__ int job_of(const JobDev* __restrict__ jobs, int njobs, size_t i) {
+  if (!MULTI) return 0;
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    size_t base = PIXEL ? jobs[mid].pixelBase : jobs[mid].pathBase;
+    if (base <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Lane refill shared by the persistent traversal kernels: lanes without a ray
+// take the next queue slots (one atomic per wave).  Returns the slot for this
+// lane or UINT32_MAX.
+__device__ __forceinline__ uint32_t refill(bool need, uint32_t n, uint32_t* fetch,
+                                           bool& exhausted) {
+  unsigned long long m = __ballot(need);
+  if (!m) return 0xffffffffu;
+  const int lane = __lane_id();
+  int leader = __ffsll((long long)__ballot(1)) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(fetch, (uint32_t)__popcll(m));
+  base = __shfl(base, leader, 64);
+  if (base + (uint32_t)__popcll(m) >= n) exhausted = true;
+  if (!need) return 0xffffffffu;
+  uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+  return slot < n ? slot : 0xffffffffu;
+}
+
+// One launch round of the FP32 precision kernels (ct_render32.hip).
+struct RoundLaunch {
+  PathSoA32 P32;
+  const JobDev* jobs;
+  int njobs;
+  size_t paths, pix;
+  int nb;  // bounce slots (maxBounces + 1)
+  int stack, block;
+  unsigned persistBlocks;
+  bool counting;
+  // profiling hook: kind 0 extend, 1 shade, 2 shadow
+  void (*mark)(void* ctx, hipStream_t st, int kind, bool begin);
+  void* markCtx;
+};
+int launch_round32(const RoundLaunch& R, bool multi, hipStream_t st);
+
+}  // namespace ct

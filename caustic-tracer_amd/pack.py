@@ -35,6 +35,7 @@ class PackedScene:
         self.primObject = primObject
         self.sky = sky
         self.order = order  # bvh.order
+        self.leafWords = None  # exact uint32 leaf words (layout extension) or None
 
 
 def _expand(scene):
@@ -175,9 +176,15 @@ def _prim_records(kind, a, b, c, r, obj):
     return P
 
 
-def buildBvh(bmin, bmax, bcen, bins=12):
+# Primitive count from which the float32 leaf words of the reference layout
+# (start*8 + count) stop being exact and the exact words travel beside them.
+EXACT_LEAF_LIMIT = 1 << 21
+
+
+def buildBvh(bmin, bmax, bcen, bins=12, leaf_words=False):
     """buildBvh (bvh.ts:299) on the GPU. Inputs are (n,3) binary64 PrimBounds
-    arrays (host numpy or DeviceBuffers). Returns (nodes f32, nodeCount, order u32)."""
+    arrays (host numpy or DeviceBuffers). Returns (nodes f32, nodeCount, order u32),
+    plus the exact uint32 leaf word of every node when leaf_words is set."""
     A.init()
     n = len(bmin) if not isinstance(bmin, DeviceBuffer) else bmin.shape[0]
     dev = [x if isinstance(x, DeviceBuffer) else
@@ -186,10 +193,18 @@ def buildBvh(bmin, bmax, bcen, bins=12):
     d_nodes = DeviceBuffer.empty((max(1, 2 * n - 1) * NODE_STRIDE,), np.float32)
     d_order = DeviceBuffer.empty((max(1, n),), np.uint32)
     nc = C.c_uint32(0)
-    check(lib().ct_bvh_build(dev[0].ptr, dev[1].ptr, dev[2].ptr, n, bins, d_nodes.ptr,
-                             C.byref(nc), d_order.ptr, None), "ct_bvh_build")
+    if leaf_words:
+        d_words = DeviceBuffer.empty((max(1, 2 * n - 1),), np.uint32)
+        check(lib().ct_bvh_build_ex(dev[0].ptr, dev[1].ptr, dev[2].ptr, n, bins, d_nodes.ptr,
+                                    C.byref(nc), d_order.ptr, d_words.ptr, None),
+              "ct_bvh_build_ex")
+    else:
+        check(lib().ct_bvh_build(dev[0].ptr, dev[1].ptr, dev[2].ptr, n, bins, d_nodes.ptr,
+                                 C.byref(nc), d_order.ptr, None), "ct_bvh_build")
     nodes = d_nodes.numpy()[: nc.value * NODE_STRIDE].copy()
     order = d_order.numpy()[:n].copy()
+    if leaf_words:
+        return nodes, nc.value, order, d_words.numpy()[: nc.value].copy()
     return nodes, nc.value, order
 
 
@@ -199,7 +214,11 @@ def packScene(scene, bins=12):
     n = len(kind)
     recs = _prim_records(kind, a, b, c, r, obj)
     mn, mx, cen = primBounds(kind, a, b, c, r)
-    nodes, nc, order = buildBvh(mn, mx, cen, bins)
+    words = None
+    if n >= EXACT_LEAF_LIMIT:
+        nodes, nc, order, words = buildBvh(mn, mx, cen, bins, leaf_words=True)
+    else:
+        nodes, nc, order = buildBvh(mn, mx, cen, bins)
     prims = np.zeros((max(1, n), PRIM_STRIDE), np.float32)
     primObject = np.zeros(max(1, n), np.uint32)
     if n:
@@ -212,5 +231,7 @@ def packScene(scene, bins=12):
         write_material(mats, i, ob["material"])
         emits[i] = material_emission(ob["material"]) is not None
     lights = np.flatnonzero(emits[primObject[:n]]).astype(np.int32) if n else np.zeros(0, np.int32)
-    return PackedScene(prims.reshape(-1), n, nodes, nc, mats, len(objs), lights, primObject,
-                       scene.get("sky", {"kind": "none"}), order)
+    ps = PackedScene(prims.reshape(-1), n, nodes, nc, mats, len(objs), lights, primObject,
+                     scene.get("sky", {"kind": "none"}), order)
+    ps.leafWords = words  # layout extension, set from EXACT_LEAF_LIMIT primitives on
+    return ps

@@ -26,10 +26,10 @@ def _status(s):
 
 class Server:
     def __init__(self, hbm_budget=0, max_paths=0, max_active=0, slice_passes=0, policy="fifo",
-                 graph=False):
+                 graph=False, precision=0):
         A.init()
         o = A.ServerOpts(int(hbm_budget), int(max_paths), int(max_active), int(slice_passes),
-                         POLICIES[policy], int(bool(graph)))
+                         POLICIES[policy], int(bool(graph)), int(precision))
         h = C.c_void_p()
         check(lib().ct_server_create(C.byref(o), C.byref(h)), "ct_server_create")
         self.handle = h

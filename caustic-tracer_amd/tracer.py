@@ -18,6 +18,9 @@ T_MIN = 1e-4  # intersect.ts:14
 WALK_STACKLESS = 0
 WALK_ORDERED = 1
 WALK_WIDE = 2
+WALK_BRUTE = 3
+PRECISION_F64 = 0
+PRECISION_F32 = 1
 
 
 class DeviceScene:
@@ -38,12 +41,17 @@ class DeviceScene:
         lights = np.ascontiguousarray(ps.lights, np.int32)
         pobj = np.ascontiguousarray(ps.primObject, np.uint32)
         h = C.c_void_p()
-        check(lib().ct_scene_create(prims.ctypes.data_as(C.c_void_p), ps.primCount,
-                                    nodes.ctypes.data_as(C.c_void_p), ps.nodeCount,
-                                    mats.ctypes.data_as(C.c_void_p), ps.matCount,
-                                    lights.ctypes.data_as(C.c_void_p), len(lights),
-                                    pobj.ctypes.data_as(C.c_void_p), C.byref(sky), C.byref(h)),
-              "ct_scene_create")
+        args = (prims.ctypes.data_as(C.c_void_p), ps.primCount, nodes.ctypes.data_as(C.c_void_p),
+                ps.nodeCount, mats.ctypes.data_as(C.c_void_p), ps.matCount,
+                lights.ctypes.data_as(C.c_void_p), len(lights), pobj.ctypes.data_as(C.c_void_p),
+                C.byref(sky))
+        words = getattr(ps, "leafWords", None)
+        if words is not None:
+            words = np.ascontiguousarray(words, np.uint32)
+            check(lib().ct_scene_create_ex(*args, words.ctypes.data_as(C.c_void_p), C.byref(h)),
+                  "ct_scene_create_ex")
+        else:
+            check(lib().ct_scene_create(*args, C.byref(h)), "ct_scene_create")
         self.handle = h
         self.primCount = ps.primCount
         self.nodeCount = ps.nodeCount
@@ -127,6 +135,58 @@ def occluded(ps, rays, tmax, tmin=T_MIN):
     check(lib().ct_occluded(ds.handle, d_rays.ptr, n, tmin, d_tmax.ptr, out.ptr, None),
           "ct_occluded")
     return out.numpy().astype(bool)
+
+
+def _rays32_on_device(rays):
+    if isinstance(rays, DeviceBuffer):
+        return rays, rays.nbytes // 24
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+    return DeviceBuffer.from_array(r), r.shape[0]
+
+
+def closest_hit32(ps, rays, tmax=None, walk=WALK_WIDE, tmin=T_MIN):
+    """Precision form: FP32 closest hit over the quantized 4-wide nodes
+    (walk=WALK_WIDE) or every primitive (WALK_BRUTE). rays: (n, 6) float32.
+    Returns host arrays prim, t, u, v (float32)."""
+    ds = device_scene(ps)
+    d_rays, n = _rays32_on_device(rays)
+    d_tmax = None if tmax is None else DeviceBuffer.from_array(np.ascontiguousarray(tmax, np.float32))
+    outs = [DeviceBuffer.empty((max(1, n),), t) for t in (np.int32, np.float32, np.float32, np.float32)]
+    check(lib().ct_closest_hit32(ds.handle, d_rays.ptr, n, C.c_float(tmin),
+                                 None if d_tmax is None else d_tmax.ptr, walk,
+                                 *[o.ptr for o in outs], None), "ct_closest_hit32")
+    return tuple(o.numpy()[:n] for o in outs)
+
+
+def closest_hit32_counts(ps, rays, tmax=None, tmin=T_MIN):
+    """Work of the quantized walk: {nodes (4 per node), tris, spheres, fallbacks}."""
+    ds = device_scene(ps)
+    d_rays, n = _rays32_on_device(rays)
+    d_tmax = None if tmax is None else DeviceBuffer.from_array(np.ascontiguousarray(tmax, np.float32))
+    c = DeviceBuffer.from_array(np.zeros(4, np.uint64))
+    check(lib().ct_closest_hit32_counts(ds.handle, d_rays.ptr, n, C.c_float(tmin),
+                                        None if d_tmax is None else d_tmax.ptr, c.ptr, None),
+          "ct_closest_hit32_counts")
+    h = c.numpy()
+    return {"nodes": int(h[0]), "tris": int(h[1]), "spheres": int(h[2]), "fallbacks": int(h[3])}
+
+
+def occluded32(ps, rays, tmax, walk=WALK_WIDE, tmin=T_MIN):
+    ds = device_scene(ps)
+    d_rays, n = _rays32_on_device(rays)
+    d_tmax = DeviceBuffer.from_array(np.ascontiguousarray(tmax, np.float32))
+    out = DeviceBuffer.empty((max(1, n),), np.uint8)
+    check(lib().ct_occluded32(ds.handle, d_rays.ptr, n, C.c_float(tmin), d_tmax.ptr, walk,
+                              out.ptr, None), "ct_occluded32")
+    return out.numpy()[:n].astype(bool)
+
+
+def image_sum_f64(img, npix):
+    """Per-channel binary64 sums of a device float32 RGB image, reduced on the GPU."""
+    out = np.zeros(3, np.float64)
+    check(lib().ct_image_sum_f64(img.ptr, int(npix), out.ctypes.data_as(C.c_void_p), None),
+          "ct_image_sum_f64")
+    return out
 
 
 def intersectClosest(ps, ox, oy, oz, dx, dy, dz, tMin, tMax, hit):
@@ -219,6 +279,10 @@ class Tracer:
 
     def set_graph(self, on):
         check(lib().ct_tracer_set_graph(self.handle, int(bool(on))), "set_graph")
+
+    def set_precision(self, precision):
+        """PRECISION_F64 (reference arithmetic) or PRECISION_F32 (precision form)."""
+        check(lib().ct_tracer_set_precision(self.handle, int(precision)), "set_precision")
 
     def set_workgroup(self, lanes):
         check(lib().ct_tracer_set_workgroup(self.handle, int(lanes)), "set_workgroup")

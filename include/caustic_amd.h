@@ -74,6 +74,17 @@ typedef struct {
 /* Ordered walk over 4-wide nodes derived from the reference BVH at upload
  * (same closest hit; path tracing only). */
 #define CT_WALK_WIDE 2
+/* Every primitive tested in packed order (intersectBruteForce,
+ * intersect.ts:423-468); FP32 queries only. */
+#define CT_WALK_BRUTE 3
+
+/* Arithmetic of a tracer's kernels. */
+#define CT_PRECISION_F64 0 /* the reference's binary64 over float32 data (default) */
+/* Precision form: 8-bit parent-relative quantized 4-wide nodes (outward
+ * rounded; conservative traversal), FP32 intersection and shading, per-pixel
+ * Neumaier-compensated FP32 sums.  Images agree with the binary64 form in
+ * distribution (same sampling), not per sample. */
+#define CT_PRECISION_F32 1
 
 /* Counters of the last render on a tracer (not part of the reference API). */
 typedef struct {
@@ -141,6 +152,17 @@ int ct_scene_create_device(const float *d_prims, uint32_t primCount, const float
                            const int32_t *d_lights, uint32_t lightCount,
                            const uint32_t *d_primObject, const ct_sky *sky, void *stream,
                            ct_scene **out);
+/* Layout extension for scenes of 2^21 primitives or more, where the float32
+ * leaf words of pack.ts (start*8 + count, bvh.ts:60-62) stop being exact:
+ * h_leafWords holds the exact word of every node as uint32 (0xffffffff on
+ * interior nodes; ct_bvh_build_ex writes it).  The 4-wide walks read only
+ * these; the binary walks of the reference layout return CT_ELIMIT on a scene
+ * whose float32 words are inexact. */
+int ct_scene_create_ex(const float *h_prims, uint32_t primCount, const float *h_nodes,
+                       uint32_t nodeCount, const float *h_mats, uint32_t matCount,
+                       const int32_t *h_lights, uint32_t lightCount,
+                       const uint32_t *h_primObject, const ct_sky *sky,
+                       const uint32_t *h_leafWords, ct_scene **out);
 int ct_scene_destroy(ct_scene *scene);
 
 /* ---- closest hit / any hit ----------------------------------------------
@@ -158,6 +180,23 @@ int ct_occluded(const ct_scene *scene, const double *d_rays, uint32_t n, double 
  * Synchronises the stream. */
 int ct_closest_hit_counts(const ct_scene *scene, const double *d_rays, uint32_t n, double tMin,
                           const double *d_tMax, int32_t walk, uint64_t counts[3], void *stream);
+
+/* FP32 queries of the precision form: rays are 6 float32 per ray, tMax per
+ * ray (NULL = +inf); walk CT_WALK_WIDE (quantized nodes) or CT_WALK_BRUTE
+ * (every primitive; the exhaustive answer the walk must reproduce).  Exact
+ * ties in t may resolve to a different primitive than the scan's. */
+int ct_closest_hit32(const ct_scene *scene, const float *d_rays, uint32_t n, float tMin,
+                     const float *d_tMax, int32_t walk, int32_t *d_prim, float *d_t, float *d_u,
+                     float *d_v, void *stream);
+int ct_occluded32(const ct_scene *scene, const float *d_rays, uint32_t n, float tMin,
+                  const float *d_tMax, int32_t walk, uint8_t *d_out, void *stream);
+/* Work of the quantized walk: adds node-slot tests (4 per node), triangle
+ * tests, sphere tests and stack fallbacks into d_counts4 (device uint64[4]). */
+int ct_closest_hit32_counts(const ct_scene *scene, const float *d_rays, uint32_t n, float tMin,
+                            const float *d_tMax, uint64_t *d_counts4, void *stream);
+/* Per-channel binary64 sums of a float32 RGB image (npix pixels) computed on
+ * the device in a fixed order (the furnace energy balance); synchronises. */
+int ct_image_sum_f64(const float *d_img, uint64_t npix, double *h_sum3, void *stream);
 
 /* ---- path tracing ---------------------------------------------------------
  * A tracer owns the wavefront path state (SoA queues in HBM) and grows it on
@@ -200,6 +239,9 @@ int ct_tracer_set_capacity(ct_tracer *tracer, uint64_t maxPaths);
 /* Capture each round's launches into a hipGraph and replay it while the
  * round's shape (paths, pixels, jobs, bounces, settings) repeats. */
 int ct_tracer_set_graph(ct_tracer *tracer, int32_t enable);
+/* CT_PRECISION_F64 or CT_PRECISION_F32 (which always walks the quantized
+ * 4-wide nodes; the walk settings then only pick the stack depth). */
+int ct_tracer_set_precision(ct_tracer *tracer, int32_t precision);
 /* Lanes per workgroup of the traversal launches: 64, 128 or 256 (default). */
 int ct_tracer_set_workgroup(ct_tracer *tracer, int32_t lanes);
 /* Counters and device times of the last accumulate/render (synchronises). */
@@ -268,6 +310,8 @@ typedef struct ct_server_opts {
   uint32_t slicePasses; /* passes one job adds per round at most (0: as many as fit) */
   int32_t policy;       /* CT_SERVE_FIFO or CT_SERVE_SRPT */
   int32_t graph;        /* replay rounds of a repeated shape as hipGraphs */
+  int32_t precision;    /* CT_PRECISION_F64 or CT_PRECISION_F32 (compensation terms
+                           double each job's buffer footprint) */
 } ct_server_opts;
 
 typedef struct ct_job_status {
@@ -316,6 +360,11 @@ int ct_server_stats_get(ct_server *server, ct_server_stats *out);
 int ct_bvh_build(const double *d_min, const double *d_max, const double *d_centroid,
                  uint32_t count, int32_t bins, float *d_nodes, uint32_t *h_nodeCount,
                  uint32_t *d_order, void *stream);
+/* ct_bvh_build plus the exact leaf word of every node in d_leafWords
+ * (room for max(1, 2*count-1) uint32; see ct_scene_create_ex). */
+int ct_bvh_build_ex(const double *d_min, const double *d_max, const double *d_centroid,
+                    uint32_t count, int32_t bins, float *d_nodes, uint32_t *h_nodeCount,
+                    uint32_t *d_order, uint32_t *d_leafWords, void *stream);
 
 #ifdef __cplusplus
 }

@@ -139,3 +139,24 @@ def test_replay_stream_latencies(ct):
         s = stream[idx]
         r = dict(reqs[idx % len(reqs)], seed=s["seed"])
         assert np.array_equal(_alone(ct, r).view(np.uint32), img.view(np.uint32))
+
+
+def test_alone_equals_in_mix_fp32(ct):
+    """Precision form through the server: the same image as the FP32 tracer
+    renders the job alone (compensation terms persist across rounds)."""
+    reqs = _requests(ct)
+    with ct.Server(max_paths=4096, slice_passes=2, precision=ct.PRECISION_F32) as srv:
+        ids = [srv.submit(r["scene"], r["frame"], r["width"], r["height"], r["spp"], seed=r["seed"],
+                          maxBounces=r["maxBounces"]) for r in reqs]
+        imgs = []
+        for jid, r in zip(ids, reqs):
+            srv.wait(jid, timeout=120)
+            imgs.append(srv.read(jid, r["width"], r["height"]))
+            srv.release(jid)
+        assert srv.stats()["rounds"] > 1
+    tr = ct.Tracer()
+    tr.set_precision(ct.PRECISION_F32)
+    for r, img in zip(reqs, imgs):
+        out = ct.DeviceBuffer.empty((r["width"] * r["height"] * 3,), np.float32)
+        tr.render_image(r["scene"], r["frame"], r, out)
+        assert np.array_equal(out.numpy().view(np.uint32), img.view(np.uint32))
