@@ -17,6 +17,11 @@ on a bounded row sample of the same frame on the host cores.
 20k), resolutions from {128,256,512,768}^2, 1-32 spp, maxBounces 8, traced
 together after the launch settings are autotuned, replayed as a hipGraph;
 beside it the same jobs rendered one after another with renderImage.
+--form precision: 256 tessellated glass spheres (16384 triangles each) on a
+65536-triangle checker plane under a 0.01-area light of radiance 500
+(4,259,842 triangles), 1024x1024, 64 spp, maxBounces 16, rendered by the
+precision form (quantized nodes, FP32, compensated sums) and by the binary64
+form; each image's RMS error against a binary64 render at --ref-spp.
 --form serve: 128 resident scenes (1k-1M triangles) and a Poisson request
 stream (random camera, 128^2-1024^2, 1-64 spp, maxBounces 4-12) through the
 serving loop; p50/p99 time-to-complete and aggregate throughput, beside the
@@ -87,7 +92,12 @@ def cpu_baseline(ct, ps, frame, w, h, spp, depth, budget_s=12.0):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--form", choices=["plain", "batch", "serve"], default="plain")
+    ap.add_argument("--form", choices=["plain", "batch", "serve", "precision", "copy"],
+                    default="plain")
+    ap.add_argument("--copy-mib", type=int, default=1024,
+                    help="copy: bytes moved by the calibration copy kernel (MiB)")
+    ap.add_argument("--ref-spp", type=int, default=4096,
+                    help="precision: spp of the binary64 reference image")
     ap.add_argument("--precision", choices=["f64", "f32"], default="f64",
                     help="f64: the reference's binary64 arithmetic; f32: the precision form "
                          "(quantized nodes, FP32 intersection and shading)")
@@ -121,10 +131,15 @@ def main():
               "MI355X); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.form == "copy":
+        print(json.dumps(copy_form(args)), flush=True)
+        return
     ct = load_package()
     ct.init(0)
     if args.form == "batch":
         res = batch_form(ct, args)
+    elif args.form == "precision":
+        res = precision_form(ct, args)
     elif args.form == "serve":
         res = serve_form(ct, args)
     else:
@@ -149,6 +164,23 @@ def extend_roofline(cnt, prof, kernel, precision="f64"):
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
             "bytes_per_launch": ext_bytes // launches,
             "avg_launch_ms": round(prof["extendMs"] / launches, 4)}
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "extend_traffic_soup1m_1080p_4spp.json")
+
+
+def measured_traffic(args):
+    """HBM bytes per extend launch for the default workload, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE runs (scaled by the copy-kernel
+    calibration); None for other workloads or settings."""
+    default = (args.tris == 1_000_000 and args.width == 1920 and args.height == 1080 and
+               args.spp == 4 and args.depth == 6 and args.walk == "wide" and args.stack == 24)
+    if not default or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    with open(TRAFFIC_PROFILE) as f:
+        prof = json.load(f)
+    entry = prof.get(args.precision)
+    return int(entry["traffic_bytes_per_launch"]) if entry else None
 
 
 def plain_form(ct, args):
@@ -220,6 +252,7 @@ def plain_form(ct, args):
     kname = (f"k_extend32<{args.stack}>" if args.precision == "f32"
              else f"k_extend<{args.walk},{args.stack}>")
     roof = extend_roofline(cnt, prof, kname, args.precision)
+    roof["traffic"] = measured_traffic(args)
     log(f"extend: {prof['extendLaunches']} launches, {prof['extendMs']:.2f} ms total, "
         f"{cnt['nodeVisits'] / max(1, cnt['extensionRays']):.1f} nodes/ray, "
         f"{cnt['triTests'] / max(1, cnt['extensionRays']):.1f} tris/ray, {roof['achieved']:.0f} GB/s, "
@@ -431,6 +464,110 @@ def serve_form(ct, args):
                                 "mrays_per_s": round(rays_plain / plain_s / 1e6, 2)},
         "speedup_vs_plain": round(sppmpix / (samples / plain_s / 1e6), 3),
         "same_as_alone": same, "roofline": None, "cpu_baseline": None,
+    }
+
+
+def copy_form(args):
+    """A plain device-to-device copy kernel of known size (torch's elementwise
+    copy), run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE to calibrate how
+    those counters report bytes that a kernel moves through HBM."""
+    import torch
+
+    n = args.copy_mib << 20
+    src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    return {"form": "copy", "bytes_read": n, "bytes_written": n, "launches": 4,
+            "gbs": round(2 * n / dt / 1e9, 1)}
+
+
+def precision_form(ct, args):
+    import numpy as np
+
+    t = time.perf_counter()
+    sc = ct.synth.caustic_scene()
+    ps = ct.packScene(sc)
+    ds = ct.DeviceScene(ps)
+    log(f"caustic scene: {ps.primCount} prims, {ps.nodeCount} nodes, exact leaf words "
+        f"{ps.leafWords is not None}, ready in {time.perf_counter() - t:.1f} s")
+    w = h = 1024
+    spp, depth = 64, 16
+    frame = ct.cameraFrame(sc["camera"], 1.0)
+    opts = {"width": w, "height": h, "spp": spp, "seed": 1, "maxBounces": depth,
+            "russianRoulette": True, "nee": True}
+    out = ct.DeviceBuffer.empty((w * h * 3,), np.float32)
+
+    ref_tr = ct.Tracer()
+    t = time.perf_counter()
+    ref_tr.render_image(ds, frame, dict(opts, spp=args.ref_spp, seed=1000), out)
+    ct.sync()
+    ref = out.numpy().astype(np.float64)
+    ref_s = time.perf_counter() - t
+    log(f"binary64 reference: {args.ref_spp} spp in {ref_s:.1f} s, mean {ref.mean():.5f}")
+    del ref_tr
+
+    res = {}
+    for prec in ("f64", "f32"):
+        tr = ct.Tracer()
+        tr.set_precision(ct.PRECISION_F32 if prec == "f32" else ct.PRECISION_F64)
+        for _ in range(args.warmup):
+            tr.render_image(ds, frame, opts, out)
+        ct.sync()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            tr.render_image(ds, frame, opts, out)
+        ct.sync()
+        total = time.perf_counter() - t
+        st = tr.stats()
+        img = out.numpy().astype(np.float64)
+        rms = float(np.sqrt(np.mean((img - ref) ** 2)))
+        rays = st["extensionRays"] + st["shadowRays"]
+        r = {"ms_per_step": total / args.steps * 1e3, "mrays": rays * args.steps / total / 1e6,
+             "sppmpix": w * h * spp * args.steps / total / 1e6, "rms": rms,
+             "rel_rms": rms / float(np.sqrt(np.mean(ref ** 2))), "rays": rays}
+        tr.set_profiling(True)
+        tr.render_image(ds, frame, opts, out)
+        prof = tr.stats()
+        tr.set_profiling(False)
+        tr.set_counting(True)
+        tr.render_image(ds, frame, opts, out)
+        cnt = tr.stats()
+        tr.set_counting(False)
+        r["roofline"] = extend_roofline(cnt, prof, "k_extend32<24>" if prec == "f32" else
+                                        "k_extend<wide,24>", prec)
+        r["fallbacks"] = cnt["stackFallbacks"]
+        log(f"{prec}: {r['ms_per_step']:.1f} ms/step, {r['mrays']:.0f} Mrays/s, "
+            f"{r['sppmpix']:.1f} spp*Mpix/s, RMS vs reference {rms:.6f} "
+            f"(relative {r['rel_rms']:.4f}), extend {prof['extendMs']:.1f} ms, "
+            f"{cnt['nodeVisits'] / max(1, cnt['extensionRays']):.1f} nodes/ray")
+        res[prec] = r
+    f32, f64 = res["f32"], res["f64"]
+    return {
+        "metric": METRIC, "value": round(f32["mrays"], 2), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(f32["ms_per_step"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"precision form: {ps.primCount}-triangle caustic scene (256 "
+                               f"tessellated glass spheres over a checker plane, 0.01-area "
+                               f"light of radiance 500), {w}x{h}, {spp} spp, maxBounces {depth}; "
+                               f"RMS against a binary64 render at {args.ref_spp} spp",
+                   "triangles": ps.primCount, "width": w, "height": h, "spp": spp,
+                   "max_bounces": depth, "parallelism": "single GPU"},
+        "spp_mpix_per_s": round(f32["sppmpix"], 3),
+        "rms_vs_reference": {"f32": f32["rms"], "f64": f64["rms"], "reference_spp": args.ref_spp,
+                             "f32_over_f64": round(f32["rms"] / f64["rms"], 4)},
+        "binary64_form": {"ms_per_step": round(f64["ms_per_step"], 3),
+                          "mrays_per_s": round(f64["mrays"], 2),
+                          "spp_mpix_per_s": round(f64["sppmpix"], 3),
+                          "roofline": f64["roofline"]},
+        "speedup_vs_binary64_at_equal_spp": round(f64["ms_per_step"] / f32["ms_per_step"], 3),
+        "roofline": f32["roofline"], "cpu_baseline": None,
     }
 
 

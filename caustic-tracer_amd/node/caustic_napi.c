@@ -17,6 +17,16 @@
  *                  russianRoulette, nee, accum)               (tracer.ts:361)
  *   renderImage(scene, frame16, width, height, spp, seed, maxBounces,
  *               russianRoulette, nee) -> Float32Array          (tracer.ts:348)
+ *   setPrecision(0|1)            arithmetic of the two calls above
+ *   createServer({hbmBudget, maxPaths, maxActive, slicePasses, policy,
+ *                 precision}) -> server handle                 (serving loop)
+ *   submitJob(server, scene, frame16, width, height, spp, seed, maxBounces,
+ *             russianRoulette, nee) -> job id
+ *   pollJob(server, id) -> {state, passesDone, spp, submitSec, admitSec, doneSec}
+ *   waitJob(server, id) -> Promise of the same status (N-API async work: the
+ *             wait runs on a libuv worker thread, the event loop stays free)
+ *   readJob(server, id, width, height) -> Float32Array (final image once done)
+ *   releaseJob(server, id); serverStats(server) -> counters
  */
 #define NAPI_VERSION 4
 #include <node_api.h>
@@ -397,6 +407,251 @@ static napi_value render_image(napi_env env, napi_callback_info info) {
   return res;
 }
 
+static napi_value set_precision(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value a[1];
+  CALL(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  if (!ensure_init(env)) return NULL;
+  int rc = 0;
+  if (!g_tracer) rc = ct_tracer_create(&g_tracer);
+  if (!rc) rc = ct_tracer_set_precision(g_tracer, (int32_t)num(env, a[0]));
+  if (rc) return throw_ct(env, rc, "ct_tracer_set_precision");
+  return NULL;
+}
+
+/* ---- serving ------------------------------------------------------------ */
+static void server_finalize(napi_env env, void *data, void *hint) {
+  (void)env;
+  (void)hint;
+  ct_server_destroy((ct_server *)data);
+}
+
+static ct_server *get_server(napi_env env, napi_value v) {
+  void *p = NULL;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, NULL, "caustic_amd: expected a server handle");
+    return NULL;
+  }
+  return (ct_server *)p;
+}
+
+static double prop_num(napi_env env, napi_value obj, const char *key, double dflt) {
+  bool has = false;
+  napi_value v;
+  if (napi_has_named_property(env, obj, key, &has) != napi_ok || !has) return dflt;
+  if (napi_get_named_property(env, obj, key, &v) != napi_ok) return dflt;
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  return t == napi_number ? num(env, v) : dflt;
+}
+
+static napi_value create_server(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value a[1];
+  CALL(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  if (!ensure_init(env)) return NULL;
+  ct_server_opts o;
+  memset(&o, 0, sizeof o);
+  napi_valuetype t = napi_undefined;
+  if (argc >= 1) napi_typeof(env, a[0], &t);
+  if (t == napi_object) {
+    o.hbmBudget = (uint64_t)prop_num(env, a[0], "hbmBudget", 0);
+    o.maxPaths = (uint64_t)prop_num(env, a[0], "maxPaths", 0);
+    o.maxActive = (uint32_t)prop_num(env, a[0], "maxActive", 0);
+    o.slicePasses = (uint32_t)prop_num(env, a[0], "slicePasses", 0);
+    o.policy = (int32_t)prop_num(env, a[0], "policy", 0);
+    o.precision = (int32_t)prop_num(env, a[0], "precision", 0);
+  }
+  ct_server *srv = NULL;
+  int rc = ct_server_create(&o, &srv);
+  if (rc) return throw_ct(env, rc, "ct_server_create");
+  napi_value ext;
+  CALL(env, napi_create_external(env, srv, server_finalize, NULL, &ext));
+  return ext;
+}
+
+static napi_value submit_job(napi_env env, napi_callback_info info) {
+  size_t argc = 10;
+  napi_value a[10];
+  CALL(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  if (argc < 10) {
+    napi_throw_error(env, NULL, "submitJob: 10 arguments required");
+    return NULL;
+  }
+  ct_server *srv = get_server(env, a[0]);
+  if (!srv) return NULL;
+  ct_scene *s = get_scene(env, a[1]);
+  if (!s) return NULL;
+  void *frame;
+  size_t nf;
+  if (!typed(env, a[2], napi_float64_array, &frame, &nf)) return NULL;
+  if (nf < 16) {
+    napi_throw_range_error(env, NULL, "submitJob: frame needs 16 values");
+    return NULL;
+  }
+  ct_job j;
+  memset(&j, 0, sizeof j);
+  j.scene = s;
+  memcpy(j.frame, frame, sizeof j.frame);
+  j.width = (uint32_t)num(env, a[3]);
+  j.height = (uint32_t)num(env, a[4]);
+  j.spp = (uint32_t)num(env, a[5]);
+  j.seed = (uint32_t)(int64_t)num(env, a[6]);
+  j.maxBounces = (int32_t)num(env, a[7]);
+  j.russianRoulette = (int32_t)num(env, a[8]);
+  j.nee = (int32_t)num(env, a[9]);
+  uint64_t id = 0;
+  int rc = ct_job_submit(srv, &j, &id);
+  if (rc) return throw_ct(env, rc, "ct_job_submit");
+  napi_value v;
+  CALL(env, napi_create_double(env, (double)id, &v));
+  return v;
+}
+
+static napi_value status_object(napi_env env, const ct_job_status *st) {
+  napi_value o, v;
+  if (napi_create_object(env, &o) != napi_ok) return NULL;
+#define SET(name, val)                                        \
+  if (napi_create_double(env, (double)(val), &v) != napi_ok || \
+      napi_set_named_property(env, o, name, v) != napi_ok)    \
+    return NULL;
+  SET("state", st->state)
+  SET("passesDone", st->passesDone)
+  SET("spp", st->spp)
+  SET("submitSec", st->submitSec)
+  SET("admitSec", st->admitSec)
+  SET("doneSec", st->doneSec)
+#undef SET
+  return o;
+}
+
+static napi_value poll_job(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value a[2];
+  CALL(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  ct_server *srv = get_server(env, a[0]);
+  if (!srv) return NULL;
+  ct_job_status st;
+  int rc = ct_job_poll(srv, (uint64_t)num(env, a[1]), &st);
+  if (rc) return throw_ct(env, rc, "ct_job_poll");
+  return status_object(env, &st);
+}
+
+typedef struct {
+  napi_async_work work;
+  napi_deferred deferred;
+  ct_server *srv;
+  uint64_t id;
+  ct_job_status st;
+  int rc;
+  char err[512];
+} WaitWork;
+
+static void wait_execute(napi_env env, void *data) {
+  (void)env;
+  WaitWork *w = (WaitWork *)data;
+  w->rc = ct_job_wait(w->srv, w->id, -1.0, &w->st);
+  if (w->rc) snprintf(w->err, sizeof w->err, "ct_job_wait failed (%d): %s", w->rc, ct_last_error());
+}
+
+static void wait_complete(napi_env env, napi_status status, void *data) {
+  WaitWork *w = (WaitWork *)data;
+  if (status == napi_ok && w->rc == 0) {
+    napi_value o = status_object(env, &w->st);
+    napi_resolve_deferred(env, w->deferred, o);
+  } else {
+    napi_value msg, err;
+    const char *m = w->rc ? w->err : "waitJob: async work failed";
+    napi_create_string_utf8(env, m, NAPI_AUTO_LENGTH, &msg);
+    napi_create_error(env, NULL, msg, &err);
+    napi_reject_deferred(env, w->deferred, err);
+  }
+  napi_delete_async_work(env, w->work);
+  free(w);
+}
+
+static napi_value wait_job(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value a[2];
+  CALL(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  ct_server *srv = get_server(env, a[0]);
+  if (!srv) return NULL;
+  WaitWork *w = (WaitWork *)calloc(1, sizeof *w);
+  if (!w) {
+    napi_throw_error(env, NULL, "waitJob: out of memory");
+    return NULL;
+  }
+  w->srv = srv;
+  w->id = (uint64_t)num(env, a[1]);
+  napi_value promise, name;
+  if (napi_create_promise(env, &w->deferred, &promise) != napi_ok ||
+      napi_create_string_utf8(env, "caustic_amd.waitJob", NAPI_AUTO_LENGTH, &name) != napi_ok ||
+      napi_create_async_work(env, NULL, name, wait_execute, wait_complete, w, &w->work) != napi_ok ||
+      napi_queue_async_work(env, w->work) != napi_ok) {
+    free(w);
+    napi_throw_error(env, NULL, "waitJob: could not queue the wait");
+    return NULL;
+  }
+  return promise;
+}
+
+static napi_value read_job(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value a[4];
+  CALL(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  ct_server *srv = get_server(env, a[0]);
+  if (!srv) return NULL;
+  size_t n = (size_t)num(env, a[2]) * (size_t)num(env, a[3]) * 3;
+  void *h;
+  napi_value res = new_typed(env, napi_float32_array, n, 4, &h);
+  if (!res) return NULL;
+  int rc = ct_job_read(srv, (uint64_t)num(env, a[1]), (float *)h, 0);
+  if (rc) return throw_ct(env, rc, "ct_job_read");
+  return res;
+}
+
+static napi_value release_job(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value a[2];
+  CALL(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  ct_server *srv = get_server(env, a[0]);
+  if (!srv) return NULL;
+  int rc = ct_job_release(srv, (uint64_t)num(env, a[1]));
+  if (rc) return throw_ct(env, rc, "ct_job_release");
+  return NULL;
+}
+
+static napi_value server_stats(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value a[1];
+  CALL(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  ct_server *srv = get_server(env, a[0]);
+  if (!srv) return NULL;
+  ct_server_stats st;
+  int rc = ct_server_stats_get(srv, &st);
+  if (rc) return throw_ct(env, rc, "ct_server_stats_get");
+  napi_value o, v;
+  CALL(env, napi_create_object(env, &o));
+#define SET(name, val)                                                   \
+  CALL(env, napi_create_double(env, (double)(val), &v));                 \
+  CALL(env, napi_set_named_property(env, o, name, v));
+  SET("submitted", st.submitted)
+  SET("completed", st.completed)
+  SET("failed", st.failed)
+  SET("queued", st.queued)
+  SET("active", st.active)
+  SET("rounds", st.rounds)
+  SET("paths", st.paths)
+  SET("extensionRays", st.extensionRays)
+  SET("shadowRays", st.shadowRays)
+  SET("pixelSamples", st.pixelSamples)
+  SET("hbmInUse", st.hbmInUse)
+  SET("busySec", st.busySec)
+  SET("nowSec", st.nowSec)
+#undef SET
+  return o;
+}
+
 static napi_value init(napi_env env, napi_value exports) {
   napi_property_descriptor props[] = {
       {"createScene", NULL, create_scene, NULL, NULL, NULL, napi_default, NULL},
@@ -405,6 +660,14 @@ static napi_value init(napi_env env, napi_value exports) {
       {"occluded", NULL, occluded, NULL, NULL, NULL, napi_default, NULL},
       {"accumulatePass", NULL, accumulate_pass, NULL, NULL, NULL, napi_default, NULL},
       {"renderImage", NULL, render_image, NULL, NULL, NULL, napi_default, NULL},
+      {"setPrecision", NULL, set_precision, NULL, NULL, NULL, napi_default, NULL},
+      {"createServer", NULL, create_server, NULL, NULL, NULL, napi_default, NULL},
+      {"submitJob", NULL, submit_job, NULL, NULL, NULL, napi_default, NULL},
+      {"pollJob", NULL, poll_job, NULL, NULL, NULL, napi_default, NULL},
+      {"waitJob", NULL, wait_job, NULL, NULL, NULL, napi_default, NULL},
+      {"readJob", NULL, read_job, NULL, NULL, NULL, napi_default, NULL},
+      {"releaseJob", NULL, release_job, NULL, NULL, NULL, napi_default, NULL},
+      {"serverStats", NULL, server_stats, NULL, NULL, NULL, napi_default, NULL},
   };
   CALL(env, napi_define_properties(env, exports, sizeof props / sizeof props[0], props));
   return exports;

@@ -91,7 +91,35 @@ function renderImage(ps, frame, opts) {
     seed, opts.maxBounces, opts.russianRoulette ? 1 : 0, opts.nee ? 1 : 0)
 }
 
+/** Arithmetic of accumulatePass / renderImage: 0 binary64 (reference), 1 FP32 precision form. */
+function setPrecision(p) {
+  addon.setPrecision(p)
+}
+
+/**
+ * Serving loop over resident scenes: jobs are renderImage calls that run with
+ * continuous batching on the library's own thread; wait() returns a Promise
+ * (N-API async work) so the event loop keeps accepting requests meanwhile.
+ */
+class Server {
+  constructor(opts) {
+    this.handle = addon.createServer(opts || {})
+  }
+  /** Queues renderImage(ps, frame, opts); returns the job id. */
+  submit(ps, frame, opts) {
+    return addon.submitJob(this.handle, deviceScene(ps), frameArray(frame), opts.width,
+      opts.height, opts.spp, opts.seed === undefined ? 0 : opts.seed, opts.maxBounces,
+      opts.russianRoulette ? 1 : 0, opts.nee ? 1 : 0)
+  }
+  poll(id) { return addon.pollJob(this.handle, id) }
+  wait(id) { return addon.waitJob(this.handle, id) }
+  read(id, width, height) { return addon.readJob(this.handle, id, width, height) }
+  release(id) { addon.releaseJob(this.handle, id) }
+  stats() { return addon.serverStats(this.handle) }
+}
+
 module.exports = {
   WALK_STACKLESS, WALK_ORDERED, T_MIN, buildBvh, intersectClosest, intersectClosestStackless,
   occluded, closestHitBatch, occludedBatch, accumulatePass, renderImage, frameArray,
+  setPrecision, Server,
 }

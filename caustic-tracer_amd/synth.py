@@ -144,6 +144,77 @@ def request_stream(n_scenes=128, rate=200.0, duration=60.0, seed=5, res_lo=128, 
     return out
 
 
+def uv_sphere(center, radius, bands=65, segments=128):
+    """Tessellated sphere: 2*segments*(bands-1) triangles (16384 for the
+    defaults), float32-rounded vertices, outward winding."""
+    th = np.linspace(0, math.pi, bands + 1)
+    ph = np.linspace(0, 2 * math.pi, segments + 1)[:-1]
+    st, ct_ = np.sin(th), np.cos(th)
+    ring = np.stack([st[:, None] * np.cos(ph)[None], np.broadcast_to(ct_[:, None], (bands + 1, segments)),
+                     st[:, None] * np.sin(ph)[None]], -1)  # (bands+1, segments, 3), y up
+    v = np.asarray(center, np.float64) + radius * ring
+    j = np.arange(segments)
+    j1 = (j + 1) % segments
+    tris = []
+    top, bot = v[0, 0], v[bands, 0]
+    tris.append(np.stack([np.broadcast_to(top, (segments, 3)), v[1, j1], v[1, j]], 1))
+    for b in range(1, bands - 1):
+        tris.append(np.stack([v[b, j], v[b, j1], v[b + 1, j1]], 1))
+        tris.append(np.stack([v[b, j], v[b + 1, j1], v[b + 1, j]], 1))
+    tris.append(np.stack([np.broadcast_to(bot, (segments, 3)), v[bands - 1, j], v[bands - 1, j1]], 1))
+    return _f32(np.concatenate(tris, 0))
+
+
+def caustic_scene(n_spheres=256, sphere_bands=65, sphere_segments=128, plane_rows=128,
+                  plane_cols=256, seed=3, light_area=0.01, light_radiance=500.0,
+                  light_height=6.0):
+    """Precision-form workload: n_spheres tessellated glass spheres (IOR 1.5,
+    radii uniform in [0.2, 1.0]) resting on a checker-textured diffuse plane of
+    2*plane_rows*plane_cols triangles, lit by one 2-triangle square light of
+    the given area and radiance facing down; no sky.  The defaults give
+    256 x 16384 + 65536 + 2 = 4,259,842 triangles."""
+    rng = np.random.default_rng(seed)
+    side = int(math.ceil(math.sqrt(n_spheres)))
+    pitch = 2.5
+    half = side * pitch / 2
+    objects = []
+    radii = rng.uniform(0.2, 1.0, n_spheres)
+    for i in range(n_spheres):
+        gx, gz = i % side, i // side
+        r = float(radii[i])
+        slack = pitch / 2 - r
+        cx = -half + (gx + 0.5) * pitch + rng.uniform(-slack, slack)
+        cz = -half + (gz + 0.5) * pitch + rng.uniform(-slack, slack)
+        tri = uv_sphere((cx, r, cz), r, sphere_bands, sphere_segments)
+        objects.append({"name": f"glass{i}", "material": {"type": "dielectric", "ior": 1.5},
+                        "shape": {"kind": "mesh", "vertices": tri.reshape(-1, 3),
+                                  "indices": np.arange(3 * len(tri))}})
+    ext = half + 2.0
+    xs = np.linspace(-ext, ext, plane_cols + 1)
+    zs = np.linspace(-ext, ext, plane_rows + 1)
+    X, Z = np.meshgrid(xs, zs)
+    P = np.stack([X, np.zeros_like(X), Z], -1)
+    a, b, c, d = P[:-1, :-1], P[:-1, 1:], P[1:, 1:], P[1:, :-1]
+    plane = np.concatenate([np.stack([a, c, b], -2).reshape(-1, 3, 3),
+                            np.stack([a, d, c], -2).reshape(-1, 3, 3)], 0)
+    plane = _f32(plane)
+    objects.append({"name": "plane", "material": {"type": "lambertian", "albedo": (0.8, 0.8, 0.8),
+                                                  "checker": {"color2": (0.1, 0.1, 0.1),
+                                                              "scale": 1.0}},
+                    "shape": {"kind": "mesh", "vertices": plane.reshape(-1, 3),
+                              "indices": np.arange(3 * len(plane))}})
+    h = math.sqrt(light_area) / 2
+    y = light_height
+    q = _f32(np.array([[[-h, y, -h], [h, y, h], [h, y, -h]], [[-h, y, -h], [-h, y, h], [h, y, h]]]))
+    objects.append({"name": "light", "material": {"type": "emissive", "color": (1.0, 1.0, 1.0),
+                                                  "strength": light_radiance},
+                    "shape": {"kind": "mesh", "vertices": q.reshape(-1, 3), "indices": np.arange(6)}})
+    dist = 34.0
+    camera = {"target": (0.0, 0.0, 0.0), "yaw": 0.35, "pitch": 0.6, "distance": dist,
+              "fovDeg": 50.0, "lensRadius": 0.0, "focusDistance": dist}
+    return {"name": "caustics", "objects": objects, "camera": camera, "sky": {"kind": "none"}}
+
+
 def random_rays(rng_stream, n, lo=-14.0, hi=14.0, tlo=-8.0, thi=8.0):
     """Rays in the shape of bvh.test.ts:94-99 from a stream of uniforms in [0,1):
     origins in [lo,hi]^3 aimed at points of [tlo,thi]^3, unit binary64 directions."""

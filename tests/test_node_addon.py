@@ -26,7 +26,7 @@ def test_addon_loads_and_exports_reference_names():
     assert out.returncode == 0, out.stderr
     names = set(out.stdout.strip().split(","))
     for n in ("buildBvh", "intersectClosest", "intersectClosestStackless", "occluded",
-              "accumulatePass", "renderImage"):
+              "accumulatePass", "renderImage", "setPrecision", "Server"):
         assert n in names
 
 
@@ -56,3 +56,39 @@ def test_node_render_matches_oracle(O, tmp_path):
     assert res["r"] and res["hit"]["prim"] == int(ref["prim"][0]) and res["hit"]["t"] == ref["t"][0]
     occ, _ = O.occluded_batch(ps, np.array([[0, 0.5, 1.6, 0, 0, -1.0]]), 100.0)
     assert res["blocked"] == bool(occ[0])
+
+
+@needs_node
+@pytest.mark.gpu
+def test_node_serving_jobs_match_oracle(O, tmp_path):
+    """Jobs through the serving loop from Node (waits are Promises resolved by
+    N-API async work) equal the oracle's renderImage of each job alone."""
+    sc = scenes.cornell_box()
+    ps = O.pack_scene(sc)
+    w, h = 32, 24
+    fr = O.camera_frame(sc["camera"], w / h)
+    jobs = [{"width": w, "height": h, "spp": 3, "seed": 1, "maxBounces": 8, "russianRoulette": True,
+             "nee": True},
+            {"width": w, "height": h, "spp": 5, "seed": 4, "maxBounces": 5, "russianRoulette": True,
+             "nee": True},
+            {"width": w, "height": h, "spp": 2, "seed": 9, "maxBounces": 8, "russianRoulette": False,
+             "nee": False}]
+    for name, arr in (("prims", ps.prims), ("nodes", ps.nodes), ("mats", ps.mats),
+                      ("lights", ps.lights), ("primObject", ps.primObject), ("frame", fr)):
+        arr.tofile(tmp_path / f"{name}.bin")
+    (tmp_path / "meta.json").write_text(json.dumps(
+        {"primCount": ps.primCount, "nodeCount": ps.nodeCount, "matCount": ps.matCount,
+         "jobs": jobs}))
+    script = os.path.join(ROOT, "tests", "node", "serve_jobs.js")
+    r = subprocess.run([NODE, script, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    res = json.loads((tmp_path / "serve.json").read_text())
+    assert all(s["state"] == 2 for s in res["statuses"])
+    assert res["stats"]["completed"] == 3 and res["stats"]["rounds"] > 1
+    for k, j in enumerate(jobs):
+        g = np.fromfile(tmp_path / f"job{k}.bin", np.float32)
+        o = O.render_image(ps, fr, w, h, j["spp"], seed=j["seed"], max_bounces=j["maxBounces"],
+                           russian_roulette=j["russianRoulette"], nee=j["nee"])
+        den = np.maximum(np.abs(g), np.abs(o)).astype(np.float64)
+        err = np.where(den > 0, np.abs(g - o) / np.where(den > 0, den, 1), 0)
+        assert err.max() <= 1e-4
