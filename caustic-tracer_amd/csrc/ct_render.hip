@@ -515,7 +515,9 @@ k_extend(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
   double oix = 0, oiy = 0, oiz = 0;
   Hit h;
   h.t = 0; h.u = 0; h.v = 0; h.prim = -1;
-  int cur = -1, sp = 0, pend = -1;
+  int cur = -1, sp = 0;
+  Parked pk;
+  pk.n = 0;
   for (;;) {
     if (!exhausted) {
       uint32_t slot = refill(!have, n, fetch, exhausted);
@@ -529,11 +531,10 @@ k_extend(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
         if (WALK == CT_WALK_STACKLESS) {
           closest_stackless<COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
           cur = -1;
-          pend = -1;
         } else {
           ix = safe_inv(dx); iy = safe_inv(dy); iz = safe_inv(dz);
           oix = ox * ix; oiy = oy * iy; oiz = oz * iz;
-          sp = 0; pend = -1;
+          sp = 0;
           cur = walk_root<WIDE>(S, ix, iy, iz, oix, oiy, oiz, h.t, cnt, COUNT);
         }
       }
@@ -544,8 +545,8 @@ k_extend(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
       if (WALK != CT_WALK_STACKLESS) {
         while (cur != -1) {
           if (cur <= -2) {
-            if (pend >= 0) break;
-            pend = -2 - cur;
+            if (pk.n == PARK_LEAVES) break;
+            park(pk, -2 - cur);
             cur = -1;
           } else if (!interior_step<WIDE, true, SD, COUNT>(S, ix, iy, iz, oix, oiy, oiz, h.t,
                                                            cur, sp, sNode, sEntry, lanes,
@@ -563,12 +564,11 @@ k_extend(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
               }
             }
           }
-          if (__all(pend >= 0 || cur == -1)) break;
+          if (__all(pk.n == PARK_LEAVES || cur == -1)) break;
         }
-        if (pend >= 0) {
-          test_leaf<COUNT>(S, pend, ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
-          pend = -1;
-        }
+        for (int k = 0; k < pk.n; k++)
+          test_leaf<COUNT>(S, parked(pk, k), ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
+        pk.n = 0;
         if (overflow) {
           if (COUNT) cnt.fallbacks++;
           closest_stackless<COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
@@ -576,7 +576,7 @@ k_extend(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
           sp = 0;
         }
       }
-      if (cur == -1 && pend < 0) {
+      if (cur == -1) {
         P.hitPrim[pid] = h.prim;
         P.hitT[pid] = h.t;
         have = false;
@@ -631,7 +631,9 @@ k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
   DevScene S = jobs[0].S;
   double ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0, tMax = 0;
   double oix = 0, oiy = 0, oiz = 0;
-  int cur = -1, sp = 0, pend = -1;
+  int cur = -1, sp = 0;
+  Parked pk;
+  pk.n = 0;
   for (;;) {
     if (!exhausted) {
       uint32_t slot = refill(!have, n, fetch, exhausted);
@@ -647,11 +649,10 @@ k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
         if (WALK == CT_WALK_STACKLESS) {
           blocked = occluded<COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, tMax, cnt);
           cur = -1;
-          pend = -1;
         } else {
           ix = safe_inv(dx); iy = safe_inv(dy); iz = safe_inv(dz);
           oix = ox * ix; oiy = oy * iy; oiz = oz * iz;
-          sp = 0; pend = -1;
+          sp = 0;
           cur = walk_root<WIDE>(S, ix, iy, iz, oix, oiy, oiz, tMax, cnt, COUNT);
         }
       }
@@ -662,8 +663,8 @@ k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
         bool overflow = false;
         while (cur != -1) {
           if (cur <= -2) {
-            if (pend >= 0) break;
-            pend = -2 - cur;
+            if (pk.n == PARK_LEAVES) break;
+            park(pk, -2 - cur);
             cur = -1;
           } else if (!interior_step<WIDE, false, SD, COUNT>(S, ix, iy, iz, oix, oiy, oiz, tMax,
                                                             cur, sp, sNode, nullptr, lanes,
@@ -676,12 +677,11 @@ k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
             sp--;
             cur = sNode[sp * lanes];
           }
-          if (__all(pend >= 0 || cur == -1)) break;
+          if (__all(pk.n == PARK_LEAVES || cur == -1)) break;
         }
-        if (pend >= 0) {
-          blocked = leaf_blocks(S, pend, ox, oy, oz, dx, dy, dz, tMax, cnt, COUNT);
-          pend = -1;
-        }
+        for (int k = 0; k < pk.n && !blocked; k++)
+          blocked = leaf_blocks(S, parked(pk, k), ox, oy, oz, dx, dy, dz, tMax, cnt, COUNT);
+        pk.n = 0;
         if (overflow && !blocked) {
           if (COUNT) cnt.fallbacks++;
           blocked = occluded<COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, tMax, cnt);
@@ -691,7 +691,7 @@ k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
           sp = 0;
         }
       }
-      if (cur == -1 && pend < 0) {
+      if (cur == -1) {
         if (!blocked) {
           P.Lr[pid] += P.sCr[slotCur];
           P.Lg[pid] += P.sCg[slotCur];

@@ -920,7 +920,9 @@ k_extend32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
   Ray R = make_ray(0, 0, 0, 1, 1, 1);
   Hit h;
   h.t = 0; h.u = 0; h.v = 0; h.prim = -1;
-  int cur = -1, sp = 0, pend = -1;
+  int cur = -1, sp = 0;
+  Parked pk;
+  pk.n = 0;
   for (;;) {
     if (!exhausted) {
       uint32_t slot = refill(!have, n, fetch, exhausted);
@@ -931,7 +933,7 @@ k_extend32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
         h.t = __builtin_inff(); h.u = 0; h.v = 0; h.prim = -1;
         have = true;
         sp = 0;
-        pend = -1;
+        pk.n = 0;
         cur = 0;
       }
     }
@@ -940,8 +942,8 @@ k_extend32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
       bool overflow = false;
       while (cur != -1) {
         if (cur <= -2) {
-          if (pend >= 0) break;
-          pend = -2 - cur;
+          if (pk.n == PARK_LEAVES) break;
+          park(pk, -2 - cur);
           cur = -1;
         } else if (!qwide_step<true, STACK, COUNT>(S, R, h.t * CULL_SLACK, cur, sp, sNode, sEntry,
                                                    lanes, cnt)) {
@@ -958,19 +960,17 @@ k_extend32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
             }
           }
         }
-        if (__all(pend >= 0 || cur == -1)) break;
+        if (__all(pk.n == PARK_LEAVES || cur == -1)) break;
       }
-      if (pend >= 0) {
-        test_leaf<COUNT>(S, pend, R, T_MIN_F, h, cnt);
-        pend = -1;
-      }
+      for (int k = 0; k < pk.n; k++) test_leaf<COUNT>(S, parked(pk, k), R, T_MIN_F, h, cnt);
+      pk.n = 0;
       if (overflow) {
         if (COUNT) cnt.fallbacks++;
         closest_stackless<COUNT>(S, R, T_MIN_F, h, cnt);
         cur = -1;
         sp = 0;
       }
-      if (cur == -1 && pend < 0) {
+      if (cur == -1) {
         P.hitPrim[pid] = h.prim;
         P.hitT[pid] = h.t;
         have = false;
@@ -1000,7 +1000,9 @@ k_shadow32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
   DevScene S = jobs[0].S;
   Ray R = make_ray(0, 0, 0, 1, 1, 1);
   float tMax = 0, tCull = 0;
-  int cur = -1, sp = 0, pend = -1;
+  int cur = -1, sp = 0;
+  Parked pk;
+  pk.n = 0;
   for (;;) {
     if (!exhausted) {
       uint32_t slot = refill(!have, n, fetch, exhausted);
@@ -1014,7 +1016,7 @@ k_shadow32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
         have = true;
         blocked = false;
         sp = 0;
-        pend = -1;
+        pk.n = 0;
         cur = 0;
       }
     }
@@ -1023,8 +1025,8 @@ k_shadow32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
       bool overflow = false;
       while (cur != -1) {
         if (cur <= -2) {
-          if (pend >= 0) break;
-          pend = -2 - cur;
+          if (pk.n == PARK_LEAVES) break;
+          park(pk, -2 - cur);
           cur = -1;
         } else if (!qwide_step<false, STACK, COUNT>(S, R, tCull, cur, sp, sNode, nullptr, lanes,
                                                     cnt)) {
@@ -1036,12 +1038,11 @@ k_shadow32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
           sp--;
           cur = sNode[sp * lanes];
         }
-        if (__all(pend >= 0 || cur == -1)) break;
+        if (__all(pk.n == PARK_LEAVES || cur == -1)) break;
       }
-      if (pend >= 0) {
-        blocked = leaf_blocks<COUNT>(S, pend, R, T_MIN_F, tMax, cnt);
-        pend = -1;
-      }
+      for (int k = 0; k < pk.n && !blocked; k++)
+        blocked = leaf_blocks<COUNT>(S, parked(pk, k), R, T_MIN_F, tMax, cnt);
+      pk.n = 0;
       if (overflow && !blocked) {
         if (COUNT) cnt.fallbacks++;
         blocked = occluded_stackless<COUNT>(S, R, T_MIN_F, tMax, cnt);
@@ -1050,7 +1051,7 @@ k_shadow32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
         cur = -1;
         sp = 0;
       }
-      if (cur == -1 && pend < 0) {
+      if (cur == -1) {
         if (!blocked) {
           P.Lr[pid] += P.sCr[slotCur];
           P.Lg[pid] += P.sCg[slotCur];

@@ -132,6 +132,37 @@ __device__ __forceinline__ uint32_t refill(bool need, uint32_t n, uint32_t* fetc
   return slot < n ? slot : 0xffffffffu;
 }
 
+// Leaves a lane of the persistent walks parks before the wave tests them
+// together: a lane keeps descending after reaching a leaf until it holds
+// PARK_LEAVES of them (or has nothing left), so the wave's interior phase and
+// its leaf phase both run with most lanes busy.  Parking only delays when the
+// closest distance shrinks: the walk visits a superset of the unparked walk's
+// boxes and returns the same minimum.
+#ifndef CT_PARK_LEAVES
+#define CT_PARK_LEAVES 3
+#endif
+constexpr int PARK_LEAVES = CT_PARK_LEAVES;
+
+struct Parked {
+  int w[PARK_LEAVES];  // leaf words
+  int n;
+};
+
+__device__ __forceinline__ void park(Parked& p, int word) {
+#pragma unroll
+  for (int k = 0; k < PARK_LEAVES; k++)
+    if (p.n == k) p.w[k] = word;
+  p.n++;
+}
+
+__device__ __forceinline__ int parked(const Parked& p, int k) {
+  int w = p.w[0];
+#pragma unroll
+  for (int j = 1; j < PARK_LEAVES; j++)
+    if (k == j) w = p.w[j];
+  return w;
+}
+
 // One launch round of the FP32 precision kernels (ct_render32.hip).
 struct RoundLaunch {
   PathSoA32 P32;
