@@ -68,15 +68,15 @@ def cpu_baseline(ct, ps, frame, w, h, spp, depth, budget_s=12.0):
                     ps.order, ps.sky)
     threads = max(1, min(16, os.cpu_count() or 1))
     acc = np.zeros(w * h * 3, np.float32)
-    y0 = h // 2
+    y0 = h // 2 - threads // 2
     # probe: one pass over `threads` rows
     t = time.perf_counter()
     c = O.accumulate_pass(view, frame, w, h, 0, acc, max_bounces=depth, threads=threads,
                           rows=(y0, y0 + threads))
-    probe = time.perf_counter() - t
-    per_row_pass = probe / threads * threads  # wall seconds per `threads` rows
-    rows = threads * max(1, int(budget_s / max(per_row_pass * spp, 1e-3)))
-    rows = min(rows, h - y0)
+    probe = time.perf_counter() - t  # wall seconds for one pass over `threads` rows
+    rows = threads * max(1, int(budget_s / max(probe * spp, 1e-3)))
+    rows = min(rows, h)
+    y0 = max(0, (h - rows) // 2)  # a band centred on the frame, the whole frame if affordable
     acc[:] = 0
     t = time.perf_counter()
     rays = 0
@@ -114,6 +114,8 @@ def main():
     ap.add_argument("--stack", type=int, default=24, help="LDS stack entries of the ordered walks")
     ap.add_argument("--persist-blocks", type=int, default=2048,
                     help="workgroups of the persistent traversal launches (0: one lane per ray)")
+    ap.add_argument("--rays", type=int, default=1 << 24,
+                    help="plain: random closest-hit queries timed beside the render (0: none)")
     ap.add_argument("--workgroup", type=int, default=256, choices=[64, 128, 256],
                     help="lanes per workgroup of the traversal launches")
     ap.add_argument("--no-cpu", action="store_true")
@@ -260,6 +262,7 @@ def plain_form(ct, args):
         f"shadow {prof['shadowMs']:.2f} ms "
         f"({cnt['shadowNodeVisits'] / max(1, cnt['shadowRays']):.1f} nodes/ray, "
         f"{cnt['shadowStackFallbacks']} fallbacks)")
+    chq = closest_hit_queries(ct, ds, ps, args.rays) if args.rays > 0 else None
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (oracle on host cores)...")
@@ -279,9 +282,59 @@ def plain_form(ct, args):
                    "max_bounces": depth, "parallelism": "single GPU"},
         "spp_mpix_per_s": round(sppmpix, 3),
         "rays_per_step": rays, "bvh_build_ms": round(build_ms, 2),
+        "closest_hit_queries": chq,
         "roofline": roof, "cpu_baseline": cpu,
     }
     return res
+
+
+def closest_hit_queries(ct, ds, ps, n):
+    """Closest-hit-only pass on n random rays over the same scene, in the shape
+    of bvh.test.ts:94-99 (origins in [-14,14]^3 aimed at points of [-8,8]^3,
+    unit binary64 directions): throughput of the 4-wide walk, whether every
+    ray's primitive and t equal the stackless walk's, and a sample checked
+    against the oracle's ordered walk."""
+    import numpy as np
+
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O  # the checker
+
+    g = np.random.default_rng(99)
+    rays = ct.synth.random_rays(g.random(6 * n), n)
+    d_rays = ct.DeviceBuffer.from_array(rays)
+    prim = ct.DeviceBuffer.empty((n,), np.int32)
+    tt = ct.DeviceBuffer.empty((n,), np.float64)
+    uu = ct.DeviceBuffer.empty((n,), np.float64)
+    vv = ct.DeviceBuffer.empty((n,), np.float64)
+    lib = ct._capi.lib()
+
+    def run():
+        ct._capi.check(lib.ct_closest_hit(ds.handle, d_rays.ptr, n, ct.T_MIN, None, ct.WALK_WIDE,
+                                          prim.ptr, tt.ptr, uu.ptr, vv.ptr, None),
+                       "ct_closest_hit")
+
+    run()  # warm
+    ct.sync()
+    reps = 3
+    t = time.perf_counter()
+    for _ in range(reps):
+        run()
+    ct.sync()
+    dt = (time.perf_counter() - t) / reps
+    pw, tw = prim.numpy(), tt.numpy()
+    s = ct.tracer.closest_hit(ds, d_rays, walk=ct.WALK_STACKLESS)
+    agree = bool(np.array_equal(pw, s["prim"]) and np.array_equal(tw, s["t"]))
+    k = 4096
+    view = O.Packed(ps.prims, ps.nodes, ps.nodeCount, ps.mats, ps.lights, ps.primObject,
+                    ps.order, ps.sky)
+    o = O.closest_batch(view, rays[:k], walk=O.WALK_ORDERED)
+    sample = bool(np.array_equal(pw[:k], o["prim"]) and np.array_equal(tw[:k], o["t"]))
+    log(f"closest-hit queries: {n} rays, {n / dt / 1e6:.0f} Mrays/s (wide walk), all equal to "
+        f"the stackless walk: {agree}, {k}-ray sample equal to the oracle: {sample}, "
+        f"hits {(pw >= 0).mean():.3f}")
+    return {"rays": n, "mrays_per_s": round(n / dt / 1e6, 1), "walk": "wide",
+            "equal_to_stackless": agree, "oracle_sample_rays": k, "equal_to_oracle": sample,
+            "hit_fraction": round(float((pw >= 0).mean()), 4)}
 
 
 def _digest(a):

@@ -4,6 +4,7 @@
 // scene is read straight from the packed float32 arrays: 2 x 16 B per node
 // and 3 x 16 B per primitive, the algorithmic minimum of the layout.
 #include "ct_internal.h"
+#include "ct_wavefront.h"
 
 namespace ct {
 
@@ -74,10 +75,112 @@ k_occluded(DevScene S, const double* __restrict__ rays, uint32_t n, double tMin,
   }
 }
 
+constexpr int WIDE_STACK = 24;
+
+// Closest hit over the derived 4-wide nodes, organised like the path
+// tracer's extend kernel: each lane walks its own rays (a fixed sequence
+// i, i + stride, ...), parks leaves until the wave's lanes all hold
+// PARK_LEAVES of them or are done, then the parked leaves are tested together;
+// a lane whose ray finishes takes its next ray at the top of the loop.
+template <bool COUNT>
+__global__ void __launch_bounds__(QUERY_BLOCK)
+k_closest_wide(DevScene S, const double* __restrict__ rays, uint32_t n, double tMin,
+               const double* __restrict__ tMax, int32_t* __restrict__ prim,
+               double* __restrict__ tOut, double* __restrict__ uOut, double* __restrict__ vOut,
+               unsigned long long* __restrict__ counts) {
+  __shared__ int sNodeAll[WIDE_STACK * QUERY_BLOCK];
+  __shared__ uint16_t sEntryAll[WIDE_STACK * QUERY_BLOCK];
+  int* sNode = sNodeAll + threadIdx.x;
+  uint16_t* sEntry = sEntryAll + threadIdx.x;
+  const int lanes = QUERY_BLOCK;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t next = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t i = 0;
+  TraversalCounts cnt = {0, 0, 0, 0};
+  bool have = false;
+  double ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0;
+  double oix = 0, oiy = 0, oiz = 0;
+  Hit h;
+  h.t = 0; h.u = 0; h.v = 0; h.prim = -1;
+  int cur = -1, sp = 0;
+  Parked pk;
+  pk.n = 0;
+  for (;;) {
+    if (!have && next < n) {
+      i = next;
+      next += stride;
+      load_ray(rays, i, ox, oy, oz, dx, dy, dz);
+      h.t = tMax ? tMax[i] : __builtin_inf();
+      h.u = 0; h.v = 0; h.prim = -1;
+      ix = safe_inv(dx); iy = safe_inv(dy); iz = safe_inv(dz);
+      oix = ox * ix; oiy = oy * iy; oiz = oz * iz;
+      sp = 0;
+      pk.n = 0;
+      cur = walk_root<true>(S, ix, iy, iz, oix, oiy, oiz, h.t, cnt, COUNT);
+      have = true;
+    }
+    if (!__any(have)) break;
+    if (have) {
+      bool overflow = false;
+      while (cur != -1) {
+        if (cur <= -2) {
+          if (pk.n == PARK_LEAVES) break;
+          park(pk, -2 - cur);
+          cur = -1;
+        } else if (!interior_step<true, true, WIDE_STACK, COUNT>(S, ix, iy, iz, oix, oiy, oiz, h.t,
+                                                                cur, sp, sNode, sEntry, lanes,
+                                                                cnt)) {
+          overflow = true;
+          cur = -1;
+          break;
+        }
+        if (cur == -1) {
+          while (sp > 0) {
+            sp--;
+            if (entry_from_bf16(sEntry[sp * lanes]) < h.t) {
+              cur = sNode[sp * lanes];
+              break;
+            }
+          }
+        }
+        if (__all(pk.n == PARK_LEAVES || cur == -1)) break;
+      }
+      for (int k = 0; k < pk.n; k++)
+        test_leaf<COUNT>(S, parked(pk, k), ox, oy, oz, dx, dy, dz, tMin, h, cnt);
+      pk.n = 0;
+      if (overflow) {
+        if (COUNT) cnt.fallbacks++;
+        closest_stackless<COUNT>(S, ox, oy, oz, dx, dy, dz, tMin, h, cnt);
+        cur = -1;
+        sp = 0;
+      }
+      if (cur == -1) {
+        if (prim) prim[i] = h.prim;
+        if (tOut) tOut[i] = h.prim < 0 ? __builtin_inf() : h.t;
+        if (uOut) uOut[i] = h.u;
+        if (vOut) vOut[i] = h.v;
+        have = false;
+      }
+    }
+  }
+  if (COUNT) {
+    wave_add(counts + 0, cnt.nodes);
+    wave_add(counts + 1, cnt.tris);
+    wave_add(counts + 2, cnt.spheres);
+  }
+}
+
 template <int WALK, bool COUNT>
 static int launch_closest(const ct_scene* s, const double* rays, uint32_t n, double tMin,
                           const double* tMax, int32_t* prim, double* t, double* u, double* v,
                           unsigned long long* counts, hipStream_t stream) {
+  if (WALK == CT_WALK_WIDE) {
+    unsigned grid = grid_for(n, QUERY_BLOCK, 2048);
+    hipLaunchKernelGGL((k_closest_wide<COUNT>), dim3(grid), dim3(QUERY_BLOCK), 0, stream, s->dev,
+                       rays, n, tMin, tMax, prim, t, u, v, counts);
+    CT_LAUNCH_CHECK("k_closest_wide");
+    return CT_OK;
+  }
   unsigned grid = grid_for(n, QUERY_BLOCK, 256 * 32);
   hipLaunchKernelGGL((k_closest<WALK, COUNT>), dim3(grid), dim3(QUERY_BLOCK), 0, stream, s->dev,
                      rays, n, tMin, tMax, prim, t, u, v, counts);
@@ -102,6 +205,8 @@ int ct_closest_hit(const ct_scene* s, const double* rays, uint32_t n, double tMi
     return launch_closest<CT_WALK_ORDERED, false>(s, rays, n, tMin, tMax, prim, t, u, v, nullptr, st);
   if (walk == CT_WALK_STACKLESS)
     return launch_closest<CT_WALK_STACKLESS, false>(s, rays, n, tMin, tMax, prim, t, u, v, nullptr, st);
+  if (walk == CT_WALK_WIDE)
+    return launch_closest<CT_WALK_WIDE, false>(s, rays, n, tMin, tMax, prim, t, u, v, nullptr, st);
   return set_error(CT_EINVAL, "ct_closest_hit: unknown walk");
 }
 
@@ -120,6 +225,8 @@ int ct_closest_hit_counts(const ct_scene* s, const double* rays, uint32_t n, dou
     rc = launch_closest<CT_WALK_ORDERED, true>(s, rays, n, tMin, tMax, nullptr, nullptr, nullptr, nullptr, d, st);
   else if (walk == CT_WALK_STACKLESS)
     rc = launch_closest<CT_WALK_STACKLESS, true>(s, rays, n, tMin, tMax, nullptr, nullptr, nullptr, nullptr, d, st);
+  else if (walk == CT_WALK_WIDE)
+    rc = launch_closest<CT_WALK_WIDE, true>(s, rays, n, tMin, tMax, nullptr, nullptr, nullptr, nullptr, d, st);
   else
     rc = set_error(CT_EINVAL, "ct_closest_hit_counts: unknown walk");
   unsigned long long h[3] = {0, 0, 0};

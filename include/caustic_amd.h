@@ -72,7 +72,7 @@ typedef struct {
 #define CT_WALK_STACKLESS 0 /* intersect.ts:290-365 */
 #define CT_WALK_ORDERED 1   /* intersect.ts:202-282 */
 /* Ordered walk over 4-wide nodes derived from the reference BVH at upload
- * (same closest hit; path tracing only). */
+ * (same closest hit as the reference walks). */
 #define CT_WALK_WIDE 2
 /* Every primitive tested in packed order (intersectBruteForce,
  * intersect.ts:423-468); FP32 queries only. */

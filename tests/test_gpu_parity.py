@@ -37,7 +37,7 @@ def many_rays(O):
     return scenes.ray_set_bvh_test(O.Rng(99), 20000)
 
 
-@pytest.mark.parametrize("walk", [0, 1], ids=["stackless", "ordered"])
+@pytest.mark.parametrize("walk", [0, 1, 2], ids=["stackless", "ordered", "wide"])
 def test_closest_hit_bit_exact(ct, O, bvh_scene, many_rays, walk):
     owalk = O.WALK_STACKLESS if walk == ct.WALK_STACKLESS else O.WALK_ORDERED
     g = ct.closest_hit(bvh_scene, many_rays, walk=walk)
@@ -45,8 +45,13 @@ def test_closest_hit_bit_exact(ct, O, bvh_scene, many_rays, walk):
     assert (g["prim"] == o["prim"]).all()
     assert np.array_equal(g["t"], o["t"])
     hit = o["prim"] >= 0
+    if walk == ct.WALK_WIDE:
+        # a sphere hit keeps u, v from the last triangle test (intersect.ts:173-176),
+        # which depends on the visiting order: compare them on triangle hits
+        kind = np.asarray(bvh_scene.prims, np.float32).reshape(-1, 16)[:, 3]
+        hit &= kind[np.maximum(o["prim"], 0)] != 0
     assert np.array_equal(g["u"][hit], o["u"][hit]) and np.array_equal(g["v"][hit], o["v"][hit])
-    assert hit.sum() > 10000
+    assert hit.sum() > 5000
 
 
 def test_closest_hit_finite_tmax_and_counts(ct, O, bvh_scene):
