@@ -7,20 +7,25 @@ intersectClosest / occluded (intersect.ts), accumulatePass / renderImage
 HIP kernels through libcaustic_amd.so; importing works without a GPU, and any
 call that needs the library fails loudly when it is missing.
 """
-from . import serve, synth
+from . import display, presets, scene_io, serve, synth
+from .presets import SCENE_PRESETS, presetById
+from .scene_io import SceneParseError, parseScene, parseSceneJSON, sceneToJSON
+from .display import decodePPM, encodePPM, tonemapBuffer
 from ._capi import CTError, DeviceBuffer, init, sync
 from .pack import PackedScene, buildBvh, packScene, primBounds
 from .tracer import (PRECISION_F32, PRECISION_F64, T_MIN, WALK_BRUTE, WALK_ORDERED,
                      WALK_STACKLESS, WALK_WIDE, DeviceScene, Tracer, accumulatePass, cameraFrame,
                      closest_hit, closest_hit32, closest_hit32_counts, closest_hit_counts,
-                     image_sum_f64, intersectClosest, occluded, occluded32, renderImage)
+                     image_sum_f64, intersectClosest, occluded, occluded32, pick, primaryRay,
+                     renderImage)
 from .serve import Server
 
 __all__ = [
-    "serve", "Server", "synth", "CTError", "DeviceBuffer", "init", "sync", "PackedScene", "buildBvh", "packScene",
+    "presets", "SCENE_PRESETS", "presetById", "scene_io", "SceneParseError", "parseScene",
+    "parseSceneJSON", "sceneToJSON", "display", "tonemapBuffer", "encodePPM", "decodePPM", "serve", "Server", "synth", "CTError", "DeviceBuffer", "init", "sync", "PackedScene", "buildBvh", "packScene",
     "primBounds", "T_MIN", "WALK_ORDERED", "WALK_STACKLESS", "WALK_WIDE", "WALK_BRUTE",
     "PRECISION_F64", "PRECISION_F32", "closest_hit32", "closest_hit32_counts", "occluded32",
-    "image_sum_f64", "DeviceScene", "Tracer",
+    "image_sum_f64", "pick", "primaryRay", "DeviceScene", "Tracer",
     "accumulatePass", "cameraFrame", "closest_hit", "closest_hit_counts", "intersectClosest",
     "occluded", "renderImage",
 ]

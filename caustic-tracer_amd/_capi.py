@@ -125,6 +125,8 @@ SIGNATURES = {
     "ct_occluded32": [vp, vp, u32, C.c_float, vp, i32, vp, vp],
     "ct_closest_hit32_counts": [vp, vp, u32, C.c_float, vp, vp, vp],
     "ct_image_sum_f64": [vp, C.c_uint64, vp, vp],
+    "ct_tonemap": [vp, u32, u32, f64, f64, i32, vp, vp],
+    "ct_job_read_rgba": [vp, C.c_uint64, f64, i32, vp, i32],
     "ct_tracer_get_config": [vp, C.POINTER(TracerConfig)],
     "ct_tracer_set_config": [vp, C.POINTER(TracerConfig)],
     "ct_tracer_autotune": [vp, vp, u32, u32, C.c_char_p, C.POINTER(TuneResult), vp],

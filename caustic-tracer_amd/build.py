@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcaustic_amd.so")
 SOURCES = ["ct_runtime.hip", "ct_trace.hip", "ct_render.hip", "ct_render32.hip", "ct_bvh.hip",
-           "ct_tune.hip", "ct_serve.hip"]
+           "ct_tune.hip", "ct_serve.hip", "ct_display.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

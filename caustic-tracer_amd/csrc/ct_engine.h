@@ -46,6 +46,10 @@ int run_plan(ct_tracer* T, std::vector<PlanJob> jobs, hipStream_t st);
 // Sets n floats to zero with a kernel (safe under graph capture).
 int zero_floats(float* p, size_t n, hipStream_t st);
 
+// tonemapBuffer on the device: width*height*3 float32 -> width*height RGBA8.
+int launch_tonemap(const float* d_accum, size_t npix, double samples, double exposureEv,
+                   int mode, uint8_t* d_rgba, hipStream_t st);
+
 // Path capacity of a tracer (paths in flight per round).
 size_t tracer_capacity(const ct_tracer* T);
 

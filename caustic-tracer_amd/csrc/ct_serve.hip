@@ -375,6 +375,37 @@ int ct_job_read(ct_server* S, uint64_t id, float* dst, int32_t dstOnDevice) {
   return CT_OK;
 }
 
+int ct_job_read_rgba(ct_server* S, uint64_t id, double exposureEv, int32_t mode, uint8_t* dst,
+                     int32_t dstOnDevice) {
+  if (!S || !dst) return set_error(CT_EINVAL, "ct_job_read_rgba: null argument");
+  std::unique_lock<std::mutex> lk(S->mu);
+  SJob* j = find(S, id);
+  if (!j) return set_error(CT_EINVAL, "ct_job_read_rgba: unknown job");
+  S->cvDone.wait(lk, [&] {
+    SJob* x = find(S, id);
+    return !x || x->inFlight == 0;
+  });
+  j = find(S, id);
+  if (!j) return set_error(CT_EINVAL, "ct_job_read_rgba: job released while reading");
+  if (!j->accum) return set_error(CT_EINVAL, "ct_job_read_rgba: the job has no buffer yet");
+  size_t npix = (size_t)j->desc.width * j->desc.height;
+  // a finished buffer holds the average, a running one the sum of its passes
+  double samples = j->state == CT_JOB_DONE ? 1.0 : (double)j->done;
+  uint8_t* d = dst;
+  if (!dstOnDevice) CT_HIP(hipMallocAsync((void**)&d, npix * 4, S->readStream));
+  int rc = launch_tonemap(j->accum, npix, samples, exposureEv, mode, d, S->readStream);
+  hipError_t e = hipSuccess;
+  if (!rc && !dstOnDevice)
+    e = hipMemcpyAsync(dst, d, npix * 4, hipMemcpyDeviceToHost, S->readStream);
+  if (!dstOnDevice) {
+    hipError_t e2 = hipFreeAsync(d, S->readStream);
+    if (e == hipSuccess) e = e2;
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(S->readStream);
+  if (rc) return rc;
+  return check_hip(e, "ct_job_read_rgba");
+}
+
 int ct_job_release(ct_server* S, uint64_t id) {
   if (!S) return set_error(CT_EINVAL, "ct_job_release: null server");
   {

@@ -72,6 +72,15 @@ class Server:
         check(lib().ct_job_read(self.handle, jid, a.ctypes.data_as(C.c_void_p), 0), "ct_job_read")
         return a
 
+    def read_rgba(self, jid, width, height, exposureEv=0.0, mode="aces"):
+        """The job's frame tone mapped on the device (tonemap.ts:41-62): the final
+        image once done, the progressive average before. RGBA8 numpy array."""
+        a = np.empty(width * height * 4, np.uint8)
+        check(lib().ct_job_read_rgba(self.handle, jid, float(exposureEv),
+                                     {"aces": 0, "reinhard": 1}[mode],
+                                     a.ctypes.data_as(C.c_void_p), 0), "ct_job_read_rgba")
+        return a
+
     def release(self, jid):
         check(lib().ct_job_release(self.handle, jid), "ct_job_release")
 

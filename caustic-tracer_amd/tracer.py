@@ -227,6 +227,37 @@ def cameraFrame(cam, aspect):
                      cam["lensRadius"], fd if fd > 1e-3 else 1e-3], np.float64)
 
 
+def primaryRay(frame, sx, sy):
+    """primaryRay (camera.ts:75-77) = generateRay(frame, sx, sy, 0, 0)
+    (camera.ts:62-72) in binary64: the pinhole ray through (sx, sy), y up."""
+    f = [float(x) for x in frame]
+    o, r, u, fw = f[0:3], f[3:6], f[6:9], f[9:12]
+    th, aspect, lens, fd = f[12], f[13], f[14], f[15]
+    px = (2 * sx - 1) * th * aspect
+    py = (2 * sy - 1) * th
+    focus = [o[k] + ((r[k] * px + u[k] * py) + fw[k]) * fd for k in range(3)]
+    offset = [r[k] * (0.0 * lens) + u[k] * (0.0 * lens) for k in range(3)]
+    origin = [o[k] + offset[k] for k in range(3)]
+    d = _normalize([focus[k] - origin[k] for k in range(3)])
+    return origin, d
+
+
+def pick(ps, frame, sx, sy):
+    """pick (picking.ts:21-33): the object under normalised screen position
+    (sx, sy) in [0, 1] (sy = 1 at the top), found with the ordered walk on the
+    GPU.  Returns {objectIndex, prim, t, point} or None on a miss."""
+    o, d = primaryRay(frame, sx, sy)
+    ray = np.array([[*o, *d]], np.float64)
+    h = closest_hit(ps, ray, walk=WALK_ORDERED)
+    prim = int(h["prim"][0])
+    if prim < 0:
+        return None
+    t = float(h["t"][0])
+    obj = np.asarray(ps.primObject, np.uint32)[prim] if not isinstance(ps, DeviceScene) else None
+    return {"objectIndex": None if obj is None else int(obj), "prim": prim, "t": t,
+            "point": (o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t)}
+
+
 # ---------------------------------------------------------------------------
 # tracer.ts
 

@@ -351,6 +351,20 @@ int ct_job_read(ct_server *server, uint64_t id, float *dst, int32_t dstOnDevice)
 int ct_job_release(ct_server *server, uint64_t id);
 int ct_server_stats_get(ct_server *server, ct_server_stats *out);
 
+/* ---- display ----------------------------------------------------------------
+ * tonemapBuffer (tonemap.ts:41-62) on the device: each channel of the float32
+ * sum (width*height*3) times 2^exposureEv / samples, tone mapped (ACES fit or
+ * Reinhard), sRGB encoded, rounded to 8 bits; alpha 255.  d_rgba holds
+ * width*height*4 bytes.  Pair with a host P6 writer (ppm.ts:4-16). */
+#define CT_TONEMAP_ACES 0
+#define CT_TONEMAP_REINHARD 1
+int ct_tonemap(const float *d_accum, uint32_t width, uint32_t height, double samples,
+               double exposureEv, int32_t mode, uint8_t *d_rgba, void *stream);
+/* A served job's frame as 8-bit RGBA (width*height*4 bytes): the final image
+ * once done, the progressive average of the passes added so far before. */
+int ct_job_read_rgba(ct_server *server, uint64_t id, double exposureEv, int32_t mode,
+                     uint8_t *dst, int32_t dstOnDevice);
+
 /* ---- BVH build --------------------------------------------------------------
  * buildBvh (bvh.ts:299-301) on the GPU from device-resident PrimBounds
  * (bvh.ts:22-28: 3 binary64 per primitive for min, max and centroid).
