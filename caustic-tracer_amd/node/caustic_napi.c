@@ -27,6 +27,8 @@
  *             wait runs on a libuv worker thread, the event loop stays free)
  *   readJob(server, id, width, height) -> Float32Array (final image once done)
  *   releaseJob(server, id); serverStats(server) -> counters
+ *   tonemap(accum, width, height, samples, exposureEv, mode)
+ *       -> Uint8ClampedArray RGBA                              (tonemap.ts:41)
  */
 #define NAPI_VERSION 4
 #include <node_api.h>
@@ -652,6 +654,40 @@ static napi_value server_stats(napi_env env, napi_callback_info info) {
   return o;
 }
 
+static napi_value tonemap(napi_env env, napi_callback_info info) {
+  size_t argc = 6;
+  napi_value a[6];
+  CALL(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  if (!ensure_init(env)) return NULL;
+  void *acc;
+  size_t na;
+  if (!typed(env, a[0], napi_float32_array, &acc, &na)) return NULL;
+  uint32_t w = (uint32_t)num(env, a[1]), h = (uint32_t)num(env, a[2]);
+  size_t n = (size_t)w * h;
+  if (na < n * 3) {
+    napi_throw_range_error(env, NULL, "tonemap: accum needs width*height*3 values");
+    return NULL;
+  }
+  int rc = 0;
+  void *d_acc = to_device(acc, n * 12, &rc);
+  void *d_out = NULL;
+  if (!rc) rc = ct_device_alloc(&d_out, n * 4);
+  if (!rc) rc = ct_tonemap((const float *)d_acc, w, h, num(env, a[3]), num(env, a[4]),
+                           (int32_t)num(env, a[5]), (uint8_t *)d_out, NULL);
+  napi_value res = NULL;
+  if (!rc) {
+    void *hbuf;
+    napi_value ab;
+    if (napi_create_arraybuffer(env, n * 4, &hbuf, &ab) == napi_ok &&
+        napi_create_typedarray(env, napi_uint8_clamped_array, n * 4, ab, 0, &res) == napi_ok)
+      rc = ct_copy_d2h(hbuf, d_out, n * 4);
+  }
+  ct_device_free(d_acc);
+  ct_device_free(d_out);
+  if (rc) return throw_ct(env, rc, "ct_tonemap");
+  return res;
+}
+
 static napi_value init(napi_env env, napi_value exports) {
   napi_property_descriptor props[] = {
       {"createScene", NULL, create_scene, NULL, NULL, NULL, napi_default, NULL},
@@ -668,6 +704,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"readJob", NULL, read_job, NULL, NULL, NULL, napi_default, NULL},
       {"releaseJob", NULL, release_job, NULL, NULL, NULL, napi_default, NULL},
       {"serverStats", NULL, server_stats, NULL, NULL, NULL, napi_default, NULL},
+      {"tonemap", NULL, tonemap, NULL, NULL, NULL, napi_default, NULL},
   };
   CALL(env, napi_define_properties(env, exports, sizeof props / sizeof props[0], props));
   return exports;

@@ -91,6 +91,37 @@ function renderImage(ps, frame, opts) {
     seed, opts.maxBounces, opts.russianRoulette ? 1 : 0, opts.nee ? 1 : 0)
 }
 
+/** tonemapBuffer (tonemap.ts:41-62) on the GPU: float32 sum -> RGBA8. */
+function tonemapBuffer(accum, width, height, samples, exposureEv, mode, out) {
+  const rgba = addon.tonemap(accum, width, height, samples, exposureEv,
+    mode === 'reinhard' ? 1 : 0)
+  if (out) {
+    out.set(rgba)
+    return out
+  }
+  return rgba
+}
+
+/** pick (picking.ts:21-33): primaryRay (camera.ts:75) in binary64, closest hit on the GPU. */
+function pick(ps, frame, sx, sy) {
+  const f = frameArray(frame)
+  const px = (2 * sx - 1) * f[12] * f[13]
+  const py = (2 * sy - 1) * f[12]
+  const o = [f[0], f[1], f[2]]
+  const d = [0, 1, 2].map(k => o[k] + ((f[3 + k] * px + f[6 + k] * py) + f[9 + k]) * f[15] - o[k])
+  const len = Math.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])
+  const inv = len > 0 ? 1 / len : 0
+  const dir = d.map(x => x * inv)
+  const hit = { t: Infinity, prim: -1, u: 0, v: 0 }
+  if (!intersectClosest(ps, o[0], o[1], o[2], dir[0], dir[1], dir[2], T_MIN, Infinity, hit)) return null
+  return {
+    objectIndex: ps.primObject[hit.prim],
+    prim: hit.prim,
+    t: hit.t,
+    point: { x: o[0] + dir[0] * hit.t, y: o[1] + dir[1] * hit.t, z: o[2] + dir[2] * hit.t },
+  }
+}
+
 /** Arithmetic of accumulatePass / renderImage: 0 binary64 (reference), 1 FP32 precision form. */
 function setPrecision(p) {
   addon.setPrecision(p)
@@ -121,5 +152,5 @@ class Server {
 module.exports = {
   WALK_STACKLESS, WALK_ORDERED, T_MIN, buildBvh, intersectClosest, intersectClosestStackless,
   occluded, closestHitBatch, occludedBatch, accumulatePass, renderImage, frameArray,
-  setPrecision, Server,
+  setPrecision, Server, tonemapBuffer, pick,
 }

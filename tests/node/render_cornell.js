@@ -22,4 +22,7 @@ fs.writeFileSync(path.join(dir, 'out.bin'), Buffer.from(img.buffer))
 const hit = { t: Infinity, prim: -1, u: 0, v: 0 }
 const r = ct.intersectClosest(ps, 0, 0.5, 1.6, 0, 0, -1, 1e-4, Infinity, hit)
 const blocked = ct.occluded(ps, 0, 0.5, 1.6, 0, 0, -1, 1e-4, 100)
-fs.writeFileSync(path.join(dir, 'hit.json'), JSON.stringify({ r, hit, blocked }))
+const rgba = ct.tonemapBuffer(img, w, h, 1, 0, 'aces')
+fs.writeFileSync(path.join(dir, 'rgba.bin'), Buffer.from(rgba.buffer))
+const picked = ct.pick(ps, frame, 0.74, 0.15)
+fs.writeFileSync(path.join(dir, 'hit.json'), JSON.stringify({ r, hit, blocked, picked }))

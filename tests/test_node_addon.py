@@ -26,7 +26,7 @@ def test_addon_loads_and_exports_reference_names():
     assert out.returncode == 0, out.stderr
     names = set(out.stdout.strip().split(","))
     for n in ("buildBvh", "intersectClosest", "intersectClosestStackless", "occluded",
-              "accumulatePass", "renderImage", "setPrecision", "Server"):
+              "accumulatePass", "renderImage", "setPrecision", "Server", "tonemapBuffer", "pick"):
         assert n in names
 
 
@@ -56,6 +56,14 @@ def test_node_render_matches_oracle(O, tmp_path):
     assert res["r"] and res["hit"]["prim"] == int(ref["prim"][0]) and res["hit"]["t"] == ref["t"][0]
     occ, _ = O.occluded_batch(ps, np.array([[0, 0.5, 1.6, 0, 0, -1.0]]), 100.0)
     assert res["blocked"] == bool(occ[0])
+    # tonemapBuffer on the GPU and pick (scenes.test.ts:94-104: the glass sphere)
+    rgba = np.fromfile(tmp_path / "rgba.bin", np.uint8)
+    ref = O.tonemap_buffer(g, w, h, 1, 0.0, "aces")
+    d = np.abs(rgba.astype(np.int32) - ref.astype(np.int32))
+    assert d.max() <= 1 and (d > 0).sum() <= 2
+    assert res["picked"] is not None
+    assert sc["objects"][res["picked"]["objectIndex"]]["name"] == "Glass sphere"
+    assert res["picked"]["t"] > 1
 
 
 @needs_node
