@@ -520,9 +520,11 @@ __device__ __forceinline__ bool interior_step(const DevScene& S, double ix, doub
     int n = (int)h0 + (int)h1 + (int)h2 + (int)h3;
     // sorting network on (entry, code), ascending entry
 #define CT_CSWAP(a, b)                                        \
-  if (e[b] < e[a]) {                                          \
-    double te = e[a]; e[a] = e[b]; e[b] = te;                 \
-    int tc = c[a]; c[a] = c[b]; c[b] = tc;                    \
+  {                                                           \
+    bool sw_ = e[b] < e[a];                                   \
+    double lo_ = sw_ ? e[b] : e[a], hi_ = sw_ ? e[a] : e[b];  \
+    int cl_ = sw_ ? c[b] : c[a], ch_ = sw_ ? c[a] : c[b];     \
+    e[a] = lo_; e[b] = hi_; c[a] = cl_; c[b] = ch_;           \
   }
     CT_CSWAP(0, 1) CT_CSWAP(2, 3) CT_CSWAP(0, 2) CT_CSWAP(1, 3) CT_CSWAP(1, 2)
 #undef CT_CSWAP

@@ -292,10 +292,12 @@ __device__ __forceinline__ bool qwide_step(const DevScene& S, const Ray& R, floa
     e[k] = hit ? tn : INF;
     n += hit ? 1 : 0;
   }
-#define CT_CSWAP(a, b)                           \
-  if (e[b] < e[a]) {                             \
-    float te = e[a]; e[a] = e[b]; e[b] = te;     \
-    int tc = c[a]; c[a] = c[b]; c[b] = tc;       \
+#define CT_CSWAP(a, b)                                        \
+  {                                                           \
+    bool sw_ = e[b] < e[a];                                   \
+    float lo_ = sw_ ? e[b] : e[a], hi_ = sw_ ? e[a] : e[b];   \
+    int cl_ = sw_ ? c[b] : c[a], ch_ = sw_ ? c[a] : c[b];     \
+    e[a] = lo_; e[b] = hi_; c[a] = cl_; c[b] = ch_;           \
   }
   CT_CSWAP(0, 1) CT_CSWAP(2, 3) CT_CSWAP(0, 2) CT_CSWAP(1, 3) CT_CSWAP(1, 2)
 #undef CT_CSWAP
