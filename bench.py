@@ -149,15 +149,16 @@ def main():
     print(json.dumps(res), flush=True)
 
 
-def extend_roofline(cnt, prof, kernel, precision="f64"):
+def extend_roofline(cnt, prof, kernel, precision="f64", quantized=True):
     """Roofline of the closest-hit launches: algorithmic bytes from the counting
     run over the summed HIP-event time of the same launches.  Per counted node
-    slot: 32 B (binary64 form: a float32 box of the 4-wide node, 128 B per
-    node) or 16 B (precision form: 64 B quantized node); 48 B per triangle and
-    32 B per sphere test; per ray the ray in and the hit out (binary64: 24 + 16 B,
-    FP32: 24 + 8 B)."""
+    slot: 16 B for the 4-wide walk (a 64 B quantized node per 4 slots, both
+    precisions) or 32 B for the binary walks (one float32 node each); 48 B per
+    triangle and 32 B per sphere test; per ray the ray in and the hit out
+    (binary64: 24 + 16 B, FP32: 24 + 8 B)."""
     launches = max(1, prof["extendLaunches"])
-    node_b, ray_b = (16, 32) if precision == "f32" else (32, 40)
+    node_b = 16 if quantized or precision == "f32" else 32
+    ray_b = 32 if precision == "f32" else 40
     ext_bytes = (node_b * cnt["nodeVisits"] + 48 * cnt["triTests"] + 32 * cnt["sphereTests"] +
                  ray_b * cnt["extensionRays"])
     ext_s = prof["extendMs"] / 1e3
@@ -253,7 +254,7 @@ def plain_form(ct, args):
     tr.set_counting(False)
     kname = (f"k_extend32<{args.stack}>" if args.precision == "f32"
              else f"k_extend<{args.walk},{args.stack}>")
-    roof = extend_roofline(cnt, prof, kname, args.precision)
+    roof = extend_roofline(cnt, prof, kname, args.precision, quantized=args.walk == "wide")
     roof["traffic"] = measured_traffic(args)
     log(f"extend: {prof['extendLaunches']} launches, {prof['extendMs']:.2f} ms total, "
         f"{cnt['nodeVisits'] / max(1, cnt['extensionRays']):.1f} nodes/ray, "
@@ -412,7 +413,8 @@ def batch_form(ct, args):
     cnt = tr.stats()
     tr.set_counting(False)
     cfg = tune["config"]
-    roof = extend_roofline(cnt, prof, f"k_extend<walk {cfg['walk']},{cfg['stackDepth']}>")
+    roof = extend_roofline(cnt, prof, f"k_extend<walk {cfg['walk']},{cfg['stackDepth']}>",
+                           quantized=cfg["walk"] == ct.WALK_WIDE)
     cpu = None
     if not args.no_cpu:
         j = max(jobs, key=lambda x: x["width"])

@@ -7,7 +7,10 @@
 // unpacked FP32, so the double-precision twin costs registers, not issue
 // slots, and closest-hit IDs, t, u, v come out bit-identical to the
 // reference's.  The library is compiled with -ffp-contract=off so no a*b+c is
-// fused; f64 division and sqrt are the correctly rounded expansions.
+// fused; f64 division and sqrt are the correctly rounded expansions.  Which
+// boxes a walk enters is not part of that contract, only that it enters every
+// box holding a primitive the reference's walk would test: the 4-wide walk
+// (ct_qwalk.h) decides boxes in FP32 with conservative slack.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -33,19 +36,17 @@ struct DevScene {
   const float4* __restrict__ mats;   // 4 float4 per material
   const int32_t* __restrict__ lights;
   const uint32_t* __restrict__ primObject;
-  // 4-wide traversal nodes derived from `nodes` at upload: 8 float4 each,
-  // [min.x[4] | min.y[4] | min.z[4] | max.x[4] | max.y[4] | max.z[4] |
-  //  child code[4] | unused]; child code >= 0 another wide node, <= -2 a leaf
-  // (-2 - leaf word of the reference node), -1 an empty slot.
-  const float4* __restrict__ wide;
   // Exact leaf words (start*8 + count; 0xffffffff on interior nodes), one per
   // reference node: the float32 words of the reference layout are exact only
   // below 2^21 primitives.
   const uint32_t* __restrict__ leafWords;
-  // Quantized 4-wide nodes of the FP32 precision form, 4 float4 (64 B) each:
-  // [origin.xyz, scale.x | scale.y, scale.z, qlo.x, qlo.y | qlo.z, qhi.x,
-  //  qhi.y, qhi.z | child code[4]]; q* hold one byte per child; a child box
-  // is origin + q*scale (power-of-two scales), rounded outward at upload.
+  // 4-wide traversal nodes derived from `nodes` at upload, quantized, 4
+  // float4 (64 B) each: [origin.xyz, scale.x | scale.y, scale.z, qlo.x, qlo.y |
+  // qlo.z, qhi.x, qhi.y, qhi.z | child code[4]]; q* hold one byte per child; a
+  // child box is origin + q*scale (power-of-two scales), rounded outward at
+  // upload from the float32 box of the reference node it stands for.  Child
+  // code >= 0 another wide node, <= -2 a leaf (-2 - leaf word), -1 an empty
+  // slot.
   const float4* __restrict__ qwide;
   int wideCount;
   int binaryExact;  // every float32 leaf word equals its exact word
@@ -470,120 +471,51 @@ __device__ __forceinline__ void closest_wave(const DevScene& S, double ox, doubl
   }
 }
 
-// Slab test of child k of a wide node (see box_enter_fma).
-__device__ __forceinline__ bool box_enter_fma6(double ix, double iy, double iz, double oix,
-                                               double oiy, double oiz, float lx, float ly,
-                                               float lz, float hx, float hy, float hz,
-                                               double tMax, double& entry) {
-  double tx0 = fma((double)lx, ix, -oix), tx1 = fma((double)hx, ix, -oix);
-  double ty0 = fma((double)ly, iy, -oiy), ty1 = fma((double)hy, iy, -oiy);
-  double tz0 = fma((double)lz, iz, -oiz), tz1 = fma((double)hz, iz, -oiz);
-  double tn = fmax(fmax(fmin(tx0, tx1), fmin(ty0, ty1)), fmax(fmin(tz0, tz1), 0.0));
-  double tf = fmin(fmin(fmax(tx0, tx1), fmax(ty0, ty1)), fmin(fmax(tz0, tz1), tMax));
-  entry = tn;
-  return tn <= tf;
-}
-
-// One interior step of an ordered walk.  Binary form: test both children of
-// reference node `cur`, continue with the nearer, defer the other.  Wide form:
-// test the four children of wide node `cur`, continue with the nearest,
-// defer the rest nearest-last.  Deferred work goes on the LDS stack with a
-// bf16 lower bound of its entry (PUSH_ENTRY) or without one (any hit).
-// Returns false when the stack would overflow.
-template <bool WIDE, bool PUSH_ENTRY, int STACK, bool COUNT>
+// One interior step of the binary ordered walk: test both children of
+// reference node `cur`, continue with the nearer, defer the other on the LDS
+// stack with a bf16 lower bound of its entry (PUSH_ENTRY) or without one (any
+// hit).  Returns false when the stack would overflow.
+template <bool PUSH_ENTRY, int STACK, bool COUNT>
 __device__ __forceinline__ bool interior_step(const DevScene& S, double ix, double iy, double iz,
                                               double oix, double oiy, double oiz, double tMax,
                                               int& cur, int& sp, int* __restrict__ sNode,
                                               uint16_t* __restrict__ sEntry, int lanes,
                                               TraversalCounts& cnt) {
-  const double INF = __builtin_inf();
-  if constexpr (WIDE) {
-    const float4* w = S.wide + 8 * (size_t)cur;
-    float4 mnx = w[0], mny = w[1], mnz = w[2], mxx = w[3], mxy = w[4], mxz = w[5], cc = w[6];
-    if (COUNT) cnt.nodes += 4;
-    double e[4];
-    int c[4] = {__float_as_int(cc.x), __float_as_int(cc.y), __float_as_int(cc.z),
-                __float_as_int(cc.w)};
-    bool h0 = box_enter_fma6(ix, iy, iz, oix, oiy, oiz, mnx.x, mny.x, mnz.x, mxx.x, mxy.x, mxz.x, tMax, e[0]);
-    bool h1 = box_enter_fma6(ix, iy, iz, oix, oiy, oiz, mnx.y, mny.y, mnz.y, mxx.y, mxy.y, mxz.y, tMax, e[1]);
-    bool h2 = box_enter_fma6(ix, iy, iz, oix, oiy, oiz, mnx.z, mny.z, mnz.z, mxx.z, mxy.z, mxz.z, tMax, e[2]);
-    bool h3 = box_enter_fma6(ix, iy, iz, oix, oiy, oiz, mnx.w, mny.w, mnz.w, mxx.w, mxy.w, mxz.w, tMax, e[3]);
-    // empty slots (code -1) never count as hits
-    h0 = h0 && c[0] != -1;
-    h1 = h1 && c[1] != -1;
-    h2 = h2 && c[2] != -1;
-    h3 = h3 && c[3] != -1;
-    if (!h0) e[0] = INF;
-    if (!h1) e[1] = INF;
-    if (!h2) e[2] = INF;
-    if (!h3) e[3] = INF;
-    int n = (int)h0 + (int)h1 + (int)h2 + (int)h3;
-    // sorting network on (entry, code), ascending entry
-#define CT_CSWAP(a, b)                                        \
-  {                                                           \
-    bool sw_ = e[b] < e[a];                                   \
-    double lo_ = sw_ ? e[b] : e[a], hi_ = sw_ ? e[a] : e[b];  \
-    int cl_ = sw_ ? c[b] : c[a], ch_ = sw_ ? c[a] : c[b];     \
-    e[a] = lo_; e[b] = hi_; c[a] = cl_; c[b] = ch_;           \
-  }
-    CT_CSWAP(0, 1) CT_CSWAP(2, 3) CT_CSWAP(0, 2) CT_CSWAP(1, 3) CT_CSWAP(1, 2)
-#undef CT_CSWAP
-    if (n == 0) {
-      cur = -1;
-      return true;
-    }
-    if (sp + n - 1 > STACK) return false;
-    for (int k = 3; k >= 1; k--) {
-      if (k < n) {
-        sNode[sp * lanes] = c[k];
-        if (PUSH_ENTRY) sEntry[sp * lanes] = entry_bf16_down(e[k]);
-        sp++;
-      }
-    }
-    cur = c[0];
-    return true;
+  int left = cur + 1;
+  float4 llo = S.nodes[2 * (size_t)left], lhi = S.nodes[2 * (size_t)left + 1];
+  int right = (int)llo.w;
+  float4 rlo = S.nodes[2 * (size_t)right], rhi = S.nodes[2 * (size_t)right + 1];
+  if (COUNT) cnt.nodes += 2;
+  double tl, tr;
+  bool hl = box_enter_fma(ix, iy, iz, oix, oiy, oiz, llo, lhi, tMax, tl);
+  bool hr = box_enter_fma(ix, iy, iz, oix, oiy, oiz, rlo, rhi, tMax, tr);
+  int el = enc_child(S, left, lhi.w), er = enc_child(S, right, rhi.w);
+  if (hl && hr) {
+    if (sp == STACK) return false;
+    bool leftFirst = tl <= tr;
+    sNode[sp * lanes] = leftFirst ? er : el;
+    if (PUSH_ENTRY) sEntry[sp * lanes] = entry_bf16_down(leftFirst ? tr : tl);
+    sp++;
+    cur = leftFirst ? el : er;
+  } else if (hl) {
+    cur = el;
+  } else if (hr) {
+    cur = er;
   } else {
-    int left = cur + 1;
-    float4 llo = S.nodes[2 * (size_t)left], lhi = S.nodes[2 * (size_t)left + 1];
-    int right = (int)llo.w;
-    float4 rlo = S.nodes[2 * (size_t)right], rhi = S.nodes[2 * (size_t)right + 1];
-    if (COUNT) cnt.nodes += 2;
-    double tl, tr;
-    bool hl = box_enter_fma(ix, iy, iz, oix, oiy, oiz, llo, lhi, tMax, tl);
-    bool hr = box_enter_fma(ix, iy, iz, oix, oiy, oiz, rlo, rhi, tMax, tr);
-    int el = enc_child(S, left, lhi.w), er = enc_child(S, right, rhi.w);
-    if (hl && hr) {
-      if (sp == STACK) return false;
-      bool leftFirst = tl <= tr;
-      sNode[sp * lanes] = leftFirst ? er : el;
-      if (PUSH_ENTRY) sEntry[sp * lanes] = entry_bf16_down(leftFirst ? tr : tl);
-      sp++;
-      cur = leftFirst ? el : er;
-    } else if (hl) {
-      cur = el;
-    } else if (hr) {
-      cur = er;
-    } else {
-      cur = -1;
-    }
-    return true;
+    cur = -1;
   }
+  return true;
 }
 
-// Code of the traversal root: the binary root (tested against its box, as
-// intersect.ts:220-227 does) or wide node 0 (its children are tested instead).
-template <bool WIDE>
+// Code of the binary walk's root: the root box is tested first, as
+// intersect.ts:220-227 does.
 __device__ __forceinline__ int walk_root(const DevScene& S, double ix, double iy, double iz,
-                                        double oix, double oiy, double oiz, double tMax,
-                                        TraversalCounts& cnt, bool count) {
-  if constexpr (WIDE) {
-    return 0;
-  } else {
-    if (count) cnt.nodes++;
-    float4 lo = S.nodes[0], hi = S.nodes[1];
-    double e;
-    return box_enter_fma(ix, iy, iz, oix, oiy, oiz, lo, hi, tMax, e) ? enc_child(S, 0, hi.w) : -1;
-  }
+                                         double oix, double oiy, double oiz, double tMax,
+                                         TraversalCounts& cnt, bool count) {
+  if (count) cnt.nodes++;
+  float4 lo = S.nodes[0], hi = S.nodes[1];
+  double e;
+  return box_enter_fma(ix, iy, iz, oix, oiy, oiz, lo, hi, tMax, e) ? enc_child(S, 0, hi.w) : -1;
 }
 
 }  // namespace ct

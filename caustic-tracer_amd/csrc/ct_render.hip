@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "ct_engine.h"
+#include "ct_qwalk.h"
 #include "ct_wavefront.h"
 
 namespace ct {
@@ -492,9 +493,12 @@ k_shade(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__
 }
 
 // Persistent closest hit.  WALK: CT_WALK_STACKLESS (the reference's skip-link
-// loop, exact mirror), CT_WALK_ORDERED (binary nodes) or CT_WALK_WIDE
-// (derived 4-wide nodes); the ordered forms park leaves until the whole wave
-// holds one (closest_wave) and keep bf16 lower bounds of deferred entries.
+// loop, exact mirror), CT_WALK_ORDERED (binary nodes, binary64 slab tests) or
+// CT_WALK_WIDE (the quantized 4-wide nodes walked in FP32 with the binary64
+// ray's rounding folded into the plane slack); the ordered forms park leaves
+// until every lane of the wave holds PARK_LEAVES of them and keep bf16 lower
+// bounds of deferred entries.  Primitive tests are binary64 in every walk, so
+// the closest hit is the reference's.
 template <int WALK, int STACK, bool COUNT, bool MULTI, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
 k_extend(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__ q,
@@ -513,6 +517,7 @@ k_extend(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
   DevScene S = jobs[0].S;
   double ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0;
   double oix = 0, oiy = 0, oiz = 0;
+  qw::QRay Q = qw::qray(0, 0, 0, 1, 1, 1);
   Hit h;
   h.t = 0; h.u = 0; h.v = 0; h.prim = -1;
   int cur = -1, sp = 0;
@@ -531,11 +536,15 @@ k_extend(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
         if (WALK == CT_WALK_STACKLESS) {
           closest_stackless<COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
           cur = -1;
+        } else if (WIDE) {
+          Q = qw::qray64(S, ox, oy, oz, dx, dy, dz);
+          sp = 0;
+          cur = 0;
         } else {
           ix = safe_inv(dx); iy = safe_inv(dy); iz = safe_inv(dz);
           oix = ox * ix; oiy = oy * iy; oiz = oz * iz;
           sp = 0;
-          cur = walk_root<WIDE>(S, ix, iy, iz, oix, oiy, oiz, h.t, cnt, COUNT);
+          cur = walk_root(S, ix, iy, iz, oix, oiy, oiz, h.t, cnt, COUNT);
         }
       }
     }
@@ -548,19 +557,25 @@ k_extend(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
             if (pk.n == PARK_LEAVES) break;
             park(pk, -2 - cur);
             cur = -1;
-          } else if (!interior_step<WIDE, true, SD, COUNT>(S, ix, iy, iz, oix, oiy, oiz, h.t,
-                                                           cur, sp, sNode, sEntry, lanes,
-                                                           cnt)) {
+          } else if (!(WIDE ? qw::qwide_step<true, SD, COUNT>(S, Q, qw::cull_bound(h.t), cur, sp,
+                                                              sNode, sEntry, lanes, cnt)
+                            : interior_step<true, SD, COUNT>(S, ix, iy, iz, oix, oiy, oiz, h.t,
+                                                             cur, sp, sNode, sEntry, lanes,
+                                                             cnt))) {
             overflow = true;
             cur = -1;
             break;
           }
           if (cur == -1) {
-            while (sp > 0) {
-              sp--;
-              if (entry_from_bf16(sEntry[sp * lanes]) < h.t) {
-                cur = sNode[sp * lanes];
-                break;
+            if (WIDE) {
+              qw::qpop(cur, sp, sNode, sEntry, lanes, qw::cull_bound(h.t));
+            } else {
+              while (sp > 0) {
+                sp--;
+                if (entry_from_bf16(sEntry[sp * lanes]) < h.t) {
+                  cur = sNode[sp * lanes];
+                  break;
+                }
               }
             }
           }
@@ -631,6 +646,7 @@ k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
   DevScene S = jobs[0].S;
   double ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0, tMax = 0;
   double oix = 0, oiy = 0, oiz = 0;
+  qw::QRay Q = qw::qray(0, 0, 0, 1, 1, 1);
   int cur = -1, sp = 0;
   Parked pk;
   pk.n = 0;
@@ -649,11 +665,15 @@ k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
         if (WALK == CT_WALK_STACKLESS) {
           blocked = occluded<COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, tMax, cnt);
           cur = -1;
+        } else if (WIDE) {
+          Q = qw::qray64(S, ox, oy, oz, dx, dy, dz);
+          sp = 0;
+          cur = 0;
         } else {
           ix = safe_inv(dx); iy = safe_inv(dy); iz = safe_inv(dz);
           oix = ox * ix; oiy = oy * iy; oiz = oz * iz;
           sp = 0;
-          cur = walk_root<WIDE>(S, ix, iy, iz, oix, oiy, oiz, tMax, cnt, COUNT);
+          cur = walk_root(S, ix, iy, iz, oix, oiy, oiz, tMax, cnt, COUNT);
         }
       }
     }
@@ -666,9 +686,11 @@ k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
             if (pk.n == PARK_LEAVES) break;
             park(pk, -2 - cur);
             cur = -1;
-          } else if (!interior_step<WIDE, false, SD, COUNT>(S, ix, iy, iz, oix, oiy, oiz, tMax,
-                                                            cur, sp, sNode, nullptr, lanes,
-                                                            cnt)) {
+          } else if (!(WIDE ? qw::qwide_step<false, SD, COUNT>(S, Q, qw::cull_bound(tMax), cur,
+                                                               sp, sNode, nullptr, lanes, cnt)
+                            : interior_step<false, SD, COUNT>(S, ix, iy, iz, oix, oiy, oiz,
+                                                              tMax, cur, sp, sNode, nullptr,
+                                                              lanes, cnt))) {
             overflow = true;
             cur = -1;
             break;

@@ -15,6 +15,7 @@
 // the top 24 bits scaled by 2^-24 (in [0, 1) exactly in binary32).
 #include <string.h>
 
+#include "ct_qwalk.h"
 #include "ct_wavefront.h"
 
 namespace ct {
@@ -24,20 +25,8 @@ constexpr float T_MIN_F = 1e-4f;
 constexpr float TWO_PI_F = 6.28318530717958647692f;
 constexpr float INV_PI_F = 0.31830988618379067154f;
 constexpr float SHADOW_SHORTEN_F = 0.999f;
-// Slab planes of a quantized node are evaluated per axis as t = q*B + A' with
-// B = scale*inv (exact: scale is a power of two), A = (origin - o)*inv and
-// A' = A -+ E for the near/far plane, E = 2^-21 * (255|B| + |A|).  The
-// rounding of A, of 1/d, of A' and of the FMA together stays below about
-// 6.1 ulp-halvings (2^-24) of 255|B| + |A|, so E (8 ulp-halvings) moves every
-// near plane below and every far plane above its exact value.  The slack is
-// per axis: an axis where d is tiny gets a huge slack on its own planes only,
-// which then lie far outside the other axes' intervals.  The same constant
-// bounds the purely relative error (3 roundings) of the float32 box test.
-constexpr float SLAB_REL = 4.76837158203125e-7f;
-// Boxes and deferred entries are culled only beyond closest*(1 + 2^-14), so
-// rounding in the FP32 primitive tests cannot make the walk skip a primitive
-// whose computed hit would be nearer.
-constexpr float CULL_SLACK = 1.00006103515625f;
+using qw::CULL_SLACK;
+using qw::SLAB_REL;
 constexpr int QUERY_BLOCK = 256;
 constexpr int QUERY_STACK = 24;
 
@@ -50,22 +39,11 @@ __device__ __forceinline__ int pick(Rng& r, int n) {
   return (int)(m < f ? m : f);
 }
 __device__ __forceinline__ float max0(float x) { return x > 0 ? x : (x != x ? x : 0.0f); }
-// 1/d with |d| below 1e-20 treated as +-1e-20 (intersect.ts:142 uses 1e-20 for d == 0).
-__device__ __forceinline__ float safe_inv(float d) {
-  return 1.0f / (fabsf(d) < 1e-20f ? (d < 0.0f ? -1e-20f : 1e-20f) : d);
-}
-
-struct Ray {
-  float ox, oy, oz, dx, dy, dz, ix, iy, iz;
-};
-
+// The walk ray is the ray whose primitives are tested: no extra slack.
+using Ray = qw::QRay;
 __device__ __forceinline__ Ray make_ray(float ox, float oy, float oz, float dx, float dy,
                                         float dz) {
-  Ray r;
-  r.ox = ox; r.oy = oy; r.oz = oz;
-  r.dx = dx; r.dy = dy; r.dz = dz;
-  r.ix = safe_inv(dx); r.iy = safe_inv(dy); r.iz = safe_inv(dz);
-  return r;
+  return qw::qray(ox, oy, oz, dx, dy, dz);
 }
 
 struct Hit {
@@ -240,82 +218,8 @@ __device__ __forceinline__ bool occluded_stackless(const DevScene& S, const Ray&
   return false;
 }
 
-__device__ __forceinline__ float qbyte(uint32_t w, int k) {
-  return (float)((w >> (8 * k)) & 0xffu);
-}
-
-__device__ __forceinline__ uint16_t entry_bf16_down(float x) {
-  return (uint16_t)(__float_as_uint(x) >> 16);  // x >= 0: truncation rounds down
-}
-__device__ __forceinline__ float entry_from_bf16(uint16_t b) {
-  return __uint_as_float((uint32_t)b << 16);
-}
-
-// One step of the ordered walk over quantized 4-wide node `cur`: the four
-// child boxes are tested (planes q*B + A per axis, near/far chosen by the
-// ray's direction signs), the nearest hit child continues, the others go on
-// the LDS stack nearest-last.  Returns false when the stack would overflow.
-template <bool PUSH_ENTRY, int STACK, bool COUNT>
-__device__ __forceinline__ bool qwide_step(const DevScene& S, const Ray& R, float tCull, int& cur,
-                                           int& sp, int* __restrict__ sNode,
-                                           uint16_t* __restrict__ sEntry, int lanes,
-                                           TraversalCounts& cnt) {
-  const float INF = __builtin_inff();
-  const float4* w = S.qwide + 4 * (size_t)cur;
-  float4 w0 = w[0], w1 = w[1], w2 = w[2];
-  int4 cc = *(const int4*)(w + 3);
-  if (COUNT) cnt.nodes += 4;
-  float Bx = w0.w * R.ix, By = w1.x * R.iy, Bz = w1.y * R.iz;
-  float Ax = (w0.x - R.ox) * R.ix, Ay = (w0.y - R.oy) * R.iy, Az = (w0.z - R.oz) * R.iz;
-  float Ex = SLAB_REL * fmaf(255.0f, fabsf(Bx), fabsf(Ax));
-  float Ey = SLAB_REL * fmaf(255.0f, fabsf(By), fabsf(Ay));
-  float Ez = SLAB_REL * fmaf(255.0f, fabsf(Bz), fabsf(Az));
-  float Anx = Ax - Ex, Any = Ay - Ey, Anz = Az - Ez;  // near planes: lower bounds
-  float Afx = Ax + Ex, Afy = Ay + Ey, Afz = Az + Ez;  // far planes: upper bounds
-  uint32_t lx = __float_as_uint(w1.z), ly = __float_as_uint(w1.w), lz = __float_as_uint(w2.x);
-  uint32_t hx = __float_as_uint(w2.y), hy = __float_as_uint(w2.z), hz = __float_as_uint(w2.w);
-  uint32_t nx = R.ix < 0 ? hx : lx, fx = R.ix < 0 ? lx : hx;
-  uint32_t ny = R.iy < 0 ? hy : ly, fy = R.iy < 0 ? ly : hy;
-  uint32_t nz = R.iz < 0 ? hz : lz, fz = R.iz < 0 ? lz : hz;
-  float e[4];
-  int c[4] = {cc.x, cc.y, cc.z, cc.w};
-  int n = 0;
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    float tn = fmaxf(fmaxf(fmaf(qbyte(nx, k), Bx, Anx), fmaf(qbyte(ny, k), By, Any)),
-                     fmaf(qbyte(nz, k), Bz, Anz));
-    float tf = fminf(fminf(fmaf(qbyte(fx, k), Bx, Afx), fmaf(qbyte(fy, k), By, Afy)),
-                     fmaf(qbyte(fz, k), Bz, Afz));
-    tn = fmaxf(tn, 0.0f);
-    tf = fminf(tf, tCull);
-    bool hit = tn <= tf && c[k] != -1;
-    e[k] = hit ? tn : INF;
-    n += hit ? 1 : 0;
-  }
-#define CT_CSWAP(a, b)                                        \
-  {                                                           \
-    bool sw_ = e[b] < e[a];                                   \
-    float lo_ = sw_ ? e[b] : e[a], hi_ = sw_ ? e[a] : e[b];   \
-    int cl_ = sw_ ? c[b] : c[a], ch_ = sw_ ? c[a] : c[b];     \
-    e[a] = lo_; e[b] = hi_; c[a] = cl_; c[b] = ch_;           \
-  }
-  CT_CSWAP(0, 1) CT_CSWAP(2, 3) CT_CSWAP(0, 2) CT_CSWAP(1, 3) CT_CSWAP(1, 2)
-#undef CT_CSWAP
-  if (n == 0) {
-    cur = -1;
-    return true;
-  }
-  if (sp + n - 1 > STACK) return false;
-  for (int k = 3; k >= 1; k--) {
-    if (k < n) {
-      sNode[sp * lanes] = c[k];
-      if (PUSH_ENTRY) sEntry[sp * lanes] = entry_bf16_down(e[k]);
-      sp++;
-    }
-  }
-  cur = c[0];
-  return true;
-}
+using qw::entry_from_bf16;
+using qw::qwide_step;
 
 // Per-lane closest hit with the quantized walk (queries).
 template <int STACK, bool COUNT>

@@ -128,16 +128,23 @@ static bool leaf_words(const float* nodes, uint32_t nodeCount, const uint32_t* g
   return exact;
 }
 
-// 4-wide nodes from the binary tree: every interior node at even depth becomes
-// a wide node whose children are its grandchildren (or its children where
-// those are leaves).  Boxes are the reference nodes' float32 boxes, copied
-// bit for bit; only the intermediate boxes are skipped, which widens the set
-// of subtrees a walk enters and never narrows it.
+// 4-wide nodes from the binary tree: a wide node starts from the two children
+// of a binary node and, while it has fewer than four, replaces its interior
+// child of largest surface area by that child's two children (the greedy
+// collapse that opens the boxes a ray is most likely to enter).  Boxes are the
+// reference nodes' float32 boxes, copied bit for bit; only the intermediate
+// boxes are skipped, which widens the set of subtrees a walk enters and never
+// narrows it.
 static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* words,
                        std::vector<float>& out) {
   out.clear();
   auto isLeaf = [&](uint32_t i) { return words[i] != 0xffffffffu; };
   auto code = [&](uint32_t i) { return -2 - (int32_t)words[i]; };
+  auto area = [&](uint32_t i) {
+    const float* b = nodes + (size_t)i * NODE_STRIDE;
+    double x = (double)b[4] - b[0], y = (double)b[5] - b[1], z = (double)b[6] - b[2];
+    return x * y + y * z + z * x;
+  };
   struct Item { uint32_t n2; uint32_t slot; };  // binary node to expand into wide node `slot`
   std::vector<Item> work;
   auto alloc = [&]() {
@@ -172,15 +179,23 @@ static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* w
     work.pop_back();
     kids.clear();
     uint32_t L = it.n2 + 1;
-    uint32_t R = (uint32_t)nodes[(size_t)L * NODE_STRIDE + 3];
-    for (uint32_t c : {L, R}) {
-      if (isLeaf(c)) {
-        kids.push_back(c);
-      } else {
-        uint32_t cl = c + 1;
-        kids.push_back(cl);
-        kids.push_back((uint32_t)nodes[(size_t)cl * NODE_STRIDE + 3]);
+    kids.push_back(L);
+    kids.push_back((uint32_t)nodes[(size_t)L * NODE_STRIDE + 3]);
+    while (kids.size() < 4) {
+      int pick = -1;
+      double best = -1.0;
+      for (size_t k = 0; k < kids.size(); k++) {
+        if (isLeaf(kids[k])) continue;
+        double a = area(kids[k]);
+        if (a > best) {
+          best = a;
+          pick = (int)k;
+        }
       }
+      if (pick < 0) break;
+      uint32_t c = kids[pick];
+      kids[pick] = c + 1;
+      kids.push_back((uint32_t)nodes[(size_t)(c + 1) * NODE_STRIDE + 3]);
     }
     for (size_t k = 0; k < kids.size(); k++) {
       uint32_t g = kids[k];
@@ -326,10 +341,9 @@ static int scene_create_impl(const float* prims, uint32_t primCount, const float
   size_t szMats = align_up(mc * MAT_STRIDE * sizeof(float));
   size_t szLights = align_up((lightCount ? lightCount : 1) * sizeof(int32_t));
   size_t szObj = align_up(pc * sizeof(uint32_t));
-  size_t szWide = align_up(wide.size() * sizeof(float));
   size_t szQwide = align_up(qwide.size() * sizeof(float));
   size_t szWords = align_up(words.size() * sizeof(uint32_t));
-  size_t total = szPrims + szNodes + szMats + szLights + szObj + szWide + szQwide + szWords;
+  size_t total = szPrims + szNodes + szMats + szLights + szObj + szQwide + szWords;
   ct_scene* s = new ct_scene();
   hipError_t e = hipMalloc(&s->buf, total);
   if (e != hipSuccess) {
@@ -344,8 +358,7 @@ static int scene_create_impl(const float* prims, uint32_t primCount, const float
   char* pMats = pNodes + szNodes;
   char* pLights = pMats + szMats;
   char* pObj = pLights + szLights;
-  char* pWide = pObj + szObj;
-  char* pQwide = pWide + szWide;
+  char* pQwide = pObj + szObj;
   char* pWords = pQwide + szQwide;
   e = hipMemsetAsync(base, 0, total, stream);
   if (e == hipSuccess && primCount)
@@ -358,9 +371,6 @@ static int scene_create_impl(const float* prims, uint32_t primCount, const float
     e = hipMemcpyAsync(pLights, lights, (size_t)lightCount * sizeof(int32_t), kind, stream);
   if (e == hipSuccess && primCount)
     e = hipMemcpyAsync(pObj, primObject, (size_t)primCount * sizeof(uint32_t), kind, stream);
-  if (e == hipSuccess && !wide.empty())
-    e = hipMemcpyAsync(pWide, wide.data(), wide.size() * sizeof(float), hipMemcpyHostToDevice,
-                       stream);
   if (e == hipSuccess && !qwide.empty())
     e = hipMemcpyAsync(pQwide, qwide.data(), qwide.size() * sizeof(float), hipMemcpyHostToDevice,
                        stream);
@@ -379,7 +389,6 @@ static int scene_create_impl(const float* prims, uint32_t primCount, const float
   d.mats = (const float4*)pMats;
   d.lights = (const int32_t*)pLights;
   d.primObject = (const uint32_t*)pObj;
-  d.wide = (const float4*)pWide;
   d.wideCount = (int)(wide.size() / 32);
   d.qwide = (const float4*)pQwide;
   d.leafWords = (const uint32_t*)pWords;

@@ -4,6 +4,7 @@
 // scene is read straight from the packed float32 arrays: 2 x 16 B per node
 // and 3 x 16 B per primitive, the algorithmic minimum of the layout.
 #include "ct_internal.h"
+#include "ct_qwalk.h"
 #include "ct_wavefront.h"
 
 namespace ct {
@@ -98,8 +99,8 @@ k_closest_wide(DevScene S, const double* __restrict__ rays, uint32_t n, double t
   size_t i = 0;
   TraversalCounts cnt = {0, 0, 0, 0};
   bool have = false;
-  double ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0;
-  double oix = 0, oiy = 0, oiz = 0;
+  double ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0;
+  qw::QRay Q = qw::qray(0, 0, 0, 1, 1, 1);
   Hit h;
   h.t = 0; h.u = 0; h.v = 0; h.prim = -1;
   int cur = -1, sp = 0;
@@ -112,11 +113,10 @@ k_closest_wide(DevScene S, const double* __restrict__ rays, uint32_t n, double t
       load_ray(rays, i, ox, oy, oz, dx, dy, dz);
       h.t = tMax ? tMax[i] : __builtin_inf();
       h.u = 0; h.v = 0; h.prim = -1;
-      ix = safe_inv(dx); iy = safe_inv(dy); iz = safe_inv(dz);
-      oix = ox * ix; oiy = oy * iy; oiz = oz * iz;
+      Q = qw::qray64(S, ox, oy, oz, dx, dy, dz);
       sp = 0;
       pk.n = 0;
-      cur = walk_root<true>(S, ix, iy, iz, oix, oiy, oiz, h.t, cnt, COUNT);
+      cur = 0;
       have = true;
     }
     if (!__any(have)) break;
@@ -127,22 +127,13 @@ k_closest_wide(DevScene S, const double* __restrict__ rays, uint32_t n, double t
           if (pk.n == PARK_LEAVES) break;
           park(pk, -2 - cur);
           cur = -1;
-        } else if (!interior_step<true, true, WIDE_STACK, COUNT>(S, ix, iy, iz, oix, oiy, oiz, h.t,
-                                                                cur, sp, sNode, sEntry, lanes,
-                                                                cnt)) {
+        } else if (!qw::qwide_step<true, WIDE_STACK, COUNT>(S, Q, qw::cull_bound(h.t), cur, sp,
+                                                           sNode, sEntry, lanes, cnt)) {
           overflow = true;
           cur = -1;
           break;
         }
-        if (cur == -1) {
-          while (sp > 0) {
-            sp--;
-            if (entry_from_bf16(sEntry[sp * lanes]) < h.t) {
-              cur = sNode[sp * lanes];
-              break;
-            }
-          }
-        }
+        if (cur == -1) qw::qpop(cur, sp, sNode, sEntry, lanes, qw::cull_bound(h.t));
         if (__all(pk.n == PARK_LEAVES || cur == -1)) break;
       }
       for (int k = 0; k < pk.n; k++)

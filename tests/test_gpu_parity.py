@@ -63,7 +63,8 @@ def test_closest_hit_finite_tmax_and_counts(ct, O, bvh_scene):
     dv /= np.linalg.norm(dv, axis=1, keepdims=True)
     rays = np.concatenate([o, dv], 1)
     tmax = 0.5 + 19.5 * d[:, 6]
-    for walk, owalk in ((ct.WALK_STACKLESS, O.WALK_STACKLESS), (ct.WALK_ORDERED, O.WALK_ORDERED)):
+    for walk, owalk in ((ct.WALK_STACKLESS, O.WALK_STACKLESS), (ct.WALK_ORDERED, O.WALK_ORDERED),
+                        (ct.WALK_WIDE, O.WALK_ORDERED)):
         g = ct.closest_hit(bvh_scene, rays, tmax, walk=walk)
         r = O.closest_batch(bvh_scene, rays, tmax, walk=owalk)
         assert (g["prim"] == r["prim"]).all() and np.array_equal(g["t"], r["t"])
@@ -90,6 +91,61 @@ def test_occluded_bit_exact(ct, O, bvh_scene):
     g = ct.occluded(bvh_scene, rays, tmax)
     r, _ = O.occluded_batch(bvh_scene, rays, tmax)
     assert (g == r).all() and 0 < g.sum() < n
+
+
+def _translated(scene, off):
+    """The scene moved by `off`: coordinates of large magnitude around small features."""
+    out = dict(scene, objects=[])
+    for ob in scene["objects"]:
+        sh = dict(ob["shape"])
+        for k in ("a", "b", "c", "center"):
+            if k in sh:
+                sh[k] = tuple(x + d for x, d in zip(sh[k], off))
+        out["objects"].append(dict(ob, shape=sh))
+    return out
+
+
+def _aimed_rays(ps, seed, n):
+    """Rays aimed at points of the scene's triangles -- vertices, edge midpoints
+    and interior points, so many hits fall on shared boundaries -- from
+    distances log-uniform in [1e-2, 1e5], a third of them with one or two
+    direction components exactly zero."""
+    rng = np.random.default_rng(seed)
+    pr = np.asarray(ps.prims, np.float32).reshape(-1, 16).astype(np.float64)
+    tri = np.nonzero(pr[:, 3] != 0)[0]
+    k = tri[rng.integers(0, len(tri), n)]
+    a, b, c = pr[k, 0:3], pr[k, 4:7], pr[k, 8:11]
+    uv = rng.random((n, 2))
+    uv = np.where(uv.sum(1, keepdims=True) > 1, 1 - uv, uv)
+    fixed = np.array([[0, 0], [1, 0], [0, 1], [0.5, 0], [0, 0.5], [0.5, 0.5]])
+    pick = rng.integers(0, len(fixed) + 4, n)
+    uv = np.where((pick < len(fixed))[:, None], fixed[np.minimum(pick, len(fixed) - 1)], uv)
+    target = a + uv[:, :1] * (b - a) + uv[:, 1:] * (c - a)
+    d = rng.normal(size=(n, 3))
+    zero = rng.integers(0, 9, n)
+    d[zero == 0, 0] = 0
+    d[zero == 1, 1] = 0
+    d[zero == 2, 2] = 0
+    d[zero == 3, :2] = 0
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    dist = 10.0 ** rng.uniform(-2, 5, n)
+    return np.concatenate([target - dist[:, None] * d, d], 1)
+
+
+@pytest.mark.parametrize("off", [(0, 0, 0), (4096.3, -3000.7, 65536.1)], ids=["origin", "far"])
+def test_wide_walk_adversarial_rays(ct, O, off):
+    """The 4-wide walk decides boxes in FP32 from the rounded binary64 ray; its
+    per-axis slack must keep every box a binary64 hit lies in.  Rays from up
+    to 1e5 away, axis-parallel rays and hits on vertices and edges, on the
+    scene at the origin and moved to coordinates around 6.5e4, give the
+    reference's closest hit (prim and t) exactly."""
+    ps = O.pack_scene(_translated(scenes.random_scene(O.Rng(77), 50, 3000), off))
+    rays = _aimed_rays(ps, 11, 20000)
+    g = ct.closest_hit(ps, rays, walk=ct.WALK_WIDE)
+    o = O.closest_batch(ps, rays, walk=O.WALK_ORDERED)
+    assert (g["prim"] == o["prim"]).all()
+    assert np.array_equal(g["t"], o["t"])
+    assert (o["prim"] >= 0).mean() > 0.7
 
 
 def test_empty_and_coincident(ct, O):
