@@ -93,11 +93,19 @@ __device__ __forceinline__ float qbyte(uint32_t w, int k) {
   return (float)((w >> (8 * k)) & 0xffu);
 }
 
-__device__ __forceinline__ uint16_t entry_bf16_down(float x) {
-  return (uint16_t)(__float_as_uint(x) >> 16);  // x >= 0: truncation rounds down
+// Deferred entries keep one byte: a lower bound of the entry distance with
+// the float's exponent and top three mantissa bits (steps of at most 12.5%)
+// over [2^-15, 1.75 * 2^16]; 0 stands for 0, and distances beyond the range
+// keep the top code, itself a lower bound.  Truncating the mantissa rounds
+// down, so a pop never skips an entry the culling bound does not exclude.
+using Entry = uint8_t;
+constexpr int ENTRY_BASE = (127 - 15) << 3;  // (exponent, 3 mantissa bits) of 2^-15
+__device__ __forceinline__ Entry entry_down(float x) {
+  int b = (int)(__float_as_uint(x) >> 20) - ENTRY_BASE + 1;
+  return (Entry)min(max(b, 0), 255);
 }
-__device__ __forceinline__ float entry_from_bf16(uint16_t b) {
-  return __uint_as_float((uint32_t)b << 16);
+__device__ __forceinline__ float entry_value(Entry e) {
+  return e == 0 ? 0.0f : __uint_as_float((uint32_t)(e - 1 + ENTRY_BASE) << 20);
 }
 
 // Culling bound of a walk whose closest distance (or segment end) is t.
@@ -107,12 +115,12 @@ __device__ __forceinline__ float cull_bound(double t) { return (float)t * CULL_S
 // One step of the ordered walk over quantized 4-wide node `cur`: the four
 // child boxes are tested (near/far planes chosen by the ray's direction
 // signs), the nearest hit child continues, the others go on the LDS stack
-// nearest-last with a bf16 lower bound of their entry (PUSH_ENTRY) or without
+// nearest-last with a one-byte lower bound of their entry (PUSH_ENTRY) or without
 // one (any hit).  Returns false when the stack would overflow.
 template <bool PUSH_ENTRY, int STACK, bool COUNT>
 __device__ __forceinline__ bool qwide_step(const DevScene& S, const QRay& R, float tCull, int& cur,
                                            int& sp, int* __restrict__ sNode,
-                                           uint16_t* __restrict__ sEntry, int lanes,
+                                           Entry* __restrict__ sEntry, int lanes,
                                            TraversalCounts& cnt) {
   const float INF = __builtin_inff();
   const float4* w = S.qwide + 4 * (size_t)cur;
@@ -163,7 +171,7 @@ __device__ __forceinline__ bool qwide_step(const DevScene& S, const QRay& R, flo
   for (int k = 3; k >= 1; k--) {
     if (k < n) {
       sNode[sp * lanes] = c[k];
-      if (PUSH_ENTRY) sEntry[sp * lanes] = entry_bf16_down(e[k]);
+      if (PUSH_ENTRY) sEntry[sp * lanes] = entry_down(e[k]);
       sp++;
     }
   }
@@ -173,11 +181,11 @@ __device__ __forceinline__ bool qwide_step(const DevScene& S, const QRay& R, flo
 
 // Pops deferred work, skipping entries beyond the culling bound.
 __device__ __forceinline__ void qpop(int& cur, int& sp, const int* __restrict__ sNode,
-                                     const uint16_t* __restrict__ sEntry, int lanes,
+                                     const Entry* __restrict__ sEntry, int lanes,
                                      float tCull) {
   while (sp > 0) {
     sp--;
-    if (entry_from_bf16(sEntry[sp * lanes]) <= tCull) {
+    if (entry_value(sEntry[sp * lanes]) <= tCull) {
       cur = sNode[sp * lanes];
       return;
     }

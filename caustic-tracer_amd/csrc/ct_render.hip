@@ -22,6 +22,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "ct_engine.h"
@@ -492,6 +493,40 @@ k_shade(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__
   }
 }
 
+// One interior step of the ordered walks: the quantized 4-wide node (FP32
+// planes with the binary64 ray's slack) or the binary node (binary64 slabs).
+template <bool WIDE, bool PUSH_ENTRY, int SD, bool COUNT, typename EntryT>
+__device__ __forceinline__ bool walk_step(const DevScene& S, const qw::QRay& Q, double ix,
+                                          double iy, double iz, double oix, double oiy, double oiz,
+                                          double tMax, int& cur, int& sp, int* __restrict__ sNode,
+                                          EntryT* __restrict__ sEntry, int lanes,
+                                          TraversalCounts& cnt) {
+  if constexpr (WIDE)
+    return qw::qwide_step<PUSH_ENTRY, SD, COUNT>(S, Q, qw::cull_bound(tMax), cur, sp, sNode,
+                                                 sEntry, lanes, cnt);
+  else
+    return interior_step<PUSH_ENTRY, SD, COUNT>(S, ix, iy, iz, oix, oiy, oiz, tMax, cur, sp,
+                                                sNode, sEntry, lanes, cnt);
+}
+
+// Pops deferred work whose entry bound is not beyond the closest hit.
+template <bool WIDE, typename EntryT>
+__device__ __forceinline__ void walk_pop(int& cur, int& sp, const int* __restrict__ sNode,
+                                         const EntryT* __restrict__ sEntry, int lanes,
+                                         double closest) {
+  if constexpr (WIDE) {
+    qw::qpop(cur, sp, sNode, sEntry, lanes, qw::cull_bound(closest));
+  } else {
+    while (sp > 0) {
+      sp--;
+      if (entry_from_bf16(sEntry[sp * lanes]) < closest) {
+        cur = sNode[sp * lanes];
+        return;
+      }
+    }
+  }
+}
+
 // Persistent closest hit.  WALK: CT_WALK_STACKLESS (the reference's skip-link
 // loop, exact mirror), CT_WALK_ORDERED (binary nodes, binary64 slab tests) or
 // CT_WALK_WIDE (the quantized 4-wide nodes walked in FP32 with the binary64
@@ -499,16 +534,21 @@ k_shade(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__
 // until every lane of the wave holds PARK_LEAVES of them and keep bf16 lower
 // bounds of deferred entries.  Primitive tests are binary64 in every walk, so
 // the closest hit is the reference's.
+// The 4-wide walk's LDS stack (4-byte code + 1-byte entry per slot, 30 KB per
+// 256-lane workgroup) leaves room for five waves per SIMD; the register cap
+// lets the VGPR budget match.
 template <int WALK, int STACK, bool COUNT, bool MULTI, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
-k_extend(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__ q,
+    __attribute__((amdgpu_waves_per_eu(WALK == CT_WALK_WIDE ? 5 : 1)))
+    k_extend(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__ q,
          const uint32_t* __restrict__ count, uint32_t* __restrict__ fetch) {
   constexpr bool WIDE = WALK == CT_WALK_WIDE;
   constexpr int SD = WALK == CT_WALK_STACKLESS ? 1 : STACK;
+  using EntryT = typename std::conditional<WIDE, qw::Entry, uint16_t>::type;
   __shared__ int sNodeAll[SD * BLOCK];
-  __shared__ uint16_t sEntryAll[SD * BLOCK];
+  __shared__ EntryT sEntryAll[SD * BLOCK];
   int* sNode = sNodeAll + threadIdx.x;
-  uint16_t* sEntry = sEntryAll + threadIdx.x;
+  EntryT* sEntry = sEntryAll + threadIdx.x;
   const int lanes = BLOCK;
   const uint32_t n = *count;
   TraversalCounts cnt = {0, 0, 0, 0};
@@ -557,28 +597,13 @@ k_extend(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
             if (pk.n == PARK_LEAVES) break;
             park(pk, -2 - cur);
             cur = -1;
-          } else if (!(WIDE ? qw::qwide_step<true, SD, COUNT>(S, Q, qw::cull_bound(h.t), cur, sp,
-                                                              sNode, sEntry, lanes, cnt)
-                            : interior_step<true, SD, COUNT>(S, ix, iy, iz, oix, oiy, oiz, h.t,
-                                                             cur, sp, sNode, sEntry, lanes,
-                                                             cnt))) {
+          } else if (!walk_step<WIDE, true, SD, COUNT>(S, Q, ix, iy, iz, oix, oiy, oiz, h.t, cur,
+                                                       sp, sNode, sEntry, lanes, cnt)) {
             overflow = true;
             cur = -1;
             break;
           }
-          if (cur == -1) {
-            if (WIDE) {
-              qw::qpop(cur, sp, sNode, sEntry, lanes, qw::cull_bound(h.t));
-            } else {
-              while (sp > 0) {
-                sp--;
-                if (entry_from_bf16(sEntry[sp * lanes]) < h.t) {
-                  cur = sNode[sp * lanes];
-                  break;
-                }
-              }
-            }
-          }
+          if (cur == -1) walk_pop<WIDE>(cur, sp, sNode, sEntry, lanes, h.t);
           if (__all(pk.n == PARK_LEAVES || cur == -1)) break;
         }
         for (int k = 0; k < pk.n; k++)
@@ -636,6 +661,7 @@ k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
          uint32_t* __restrict__ fetch) {
   constexpr bool WIDE = WALK == CT_WALK_WIDE;
   constexpr int SD = WALK == CT_WALK_STACKLESS ? 1 : STACK;
+  using EntryT = typename std::conditional<WIDE, qw::Entry, uint16_t>::type;
   __shared__ int sNodeAll[SD * BLOCK];
   int* sNode = sNodeAll + threadIdx.x;
   const int lanes = BLOCK;
@@ -686,11 +712,9 @@ k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
             if (pk.n == PARK_LEAVES) break;
             park(pk, -2 - cur);
             cur = -1;
-          } else if (!(WIDE ? qw::qwide_step<false, SD, COUNT>(S, Q, qw::cull_bound(tMax), cur,
-                                                               sp, sNode, nullptr, lanes, cnt)
-                            : interior_step<false, SD, COUNT>(S, ix, iy, iz, oix, oiy, oiz,
-                                                              tMax, cur, sp, sNode, nullptr,
-                                                              lanes, cnt))) {
+          } else if (!walk_step<WIDE, false, SD, COUNT>(S, Q, ix, iy, iz, oix, oiy, oiz, tMax,
+                                                        cur, sp, sNode, (EntryT*)nullptr, lanes,
+                                                        cnt)) {
             overflow = true;
             cur = -1;
             break;

@@ -218,13 +218,14 @@ __device__ __forceinline__ bool occluded_stackless(const DevScene& S, const Ray&
   return false;
 }
 
-using qw::entry_from_bf16;
+using qw::Entry;
+using qw::entry_value;
 using qw::qwide_step;
 
 // Per-lane closest hit with the quantized walk (queries).
 template <int STACK, bool COUNT>
 __device__ void closest_q(const DevScene& S, const Ray& R, float tMin, Hit& h,
-                          int* __restrict__ sNode, uint16_t* __restrict__ sEntry, int lanes,
+                          int* __restrict__ sNode, Entry* __restrict__ sEntry, int lanes,
                           TraversalCounts& cnt) {
   int cur = 0, sp = 0;
   for (;;) {
@@ -241,7 +242,7 @@ __device__ void closest_q(const DevScene& S, const Ray& R, float tMin, Hit& h,
     } else {
       while (sp > 0) {
         sp--;
-        if (entry_from_bf16(sEntry[sp * lanes]) <= h.t * CULL_SLACK) {
+        if (entry_value(sEntry[sp * lanes]) <= h.t * CULL_SLACK) {
           cur = sNode[sp * lanes];
           break;
         }
@@ -283,7 +284,7 @@ k_closest32(DevScene S, const float* __restrict__ rays, uint32_t n, float tMin,
             float* __restrict__ uOut, float* __restrict__ vOut,
             unsigned long long* __restrict__ counts) {
   __shared__ int sNode[BRUTE ? 1 : QUERY_STACK * QUERY_BLOCK];
-  __shared__ uint16_t sEntry[BRUTE ? 1 : QUERY_STACK * QUERY_BLOCK];
+  __shared__ Entry sEntry[BRUTE ? 1 : QUERY_STACK * QUERY_BLOCK];
   TraversalCounts cnt = {0, 0, 0, 0};
   size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -814,9 +815,9 @@ __global__ void __launch_bounds__(BLOCK)
 k_extend32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__ q,
            const uint32_t* __restrict__ count, uint32_t* __restrict__ fetch) {
   __shared__ int sNodeAll[STACK * BLOCK];
-  __shared__ uint16_t sEntryAll[STACK * BLOCK];
+  __shared__ Entry sEntryAll[STACK * BLOCK];
   int* sNode = sNodeAll + threadIdx.x;
-  uint16_t* sEntry = sEntryAll + threadIdx.x;
+  Entry* sEntry = sEntryAll + threadIdx.x;
   const int lanes = BLOCK;
   const uint32_t n = *count;
   TraversalCounts cnt = {0, 0, 0, 0};
@@ -860,7 +861,7 @@ k_extend32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
         if (cur == -1) {
           while (sp > 0) {
             sp--;
-            if (entry_from_bf16(sEntry[sp * lanes]) <= h.t * CULL_SLACK) {
+            if (entry_value(sEntry[sp * lanes]) <= h.t * CULL_SLACK) {
               cur = sNode[sp * lanes];
               break;
             }

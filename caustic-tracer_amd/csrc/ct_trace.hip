@@ -84,15 +84,15 @@ constexpr int WIDE_STACK = 24;
 // PARK_LEAVES of them or are done, then the parked leaves are tested together;
 // a lane whose ray finishes takes its next ray at the top of the loop.
 template <bool COUNT>
-__global__ void __launch_bounds__(QUERY_BLOCK)
+__global__ void __launch_bounds__(QUERY_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
 k_closest_wide(DevScene S, const double* __restrict__ rays, uint32_t n, double tMin,
                const double* __restrict__ tMax, int32_t* __restrict__ prim,
                double* __restrict__ tOut, double* __restrict__ uOut, double* __restrict__ vOut,
                unsigned long long* __restrict__ counts) {
   __shared__ int sNodeAll[WIDE_STACK * QUERY_BLOCK];
-  __shared__ uint16_t sEntryAll[WIDE_STACK * QUERY_BLOCK];
+  __shared__ qw::Entry sEntryAll[WIDE_STACK * QUERY_BLOCK];
   int* sNode = sNodeAll + threadIdx.x;
-  uint16_t* sEntry = sEntryAll + threadIdx.x;
+  qw::Entry* sEntry = sEntryAll + threadIdx.x;
   const int lanes = QUERY_BLOCK;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   size_t next = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
