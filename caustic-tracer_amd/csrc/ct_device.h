@@ -102,12 +102,36 @@ struct PrimGeom {
   float4 c;  // c.xyz, area
 };
 
+// Loads through the global address space.  A scene view read out of device
+// memory (the job table) carries generic pointers, and a load through one is a
+// FLAT access, which also counts against the LDS wait counter: LDS stack reads
+// would wait for node fetches in flight.  The scene arrays are global memory,
+// so their loads go through address-space-1 pointers of native vector types
+// (a load of HIP's float4 class would convert back to a generic reference).
+typedef float ct_nf4 __attribute__((ext_vector_type(4)));
+typedef int ct_ni4 __attribute__((ext_vector_type(4)));
+#define CT_GLOBAL __attribute__((address_space(1)))
+__device__ __forceinline__ float4 gload(const float4* p) {
+  ct_nf4 v = *(const CT_GLOBAL ct_nf4*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ int4 gload(const int4* p) {
+  ct_ni4 v = *(const CT_GLOBAL ct_ni4*)p;
+  return make_int4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint32_t gload(const uint32_t* p) {
+  return *(const CT_GLOBAL uint32_t*)p;
+}
+__device__ __forceinline__ int32_t gload(const int32_t* p) {
+  return *(const CT_GLOBAL int32_t*)p;
+}
+
 __device__ __forceinline__ PrimGeom load_prim(const DevScene& S, int p) {
   const float4* r = S.prims + (size_t)p * 4;
   PrimGeom g;
-  g.a = r[0];
-  g.b = r[1];
-  g.c = r[2];
+  g.a = gload(r);
+  g.b = gload(r + 1);
+  g.c = gload(r + 2);
   return g;
 }
 
@@ -264,15 +288,16 @@ __device__ __forceinline__ void closest_stackless(const DevScene& S, double ox, 
   int node = 0;
   const int nodeCount = S.nodeCount;
   while (node < nodeCount) {
-    float4 lo = S.nodes[2 * (size_t)node];
-    float4 hi = S.nodes[2 * (size_t)node + 1];
+    float4 lo = gload(S.nodes + 2 * (size_t)node);
+    float4 hi = gload(S.nodes + 2 * (size_t)node + 1);
     if (COUNT) cnt.nodes++;
     double entry = ray_aabb(ox, oy, oz, idx, idy, idz, lo, hi, h.t);
     if (entry == __builtin_inf()) {
       node = (int)lo.w;
       continue;
     }
-    if (hi.w >= 0.0f) test_leaf<COUNT>(S, (int)S.leafWords[node], ox, oy, oz, dx, dy, dz, tMin, h, cnt);
+    if (hi.w >= 0.0f)
+      test_leaf<COUNT>(S, (int)gload(S.leafWords + node), ox, oy, oz, dx, dy, dz, tMin, h, cnt);
     node++;
   }
 }
@@ -286,8 +311,8 @@ __device__ __forceinline__ bool occluded(const DevScene& S, double ox, double oy
   int node = 0;
   const int nodeCount = S.nodeCount;
   while (node < nodeCount) {
-    float4 lo = S.nodes[2 * (size_t)node];
-    float4 hi = S.nodes[2 * (size_t)node + 1];
+    float4 lo = gload(S.nodes + 2 * (size_t)node);
+    float4 hi = gload(S.nodes + 2 * (size_t)node + 1);
     if (COUNT) cnt.nodes++;
     double entry = ray_aabb(ox, oy, oz, idx, idy, idz, lo, hi, tMax);
     if (entry == __builtin_inf()) {
@@ -295,7 +320,7 @@ __device__ __forceinline__ bool occluded(const DevScene& S, double ox, double oy
       continue;
     }
     if (hi.w >= 0.0f) {
-      int li = (int)S.leafWords[node];
+      int li = (int)gload(S.leafWords + node);
       int count = li & 7;
       int start = li >> 3;
       for (int p = start; p < start + count; p++) {
@@ -383,7 +408,7 @@ __device__ __forceinline__ double entry_from_bf16(uint16_t b) {
 // Work item encoding shared by the stack and the current-node register:
 // >= 0 interior node index, <= -2 a leaf (-2 - leafWord), -1 nothing.
 __device__ __forceinline__ int enc_child(const DevScene& S, int idx, float leafWord) {
-  return leafWord >= 0.0f ? -2 - (int)S.leafWords[idx] : idx;
+  return leafWord >= 0.0f ? -2 - (int)gload(S.leafWords + idx) : idx;
 }
 
 // Closest hit with the ordered walk of intersect.ts:202-282 (nearer child
@@ -403,7 +428,7 @@ __device__ __forceinline__ void closest_wave(const DevScene& S, double ox, doubl
   if (COUNT) cnt.nodes++;
   int cur = -1;
   {
-    float4 lo = S.nodes[0], hi = S.nodes[1];
+    float4 lo = gload(S.nodes + 0), hi = gload(S.nodes + 1);
     double e;
     if (box_enter(ox, oy, oz, ix, iy, iz, lo, hi, h.t, e)) cur = enc_child(S, 0, hi.w);
   }
@@ -419,9 +444,11 @@ __device__ __forceinline__ void closest_wave(const DevScene& S, double ox, doubl
         cur = -1;
       } else {
         int left = cur + 1;
-        float4 llo = S.nodes[2 * (size_t)left], lhi = S.nodes[2 * (size_t)left + 1];
+        float4 llo = gload(S.nodes + 2 * (size_t)left);
+        float4 lhi = gload(S.nodes + 2 * (size_t)left + 1);
         int right = (int)llo.w;
-        float4 rlo = S.nodes[2 * (size_t)right], rhi = S.nodes[2 * (size_t)right + 1];
+        float4 rlo = gload(S.nodes + 2 * (size_t)right);
+        float4 rhi = gload(S.nodes + 2 * (size_t)right + 1);
         if (COUNT) cnt.nodes += 2;
         double tl, tr;
         bool hl = box_enter(ox, oy, oz, ix, iy, iz, llo, lhi, h.t, tl);
@@ -482,9 +509,9 @@ __device__ __forceinline__ bool interior_step(const DevScene& S, double ix, doub
                                               uint16_t* __restrict__ sEntry, int lanes,
                                               TraversalCounts& cnt) {
   int left = cur + 1;
-  float4 llo = S.nodes[2 * (size_t)left], lhi = S.nodes[2 * (size_t)left + 1];
+  float4 llo = gload(S.nodes + 2 * (size_t)left), lhi = gload(S.nodes + 2 * (size_t)left + 1);
   int right = (int)llo.w;
-  float4 rlo = S.nodes[2 * (size_t)right], rhi = S.nodes[2 * (size_t)right + 1];
+  float4 rlo = gload(S.nodes + 2 * (size_t)right), rhi = gload(S.nodes + 2 * (size_t)right + 1);
   if (COUNT) cnt.nodes += 2;
   double tl, tr;
   bool hl = box_enter_fma(ix, iy, iz, oix, oiy, oiz, llo, lhi, tMax, tl);
@@ -513,7 +540,7 @@ __device__ __forceinline__ int walk_root(const DevScene& S, double ix, double iy
                                          double oix, double oiy, double oiz, double tMax,
                                          TraversalCounts& cnt, bool count) {
   if (count) cnt.nodes++;
-  float4 lo = S.nodes[0], hi = S.nodes[1];
+  float4 lo = gload(S.nodes + 0), hi = gload(S.nodes + 1);
   double e;
   return box_enter_fma(ix, iy, iz, oix, oiy, oiz, lo, hi, tMax, e) ? enc_child(S, 0, hi.w) : -1;
 }

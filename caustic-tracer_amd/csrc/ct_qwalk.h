@@ -77,7 +77,7 @@ __device__ __forceinline__ float axis_slack(double o, float of, double d, float 
 __device__ __forceinline__ QRay qray64(const DevScene& S, double ox, double oy, double oz,
                                        double dx, double dy, double dz) {
   QRay r = qray((float)ox, (float)oy, (float)oz, (float)dx, (float)dy, (float)dz);
-  const float4 lo = S.nodes[0], hi = S.nodes[1];
+  const float4 lo = gload(S.nodes), hi = gload(S.nodes + 1);
   const double cx = 0.5 * ((double)lo.x + (double)hi.x), hx = 0.5 * ((double)hi.x - (double)lo.x);
   const double cy = 0.5 * ((double)lo.y + (double)hi.y), hy = 0.5 * ((double)hi.y - (double)lo.y);
   const double cz = 0.5 * ((double)lo.z + (double)hi.z), hz = 0.5 * ((double)hi.z - (double)lo.z);
@@ -124,8 +124,8 @@ __device__ __forceinline__ bool qwide_step(const DevScene& S, const QRay& R, flo
                                            TraversalCounts& cnt) {
   const float INF = __builtin_inff();
   const float4* w = S.qwide + 4 * (size_t)cur;
-  float4 w0 = w[0], w1 = w[1], w2 = w[2];
-  int4 cc = *(const int4*)(w + 3);
+  float4 w0 = gload(w), w1 = gload(w + 1), w2 = gload(w + 2);
+  int4 cc = gload((const int4*)(w + 3));
   if (COUNT) cnt.nodes += 4;
   float Bx = w0.w * R.ix, By = w1.x * R.iy, Bz = w1.y * R.iz;
   float Ax = (w0.x - R.ox) * R.ix, Ay = (w0.y - R.oy) * R.iy, Az = (w0.z - R.oz) * R.iz;

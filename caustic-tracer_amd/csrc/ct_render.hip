@@ -156,9 +156,9 @@ __device__ bool sample_light(const DevScene& S, double px, double py, double pz,
                              LightSample& L) {
   int lightCount = S.lightCount;
   if (lightCount == 0) return false;
-  int prim = S.lights[rng.pick(lightCount)];
+  int prim = gload(S.lights + rng.pick(lightCount));
   PrimGeom g = load_prim(S, prim);
-  float4 em = S.mats[(size_t)S.primObject[prim] * 4 + 3];
+  float4 em = gload(S.mats + (size_t)gload(S.primObject + prim) * 4 + 3);
   L.e[0] = em.x; L.e[1] = em.y; L.e[2] = em.z;
   if (g.a.w == 0.0f) {
     double cx = (double)g.a.x - px, cy = (double)g.a.y - py, cz = (double)g.a.z - pz;
@@ -222,7 +222,7 @@ __device__ bool sample_light(const DevScene& S, double px, double py, double pz,
   if (dist2 <= 0) return false;
   double dist = sqrt(dist2);
   double wx = tx / dist, wy = ty / dist, wz = tz / dist;
-  float4 nrm = S.prims[(size_t)prim * 4 + 3];
+  float4 nrm = gload(S.prims + (size_t)prim * 4 + 3);
   double cosL = fabs((double)nrm.x * wx + (double)nrm.y * wy + (double)nrm.z * wz);
   if (cosL < 1e-6) return false;
   L.w[0] = wx; L.w[1] = wy; L.w[2] = wz;
@@ -310,7 +310,7 @@ k_shade(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__
           ny = (py - (double)g.a.y) * invR;
           nz = (pz - (double)g.a.z) * invR;
         } else {
-          float4 nrm = S.prims[(size_t)prim * 4 + 3];
+          float4 nrm = gload(S.prims + (size_t)prim * 4 + 3);
           nx = nrm.x;
           ny = nrm.y;
           nz = nrm.z;
@@ -319,8 +319,8 @@ k_shade(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__
         double sx = frontFace ? nx : -nx;
         double sy = frontFace ? ny : -ny;
         double sz = frontFace ? nz : -nz;
-        size_t m = (size_t)S.primObject[prim] * 4;
-        float4 m0 = S.mats[m], m1 = S.mats[m + 1], m3 = S.mats[m + 3];
+        size_t m = (size_t)gload(S.primObject + prim) * 4;
+        float4 m0 = gload(S.mats + m), m1 = gload(S.mats + m + 1), m3 = gload(S.mats + m + 3);
         double er = m3.x, eg = m3.y, eb = m3.z;
         if (er > 0 || eg > 0 || eb > 0) {
           double w = 1;
@@ -341,7 +341,7 @@ k_shade(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__
             double cs = m1.y;
             double sum = floor(px * cs + 0.5) + floor(py * cs + 0.5) + floor(pz * cs + 0.5);
             if (fmod(fmod(sum, 2.0) + 2.0, 2.0) != 0) {  // !checkerPrimary
-              float4 m2 = S.mats[m + 2];
+              float4 m2 = gload(S.mats + m + 2);
               ar = m2.x;
               ag = m2.y;
               ab = m2.z;

@@ -185,14 +185,14 @@ __device__ __forceinline__ void closest_stackless(const DevScene& S, const Ray& 
   int node = 0;
   const int nodeCount = S.nodeCount;
   while (node < nodeCount) {
-    float4 lo = S.nodes[2 * (size_t)node];
-    float4 hi = S.nodes[2 * (size_t)node + 1];
+    float4 lo = gload(S.nodes + 2 * (size_t)node);
+    float4 hi = gload(S.nodes + 2 * (size_t)node + 1);
     if (COUNT) cnt.nodes++;
     if (!box_hit(R, lo, hi, h.t * CULL_SLACK)) {
       node = (int)lo.w;
       continue;
     }
-    if (hi.w >= 0.0f) test_leaf<COUNT>(S, (int)S.leafWords[node], R, tMin, h, cnt);
+    if (hi.w >= 0.0f) test_leaf<COUNT>(S, (int)gload(S.leafWords + node), R, tMin, h, cnt);
     node++;
   }
 }
@@ -204,14 +204,14 @@ __device__ __forceinline__ bool occluded_stackless(const DevScene& S, const Ray&
   const int nodeCount = S.nodeCount;
   const float tCull = tMax * CULL_SLACK;
   while (node < nodeCount) {
-    float4 lo = S.nodes[2 * (size_t)node];
-    float4 hi = S.nodes[2 * (size_t)node + 1];
+    float4 lo = gload(S.nodes + 2 * (size_t)node);
+    float4 hi = gload(S.nodes + 2 * (size_t)node + 1);
     if (COUNT) cnt.nodes++;
     if (!box_hit(R, lo, hi, tCull)) {
       node = (int)lo.w;
       continue;
     }
-    if (hi.w >= 0.0f && leaf_blocks<COUNT>(S, (int)S.leafWords[node], R, tMin, tMax, cnt))
+    if (hi.w >= 0.0f && leaf_blocks<COUNT>(S, (int)gload(S.leafWords + node), R, tMin, tMax, cnt))
       return true;
     node++;
   }
@@ -468,9 +468,9 @@ __device__ bool sample_light(const DevScene& S, float px, float py, float pz, Rn
                              LightSample& L) {
   int lightCount = S.lightCount;
   if (lightCount == 0) return false;
-  int prim = S.lights[pick(rng, lightCount)];
+  int prim = gload(S.lights + pick(rng, lightCount));
   PrimGeom g = load_prim(S, prim);
-  float4 em = S.mats[(size_t)S.primObject[prim] * 4 + 3];
+  float4 em = gload(S.mats + (size_t)gload(S.primObject + prim) * 4 + 3);
   L.e[0] = em.x; L.e[1] = em.y; L.e[2] = em.z;
   if (g.a.w == 0.0f) {
     float cx = g.a.x - px, cy = g.a.y - py, cz = g.a.z - pz;
@@ -536,7 +536,7 @@ __device__ bool sample_light(const DevScene& S, float px, float py, float pz, Rn
   if (dist2 <= 0) return false;
   float dist = sqrtf(dist2);
   float wx = tx / dist, wy = ty / dist, wz = tz / dist;
-  float4 nrm = S.prims[(size_t)prim * 4 + 3];
+  float4 nrm = gload(S.prims + (size_t)prim * 4 + 3);
   float cosL = fabsf(nrm.x * wx + nrm.y * wy + nrm.z * wz);
   if (cosL < 1e-6f) return false;
   L.w[0] = wx; L.w[1] = wy; L.w[2] = wz;
@@ -623,7 +623,7 @@ k_shade32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restri
           ny = (py - g.a.y) * invR;
           nz = (pz - g.a.z) * invR;
         } else {
-          float4 nrm = S.prims[(size_t)prim * 4 + 3];
+          float4 nrm = gload(S.prims + (size_t)prim * 4 + 3);
           nx = nrm.x;
           ny = nrm.y;
           nz = nrm.z;
@@ -632,8 +632,8 @@ k_shade32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restri
         float sx = frontFace ? nx : -nx;
         float sy = frontFace ? ny : -ny;
         float sz = frontFace ? nz : -nz;
-        size_t m = (size_t)S.primObject[prim] * 4;
-        float4 m0 = S.mats[m], m1 = S.mats[m + 1], m3 = S.mats[m + 3];
+        size_t m = (size_t)gload(S.primObject + prim) * 4;
+        float4 m0 = gload(S.mats + m), m1 = gload(S.mats + m + 1), m3 = gload(S.mats + m + 3);
         float er = m3.x, eg = m3.y, eb = m3.z;
         if (er > 0 || eg > 0 || eb > 0) {
           float w = 1;
@@ -654,7 +654,7 @@ k_shade32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restri
             float cs = m1.y;
             float sum = floorf(px * cs + 0.5f) + floorf(py * cs + 0.5f) + floorf(pz * cs + 0.5f);
             if (fmodf(fmodf(sum, 2.0f) + 2.0f, 2.0f) != 0) {  // !checkerPrimary
-              float4 m2 = S.mats[m + 2];
+              float4 m2 = gload(S.mats + m + 2);
               ar = m2.x;
               ag = m2.y;
               ab = m2.z;
