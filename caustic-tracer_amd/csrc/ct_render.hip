@@ -816,6 +816,8 @@ struct ct_tracer {
   size_t maxPaths = size_t(1) << 25; // paths in flight per round (~190 B of state each)
   bool graph = false;                // capture rounds into a hipGraph and replay them
   hipStream_t ownStream = nullptr;   // capture stream
+  bool overlap = true;               // NEE launches on a second stream (SideStream)
+  SideStream side;                   // created on first use
   hipGraphExec_t graphExec = nullptr;
   std::vector<char> graphKey;
   // stats of the last call
@@ -1016,13 +1018,30 @@ static void mark_cb(void* ctx, hipStream_t st, int kind, bool begin) {
   mark((ct_tracer*)ctx, st, kind, begin);
 }
 
+// The tracer's side stream for the NEE launches, or none: profiling runs keep
+// one stream so that each launch's events bracket that launch alone.
+static int side_stream(ct_tracer* T, SideStream& out) {
+  out = SideStream();
+  if (!T->overlap || (T->profiling && !T->graph)) return CT_OK;
+  if (!T->side.side) {
+    CT_HIP(hipStreamCreateWithFlags(&T->side.side, hipStreamNonBlocking));
+    CT_HIP(hipEventCreateWithFlags(&T->side.fork, hipEventDisableTiming));
+    CT_HIP(hipEventCreateWithFlags(&T->side.join, hipEventDisableTiming));
+  }
+  out = T->side;
+  return CT_OK;
+}
+
 // Kernel launches of one round (all on `st`; graph-capturable).
 template <bool MULTI>
 static int launch_round(ct_tracer* T, int njobs, size_t paths, size_t pix, int nb,
                         hipStream_t st) {
+  SideStream side;
+  int rc = side_stream(T, side);
+  if (rc) return rc;
   if (T->precision == CT_PRECISION_F32) {
     RoundLaunch R{T->P32, T->jobsDev, njobs, paths, pix, nb, T->stack, T->block,
-                  T->persistBlocks, T->counting, mark_cb, T};
+                  T->persistBlocks, T->counting, mark_cb, T, side};
     return launch_round32(R, MULTI, st);
   }
   PathSoA P = T->P;
@@ -1046,17 +1065,24 @@ static int launch_round(ct_tracer* T, int njobs, size_t paths, size_t pix, int n
                           P, jobs, qin, cin, fExt, scount, fSh, false);
     CT_LAUNCH_CHECK("k_extend");
     mark(T, st, 0, false);
+    if (side.side && b > 0) CT_HIP(side_join(side, st));  // NEE rays of bounce b-1 done
     mark(T, st, 1, true);
     hipLaunchKernelGGL((k_shade<MULTI>), dim3(gridPaths), dim3(RBLOCK), 0, st, P, jobs, qin,
                        cin, qout, cout, scount, b);
     CT_LAUNCH_CHECK("k_shade");
     mark(T, st, 1, false);
-    mark(T, st, 2, true);
+    hipStream_t sst = st;
+    if (side.side) {
+      CT_HIP(side_fork(side, st));
+      sst = side.side;
+    }
+    mark(T, sst, 2, true);
     dispatch_trace<MULTI>(T->shadowWalk, T->stack, T->counting, T->block, T->persistBlocks,
-                          paths, st, P, jobs, qin, cin, fExt, scount, fSh, true);
+                          paths, sst, P, jobs, qin, cin, fExt, scount, fSh, true);
     CT_LAUNCH_CHECK("k_shadow");
-    mark(T, st, 2, false);
+    mark(T, sst, 2, false);
   }
+  if (side.side) CT_HIP(side_join(side, st));
   unsigned gridPix = grid_for(pix, RBLOCK, 256 * 16);
   hipLaunchKernelGGL((k_finalize<MULTI>), dim3(gridPix), dim3(RBLOCK), 0, st, P, jobs, njobs,
                      pix);
@@ -1118,7 +1144,7 @@ static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t s
     std::vector<char> key(sizeof(size_t) * 2 + sizeof(int) * 10);
     size_t kk[2] = {paths, pix};
     int ki[10] = {nb, (int)R.size(), T->walk, T->shadowWalk, T->stack, (int)T->persistBlocks,
-                  T->counting, multi, T->block, T->precision};
+                  T->counting, multi, T->block, T->precision * 2 + (T->overlap ? 1 : 0)};
     memcpy(key.data(), kk, sizeof kk);
     memcpy(key.data() + sizeof kk, ki, sizeof ki);
     if (!T->graphExec || key != T->graphKey) {
@@ -1127,6 +1153,9 @@ static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t s
         T->graphExec = nullptr;
       }
       if (!T->ownStream) CT_HIP(hipStreamCreateWithFlags(&T->ownStream, hipStreamNonBlocking));
+      SideStream created;  // stream and events exist before the capture starts
+      rc = side_stream(T, created);
+      if (rc) return rc;
       CT_HIP(hipStreamSynchronize(st));
       CT_HIP(hipStreamBeginCapture(T->ownStream, hipStreamCaptureModeThreadLocal));
       rc = multi ? launch_round<true>(T, (int)R.size(), paths, pix, nb, T->ownStream)
@@ -1284,6 +1313,9 @@ int ct_tracer_destroy(ct_tracer* T) {
   if (T->jobsHost) (void)hipHostFree(T->jobsHost);
   if (T->compBuf) (void)hipFree(T->compBuf);
   if (T->ownStream) (void)hipStreamDestroy(T->ownStream);
+  if (T->side.side) (void)hipStreamDestroy(T->side.side);
+  if (T->side.fork) (void)hipEventDestroy(T->side.fork);
+  if (T->side.join) (void)hipEventDestroy(T->side.join);
   delete T;
   return check_hip(e, "ct_tracer_destroy");
 }

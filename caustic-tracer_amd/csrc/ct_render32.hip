@@ -1117,22 +1117,30 @@ static int round32(const RoundLaunch& R, hipStream_t st) {
     uint32_t* scount = P.ctr + MAX_BOUNCE_SLOTS + b;
     uint32_t* fExt = P.ctr + 2 * MAX_BOUNCE_SLOTS + b;
     uint32_t* fSh = P.ctr + 3 * MAX_BOUNCE_SLOTS + b;
+    const bool ov = R.side.side != nullptr;
     if (R.mark) R.mark(R.markCtx, st, 0, true);
     dispatch_trace<MULTI>(R.stack, R.counting, R.block, R.persistBlocks, R.paths, st, P, jobs,
                           qin, cin, fExt, scount, fSh, false);
     CT_LAUNCH_CHECK("k_extend32");
     if (R.mark) R.mark(R.markCtx, st, 0, false);
+    if (ov && b > 0) CT_HIP(side_join(R.side, st));  // NEE rays of bounce b-1 done
     if (R.mark) R.mark(R.markCtx, st, 1, true);
     hipLaunchKernelGGL((k_shade32<MULTI>), dim3(gridPaths), dim3(RBLOCK), 0, st, P, jobs, qin,
                        cin, qout, cout, scount, b);
     CT_LAUNCH_CHECK("k_shade32");
     if (R.mark) R.mark(R.markCtx, st, 1, false);
-    if (R.mark) R.mark(R.markCtx, st, 2, true);
-    dispatch_trace<MULTI>(R.stack, R.counting, R.block, R.persistBlocks, R.paths, st, P, jobs,
+    hipStream_t sst = st;
+    if (ov) {
+      CT_HIP(side_fork(R.side, st));
+      sst = R.side.side;
+    }
+    if (R.mark) R.mark(R.markCtx, sst, 2, true);
+    dispatch_trace<MULTI>(R.stack, R.counting, R.block, R.persistBlocks, R.paths, sst, P, jobs,
                           qin, cin, fExt, scount, fSh, true);
     CT_LAUNCH_CHECK("k_shadow32");
-    if (R.mark) R.mark(R.markCtx, st, 2, false);
+    if (R.mark) R.mark(R.markCtx, sst, 2, false);
   }
+  if (R.side.side) CT_HIP(side_join(R.side, st));
   unsigned gridPix = grid_for(R.pix, RBLOCK, 256 * 16);
   hipLaunchKernelGGL((k_finalize32<MULTI>), dim3(gridPix), dim3(RBLOCK), 0, st, P, jobs, R.njobs,
                      R.pix);

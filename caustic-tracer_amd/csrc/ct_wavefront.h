@@ -163,6 +163,27 @@ __device__ __forceinline__ int parked(const Parked& p, int k) {
   return w;
 }
 
+// Second stream for the NEE launches.  The shadow rays of bounce b and the
+// extension rays of bounce b+1 are independent (one adds to the path's
+// radiance, the other writes its hit), so the two persistent launches run side
+// by side and each fills the other's tail; the shading of bounce b+1 waits for
+// both.  `side` null: one stream, launches in order.
+struct SideStream {
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+
+// Work queued on `st` so far happens before what is queued on the side stream next.
+inline hipError_t side_fork(const SideStream& s, hipStream_t st) {
+  hipError_t e = hipEventRecord(s.fork, st);
+  return e == hipSuccess ? hipStreamWaitEvent(s.side, s.fork, 0) : e;
+}
+// Work queued on the side stream so far happens before what is queued on `st` next.
+inline hipError_t side_join(const SideStream& s, hipStream_t st) {
+  hipError_t e = hipEventRecord(s.join, s.side);
+  return e == hipSuccess ? hipStreamWaitEvent(st, s.join, 0) : e;
+}
+
 // One launch round of the FP32 precision kernels (ct_render32.hip).
 struct RoundLaunch {
   PathSoA32 P32;
@@ -176,6 +197,7 @@ struct RoundLaunch {
   // profiling hook: kind 0 extend, 1 shade, 2 shadow
   void (*mark)(void* ctx, hipStream_t st, int kind, bool begin);
   void* markCtx;
+  SideStream side;
 };
 int launch_round32(const RoundLaunch& R, bool multi, hipStream_t st);
 
