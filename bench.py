@@ -118,6 +118,9 @@ def main():
                     help="plain: random closest-hit queries timed beside the render (0: none)")
     ap.add_argument("--workgroup", type=int, default=256, choices=[64, 128, 256],
                     help="lanes per workgroup of the traversal launches")
+    ap.add_argument("--path-groups", type=int, default=0, choices=[0, 1, 2],
+                    help="plain: path groups per round on their own streams (0: the tracer's "
+                         "default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--scenes", type=int, default=0, help="batch/serve: scene count (0: 64/128)")
@@ -221,6 +224,10 @@ def plain_form(ct, args):
     tr.set_schedule(args.persist_blocks > 0, max(1, args.persist_blocks))
     tr.set_precision(ct.PRECISION_F32 if args.precision == "f32" else ct.PRECISION_F64)
     tr.set_workgroup(args.workgroup)
+    if args.path_groups:
+        cfg = tr.get_config()
+        cfg["pathGroups"] = args.path_groups
+        tr.set_config(cfg)
     out = ct.DeviceBuffer.empty((w * h * 3,), np.float32)
     opts = {"width": w, "height": h, "spp": spp, "seed": 1, "maxBounces": depth,
             "russianRoulette": True, "nee": True}
@@ -278,7 +285,8 @@ def plain_form(ct, args):
                                f"{spp} spp, maxBounces {depth}, RR+NEE, walk {args.walk}, "
                                f"precision {args.precision}, "
                                f"shadow walk {args.shadow_walk}, stack {args.stack}, "
-                               f"persistent blocks {args.persist_blocks}",
+                               f"persistent blocks {args.persist_blocks}, "
+                               f"path groups {tr.get_config()['pathGroups']}",
                    "triangles": args.tris, "width": w, "height": h, "spp": spp,
                    "max_bounces": depth, "parallelism": "single GPU"},
         "spp_mpix_per_s": round(sppmpix, 3),

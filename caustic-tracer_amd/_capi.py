@@ -56,7 +56,7 @@ class PassJob(C.Structure):
 class TracerConfig(C.Structure):
     _fields_ = [("maxPaths", C.c_uint64), ("walk", C.c_int32), ("shadowWalk", C.c_int32),
                 ("stackDepth", C.c_int32), ("persistent", C.c_int32), ("blocks", C.c_int32),
-                ("workgroup", C.c_int32)]
+                ("workgroup", C.c_int32), ("pathGroups", C.c_int32)]
 
 
 class TuneResult(C.Structure):

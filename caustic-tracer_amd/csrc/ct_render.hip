@@ -40,9 +40,10 @@ __global__ void k_clear(uint32_t* ctr, int n, unsigned long long* ctr64, int n64
 // camera.ts:62-72 generateRay with the pixel/lens draws of tracer.ts:373-384.
 template <bool MULTI>
 __global__ void __launch_bounds__(RBLOCK)
-k_generate(PathSoA P, const JobDev* __restrict__ jobs, int njobs, size_t total) {
+k_generate(PathSoA P, const JobDev* __restrict__ jobs, int njobs, size_t base, size_t count) {
   size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t pid = (size_t)blockIdx.x * blockDim.x + threadIdx.x; pid < total; pid += stride) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    const size_t pid = base + i;
     int j = job_of<MULTI, false>(jobs, njobs, pid);
     const JobDev& J = jobs[j];
     const Frame& F = J.F;
@@ -85,9 +86,9 @@ k_generate(PathSoA P, const JobDev* __restrict__ jobs, int njobs, size_t total) 
     P.specular[pid] = 1;
     P.rng[pid] = rng.s;
     if (MULTI) P.jobOf[pid] = (uint32_t)j;
-    P.q0[pid] = (uint32_t)pid;
+    P.q0[i] = (uint32_t)pid;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) P.ctr[0] = (uint32_t)total;
+  if (blockIdx.x == 0 && threadIdx.x == 0) P.ctr[0] = (uint32_t)count;
 }
 
 // lightPdfScalar (tracer.ts:61-90)
@@ -816,9 +817,10 @@ struct ct_tracer {
   size_t maxPaths = size_t(1) << 25; // paths in flight per round (~190 B of state each)
   bool graph = false;                // capture rounds into a hipGraph and replay them
   hipStream_t ownStream = nullptr;   // capture stream
-  bool overlap = true;               // NEE launches on a second stream (SideStream)
-  SideStream side;                   // created on first use
-  hipGraphExec_t graphExec = nullptr;
+  bool overlap = true;               // NEE launches on side streams (SideStream)
+  int groups = 0;                    // path groups per round (0: 1 binary64, 2 FP32)
+  SideStream side, sideB, groupB;    // streams and events, created on first use
+  hipGraphExec_t graphExec[2] = {nullptr, nullptr};  // one per path group
   std::vector<char> graphKey;
   // stats of the last call
   int lastBounces = 0;
@@ -831,6 +833,14 @@ struct ct_tracer {
   hipStream_t lastStream = nullptr;
 };
 
+static void drop_graphs(ct_tracer* T) {
+  for (hipGraphExec_t& g : T->graphExec) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+  T->graphKey.clear();
+}
+
 static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 
 static int tracer_reserve(ct_tracer* T, size_t paths) {
@@ -842,7 +852,7 @@ static int tracer_reserve(ct_tracer* T, size_t paths) {
   }
   size_t n = paths;
   size_t d = al(n * 8), u32 = al(n * 4), u8 = al(n);
-  size_t total = 21 * d + 6 * u32 + u8 + al(4 * MAX_BOUNCE_SLOTS * 4) + al(16 * 8);
+  size_t total = 21 * d + 6 * u32 + u8 + al(2 * GROUP_COUNTERS * 4) + al(16 * 8);
   CT_HIP(hipMalloc(&T->buf, total));
   char* p = (char*)T->buf;
   auto takeD = [&]() { double* r = (double*)p; p += d; return r; };
@@ -860,7 +870,7 @@ static int tracer_reserve(ct_tracer* T, size_t paths) {
   P.jobOf = takeU();
   P.q0 = takeU(); P.q1 = takeU(); P.sPath = takeU();
   P.specular = (uint8_t*)p; p += u8;
-  P.ctr = (uint32_t*)p; p += al(4 * MAX_BOUNCE_SLOTS * 4);
+  P.ctr = (uint32_t*)p; p += al(2 * GROUP_COUNTERS * 4);
   P.ctr64 = (unsigned long long*)p;
   // the FP32 state aliases the start of the same allocation (one precision
   // per round): 21 float arrays, 6 uint32 arrays and the flags fit in the
@@ -888,11 +898,7 @@ static int tracer_reserve(ct_tracer* T, size_t paths) {
   }
   T->capPaths = paths;
   // path buffers moved: a captured graph refers to the old ones
-  if (T->graphExec) {
-    (void)hipGraphExecDestroy(T->graphExec);
-    T->graphExec = nullptr;
-    T->graphKey.clear();
-  }
+  drop_graphs(T);
   return CT_OK;
 }
 
@@ -906,11 +912,7 @@ static int jobs_reserve(ct_tracer* T, size_t n) {
   CT_HIP(hipMalloc((void**)&T->jobsDev, cap * sizeof(JobDev)));
   CT_HIP(hipHostMalloc((void**)&T->jobsHost, cap * sizeof(JobDev), hipHostMallocDefault));
   T->capJobs = cap;
-  if (T->graphExec) {
-    (void)hipGraphExecDestroy(T->graphExec);
-    T->graphExec = nullptr;
-    T->graphKey.clear();
-  }
+  drop_graphs(T);
   return CT_OK;
 }
 
@@ -1018,39 +1020,52 @@ static void mark_cb(void* ctx, hipStream_t st, int kind, bool begin) {
   mark((ct_tracer*)ctx, st, kind, begin);
 }
 
-// The tracer's side stream for the NEE launches, or none: profiling runs keep
-// one stream so that each launch's events bracket that launch alone.
-static int side_stream(ct_tracer* T, SideStream& out) {
-  out = SideStream();
-  if (!T->overlap || (T->profiling && !T->graph)) return CT_OK;
-  if (!T->side.side) {
-    CT_HIP(hipStreamCreateWithFlags(&T->side.side, hipStreamNonBlocking));
-    CT_HIP(hipEventCreateWithFlags(&T->side.fork, hipEventDisableTiming));
-    CT_HIP(hipEventCreateWithFlags(&T->side.join, hipEventDisableTiming));
-  }
-  out = T->side;
+static int make_side(SideStream& ss) {
+  if (ss.side) return CT_OK;
+  CT_HIP(hipStreamCreateWithFlags(&ss.side, hipStreamNonBlocking));
+  CT_HIP(hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming));
+  CT_HIP(hipEventCreateWithFlags(&ss.join, hipEventDisableTiming));
   return CT_OK;
 }
 
-// Kernel launches of one round (all on `st`; graph-capturable).
-template <bool MULTI>
-static int launch_round(ct_tracer* T, int njobs, size_t paths, size_t pix, int nb,
-                        hipStream_t st) {
-  SideStream side;
-  int rc = side_stream(T, side);
-  if (rc) return rc;
-  if (T->precision == CT_PRECISION_F32) {
-    RoundLaunch R{T->P32, T->jobsDev, njobs, paths, pix, nb, T->stack, T->block,
-                  T->persistBlocks, T->counting, mark_cb, T, side};
-    return launch_round32(R, MULTI, st);
+// Smallest round traced as two path groups.
+constexpr size_t SPLIT_MIN_PATHS = size_t(1) << 20;
+
+// The streams of a round of `paths` paths.  Profiling runs keep one stream so
+// that each launch's events bracket that launch alone.
+static int round_streams(ct_tracer* T, size_t paths, RoundStreams& rs) {
+  rs = RoundStreams();
+  if (T->profiling && !T->graph) return CT_OK;
+  int rc;
+  // measured on a 1080p frame: two groups pay for the FP32 kernels (+5%), not
+  // for the binary64 ones (-1%), whose shading runs at three waves per SIMD
+  const int groups = T->groups ? T->groups : (T->precision == CT_PRECISION_F32 ? 2 : 1);
+  if (groups > 1 && paths >= SPLIT_MIN_PATHS) {
+    if ((rc = make_side(T->groupB))) return rc;
+    rs.groups = 2;
+    rs.split = T->groupB;
   }
-  PathSoA P = T->P;
+  if (T->overlap) {
+    if ((rc = make_side(T->side))) return rc;
+    rs.shadow[0] = T->side;
+    if (rs.groups == 2) {
+      if ((rc = make_side(T->sideB))) return rc;
+      rs.shadow[1] = T->sideB;
+    }
+  }
+  return CT_OK;
+}
+
+
+// Kernel launches of one path group: generate, then per bounce extend, shade
+// and the NEE launch (on the group's side stream when it has one).
+template <bool MULTI>
+static int launch_group(ct_tracer* T, const PathSoA& P, int njobs, size_t base, size_t count,
+                        int nb, hipStream_t st, const SideStream& side) {
   const JobDev* jobs = T->jobsDev;
-  hipLaunchKernelGGL(k_clear, dim3(4), dim3(256), 0, st, P.ctr, 4 * MAX_BOUNCE_SLOTS, P.ctr64, 0);
-  CT_LAUNCH_CHECK("k_clear");
-  unsigned gridPaths = grid_for(paths, RBLOCK, 256 * 16);
+  unsigned gridPaths = grid_for(count, RBLOCK, 256 * 16);
   hipLaunchKernelGGL((k_generate<MULTI>), dim3(gridPaths), dim3(RBLOCK), 0, st, P, jobs, njobs,
-                     paths);
+                     base, count);
   CT_LAUNCH_CHECK("k_generate");
   for (int b = 0; b < nb; b++) {
     uint32_t* qin = (b & 1) ? P.q1 : P.q0;
@@ -1061,7 +1076,7 @@ static int launch_round(ct_tracer* T, int njobs, size_t paths, size_t pix, int n
     uint32_t* fExt = P.ctr + 2 * MAX_BOUNCE_SLOTS + b;
     uint32_t* fSh = P.ctr + 3 * MAX_BOUNCE_SLOTS + b;
     mark(T, st, 0, true);
-    dispatch_trace<MULTI>(T->walk, T->stack, T->counting, T->block, T->persistBlocks, paths, st,
+    dispatch_trace<MULTI>(T->walk, T->stack, T->counting, T->block, T->persistBlocks, count, st,
                           P, jobs, qin, cin, fExt, scount, fSh, false);
     CT_LAUNCH_CHECK("k_extend");
     mark(T, st, 0, false);
@@ -1078,16 +1093,66 @@ static int launch_round(ct_tracer* T, int njobs, size_t paths, size_t pix, int n
     }
     mark(T, sst, 2, true);
     dispatch_trace<MULTI>(T->shadowWalk, T->stack, T->counting, T->block, T->persistBlocks,
-                          paths, sst, P, jobs, qin, cin, fExt, scount, fSh, true);
+                          count, sst, P, jobs, qin, cin, fExt, scount, fSh, true);
     CT_LAUNCH_CHECK("k_shadow");
     mark(T, sst, 2, false);
   }
   if (side.side) CT_HIP(side_join(side, st));
-  unsigned gridPix = grid_for(pix, RBLOCK, 256 * 16);
-  hipLaunchKernelGGL((k_finalize<MULTI>), dim3(gridPix), dim3(RBLOCK), 0, st, P, jobs, njobs,
-                     pix);
-  CT_LAUNCH_CHECK("k_finalize");
   return CT_OK;
+}
+
+// One part of a round's launches (RoundPart), graph-capturable.
+template <bool MULTI>
+static int launch_part(ct_tracer* T, const RoundStreams& rs, int njobs, size_t paths, size_t pix,
+                       int nb, int part, hipStream_t st) {
+  if (T->precision == CT_PRECISION_F32) {
+    RoundLaunch R{T->P32, T->jobsDev, njobs, paths, pix, nb, T->stack, T->block,
+                  T->persistBlocks, T->counting, mark_cb, T, rs};
+    return launch_part32(R, MULTI, part, st);
+  }
+  const PathSoA& P = T->P;
+  if (part == ROUND_CLEAR) {
+    hipLaunchKernelGGL(k_clear, dim3(8), dim3(256), 0, st, P.ctr, 2 * GROUP_COUNTERS, P.ctr64,
+                       0);
+    CT_LAUNCH_CHECK("k_clear");
+  } else if (part == ROUND_FINAL) {
+    unsigned gridPix = grid_for(pix, RBLOCK, 256 * 16);
+    hipLaunchKernelGGL((k_finalize<MULTI>), dim3(gridPix), dim3(RBLOCK), 0, st, P, T->jobsDev,
+                       njobs, pix);
+    CT_LAUNCH_CHECK("k_finalize");
+  } else {
+    const int g = part - ROUND_GROUP0;
+    size_t base, count;
+    group_range(rs, paths, g, base, count);
+    return launch_group<MULTI>(T, group_view(P, base, g), njobs, base, count, nb, st,
+                               rs.shadow[g]);
+  }
+  return CT_OK;
+}
+
+// The parts of a round around its path groups: counters cleared on `st`, the
+// groups on their streams (forked from and joined back to `st`), the sums
+// finalized on `st`.  `group` launches group g on its stream.
+template <typename GroupFn>
+static int round_around_groups(const RoundStreams& rs, hipStream_t st, GroupFn group) {
+  if (rs.groups == 2) CT_HIP(side_fork(rs.split, st));
+  int rc = group(0, st);
+  if (!rc && rs.groups == 2) rc = group(1, rs.split.side);
+  if (rc) return rc;
+  if (rs.groups == 2) CT_HIP(side_join(rs.split, st));
+  return CT_OK;
+}
+
+template <bool MULTI>
+static int launch_round(ct_tracer* T, const RoundStreams& rs, int njobs, size_t paths,
+                        size_t pix, int nb, hipStream_t st) {
+  int rc = launch_part<MULTI>(T, rs, njobs, paths, pix, nb, ROUND_CLEAR, st);
+  if (!rc)
+    rc = round_around_groups(rs, st, [&](int g, hipStream_t gs) {
+      return launch_part<MULTI>(T, rs, njobs, paths, pix, nb, ROUND_GROUP0 + g, gs);
+    });
+  if (!rc) rc = launch_part<MULTI>(T, rs, njobs, paths, pix, nb, ROUND_FINAL, st);
+  return rc;
 }
 
 struct BatchTotals {
@@ -1138,53 +1203,73 @@ static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t s
                         hipMemcpyHostToDevice, st));
   int nb = maxB + 1;
   bool multi = R.size() > 1;
+  RoundStreams rs;
+  rc = round_streams(T, paths, rs);
+  if (rc) return rc;
   if (T->graph) {
-    // the captured launches depend only on the round's shape and the tracer's
-    // settings; the job table contents are read from device memory at replay
+    // each path group's launches are one captured graph, replayed on the
+    // group's stream; they depend only on the round's shape and the tracer's
+    // settings, and read the job table from device memory at replay
     std::vector<char> key(sizeof(size_t) * 2 + sizeof(int) * 10);
     size_t kk[2] = {paths, pix};
     int ki[10] = {nb, (int)R.size(), T->walk, T->shadowWalk, T->stack, (int)T->persistBlocks,
-                  T->counting, multi, T->block, T->precision * 2 + (T->overlap ? 1 : 0)};
+                  T->counting, multi, T->block,
+                  T->precision * 64 + (T->overlap ? 32 : 0) + rs.groups};
     memcpy(key.data(), kk, sizeof kk);
     memcpy(key.data() + sizeof kk, ki, sizeof ki);
-    if (!T->graphExec || key != T->graphKey) {
-      if (T->graphExec) {
-        (void)hipGraphExecDestroy(T->graphExec);
-        T->graphExec = nullptr;
-      }
+    if (!T->graphExec[0] || key != T->graphKey) {
+      drop_graphs(T);
       if (!T->ownStream) CT_HIP(hipStreamCreateWithFlags(&T->ownStream, hipStreamNonBlocking));
-      SideStream created;  // stream and events exist before the capture starts
-      rc = side_stream(T, created);
-      if (rc) return rc;
       CT_HIP(hipStreamSynchronize(st));
-      CT_HIP(hipStreamBeginCapture(T->ownStream, hipStreamCaptureModeThreadLocal));
-      rc = multi ? launch_round<true>(T, (int)R.size(), paths, pix, nb, T->ownStream)
-                 : launch_round<false>(T, (int)R.size(), paths, pix, nb, T->ownStream);
-      hipGraph_t g = nullptr;
-      hipError_t e = hipStreamEndCapture(T->ownStream, &g);
-      if (rc) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
+      for (int g = 0; g < rs.groups; g++) {
+        hipStream_t cs = g == 0 ? T->ownStream : rs.split.side;
+        CT_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        rc = multi ? launch_part<true>(T, rs, (int)R.size(), paths, pix, nb, ROUND_GROUP0 + g, cs)
+                   : launch_part<false>(T, rs, (int)R.size(), paths, pix, nb, ROUND_GROUP0 + g,
+                                        cs);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(cs, &graph);
+        if (rc) {
+          if (graph) (void)hipGraphDestroy(graph);
+          drop_graphs(T);
+          return rc;
+        }
+        if (e != hipSuccess) {
+          drop_graphs(T);
+          return check_hip(e, "hipStreamEndCapture");
+        }
+        e = hipGraphInstantiate(&T->graphExec[g], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+          drop_graphs(T);
+          return check_hip(e, "hipGraphInstantiate");
+        }
       }
-      if (e != hipSuccess) return check_hip(e, "hipStreamEndCapture");
-      e = hipGraphInstantiate(&T->graphExec, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (e != hipSuccess) return check_hip(e, "hipGraphInstantiate");
       T->graphKey = key;
     }
-    CT_HIP(hipGraphLaunch(T->graphExec, st));
+    rc = multi ? launch_part<true>(T, rs, (int)R.size(), paths, pix, nb, ROUND_CLEAR, st)
+               : launch_part<false>(T, rs, (int)R.size(), paths, pix, nb, ROUND_CLEAR, st);
+    if (!rc)
+      rc = round_around_groups(rs, st, [&](int g, hipStream_t gs) {
+        return check_hip(hipGraphLaunch(T->graphExec[g], gs), "hipGraphLaunch");
+      });
+    if (!rc)
+      rc = multi ? launch_part<true>(T, rs, (int)R.size(), paths, pix, nb, ROUND_FINAL, st)
+                 : launch_part<false>(T, rs, (int)R.size(), paths, pix, nb, ROUND_FINAL, st);
+    if (rc) return rc;
   } else {
-    rc = multi ? launch_round<true>(T, (int)R.size(), paths, pix, nb, st)
-               : launch_round<false>(T, (int)R.size(), paths, pix, nb, st);
+    rc = multi ? launch_round<true>(T, rs, (int)R.size(), paths, pix, nb, st)
+               : launch_round<false>(T, rs, (int)R.size(), paths, pix, nb, st);
     if (rc) return rc;
   }
-  std::vector<uint32_t> h(2 * MAX_BOUNCE_SLOTS);
+  std::vector<uint32_t> h(2 * GROUP_COUNTERS);
   CT_HIP(hipMemcpyAsync(h.data(), T->P.ctr, h.size() * 4, hipMemcpyDeviceToHost, st));
   CT_HIP(hipStreamSynchronize(st));
-  for (int b = 0; b < nb; b++) {
-    tot.ext += h[b];
-    tot.shadow += h[MAX_BOUNCE_SLOTS + b];
-  }
+  for (int g = 0; g < 2; g++)
+    for (int b = 0; b < nb; b++) {
+      tot.ext += h[g * GROUP_COUNTERS + b];
+      tot.shadow += h[g * GROUP_COUNTERS + MAX_BOUNCE_SLOTS + b];
+    }
   T->lastBounces = std::max(T->lastBounces, nb);
   T->lastRounds++;
   return CT_OK;
@@ -1307,15 +1392,17 @@ int ct_tracer_destroy(ct_tracer* T) {
   if (!T) return CT_OK;
   clear_events(T);
   hipError_t e = hipSuccess;
-  if (T->graphExec) (void)hipGraphExecDestroy(T->graphExec);
+  drop_graphs(T);
   if (T->buf) e = hipFree(T->buf);
   if (T->jobsDev) (void)hipFree(T->jobsDev);
   if (T->jobsHost) (void)hipHostFree(T->jobsHost);
   if (T->compBuf) (void)hipFree(T->compBuf);
   if (T->ownStream) (void)hipStreamDestroy(T->ownStream);
-  if (T->side.side) (void)hipStreamDestroy(T->side.side);
-  if (T->side.fork) (void)hipEventDestroy(T->side.fork);
-  if (T->side.join) (void)hipEventDestroy(T->side.join);
+  for (SideStream* ss : {&T->side, &T->sideB, &T->groupB}) {
+    if (ss->side) (void)hipStreamDestroy(ss->side);
+    if (ss->fork) (void)hipEventDestroy(ss->fork);
+    if (ss->join) (void)hipEventDestroy(ss->join);
+  }
   delete T;
   return check_hip(e, "ct_tracer_destroy");
 }
@@ -1377,6 +1464,7 @@ int ct_tracer_get_config(const ct_tracer* T, ct_tracer_config* out) {
   out->persistent = T->persistBlocks ? 1 : 0;
   out->blocks = T->persistBlocks ? (int32_t)T->persistBlocks : 2048;
   out->workgroup = T->block;
+  out->pathGroups = T->groups;
   return CT_OK;
 }
 
@@ -1389,6 +1477,9 @@ int ct_tracer_set_config(ct_tracer* T, const ct_tracer_config* c) {
   if (!rc) rc = ct_tracer_set_schedule(T, c->persistent, c->blocks);
   if (!rc) rc = ct_tracer_set_workgroup(T, c->workgroup);
   if (!rc) rc = ct_tracer_set_capacity(T, c->maxPaths);
+  if (!rc && (c->pathGroups < 0 || c->pathGroups > 2))
+    rc = set_error(CT_EINVAL, "ct_tracer_set_config: pathGroups must be 0, 1 or 2");
+  if (!rc) T->groups = c->pathGroups;
   if (rc) {  // all or nothing
     std::string msg = ct_last_error();
     T->walk = old.walk;
@@ -1397,6 +1488,7 @@ int ct_tracer_set_config(ct_tracer* T, const ct_tracer_config* c) {
     T->persistBlocks = old.persistent ? (unsigned)old.blocks : 0u;
     T->block = old.workgroup;
     T->maxPaths = old.maxPaths;
+    T->groups = old.pathGroups;
     return set_error(rc, msg);
   }
   return CT_OK;
@@ -1413,11 +1505,7 @@ int ct_tracer_set_capacity(ct_tracer* T, uint64_t maxPaths) {
 int ct_tracer_set_graph(ct_tracer* T, int32_t enable) {
   if (!T) return set_error(CT_EINVAL, "null tracer");
   T->graph = enable != 0;
-  if (!T->graph && T->graphExec) {
-    (void)hipGraphExecDestroy(T->graphExec);
-    T->graphExec = nullptr;
-    T->graphKey.clear();
-  }
+  if (!T->graph) drop_graphs(T);
   return CT_OK;
 }
 

@@ -346,9 +346,11 @@ __global__ void k_clear32(uint32_t* ctr, int n) {
 // generateRay (camera.ts:62-72) with the pixel/lens draws of tracer.ts:373-384.
 template <bool MULTI>
 __global__ void __launch_bounds__(RBLOCK)
-k_generate32(PathSoA32 P, const JobDev* __restrict__ jobs, int njobs, size_t total) {
+k_generate32(PathSoA32 P, const JobDev* __restrict__ jobs, int njobs, size_t base,
+             size_t count) {
   size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t pid = (size_t)blockIdx.x * blockDim.x + threadIdx.x; pid < total; pid += stride) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    const size_t pid = base + i;
     int j = job_of<MULTI, false>(jobs, njobs, pid);
     const JobDev& J = jobs[j];
     const Frame& F = J.F;
@@ -396,9 +398,9 @@ k_generate32(PathSoA32 P, const JobDev* __restrict__ jobs, int njobs, size_t tot
     P.specular[pid] = 1;
     P.rng[pid] = rng.s;
     if (MULTI) P.jobOf[pid] = (uint32_t)j;
-    P.q0[pid] = (uint32_t)pid;
+    P.q0[i] = (uint32_t)pid;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) P.ctr[0] = (uint32_t)total;
+  if (blockIdx.x == 0 && threadIdx.x == 0) P.ctr[0] = (uint32_t)count;
 }
 
 // lightPdfScalar (tracer.ts:61-90)
@@ -1100,14 +1102,12 @@ static void dispatch_trace(int stack, bool counting, int block, unsigned blocks,
 }
 
 template <bool MULTI>
-static int round32(const RoundLaunch& R, hipStream_t st) {
-  const PathSoA32& P = R.P32;
+static int group32(const RoundLaunch& R, const PathSoA32& P, size_t base, size_t count,
+                   hipStream_t st, const SideStream& side) {
   const JobDev* jobs = R.jobs;
-  hipLaunchKernelGGL(k_clear32, dim3(4), dim3(256), 0, st, P.ctr, 4 * MAX_BOUNCE_SLOTS);
-  CT_LAUNCH_CHECK("k_clear32");
-  unsigned gridPaths = grid_for(R.paths, RBLOCK, 256 * 16);
+  unsigned gridPaths = grid_for(count, RBLOCK, 256 * 16);
   hipLaunchKernelGGL((k_generate32<MULTI>), dim3(gridPaths), dim3(RBLOCK), 0, st, P, jobs,
-                     R.njobs, R.paths);
+                     R.njobs, base, count);
   CT_LAUNCH_CHECK("k_generate32");
   for (int b = 0; b < R.nb; b++) {
     uint32_t* qin = (b & 1) ? P.q1 : P.q0;
@@ -1117,41 +1117,57 @@ static int round32(const RoundLaunch& R, hipStream_t st) {
     uint32_t* scount = P.ctr + MAX_BOUNCE_SLOTS + b;
     uint32_t* fExt = P.ctr + 2 * MAX_BOUNCE_SLOTS + b;
     uint32_t* fSh = P.ctr + 3 * MAX_BOUNCE_SLOTS + b;
-    const bool ov = R.side.side != nullptr;
     if (R.mark) R.mark(R.markCtx, st, 0, true);
-    dispatch_trace<MULTI>(R.stack, R.counting, R.block, R.persistBlocks, R.paths, st, P, jobs,
+    dispatch_trace<MULTI>(R.stack, R.counting, R.block, R.persistBlocks, count, st, P, jobs,
                           qin, cin, fExt, scount, fSh, false);
     CT_LAUNCH_CHECK("k_extend32");
     if (R.mark) R.mark(R.markCtx, st, 0, false);
-    if (ov && b > 0) CT_HIP(side_join(R.side, st));  // NEE rays of bounce b-1 done
+    if (side.side && b > 0) CT_HIP(side_join(side, st));  // NEE rays of bounce b-1 done
     if (R.mark) R.mark(R.markCtx, st, 1, true);
     hipLaunchKernelGGL((k_shade32<MULTI>), dim3(gridPaths), dim3(RBLOCK), 0, st, P, jobs, qin,
                        cin, qout, cout, scount, b);
     CT_LAUNCH_CHECK("k_shade32");
     if (R.mark) R.mark(R.markCtx, st, 1, false);
     hipStream_t sst = st;
-    if (ov) {
-      CT_HIP(side_fork(R.side, st));
-      sst = R.side.side;
+    if (side.side) {
+      CT_HIP(side_fork(side, st));
+      sst = side.side;
     }
     if (R.mark) R.mark(R.markCtx, sst, 2, true);
-    dispatch_trace<MULTI>(R.stack, R.counting, R.block, R.persistBlocks, R.paths, sst, P, jobs,
+    dispatch_trace<MULTI>(R.stack, R.counting, R.block, R.persistBlocks, count, sst, P, jobs,
                           qin, cin, fExt, scount, fSh, true);
     CT_LAUNCH_CHECK("k_shadow32");
     if (R.mark) R.mark(R.markCtx, sst, 2, false);
   }
-  if (R.side.side) CT_HIP(side_join(R.side, st));
-  unsigned gridPix = grid_for(R.pix, RBLOCK, 256 * 16);
-  hipLaunchKernelGGL((k_finalize32<MULTI>), dim3(gridPix), dim3(RBLOCK), 0, st, P, jobs, R.njobs,
-                     R.pix);
-  CT_LAUNCH_CHECK("k_finalize32");
+  if (side.side) CT_HIP(side_join(side, st));
+  return CT_OK;
+}
+
+template <bool MULTI>
+static int part32(const RoundLaunch& R, int part, hipStream_t st) {
+  const PathSoA32& P = R.P32;
+  const RoundStreams& rs = R.rs;
+  if (part == ROUND_CLEAR) {
+    hipLaunchKernelGGL(k_clear32, dim3(8), dim3(256), 0, st, P.ctr, 2 * GROUP_COUNTERS);
+    CT_LAUNCH_CHECK("k_clear32");
+  } else if (part == ROUND_FINAL) {
+    unsigned gridPix = grid_for(R.pix, RBLOCK, 256 * 16);
+    hipLaunchKernelGGL((k_finalize32<MULTI>), dim3(gridPix), dim3(RBLOCK), 0, st, P, R.jobs,
+                       R.njobs, R.pix);
+    CT_LAUNCH_CHECK("k_finalize32");
+  } else {
+    const int g = part - ROUND_GROUP0;
+    size_t base, count;
+    group_range(rs, R.paths, g, base, count);
+    return group32<MULTI>(R, group_view(P, base, g), base, count, st, rs.shadow[g]);
+  }
   return CT_OK;
 }
 
 }  // namespace f32
 
-int launch_round32(const RoundLaunch& R, bool multi, hipStream_t st) {
-  return multi ? f32::round32<true>(R, st) : f32::round32<false>(R, st);
+int launch_part32(const RoundLaunch& R, bool multi, int part, hipStream_t st) {
+  return multi ? f32::part32<true>(R, part, st) : f32::part32<false>(R, part, st);
 }
 
 }  // namespace ct

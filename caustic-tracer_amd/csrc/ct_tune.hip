@@ -54,7 +54,8 @@ std::string signature(const ct_job* jobs, uint32_t n) {
   return buf;
 }
 
-// Cache line: "<signature> maxPaths walk shadowWalk stack persistent blocks workgroup ms".
+// Cache line: "<signature> maxPaths walk shadowWalk stack persistent blocks workgroup
+// pathGroups ms".
 bool cache_lookup(const char* path, const std::string& sig, ct_tracer_config& c, double& ms) {
   std::ifstream f(path);
   if (!f) return false;
@@ -67,7 +68,7 @@ bool cache_lookup(const char* path, const std::string& sig, ct_tracer_config& c,
     unsigned long long mp = 0;
     double m = 0;
     if (!(is >> s >> mp >> r.walk >> r.shadowWalk >> r.stackDepth >> r.persistent >> r.blocks >>
-          r.workgroup >> m))
+          r.workgroup >> r.pathGroups >> m))
       continue;
     if (s != sig) continue;
     r.maxPaths = mp;
@@ -81,8 +82,9 @@ bool cache_lookup(const char* path, const std::string& sig, ct_tracer_config& c,
 void cache_append(const char* path, const std::string& sig, const ct_tracer_config& c, double ms) {
   FILE* f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s %llu %d %d %d %d %d %d %.4f\n", sig.c_str(), (unsigned long long)c.maxPaths,
-          c.walk, c.shadowWalk, c.stackDepth, c.persistent, c.blocks, c.workgroup, ms);
+  fprintf(f, "%s %llu %d %d %d %d %d %d %d %.4f\n", sig.c_str(), (unsigned long long)c.maxPaths,
+          c.walk, c.shadowWalk, c.stackDepth, c.persistent, c.blocks, c.workgroup, c.pathGroups,
+          ms);
   fclose(f);
 }
 
@@ -99,7 +101,7 @@ int time_batch(ct_tracer* T, const ct_job* jobs, uint32_t n, hipStream_t st, dou
 bool same(const ct_tracer_config& a, const ct_tracer_config& b) {
   return a.maxPaths == b.maxPaths && a.walk == b.walk && a.shadowWalk == b.shadowWalk &&
          a.stackDepth == b.stackDepth && a.persistent == b.persistent && a.blocks == b.blocks &&
-         a.workgroup == b.workgroup;
+         a.workgroup == b.workgroup && a.pathGroups == b.pathGroups;
 }
 
 }  // namespace
@@ -215,6 +217,11 @@ extern "C" int ct_tracer_autotune(ct_tracer* T, const ct_job* jobs, uint32_t n, 
     ct_tracer_config c = cur;
     c.persistent = 1;
     c.blocks = std::max(1, lanes / c.workgroup);
+    if ((rc = attempt(c))) return rc;
+  }
+  for (int groups : {1, 2}) {
+    ct_tracer_config c = cur;
+    c.pathGroups = groups;
     if ((rc = attempt(c))) return rc;
   }
   rc = ct_tracer_set_config(T, &cur);

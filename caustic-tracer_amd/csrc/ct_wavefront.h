@@ -41,8 +41,8 @@ struct PathSoA {
   uint32_t* sPath;
   double *sdx, *sdy, *sdz, *sTmax, *sCr, *sCg, *sCb;
   // counters: live[b] at [b], shadow[b] at [MAX_BOUNCE_SLOTS + b], ray-fetch
-  // cursors of the extend/shadow launches at [2*MAX + b] and [3*MAX + b];
-  // 64-bit work counters at ctr64
+  // cursors of the extend/shadow launches at [2*MAX + b] and [3*MAX + b], one
+  // such block per path group; 64-bit work counters at ctr64
   uint32_t* ctr;
   unsigned long long* ctr64;
 };
@@ -184,6 +184,48 @@ inline hipError_t side_join(const SideStream& s, hipStream_t st) {
   return e == hipSuccess ? hipStreamWaitEvent(st, s.join, 0) : e;
 }
 
+// Streams of one round.  The round's paths are traced as `groups` (1 or 2)
+// independent groups, each on its own stream (group 0 on the caller's stream,
+// group 1 on `split.side`, forked from it at the start and joined before the
+// sums are finalized), so that one group's shading overlaps the other's
+// traversal and each launch's tail is filled by the other group's work.
+// Within a group the NEE launches may run on a side stream of their own.
+struct RoundStreams {
+  int groups = 1;
+  SideStream split;
+  SideStream shadow[2];
+};
+
+// Counters per path group: live[b], shadow[b], extend and shadow fetch cursors.
+constexpr int GROUP_COUNTERS = 4 * MAX_BOUNCE_SLOTS;
+
+// Parts of a round's launches: clearing the counters, each path group's
+// launches (generate, then per bounce extend, shade and NEE), and the
+// per-pixel sums once every group is done.  A captured round holds one graph
+// per path group, launched on that group's stream between the other parts.
+enum RoundPart { ROUND_CLEAR = 0, ROUND_GROUP0 = 1, ROUND_GROUP1 = 2, ROUND_FINAL = 3 };
+
+// Paths [base, base + count) of path group g of a round of `paths` paths.
+inline void group_range(const RoundStreams& rs, size_t paths, int g, size_t& base,
+                        size_t& count) {
+  size_t half = rs.groups == 2 ? paths / 2 : paths;
+  base = g == 0 ? 0 : half;
+  count = g == 0 ? half : paths - half;
+}
+
+// The queue and counter view of path group g whose paths start at `base`:
+// queues and the shadow queue hold the group's entries from `base` on; the
+// per-path arrays stay indexed by path id.
+template <typename SoA>
+inline SoA group_view(const SoA& P, size_t base, int g) {
+  SoA Q = P;
+  Q.q0 += base; Q.q1 += base; Q.sPath += base;
+  Q.sdx += base; Q.sdy += base; Q.sdz += base; Q.sTmax += base;
+  Q.sCr += base; Q.sCg += base; Q.sCb += base;
+  Q.ctr += (size_t)g * GROUP_COUNTERS;
+  return Q;
+}
+
 // One launch round of the FP32 precision kernels (ct_render32.hip).
 struct RoundLaunch {
   PathSoA32 P32;
@@ -197,8 +239,8 @@ struct RoundLaunch {
   // profiling hook: kind 0 extend, 1 shade, 2 shadow
   void (*mark)(void* ctx, hipStream_t st, int kind, bool begin);
   void* markCtx;
-  SideStream side;
+  RoundStreams rs;
 };
-int launch_round32(const RoundLaunch& R, bool multi, hipStream_t st);
+int launch_part32(const RoundLaunch& R, bool multi, int part, hipStream_t st);
 
 }  // namespace ct

@@ -256,6 +256,10 @@ typedef struct ct_tracer_config {
   int32_t persistent; /* 1: persistent traversal launches pulling from the queue */
   int32_t blocks;     /* workgroups of the persistent launches */
   int32_t workgroup;  /* lanes per traversal workgroup: 64, 128, 256 */
+  int32_t pathGroups; /* 1 or 2: a round's paths traced as that many groups, each on its
+                         own stream (its NEE launches on a side stream), so that one
+                         group's shading overlaps the other's traversal; 0: by precision
+                         (1 for binary64, 2 for the FP32 form) */
 } ct_tracer_config;
 int ct_tracer_get_config(const ct_tracer *tracer, ct_tracer_config *out);
 int ct_tracer_set_config(ct_tracer *tracer, const ct_tracer_config *config);
@@ -266,7 +270,8 @@ int ct_tracer_set_config(ct_tracer *tracer, const ct_tracer_config *config);
  * capacity (paths per round), the traversal policy (ordered walk with an LDS
  * stack of 8/16/24 entries that restarts a deep ray with the stackless walk,
  * or the stackless walk alone; binary or 4-wide nodes), the workgroup size
- * (64/128/256 lanes) and the lanes kept in flight by the persistent launches.
+ * (64/128/256 lanes), the lanes kept in flight by the persistent launches and
+ * the number of path groups traced side by side.
  * Every trial renders the jobs, so on return the outputs hold the batch's
  * images (identical under every setting).  The best setting is applied to
  * the tracer.  With cachePath set, a file of earlier results keyed by the

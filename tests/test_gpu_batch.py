@@ -133,7 +133,8 @@ def _images(jobs):
 @pytest.mark.parametrize("setting", [
     {"workgroup": 64}, {"workgroup": 128}, {"walk": 1, "shadowWalk": 1, "stackDepth": 8},
     {"walk": 0, "shadowWalk": 0}, {"walk": 2, "shadowWalk": 0, "stackDepth": 16},
-    {"persistent": 0}, {"maxPaths": 4096}, {"blocks": 64, "workgroup": 64}])
+    {"persistent": 0}, {"maxPaths": 4096}, {"blocks": 64, "workgroup": 64},
+    {"pathGroups": 1}, {"pathGroups": 2, "maxPaths": 1 << 21}])
 def test_launch_settings_do_not_change_images(ct, O, setting):
     jobs = _family_jobs(ct)
     ref_tr = ct.Tracer()
