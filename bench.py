@@ -112,7 +112,7 @@ def main():
     ap.add_argument("--walk", choices=["ordered", "stackless", "wide"], default="wide")
     ap.add_argument("--shadow-walk", choices=["ordered", "stackless", "wide"], default="wide")
     ap.add_argument("--stack", type=int, default=24, help="LDS stack entries of the ordered walks")
-    ap.add_argument("--persist-blocks", type=int, default=2048,
+    ap.add_argument("--persist-blocks", type=int, default=768,
                     help="workgroups of the persistent traversal launches (0: one lane per ray)")
     ap.add_argument("--rays", type=int, default=1 << 24,
                     help="plain: random closest-hit queries timed beside the render (0: none)")
@@ -170,6 +170,19 @@ def extend_roofline(cnt, prof, kernel, precision="f64", quantized=True):
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
             "bytes_per_launch": ext_bytes // launches,
             "avg_launch_ms": round(prof["extendMs"] / launches, 4)}
+
+
+def traversal_bytes(cnt, precision, quantized):
+    """Algorithmic bytes of all traversal work of one render (extension and NEE
+    rays), per the same per-node/primitive/ray figures as extend_roofline; an
+    NEE ray also reads its direction, segment end and pending term (56 B)."""
+    node_b = 16 if quantized or precision == "f32" else 32
+    ray_b = 32 if precision == "f32" else 40
+    ext = (node_b * cnt["nodeVisits"] + 48 * cnt["triTests"] + 32 * cnt["sphereTests"] +
+           ray_b * cnt["extensionRays"])
+    sh = (node_b * cnt["shadowNodeVisits"] + 48 * cnt["shadowTriTests"] +
+          32 * cnt["shadowSphereTests"] + (ray_b + 56) * cnt["shadowRays"])
+    return ext + sh
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "extend_traffic_soup1m_1080p_4spp.json")
@@ -236,12 +249,17 @@ def plain_form(ct, args):
         ct.sync()
         log(f"warmup {i} done")
     ct.sync()
+    # HIP events bracket every launch on its own stream during the timed steps;
+    # the last step's events give the per-launch device time of each kernel
+    tr.set_profiling(True)
     t = time.perf_counter()
     for i in range(args.steps):
         tr.render_image(ds, frame, opts, out)
     ct.sync()
     total = time.perf_counter() - t
     st = tr.stats()
+    prof = st
+    tr.set_profiling(False)
     rays = st["extensionRays"] + st["shadowRays"]
     ms_step = total / args.steps * 1e3
     mrays = rays * args.steps / total / 1e6
@@ -249,11 +267,6 @@ def plain_form(ct, args):
     log(f"timed {args.steps} steps: {ms_step:.1f} ms/step, {mrays:.0f} Mrays/s, "
         f"{sppmpix:.1f} spp*Mpix/s, rays/step ext {st['extensionRays']} shadow {st['shadowRays']}")
 
-    # per-launch device time of the dominant kernel (HIP events on the launch stream)
-    tr.set_profiling(True)
-    tr.render_image(ds, frame, opts, out)
-    prof = tr.stats()
-    tr.set_profiling(False)
     # algorithmic work of the same launches (counting build of the kernels, untimed)
     tr.set_counting(True)
     tr.render_image(ds, frame, opts, out)
@@ -263,6 +276,12 @@ def plain_form(ct, args):
              else f"k_extend<{args.walk},{args.stack}>")
     roof = extend_roofline(cnt, prof, kname, args.precision, quantized=args.walk == "wide")
     roof["traffic"] = measured_traffic(args)
+    # launches of both path groups and the NEE launches run side by side, so a
+    # launch's duration is a shared-chip figure; beside it, the algorithmic
+    # traversal bytes of a whole step (extension + NEE rays) over the step time
+    chip = traversal_bytes(cnt, args.precision, args.walk == "wide") / (ms_step / 1e3) / 1e9
+    roof["concurrent_launches"] = True
+    roof["step_traversal_gbs"] = round(chip, 1)
     log(f"extend: {prof['extendLaunches']} launches, {prof['extendMs']:.2f} ms total, "
         f"{cnt['nodeVisits'] / max(1, cnt['extensionRays']):.1f} nodes/ray, "
         f"{cnt['triTests'] / max(1, cnt['extensionRays']):.1f} tris/ray, {roof['achieved']:.0f} GB/s, "

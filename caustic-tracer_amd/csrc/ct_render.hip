@@ -812,13 +812,16 @@ struct ct_tracer {
   int walk = CT_WALK_WIDE;           // closest-hit walk of extension rays
   int shadowWalk = CT_WALK_WIDE;     // any-hit walk of NEE rays
   int stack = 24;                    // LDS stack entries of the ordered walks
-  unsigned persistBlocks = 256 * 8;  // workgroups of the traversal launches (0: one lane per ray)
+  // workgroups of each traversal launch (0: one lane per ray); up to four run
+  // at once (two path groups, each with its NEE launch beside its extension
+  // launch), so each takes a share of the chip: 768 x 256 lanes measured best
+  unsigned persistBlocks = 768;
   int block = 256;                   // lanes per workgroup of the traversal launches
   size_t maxPaths = size_t(1) << 25; // paths in flight per round (~190 B of state each)
   bool graph = false;                // capture rounds into a hipGraph and replay them
   hipStream_t ownStream = nullptr;   // capture stream
   bool overlap = true;               // NEE launches on side streams (SideStream)
-  int groups = 0;                    // path groups per round (0: 1 binary64, 2 FP32)
+  int groups = 2;                    // path groups per round, each on its own stream
   SideStream side, sideB, groupB;    // streams and events, created on first use
   hipGraphExec_t graphExec[2] = {nullptr, nullptr};  // one per path group
   std::vector<char> graphKey;
@@ -828,6 +831,7 @@ struct ct_tracer {
   uint64_t lastExtension = 0;
   uint64_t lastShadow = 0;
   uint32_t lastRounds = 0;
+  uint32_t lastExtendLaunches = 0;
   std::vector<hipEvent_t> events;  // (begin, end) pairs per launch when profiling
   std::vector<int> eventKind;      // 0 extend, 1 shade, 2 shadow
   hipStream_t lastStream = nullptr;
@@ -1031,21 +1035,21 @@ static int make_side(SideStream& ss) {
 // Smallest round traced as two path groups.
 constexpr size_t SPLIT_MIN_PATHS = size_t(1) << 20;
 
-// The streams of a round of `paths` paths.  Profiling runs keep one stream so
-// that each launch's events bracket that launch alone.
+// The streams of a round of `paths` paths.  Profiling events are recorded on
+// each launch's own stream, so they bracket that launch as it runs beside the
+// others.
 static int round_streams(ct_tracer* T, size_t paths, RoundStreams& rs) {
   rs = RoundStreams();
-  if (T->profiling && !T->graph) return CT_OK;
   int rc;
-  // measured on a 1080p frame: two groups pay for the FP32 kernels (+5%), not
-  // for the binary64 ones (-1%), whose shading runs at three waves per SIMD
-  const int groups = T->groups ? T->groups : (T->precision == CT_PRECISION_F32 ? 2 : 1);
-  if (groups > 1 && paths >= SPLIT_MIN_PATHS) {
+  if (T->groups > 1 && paths >= SPLIT_MIN_PATHS) {
     if ((rc = make_side(T->groupB))) return rc;
     rs.groups = 2;
     rs.split = T->groupB;
   }
-  if (T->overlap) {
+  // NEE side streams pay in eager rounds; inside a captured graph the side
+  // branch replays behind its group's chain on this ROCm and only adds the
+  // fork/join nodes (64-scene batch: 179.5 vs 176.6 ms)
+  if (T->overlap && !T->graph) {
     if ((rc = make_side(T->side))) return rc;
     rs.shadow[0] = T->side;
     if (rs.groups == 2) {
@@ -1272,6 +1276,7 @@ static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t s
     }
   T->lastBounces = std::max(T->lastBounces, nb);
   T->lastRounds++;
+  T->lastExtendLaunches += (uint32_t)(nb * rs.groups);
   return CT_OK;
 }
 
@@ -1282,6 +1287,7 @@ int ct::run_plan(ct_tracer* T, std::vector<PlanJob> jobs, hipStream_t st) {
   T->lastStream = st;
   T->lastBounces = 0;
   T->lastRounds = 0;
+  T->lastExtendLaunches = 0;
   BatchTotals tot;
   uint64_t paths = 0;
   for (const PlanJob& j : jobs) {
@@ -1462,7 +1468,7 @@ int ct_tracer_get_config(const ct_tracer* T, ct_tracer_config* out) {
   out->shadowWalk = T->shadowWalk;
   out->stackDepth = T->stack;
   out->persistent = T->persistBlocks ? 1 : 0;
-  out->blocks = T->persistBlocks ? (int32_t)T->persistBlocks : 2048;
+  out->blocks = T->persistBlocks ? (int32_t)T->persistBlocks : 768;
   out->workgroup = T->block;
   out->pathGroups = T->groups;
   return CT_OK;
@@ -1477,8 +1483,8 @@ int ct_tracer_set_config(ct_tracer* T, const ct_tracer_config* c) {
   if (!rc) rc = ct_tracer_set_schedule(T, c->persistent, c->blocks);
   if (!rc) rc = ct_tracer_set_workgroup(T, c->workgroup);
   if (!rc) rc = ct_tracer_set_capacity(T, c->maxPaths);
-  if (!rc && (c->pathGroups < 0 || c->pathGroups > 2))
-    rc = set_error(CT_EINVAL, "ct_tracer_set_config: pathGroups must be 0, 1 or 2");
+  if (!rc && (c->pathGroups < 1 || c->pathGroups > 2))
+    rc = set_error(CT_EINVAL, "ct_tracer_set_config: pathGroups must be 1 or 2");
   if (!rc) T->groups = c->pathGroups;
   if (rc) {  // all or nothing
     std::string msg = ct_last_error();
@@ -1603,7 +1609,7 @@ int ct_tracer_stats(ct_tracer* T, ct_render_stats* out) {
       }
     }
   } else {
-    out->extendLaunches = (uint32_t)T->lastBounces * T->lastRounds;
+    out->extendLaunches = T->lastExtendLaunches;
   }
   if (T->counting && T->buf) {
     unsigned long long h[8];

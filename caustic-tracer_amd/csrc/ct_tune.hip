@@ -164,7 +164,7 @@ extern "C" int ct_tracer_autotune(ct_tracer* T, const ct_job* jobs, uint32_t n, 
                              {CT_WALK_ORDERED, CT_WALK_ORDERED, 24}, {CT_WALK_WIDE, CT_WALK_STACKLESS, 24},
                              {CT_WALK_STACKLESS, CT_WALK_STACKLESS, 24}};
   const int workgroups[] = {64, 128, 256};
-  const int lanesInFlight[] = {1 << 18, 1 << 19, 1 << 20};
+  const int lanesInFlight[] = {1 << 17, 3 << 16, 1 << 18, 1 << 19};
 
   // Time the entry setting twice: the first call also allocates the path
   // state and loads the kernels.

@@ -277,7 +277,7 @@ class Tracer:
     def set_traversal(self, shadow_walk=WALK_WIDE, stack=24):
         check(lib().ct_tracer_set_traversal(self.handle, shadow_walk, stack), "set_traversal")
 
-    def set_schedule(self, persistent=True, blocks=2048):
+    def set_schedule(self, persistent=True, blocks=768):
         check(lib().ct_tracer_set_schedule(self.handle, int(bool(persistent)), blocks), "schedule")
 
     def set_profiling(self, on):

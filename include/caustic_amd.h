@@ -258,8 +258,7 @@ typedef struct ct_tracer_config {
   int32_t workgroup;  /* lanes per traversal workgroup: 64, 128, 256 */
   int32_t pathGroups; /* 1 or 2: a round's paths traced as that many groups, each on its
                          own stream (its NEE launches on a side stream), so that one
-                         group's shading overlaps the other's traversal; 0: by precision
-                         (1 for binary64, 2 for the FP32 form) */
+                         group's shading overlaps the other's traversal (default 2) */
 } ct_tracer_config;
 int ct_tracer_get_config(const ct_tracer *tracer, ct_tracer_config *out);
 int ct_tracer_set_config(ct_tracer *tracer, const ct_tracer_config *config);
