@@ -607,8 +607,9 @@ __global__ void __launch_bounds__(BLOCK)
           if (cur == -1) walk_pop<WIDE>(cur, sp, sNode, sEntry, lanes, h.t);
           if (__all(pk.n == PARK_LEAVES || cur == -1)) break;
         }
-        for (int k = 0; k < pk.n; k++)
-          test_leaf<COUNT>(S, parked(pk, k), ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
+#pragma unroll
+        for (int k = 0; k < PARK_LEAVES; k++)  // constant slots: the words stay in registers
+          if (k < pk.n) test_leaf<COUNT>(S, pk.w[k], ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
         pk.n = 0;
         if (overflow) {
           if (COUNT) cnt.fallbacks++;
@@ -726,8 +727,10 @@ k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
           }
           if (__all(pk.n == PARK_LEAVES || cur == -1)) break;
         }
-        for (int k = 0; k < pk.n && !blocked; k++)
-          blocked = leaf_blocks(S, parked(pk, k), ox, oy, oz, dx, dy, dz, tMax, cnt, COUNT);
+#pragma unroll
+        for (int k = 0; k < PARK_LEAVES; k++)
+          if (k < pk.n && !blocked)
+            blocked = leaf_blocks(S, pk.w[k], ox, oy, oz, dx, dy, dz, tMax, cnt, COUNT);
         pk.n = 0;
         if (overflow && !blocked) {
           if (COUNT) cnt.fallbacks++;

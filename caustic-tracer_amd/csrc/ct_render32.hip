@@ -871,7 +871,9 @@ k_extend32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
         }
         if (__all(pk.n == PARK_LEAVES || cur == -1)) break;
       }
-      for (int k = 0; k < pk.n; k++) test_leaf<COUNT>(S, parked(pk, k), R, T_MIN_F, h, cnt);
+#pragma unroll
+      for (int k = 0; k < PARK_LEAVES; k++)  // constant slots: the words stay in registers
+        if (k < pk.n) test_leaf<COUNT>(S, pk.w[k], R, T_MIN_F, h, cnt);
       pk.n = 0;
       if (overflow) {
         if (COUNT) cnt.fallbacks++;
@@ -949,8 +951,9 @@ k_shadow32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
         }
         if (__all(pk.n == PARK_LEAVES || cur == -1)) break;
       }
-      for (int k = 0; k < pk.n && !blocked; k++)
-        blocked = leaf_blocks<COUNT>(S, parked(pk, k), R, T_MIN_F, tMax, cnt);
+#pragma unroll
+      for (int k = 0; k < PARK_LEAVES; k++)
+        if (k < pk.n && !blocked) blocked = leaf_blocks<COUNT>(S, pk.w[k], R, T_MIN_F, tMax, cnt);
       pk.n = 0;
       if (overflow && !blocked) {
         if (COUNT) cnt.fallbacks++;

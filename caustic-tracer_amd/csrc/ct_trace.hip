@@ -136,8 +136,9 @@ k_closest_wide(DevScene S, const double* __restrict__ rays, uint32_t n, double t
         if (cur == -1) qw::qpop(cur, sp, sNode, sEntry, lanes, qw::cull_bound(h.t));
         if (__all(pk.n == PARK_LEAVES || cur == -1)) break;
       }
-      for (int k = 0; k < pk.n; k++)
-        test_leaf<COUNT>(S, parked(pk, k), ox, oy, oz, dx, dy, dz, tMin, h, cnt);
+#pragma unroll
+      for (int k = 0; k < PARK_LEAVES; k++)  // constant slots: the words stay in registers
+        if (k < pk.n) test_leaf<COUNT>(S, pk.w[k], ox, oy, oz, dx, dy, dz, tMin, h, cnt);
       pk.n = 0;
       if (overflow) {
         if (COUNT) cnt.fallbacks++;

@@ -155,13 +155,6 @@ __device__ __forceinline__ void park(Parked& p, int word) {
   p.n++;
 }
 
-__device__ __forceinline__ int parked(const Parked& p, int k) {
-  int w = p.w[0];
-#pragma unroll
-  for (int j = 1; j < PARK_LEAVES; j++)
-    if (k == j) w = p.w[j];
-  return w;
-}
 
 // Second stream for the NEE launches.  The shadow rays of bounce b and the
 // extension rays of bounce b+1 are independent (one adds to the path's
