@@ -341,7 +341,10 @@ k_shade(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__
           if (m1.z > 0.5f) {
             double cs = m1.y;
             double sum = floor(px * cs + 0.5) + floor(py * cs + 0.5) + floor(pz * cs + 0.5);
-            if (fmod(fmod(sum, 2.0) + 2.0, 2.0) != 0) {  // !checkerPrimary
+            // ((sum % 2) + 2) % 2 != 0 (materials.ts checker parity): sum is an
+            // integer, so sum - 2*floor(sum/2) is exactly 1 when odd, 0 when even
+            // (NaN for NaN or infinities, as the remainders give)
+            if (sum - 2.0 * floor(0.5 * sum) != 0) {  // !checkerPrimary
               float4 m2 = gload(S.mats + m + 2);
               ar = m2.x;
               ag = m2.y;
@@ -1067,8 +1070,26 @@ static int round_streams(ct_tracer* T, size_t paths, RoundStreams& rs) {
 // Kernel launches of one path group: generate, then per bounce extend, shade
 // and the NEE launch (on the group's side stream when it has one).
 template <bool MULTI>
+static int launch_group_body(ct_tracer* T, const PathSoA& P, int njobs, size_t base,
+                             size_t count, int nb, hipStream_t st, const SideStream& side);
+
+// A group's launches; its NEE side stream is joined back to `st` on every
+// path out, so no later work on `st` can overlap a launch still on the side
+// stream.
+template <bool MULTI>
 static int launch_group(ct_tracer* T, const PathSoA& P, int njobs, size_t base, size_t count,
                         int nb, hipStream_t st, const SideStream& side) {
+  int rc = launch_group_body<MULTI>(T, P, njobs, base, count, nb, st, side);
+  if (side.side) {
+    int rj = check_hip(side_join(side, st), "side_join");
+    if (!rc) rc = rj;
+  }
+  return rc;
+}
+
+template <bool MULTI>
+static int launch_group_body(ct_tracer* T, const PathSoA& P, int njobs, size_t base,
+                             size_t count, int nb, hipStream_t st, const SideStream& side) {
   const JobDev* jobs = T->jobsDev;
   unsigned gridPaths = grid_for(count, RBLOCK, 256 * 16);
   hipLaunchKernelGGL((k_generate<MULTI>), dim3(gridPaths), dim3(RBLOCK), 0, st, P, jobs, njobs,
@@ -1104,7 +1125,6 @@ static int launch_group(ct_tracer* T, const PathSoA& P, int njobs, size_t base, 
     CT_LAUNCH_CHECK("k_shadow");
     mark(T, sst, 2, false);
   }
-  if (side.side) CT_HIP(side_join(side, st));
   return CT_OK;
 }
 
@@ -1145,9 +1165,11 @@ static int round_around_groups(const RoundStreams& rs, hipStream_t st, GroupFn g
   if (rs.groups == 2) CT_HIP(side_fork(rs.split, st));
   int rc = group(0, st);
   if (!rc && rs.groups == 2) rc = group(1, rs.split.side);
-  if (rc) return rc;
-  if (rs.groups == 2) CT_HIP(side_join(rs.split, st));
-  return CT_OK;
+  if (rs.groups == 2) {  // joined on every path out
+    int rj = check_hip(side_join(rs.split, st), "side_join");
+    if (!rc) rc = rj;
+  }
+  return rc;
 }
 
 template <bool MULTI>

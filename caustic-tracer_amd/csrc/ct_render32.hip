@@ -655,7 +655,8 @@ k_shade32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restri
           if (m1.z > 0.5f) {
             float cs = m1.y;
             float sum = floorf(px * cs + 0.5f) + floorf(py * cs + 0.5f) + floorf(pz * cs + 0.5f);
-            if (fmodf(fmodf(sum, 2.0f) + 2.0f, 2.0f) != 0) {  // !checkerPrimary
+            // checker parity of the integer sum, exactly (see the binary64 kernel)
+            if (sum - 2.0f * floorf(0.5f * sum) != 0) {  // !checkerPrimary
               float4 m2 = gload(S.mats + m + 2);
               ar = m2.x;
               ag = m2.y;
@@ -1105,8 +1106,24 @@ static void dispatch_trace(int stack, bool counting, int block, unsigned blocks,
 }
 
 template <bool MULTI>
+static int group32_body(const RoundLaunch& R, const PathSoA32& P, size_t base, size_t count,
+                        hipStream_t st, const SideStream& side);
+
+// A group's launches; its NEE side stream is joined back to `st` on every path out.
+template <bool MULTI>
 static int group32(const RoundLaunch& R, const PathSoA32& P, size_t base, size_t count,
                    hipStream_t st, const SideStream& side) {
+  int rc = group32_body<MULTI>(R, P, base, count, st, side);
+  if (side.side) {
+    int rj = check_hip(side_join(side, st), "side_join");
+    if (!rc) rc = rj;
+  }
+  return rc;
+}
+
+template <bool MULTI>
+static int group32_body(const RoundLaunch& R, const PathSoA32& P, size_t base, size_t count,
+                        hipStream_t st, const SideStream& side) {
   const JobDev* jobs = R.jobs;
   unsigned gridPaths = grid_for(count, RBLOCK, 256 * 16);
   hipLaunchKernelGGL((k_generate32<MULTI>), dim3(gridPaths), dim3(RBLOCK), 0, st, P, jobs,
@@ -1142,7 +1159,6 @@ static int group32(const RoundLaunch& R, const PathSoA32& P, size_t base, size_t
     CT_LAUNCH_CHECK("k_shadow32");
     if (R.mark) R.mark(R.markCtx, sst, 2, false);
   }
-  if (side.side) CT_HIP(side_join(side, st));
   return CT_OK;
 }
 
