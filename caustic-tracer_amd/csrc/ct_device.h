@@ -28,6 +28,13 @@ constexpr double PI_D = 3.141592653589793;
 constexpr double TWO_PI = 2 * PI_D;
 constexpr double INV_PI = 1 / PI_D;
 constexpr double SHADOW_SHORTEN = 1 - 1e-3;  // tracer.ts:36
+// Quantized 4-wide nodes numbered breadth first at upload: the root and the
+// levels below it, up to this many nodes, form one block at the start of the
+// array, which single-scene traversal launches copy into LDS.  16 nodes (1 KB)
+// keep the 24-entry stack kernels within 31 KB of LDS per 256-lane workgroup;
+// 32 nodes (32 KB) cost the binary64 extend launches 5% on the 1M-triangle
+// soup, which reads as one workgroup fewer per CU.
+constexpr int TOP_NODES = 16;
 
 // Read-only view of a device-resident packed scene.
 struct DevScene {
@@ -118,6 +125,17 @@ __device__ __forceinline__ float4 gload(const float4* p) {
 __device__ __forceinline__ int4 gload(const int4* p) {
   ct_ni4 v = *(const CT_GLOBAL ct_ni4*)p;
   return make_int4(v.x, v.y, v.z, v.w);
+}
+// LDS block of the top traversal nodes, read and written through
+// address-space-3 pointers (ds_read/ds_write, never FLAT).
+#define CT_LDS __attribute__((address_space(3)))
+__device__ __forceinline__ float4 lload(const float4* p) {
+  ct_nf4 v = *(const CT_LDS ct_nf4*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void lstore(float4* p, float4 x) {
+  ct_nf4 v = {x.x, x.y, x.z, x.w};
+  *(CT_LDS ct_nf4*)p = v;
 }
 __device__ __forceinline__ uint32_t gload(const uint32_t* p) {
   return *(const CT_GLOBAL uint32_t*)p;

@@ -112,20 +112,45 @@ __device__ __forceinline__ float entry_value(Entry e) {
 __device__ __forceinline__ float cull_bound(float t) { return t * CULL_SLACK; }
 __device__ __forceinline__ float cull_bound(double t) { return (float)t * CULL_SLACK; }
 
+// Copies the first min(TOP, wideCount) quantized nodes of S (the top levels:
+// they are numbered breadth first) into the workgroup's LDS block `top`; every
+// lane of the workgroup must call it, before its first walk step.
+template <int TOP>
+__device__ __forceinline__ void stage_top(float4* __restrict__ top, const DevScene& S) {
+  if constexpr (TOP > 0) {
+    const int n = 4 * min(TOP, S.wideCount);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) lstore(top + i, gload(S.qwide + i));
+    __syncthreads();
+  }
+}
+
 // One step of the ordered walk over quantized 4-wide node `cur`: the four
 // child boxes are tested (near/far planes chosen by the ray's direction
 // signs), the nearest hit child continues, the others go on the LDS stack
 // nearest-last with a one-byte lower bound of their entry (PUSH_ENTRY) or without
-// one (any hit).  Returns false when the stack would overflow.
-template <bool PUSH_ENTRY, int STACK, bool COUNT>
+// one (any hit).  Returns false when the stack would overflow.  With TOP > 0
+// a step in which every active lane of the wave is at a node below TOP reads
+// the LDS copy `top` (stage_top).
+template <bool PUSH_ENTRY, int STACK, bool COUNT, int TOP = 0>
 __device__ __forceinline__ bool qwide_step(const DevScene& S, const QRay& R, float tCull, int& cur,
                                            int& sp, int* __restrict__ sNode,
                                            Entry* __restrict__ sEntry, int lanes,
-                                           TraversalCounts& cnt) {
+                                           TraversalCounts& cnt,
+                                           const float4* __restrict__ top = nullptr) {
   const float INF = __builtin_inff();
-  const float4* w = S.qwide + 4 * (size_t)cur;
-  float4 w0 = gload(w), w1 = gload(w + 1), w2 = gload(w + 2);
-  int4 cc = gload((const int4*)(w + 3));
+  float4 w0, w1, w2;
+  int4 cc;
+  if (TOP > 0 && __all(cur < TOP)) {  // wave-uniform: no lane waits on both paths
+    const float4* w = top + 4 * cur;
+    w0 = lload(w); w1 = lload(w + 1); w2 = lload(w + 2);
+    const float4 c4 = lload(w + 3);
+    cc = make_int4(__float_as_int(c4.x), __float_as_int(c4.y), __float_as_int(c4.z),
+                   __float_as_int(c4.w));
+  } else {
+    const float4* w = S.qwide + 4 * (size_t)cur;
+    w0 = gload(w); w1 = gload(w + 1); w2 = gload(w + 2);
+    cc = gload((const int4*)(w + 3));
+  }
   if (COUNT) cnt.nodes += 4;
   float Bx = w0.w * R.ix, By = w1.x * R.iy, Bz = w1.y * R.iz;
   float Ax = (w0.x - R.ox) * R.ix, Ay = (w0.y - R.oy) * R.iy, Az = (w0.z - R.oz) * R.iz;

@@ -499,15 +499,15 @@ k_shade(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__
 
 // One interior step of the ordered walks: the quantized 4-wide node (FP32
 // planes with the binary64 ray's slack) or the binary node (binary64 slabs).
-template <bool WIDE, bool PUSH_ENTRY, int SD, bool COUNT, typename EntryT>
+template <bool WIDE, bool PUSH_ENTRY, int SD, bool COUNT, int TOP, typename EntryT>
 __device__ __forceinline__ bool walk_step(const DevScene& S, const qw::QRay& Q, double ix,
                                           double iy, double iz, double oix, double oiy, double oiz,
                                           double tMax, int& cur, int& sp, int* __restrict__ sNode,
                                           EntryT* __restrict__ sEntry, int lanes,
-                                          TraversalCounts& cnt) {
+                                          TraversalCounts& cnt, const float4* __restrict__ top) {
   if constexpr (WIDE)
-    return qw::qwide_step<PUSH_ENTRY, SD, COUNT>(S, Q, qw::cull_bound(tMax), cur, sp, sNode,
-                                                 sEntry, lanes, cnt);
+    return qw::qwide_step<PUSH_ENTRY, SD, COUNT, TOP>(S, Q, qw::cull_bound(tMax), cur, sp, sNode,
+                                                      sEntry, lanes, cnt, top);
   else
     return interior_step<PUSH_ENTRY, SD, COUNT>(S, ix, iy, iz, oix, oiy, oiz, tMax, cur, sp,
                                                 sNode, sEntry, lanes, cnt);
@@ -549,6 +549,8 @@ __global__ void __launch_bounds__(BLOCK)
   constexpr bool WIDE = WALK == CT_WALK_WIDE;
   constexpr int SD = WALK == CT_WALK_STACKLESS ? 1 : STACK;
   using EntryT = typename std::conditional<WIDE, qw::Entry, uint16_t>::type;
+  // single-scene launches read the top levels of the 4-wide tree from LDS
+  constexpr int TOP = (WIDE && !MULTI) ? TOP_NODES : 0;
   __shared__ int sNodeAll[SD * BLOCK];
   __shared__ EntryT sEntryAll[SD * BLOCK];
   int* sNode = sNodeAll + threadIdx.x;
@@ -559,6 +561,12 @@ __global__ void __launch_bounds__(BLOCK)
   bool have = false, exhausted = false;
   uint32_t pid = 0;
   DevScene S = jobs[0].S;
+  float4* top = nullptr;
+  if constexpr (TOP > 0) {
+    __shared__ float4 sTop[4 * (TOP > 0 ? TOP : 1)];
+    top = sTop;
+    qw::stage_top<TOP>(top, S);
+  }
   double ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0;
   double oix = 0, oiy = 0, oiz = 0;
   qw::QRay Q = qw::qray(0, 0, 0, 1, 1, 1);
@@ -601,8 +609,9 @@ __global__ void __launch_bounds__(BLOCK)
             if (pk.n == PARK_LEAVES) break;
             park(pk, -2 - cur);
             cur = -1;
-          } else if (!walk_step<WIDE, true, SD, COUNT>(S, Q, ix, iy, iz, oix, oiy, oiz, h.t, cur,
-                                                       sp, sNode, sEntry, lanes, cnt)) {
+          } else if (!walk_step<WIDE, true, SD, COUNT, TOP>(S, Q, ix, iy, iz, oix, oiy, oiz, h.t,
+                                                            cur, sp, sNode, sEntry, lanes, cnt,
+                                                            top)) {
             overflow = true;
             cur = -1;
             break;
@@ -667,6 +676,7 @@ k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
   constexpr bool WIDE = WALK == CT_WALK_WIDE;
   constexpr int SD = WALK == CT_WALK_STACKLESS ? 1 : STACK;
   using EntryT = typename std::conditional<WIDE, qw::Entry, uint16_t>::type;
+  constexpr int TOP = (WIDE && !MULTI) ? TOP_NODES : 0;
   __shared__ int sNodeAll[SD * BLOCK];
   int* sNode = sNodeAll + threadIdx.x;
   const int lanes = BLOCK;
@@ -675,6 +685,12 @@ k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
   bool have = false, exhausted = false, blocked = false;
   uint32_t slotCur = 0, pid = 0;
   DevScene S = jobs[0].S;
+  float4* top = nullptr;
+  if constexpr (TOP > 0) {
+    __shared__ float4 sTop[4 * (TOP > 0 ? TOP : 1)];
+    top = sTop;
+    qw::stage_top<TOP>(top, S);
+  }
   double ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0, tMax = 0;
   double oix = 0, oiy = 0, oiz = 0;
   qw::QRay Q = qw::qray(0, 0, 0, 1, 1, 1);
@@ -717,9 +733,9 @@ k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
             if (pk.n == PARK_LEAVES) break;
             park(pk, -2 - cur);
             cur = -1;
-          } else if (!walk_step<WIDE, false, SD, COUNT>(S, Q, ix, iy, iz, oix, oiy, oiz, tMax,
-                                                        cur, sp, sNode, (EntryT*)nullptr, lanes,
-                                                        cnt)) {
+          } else if (!walk_step<WIDE, false, SD, COUNT, TOP>(S, Q, ix, iy, iz, oix, oiy, oiz,
+                                                             tMax, cur, sp, sNode,
+                                                             (EntryT*)nullptr, lanes, cnt, top)) {
             overflow = true;
             cur = -1;
             break;

@@ -817,6 +817,8 @@ template <int STACK, bool COUNT, bool MULTI, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
 k_extend32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__ q,
            const uint32_t* __restrict__ count, uint32_t* __restrict__ fetch) {
+  // single-scene launches read the top levels of the 4-wide tree from LDS
+  constexpr int TOP = MULTI ? 0 : TOP_NODES;
   __shared__ int sNodeAll[STACK * BLOCK];
   __shared__ Entry sEntryAll[STACK * BLOCK];
   int* sNode = sNodeAll + threadIdx.x;
@@ -827,6 +829,12 @@ k_extend32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
   bool have = false, exhausted = false;
   uint32_t pid = 0;
   DevScene S = jobs[0].S;
+  float4* top = nullptr;
+  if constexpr (TOP > 0) {
+    __shared__ float4 sTop[4 * (TOP > 0 ? TOP : 1)];
+    top = sTop;
+    qw::stage_top<TOP>(top, S);
+  }
   Ray R = make_ray(0, 0, 0, 1, 1, 1);
   Hit h;
   h.t = 0; h.u = 0; h.v = 0; h.prim = -1;
@@ -855,8 +863,8 @@ k_extend32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
           if (pk.n == PARK_LEAVES) break;
           park(pk, -2 - cur);
           cur = -1;
-        } else if (!qwide_step<true, STACK, COUNT>(S, R, h.t * CULL_SLACK, cur, sp, sNode, sEntry,
-                                                   lanes, cnt)) {
+        } else if (!qwide_step<true, STACK, COUNT, TOP>(S, R, h.t * CULL_SLACK, cur, sp, sNode,
+                                                        sEntry, lanes, cnt, top)) {
           overflow = true;
           cur = -1;
           break;
@@ -902,6 +910,7 @@ template <int STACK, bool COUNT, bool MULTI, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
 k_shadow32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__ count,
            uint32_t* __restrict__ fetch) {
+  constexpr int TOP = MULTI ? 0 : TOP_NODES;
   __shared__ int sNodeAll[STACK * BLOCK];
   int* sNode = sNodeAll + threadIdx.x;
   const int lanes = BLOCK;
@@ -910,6 +919,12 @@ k_shadow32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
   bool have = false, exhausted = false, blocked = false;
   uint32_t slotCur = 0, pid = 0;
   DevScene S = jobs[0].S;
+  float4* top = nullptr;
+  if constexpr (TOP > 0) {
+    __shared__ float4 sTop[4 * (TOP > 0 ? TOP : 1)];
+    top = sTop;
+    qw::stage_top<TOP>(top, S);
+  }
   Ray R = make_ray(0, 0, 0, 1, 1, 1);
   float tMax = 0, tCull = 0;
   int cur = -1, sp = 0;
@@ -940,8 +955,8 @@ k_shadow32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
           if (pk.n == PARK_LEAVES) break;
           park(pk, -2 - cur);
           cur = -1;
-        } else if (!qwide_step<false, STACK, COUNT>(S, R, tCull, cur, sp, sNode, nullptr, lanes,
-                                                    cnt)) {
+        } else if (!qwide_step<false, STACK, COUNT, TOP>(S, R, tCull, cur, sp, sNode, nullptr,
+                                                         lanes, cnt, top)) {
           overflow = true;
           cur = -1;
           break;

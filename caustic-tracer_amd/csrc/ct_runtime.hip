@@ -134,7 +134,9 @@ static bool leaf_words(const float* nodes, uint32_t nodeCount, const uint32_t* g
 // collapse that opens the boxes a ray is most likely to enter).  Boxes are the
 // reference nodes' float32 boxes, copied bit for bit; only the intermediate
 // boxes are skipped, which widens the set of subtrees a walk enters and never
-// narrows it.
+// narrows it.  Wide nodes are numbered breadth first until the first
+// TOP_NODES slots are taken (the block traversal launches copy into LDS), then
+// depth first; the numbering changes no walk's result.
 static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* words,
                        std::vector<float>& out) {
   out.clear();
@@ -173,10 +175,16 @@ static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* w
     return;
   }
   work.push_back({0, root});
+  size_t head = 0;  // work[head..] pending; FIFO while the front slot is below TOP_NODES
   std::vector<uint32_t> kids;
-  while (!work.empty()) {
-    Item it = work.back();
-    work.pop_back();
+  while (head < work.size()) {
+    Item it;
+    if (work[head].slot < (uint32_t)TOP_NODES) {
+      it = work[head++];
+    } else {
+      it = work.back();
+      work.pop_back();
+    }
     kids.clear();
     uint32_t L = it.n2 + 1;
     kids.push_back(L);
