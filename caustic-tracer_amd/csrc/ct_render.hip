@@ -733,7 +733,7 @@ k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
         bool overflow = false;
         while (cur != -1) {
           if (cur <= -2) {
-            if (pk.n == PARK_LEAVES) break;
+            if (pk.n == SHADOW_PARK_LEAVES) break;
             park(pk, -2 - cur);
             cur = -1;
           } else if (!walk_step<WIDE, false, SD, COUNT, TOP>(S, Q, ix, iy, iz, oix, oiy, oiz,
@@ -747,10 +747,10 @@ k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
             sp--;
             cur = sNode[sp * lanes];
           }
-          if (__all(pk.n == PARK_LEAVES || cur == -1)) break;
+          if (__all(pk.n == SHADOW_PARK_LEAVES || cur == -1)) break;
         }
 #pragma unroll
-        for (int k = 0; k < PARK_LEAVES; k++)
+        for (int k = 0; k < SHADOW_PARK_LEAVES; k++)
           if (k < pk.n && !blocked)
             blocked = leaf_blocks(S, pk.w[k], ox, oy, oz, dx, dy, dz, tMax, cnt, COUNT);
         pk.n = 0;

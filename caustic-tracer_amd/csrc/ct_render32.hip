@@ -952,7 +952,7 @@ k_shadow32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
       bool overflow = false;
       while (cur != -1) {
         if (cur <= -2) {
-          if (pk.n == PARK_LEAVES) break;
+          if (pk.n == SHADOW_PARK_LEAVES) break;
           park(pk, -2 - cur);
           cur = -1;
         } else if (!qwide_step<false, STACK, COUNT, TOP>(S, R, tCull, cur, sp, sNode, nullptr,
@@ -965,10 +965,10 @@ k_shadow32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
           sp--;
           cur = sNode[sp * lanes];
         }
-        if (__all(pk.n == PARK_LEAVES || cur == -1)) break;
+        if (__all(pk.n == SHADOW_PARK_LEAVES || cur == -1)) break;
       }
 #pragma unroll
-      for (int k = 0; k < PARK_LEAVES; k++)
+      for (int k = 0; k < SHADOW_PARK_LEAVES; k++)
         if (k < pk.n && !blocked) blocked = leaf_blocks<COUNT>(S, pk.w[k], R, T_MIN_F, tMax, cnt);
       pk.n = 0;
       if (overflow && !blocked) {

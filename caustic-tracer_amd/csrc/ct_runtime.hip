@@ -128,55 +128,129 @@ static bool leaf_words(const float* nodes, uint32_t nodeCount, const uint32_t* g
   return exact;
 }
 
-// 4-wide nodes from the binary tree: a wide node starts from the two children
-// of a binary node and, while it has fewer than four, replaces its interior
-// child of largest surface area by that child's two children (the greedy
-// collapse that opens the boxes a ray is most likely to enter).  Boxes are the
-// reference nodes' float32 boxes, copied bit for bit; only the intermediate
-// boxes are skipped, which widens the set of subtrees a walk enters and never
-// narrows it.  Wide nodes are numbered breadth first until the first
+// 4-wide nodes from the binary tree by the surface-area cost model, collapsed
+// optimally by dynamic programming over the binary tree (the collapse of
+// Ylitie, Karras and Laine, "Efficient incoherent ray traversal on GPUs through
+// compressed wide BVHs", HPG 2017, for four children):
+//   D(n, k)  cheapest cover of the subtree of n by at most k roots, a root
+//            being a reference leaf or a wide node
+//   wide(n)  = A(n) + min over k of D(left, k) + D(right, 4 - k)
+//   leaf(n)  = A(n) * C_PRIM * count(n)
+// with A the surface area of the reference node's float32 box, in units of one
+// 4-wide node step.  Every primitive stays in its reference leaf and every box
+// is a reference node's box, so the walks over these nodes visit a superset
+// of the boxes on the path to any primitive and return the reference's
+// closest hit.  On the 1M-triangle soup this takes 1.7% fewer node visits per
+// ray than opening the largest child first.  Leaves are not merged: merging
+// subtrees of up to 7 primitives (one run of the leaf order) at C_PRIM = 0.3
+// cost 4% on that soup.  Wide nodes are numbered breadth first until the first
 // TOP_NODES slots are taken (the block traversal launches copy into LDS), then
-// depth first; the numbering changes no walk's result.
+// depth first.
+constexpr double C_PRIM = 0.5;  // one binary64 primitive test, in node steps
+
 static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* words,
                        std::vector<float>& out) {
   out.clear();
+  const uint32_t N = nodeCount;
   auto isLeaf = [&](uint32_t i) { return words[i] != 0xffffffffu; };
-  auto code = [&](uint32_t i) { return -2 - (int32_t)words[i]; };
+  auto rightOf = [&](uint32_t i) { return (uint32_t)nodes[(size_t)(i + 1) * NODE_STRIDE + 3]; };
   auto area = [&](uint32_t i) {
     const float* b = nodes + (size_t)i * NODE_STRIDE;
     double x = (double)b[4] - b[0], y = (double)b[5] - b[1], z = (double)b[6] - b[2];
     return x * y + y * z + z * x;
   };
-  struct Item { uint32_t n2; uint32_t slot; };  // binary node to expand into wide node `slot`
-  std::vector<Item> work;
+  // Children have larger indices than their parent (skip links move forward),
+  // so one pass from the last node to the first sees children before parents.
+  std::vector<float> D((size_t)N * 4);    // D[4n + k-1] = D(n, k)
+  std::vector<uint8_t> how((size_t)N * 4);  // k = 1: 0 leaf, 1 wide; k > 1: roots given
+                                            // to the left child (0: as for k - 1)
+  std::vector<uint8_t> split(N);          // wide(n): slots given to the left child
+  const float INF = __builtin_inff();
+  for (int64_t ii = (int64_t)N - 1; ii >= 0; ii--) {
+    const uint32_t i = (uint32_t)ii;
+    float* d = &D[(size_t)i * 4];
+    uint8_t* h = &how[(size_t)i * 4];
+    const double a = area(i);
+    if (isLeaf(i)) {
+      d[0] = d[1] = d[2] = d[3] = (float)(a * C_PRIM * (words[i] & 7));
+      h[0] = h[1] = h[2] = h[3] = 0;
+      continue;
+    }
+    const uint32_t l = i + 1, r = rightOf(i);
+    const float* dl = &D[(size_t)l * 4];
+    const float* dr = &D[(size_t)r * 4];
+    float bestF = INF;
+    uint8_t bestK = 1;
+    for (int k = 1; k <= 3; k++) {
+      float c = dl[k - 1] + dr[4 - k - 1];
+      if (c < bestF) {
+        bestF = c;
+        bestK = (uint8_t)k;
+      }
+    }
+    split[i] = bestK;
+    d[0] = (float)a + bestF;  // n as a wide node
+    h[0] = 1;
+    for (int k = 2; k <= 4; k++) {
+      d[k - 1] = d[k - 2];
+      h[k - 1] = 0;
+      for (int kl = 1; kl < k; kl++) {
+        float c = dl[kl - 1] + dr[k - kl - 1];
+        if (c < d[k - 1]) {
+          d[k - 1] = c;
+          h[k - 1] = (uint8_t)kl;
+        }
+      }
+    }
+  }
+  // the roots covering subtree m with at most k of them
+  std::vector<uint32_t> roots;
+  std::vector<std::pair<uint32_t, int>> pend;
+  auto expand = [&](uint32_t m, int k) {
+    pend.clear();
+    pend.push_back({m, k});
+    while (!pend.empty()) {
+      auto [n, kk] = pend.back();
+      pend.pop_back();
+      while (kk > 1 && how[(size_t)n * 4 + kk - 1] == 0) kk--;
+      if (kk == 1) {
+        roots.push_back(n);
+      } else {
+        int kl = how[(size_t)n * 4 + kk - 1];
+        pend.push_back({rightOf(n), kk - kl});
+        pend.push_back({n + 1, kl});  // left first
+      }
+    }
+  };
   auto alloc = [&]() {
     uint32_t slot = (uint32_t)(out.size() / 32);
     out.resize(out.size() + 32);
-    float* r = &out[(size_t)slot * 32];
+    float* rr = &out[(size_t)slot * 32];
     for (int k = 0; k < 4; k++) {
       // an empty slot: a box at +inf on every axis, which no ray enters
-      r[k] = r[4 + k] = r[8 + k] = __builtin_inff();
-      r[12 + k] = r[16 + k] = r[20 + k] = __builtin_inff();
+      rr[k] = rr[4 + k] = rr[8 + k] = __builtin_inff();
+      rr[12 + k] = rr[16 + k] = rr[20 + k] = __builtin_inff();
       int32_t empty = -1;
-      memcpy(&r[24 + k], &empty, 4);
+      memcpy(&rr[24 + k], &empty, 4);
     }
     return slot;
   };
   auto put = [&](uint32_t slot, int k, uint32_t n2, int32_t c) {
-    float* r = &out[(size_t)slot * 32];
+    float* rr = &out[(size_t)slot * 32];
     const float* b = nodes + (size_t)n2 * NODE_STRIDE;
-    r[k] = b[0]; r[4 + k] = b[1]; r[8 + k] = b[2];
-    r[12 + k] = b[4]; r[16 + k] = b[5]; r[20 + k] = b[6];
-    memcpy(&r[24 + k], &c, 4);
+    rr[k] = b[0]; rr[4 + k] = b[1]; rr[8 + k] = b[2];
+    rr[12 + k] = b[4]; rr[16 + k] = b[5]; rr[20 + k] = b[6];
+    memcpy(&rr[24 + k], &c, 4);
   };
   uint32_t root = alloc();
   if (isLeaf(0)) {
-    put(root, 0, 0, code(0));
+    put(root, 0, 0, -2 - (int32_t)words[0]);
     return;
   }
+  struct Item { uint32_t n2; uint32_t slot; };  // binary node expanded into wide node `slot`
+  std::vector<Item> work;
   work.push_back({0, root});
   size_t head = 0;  // work[head..] pending; FIFO while the front slot is below TOP_NODES
-  std::vector<uint32_t> kids;
   while (head < work.size()) {
     Item it;
     if (work[head].slot < (uint32_t)TOP_NODES) {
@@ -185,30 +259,13 @@ static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* w
       it = work.back();
       work.pop_back();
     }
-    kids.clear();
-    uint32_t L = it.n2 + 1;
-    kids.push_back(L);
-    kids.push_back((uint32_t)nodes[(size_t)L * NODE_STRIDE + 3]);
-    while (kids.size() < 4) {
-      int pick = -1;
-      double best = -1.0;
-      for (size_t k = 0; k < kids.size(); k++) {
-        if (isLeaf(kids[k])) continue;
-        double a = area(kids[k]);
-        if (a > best) {
-          best = a;
-          pick = (int)k;
-        }
-      }
-      if (pick < 0) break;
-      uint32_t c = kids[pick];
-      kids[pick] = c + 1;
-      kids.push_back((uint32_t)nodes[(size_t)(c + 1) * NODE_STRIDE + 3]);
-    }
-    for (size_t k = 0; k < kids.size(); k++) {
-      uint32_t g = kids[k];
+    roots.clear();
+    expand(it.n2 + 1, split[it.n2]);
+    expand(rightOf(it.n2), 4 - split[it.n2]);
+    for (size_t k = 0; k < roots.size(); k++) {
+      uint32_t g = roots[k];
       if (isLeaf(g)) {
-        put(it.slot, (int)k, g, code(g));
+        put(it.slot, (int)k, g, -2 - (int32_t)words[g]);
       } else {
         uint32_t ws = alloc();
         put(it.slot, (int)k, g, (int32_t)ws);
@@ -216,7 +273,6 @@ static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* w
       }
     }
   }
-  (void)nodeCount;
 }
 
 // Exact difference of two floats in binary64, when it is representable.

@@ -142,6 +142,13 @@ __device__ __forceinline__ uint32_t refill(bool need, uint32_t n, uint32_t* fetc
 #define CT_PARK_LEAVES 3
 #endif
 constexpr int PARK_LEAVES = CT_PARK_LEAVES;
+// Leaves an any-hit (NEE) ray parks before the wave tests them; the first
+// blocking primitive ends the ray.
+#ifndef CT_SHADOW_PARK_LEAVES
+#define CT_SHADOW_PARK_LEAVES 3
+#endif
+constexpr int SHADOW_PARK_LEAVES = CT_SHADOW_PARK_LEAVES;
+static_assert(SHADOW_PARK_LEAVES >= 1 && SHADOW_PARK_LEAVES <= PARK_LEAVES, "parking slots");
 
 struct Parked {
   int w[PARK_LEAVES];  // leaf words
