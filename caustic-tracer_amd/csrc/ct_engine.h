@@ -53,4 +53,12 @@ int launch_tonemap(const float* d_accum, size_t npix, double samples, double exp
 // Path capacity of a tracer (paths in flight per round).
 size_t tracer_capacity(const ct_tracer* T);
 
+// Creates the tracer's side streams (second path group, NEE launches) now
+// rather than at its first round.  A process gets few hardware queues
+// (GPU_MAX_HW_QUEUES, 4 by default) and a stream created once they are all
+// taken shares one, serialising its launches with the other stream's: a
+// caller that creates its own round stream should call this right after, so
+// that the round's four streams hold four queues and later streams share.
+int tracer_streams(ct_tracer* T);
+
 }  // namespace ct

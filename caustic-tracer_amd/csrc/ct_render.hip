@@ -1057,6 +1057,13 @@ static int make_side(SideStream& ss) {
   return CT_OK;
 }
 
+int ct::tracer_streams(ct_tracer* T) {
+  int rc = make_side(T->groupB);
+  if (!rc) rc = make_side(T->side);
+  if (!rc) rc = make_side(T->sideB);
+  return rc;
+}
+
 // Smallest round traced as two path groups.
 constexpr size_t SPLIT_MIN_PATHS = size_t(1) << 20;
 
