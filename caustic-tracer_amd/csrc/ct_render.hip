@@ -538,8 +538,8 @@ __device__ __forceinline__ void walk_pop(int& cur, int& sp, const int* __restric
 // loop, exact mirror), CT_WALK_ORDERED (binary nodes, binary64 slab tests) or
 // CT_WALK_WIDE (the quantized 4-wide nodes walked in FP32 with the binary64
 // ray's rounding folded into the plane slack); the ordered forms park leaves
-// until every lane of the wave holds PARK_LEAVES of them and keep bf16 lower
-// bounds of deferred entries.  Primitive tests are binary64 in every walk, so
+// (up to PARK_SLOTS) until every lane of the wave holds PARK_LEAVES of them
+// and keep one-byte or bf16 lower bounds of deferred entries.  Primitive tests are binary64 in every walk, so
 // the closest hit is the reference's.
 // The 4-wide walk's LDS stack (4-byte code + 1-byte entry per slot, 30 KB per
 // 256-lane workgroup) leaves room for five waves per SIMD; the register cap
@@ -609,7 +609,7 @@ __global__ void __launch_bounds__(BLOCK)
       if (WALK != CT_WALK_STACKLESS) {
         while (cur != -1) {
           if (cur <= -2) {
-            if (pk.n == PARK_LEAVES) break;
+            if (pk.n == PARK_SLOTS) break;
             park(pk, -2 - cur);
             cur = -1;
           } else if (!walk_step<WIDE, true, SD, COUNT, TOP>(S, Q, ix, iy, iz, oix, oiy, oiz, h.t,
@@ -620,10 +620,10 @@ __global__ void __launch_bounds__(BLOCK)
             break;
           }
           if (cur == -1) walk_pop<WIDE>(cur, sp, sNode, sEntry, lanes, h.t);
-          if (__all(pk.n == PARK_LEAVES || cur == -1)) break;
+          if (__all(pk.n >= PARK_LEAVES || cur == -1)) break;
         }
 #pragma unroll
-        for (int k = 0; k < PARK_LEAVES; k++)  // constant slots: the words stay in registers
+        for (int k = 0; k < PARK_SLOTS; k++)  // constant slots: the words stay in registers
           if (k < pk.n) test_leaf<COUNT>(S, pk.w[k], ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
         pk.n = 0;
         if (overflow) {
@@ -733,7 +733,7 @@ k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
         bool overflow = false;
         while (cur != -1) {
           if (cur <= -2) {
-            if (pk.n == SHADOW_PARK_LEAVES) break;
+            if (pk.n == SHADOW_PARK_SLOTS) break;
             park(pk, -2 - cur);
             cur = -1;
           } else if (!walk_step<WIDE, false, SD, COUNT, TOP>(S, Q, ix, iy, iz, oix, oiy, oiz,
@@ -747,10 +747,10 @@ k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
             sp--;
             cur = sNode[sp * lanes];
           }
-          if (__all(pk.n == SHADOW_PARK_LEAVES || cur == -1)) break;
+          if (__all(pk.n >= SHADOW_PARK_LEAVES || cur == -1)) break;
         }
 #pragma unroll
-        for (int k = 0; k < SHADOW_PARK_LEAVES; k++)
+        for (int k = 0; k < SHADOW_PARK_SLOTS; k++)
           if (k < pk.n && !blocked)
             blocked = leaf_blocks(S, pk.w[k], ox, oy, oz, dx, dy, dz, tMax, cnt, COUNT);
         pk.n = 0;

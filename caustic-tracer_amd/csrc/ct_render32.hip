@@ -860,7 +860,7 @@ k_extend32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
       bool overflow = false;
       while (cur != -1) {
         if (cur <= -2) {
-          if (pk.n == PARK_LEAVES) break;
+          if (pk.n == PARK_SLOTS) break;
           park(pk, -2 - cur);
           cur = -1;
         } else if (!qwide_step<true, STACK, COUNT, TOP>(S, R, h.t * CULL_SLACK, cur, sp, sNode,
@@ -878,10 +878,10 @@ k_extend32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
             }
           }
         }
-        if (__all(pk.n == PARK_LEAVES || cur == -1)) break;
+        if (__all(pk.n >= PARK_LEAVES || cur == -1)) break;
       }
 #pragma unroll
-      for (int k = 0; k < PARK_LEAVES; k++)  // constant slots: the words stay in registers
+      for (int k = 0; k < PARK_SLOTS; k++)  // constant slots: the words stay in registers
         if (k < pk.n) test_leaf<COUNT>(S, pk.w[k], R, T_MIN_F, h, cnt);
       pk.n = 0;
       if (overflow) {
@@ -952,7 +952,7 @@ k_shadow32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
       bool overflow = false;
       while (cur != -1) {
         if (cur <= -2) {
-          if (pk.n == SHADOW_PARK_LEAVES) break;
+          if (pk.n == SHADOW_PARK_SLOTS32) break;
           park(pk, -2 - cur);
           cur = -1;
         } else if (!qwide_step<false, STACK, COUNT, TOP>(S, R, tCull, cur, sp, sNode, nullptr,
@@ -965,10 +965,10 @@ k_shadow32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
           sp--;
           cur = sNode[sp * lanes];
         }
-        if (__all(pk.n == SHADOW_PARK_LEAVES || cur == -1)) break;
+        if (__all(pk.n >= SHADOW_PARK_LEAVES || cur == -1)) break;
       }
 #pragma unroll
-      for (int k = 0; k < SHADOW_PARK_LEAVES; k++)
+      for (int k = 0; k < SHADOW_PARK_SLOTS32; k++)
         if (k < pk.n && !blocked) blocked = leaf_blocks<COUNT>(S, pk.w[k], R, T_MIN_F, tMax, cnt);
       pk.n = 0;
       if (overflow && !blocked) {

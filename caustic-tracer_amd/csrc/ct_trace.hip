@@ -80,8 +80,8 @@ constexpr int WIDE_STACK = 24;
 
 // Closest hit over the derived 4-wide nodes, organised like the path
 // tracer's extend kernel: each lane walks its own rays (a fixed sequence
-// i, i + stride, ...), parks leaves until the wave's lanes all hold
-// PARK_LEAVES of them or are done, then the parked leaves are tested together;
+// i, i + stride, ...), parks leaves (up to PARK_SLOTS) until the wave's lanes
+// all hold PARK_LEAVES of them or are done, then the parked leaves are tested together;
 // a lane whose ray finishes takes its next ray at the top of the loop.
 template <bool COUNT>
 __global__ void __launch_bounds__(QUERY_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
@@ -124,7 +124,7 @@ k_closest_wide(DevScene S, const double* __restrict__ rays, uint32_t n, double t
       bool overflow = false;
       while (cur != -1) {
         if (cur <= -2) {
-          if (pk.n == PARK_LEAVES) break;
+          if (pk.n == PARK_SLOTS) break;
           park(pk, -2 - cur);
           cur = -1;
         } else if (!qw::qwide_step<true, WIDE_STACK, COUNT>(S, Q, qw::cull_bound(h.t), cur, sp,
@@ -134,10 +134,10 @@ k_closest_wide(DevScene S, const double* __restrict__ rays, uint32_t n, double t
           break;
         }
         if (cur == -1) qw::qpop(cur, sp, sNode, sEntry, lanes, qw::cull_bound(h.t));
-        if (__all(pk.n == PARK_LEAVES || cur == -1)) break;
+        if (__all(pk.n >= PARK_LEAVES || cur == -1)) break;
       }
 #pragma unroll
-      for (int k = 0; k < PARK_LEAVES; k++)  // constant slots: the words stay in registers
+      for (int k = 0; k < PARK_SLOTS; k++)  // constant slots: the words stay in registers
         if (k < pk.n) test_leaf<COUNT>(S, pk.w[k], ox, oy, oz, dx, dy, dz, tMin, h, cnt);
       pk.n = 0;
       if (overflow) {
