@@ -132,40 +132,38 @@ __device__ __forceinline__ uint32_t refill(bool need, uint32_t n, uint32_t* fetc
   return slot < n ? slot : 0xffffffffu;
 }
 
-// Leaves a lane of the persistent walks parks before the wave tests them
-// together: a lane keeps descending after reaching a leaf until it holds
-// PARK_LEAVES of them (or has nothing left), so the wave's interior phase and
-// its leaf phase both run with most lanes busy.  Parking only delays when the
-// closest distance shrinks: the walk visits a superset of the unparked walk's
-// boxes and returns the same minimum.
+// Leaf parking of the persistent walks.  A lane that reaches a leaf parks it
+// and keeps walking; the wave leaves its interior phase once every lane holds
+// at least PARK_LEAVES parked leaves (or has nothing left), and lanes that
+// already hold them go on walking and parking, up to PARK_SLOTS, while the
+// wave waits for the others: steps that would idle find leaves ahead of time.
+// Then the wave tests all parked leaves together.  Parking only delays when
+// the closest distance shrinks: the walk visits a superset of the unparked
+// walk's boxes and returns the same minimum.  Measured on the 1080p soup frame
+// (binary64 / FP32 Mrays/s): exit 3 with 3 slots 1036 / 1115, exit 3 with 5
+// slots 1065 / 1160, exit 2 with 5 slots 1083 / 1214, exit 1 with 4 slots
+// 1131 / 1236 (3 slots: 1128 / 1230; 2 slots: 1051 / 1140).
 #ifndef CT_PARK_LEAVES
-#define CT_PARK_LEAVES 3
+#define CT_PARK_LEAVES 1
 #endif
 constexpr int PARK_LEAVES = CT_PARK_LEAVES;
-// Parking slots of a closest-hit ray: a lane that already holds PARK_LEAVES
-// leaves keeps walking, and parking, while the wave waits for its other lanes
-// (speculative leaves: on the 1080p soup frame 4.6% more node visits per ray,
-// +2.7% Mrays/s in binary64 and +4% in FP32 with five slots; four and six
-// slots measure the same within noise).
 #ifndef CT_PARK_SLOTS
-#define CT_PARK_SLOTS 5
+#define CT_PARK_SLOTS 4
 #endif
 constexpr int PARK_SLOTS = CT_PARK_SLOTS;
 static_assert(PARK_SLOTS >= PARK_LEAVES, "parking slots");
-// Leaves an any-hit (NEE) ray parks before the wave tests them; the first
-// blocking primitive ends the ray.
+// The same for any-hit (NEE) rays, whose first blocking primitive ends the
+// ray; their slots differ between the binary64 and the FP32 kernels (with
+// exit 1: 3 and 4 slots; more slots cost the binary64 kernel 3-4%).
 #ifndef CT_SHADOW_PARK_LEAVES
-#define CT_SHADOW_PARK_LEAVES 3
+#define CT_SHADOW_PARK_LEAVES 1
 #endif
 constexpr int SHADOW_PARK_LEAVES = CT_SHADOW_PARK_LEAVES;
-// Parking slots of an any-hit ray in the binary64 and in the FP32 kernels:
-// speculative leaves pay in the FP32 kernel (+2% with five slots on the 1080p
-// soup frame) and cost the binary64 one (-1% to -2% with four or five).
 #ifndef CT_SHADOW_PARK_SLOTS
 #define CT_SHADOW_PARK_SLOTS 3
 #endif
 #ifndef CT_SHADOW_PARK_SLOTS32
-#define CT_SHADOW_PARK_SLOTS32 5
+#define CT_SHADOW_PARK_SLOTS32 4
 #endif
 constexpr int SHADOW_PARK_SLOTS = CT_SHADOW_PARK_SLOTS;
 constexpr int SHADOW_PARK_SLOTS32 = CT_SHADOW_PARK_SLOTS32;
