@@ -112,8 +112,9 @@ def main():
     ap.add_argument("--walk", choices=["ordered", "stackless", "wide"], default="wide")
     ap.add_argument("--shadow-walk", choices=["ordered", "stackless", "wide"], default="wide")
     ap.add_argument("--stack", type=int, default=24, help="LDS stack entries of the ordered walks")
-    ap.add_argument("--persist-blocks", type=int, default=768,
-                    help="workgroups of the persistent traversal launches (0: one lane per ray)")
+    ap.add_argument("--persist-blocks", type=int, default=None,
+                    help="workgroups of the persistent traversal launches (0: one lane per ray; "
+                         "default: the tracer's, 640 binary64 / 768 FP32)")
     ap.add_argument("--rays", type=int, default=1 << 24,
                     help="plain: random closest-hit queries timed beside the render (0: none)")
     ap.add_argument("--workgroup", type=int, default=256, choices=[64, 128, 256],
@@ -234,7 +235,8 @@ def plain_form(ct, args):
     walk = walks[args.walk]
     tr = ct.Tracer(walk=walk)
     tr.set_traversal(walks[args.shadow_walk], args.stack)
-    tr.set_schedule(args.persist_blocks > 0, max(1, args.persist_blocks))
+    if args.persist_blocks is not None:
+        tr.set_schedule(args.persist_blocks > 0, max(1, args.persist_blocks))
     tr.set_precision(ct.PRECISION_F32 if args.precision == "f32" else ct.PRECISION_F64)
     tr.set_workgroup(args.workgroup)
     if args.path_groups:
@@ -304,7 +306,7 @@ def plain_form(ct, args):
                                f"{spp} spp, maxBounces {depth}, RR+NEE, walk {args.walk}, "
                                f"precision {args.precision}, "
                                f"shadow walk {args.shadow_walk}, stack {args.stack}, "
-                               f"persistent blocks {args.persist_blocks}, "
+                               f"persistent blocks {tr.get_config()['blocks']}, "
                                f"path groups {tr.get_config()['pathGroups']}",
                    "triangles": args.tris, "width": w, "height": h, "spp": spp,
                    "max_bounces": depth, "parallelism": "single GPU"},

@@ -839,8 +839,10 @@ struct ct_tracer {
   int stack = 24;                    // LDS stack entries of the ordered walks
   // workgroups of each traversal launch (0: one lane per ray); up to four run
   // at once (two path groups, each with its NEE launch beside its extension
-  // launch), so each takes a share of the chip: 768 x 256 lanes measured best
+  // launch), so each takes a share of the chip.  Until a caller sets it, the
+  // count follows the precision (traversal_blocks).
   unsigned persistBlocks = 768;
+  bool blocksAuto = true;
   int block = 256;                   // lanes per workgroup of the traversal launches
   size_t maxPaths = size_t(1) << 25; // paths in flight per round (~190 B of state each)
   bool graph = false;                // capture rounds into a hipGraph and replay them
@@ -861,6 +863,15 @@ struct ct_tracer {
   std::vector<int> eventKind;      // 0 extend, 1 shade, 2 shadow
   hipStream_t lastStream = nullptr;
 };
+
+// Workgroups of a traversal launch: the caller's setting, or by default 640
+// for the binary64 kernels and 768 for the FP32 ones (measured best on the
+// 1080p soup frame: binary64 1147 Mrays/s at 640 against 1130 at 768 and 1134
+// at 576; FP32 1237 at 768 against 1214 at 640).
+static unsigned traversal_blocks(const ct_tracer* T) {
+  if (!T->blocksAuto) return T->persistBlocks;
+  return T->precision == CT_PRECISION_F32 ? 768u : 640u;
+}
 
 static void drop_graphs(ct_tracer* T) {
   for (hipGraphExec_t& g : T->graphExec) {
@@ -1130,7 +1141,7 @@ static int launch_group_body(ct_tracer* T, const PathSoA& P, int njobs, size_t b
     uint32_t* fExt = P.ctr + 2 * MAX_BOUNCE_SLOTS + b;
     uint32_t* fSh = P.ctr + 3 * MAX_BOUNCE_SLOTS + b;
     mark(T, st, 0, true);
-    dispatch_trace<MULTI>(T->walk, T->stack, T->counting, T->block, T->persistBlocks, count, st,
+    dispatch_trace<MULTI>(T->walk, T->stack, T->counting, T->block, traversal_blocks(T), count, st,
                           P, jobs, qin, cin, fExt, scount, fSh, false);
     CT_LAUNCH_CHECK("k_extend");
     mark(T, st, 0, false);
@@ -1146,7 +1157,7 @@ static int launch_group_body(ct_tracer* T, const PathSoA& P, int njobs, size_t b
       sst = side.side;
     }
     mark(T, sst, 2, true);
-    dispatch_trace<MULTI>(T->shadowWalk, T->stack, T->counting, T->block, T->persistBlocks,
+    dispatch_trace<MULTI>(T->shadowWalk, T->stack, T->counting, T->block, traversal_blocks(T),
                           count, sst, P, jobs, qin, cin, fExt, scount, fSh, true);
     CT_LAUNCH_CHECK("k_shadow");
     mark(T, sst, 2, false);
@@ -1160,7 +1171,7 @@ static int launch_part(ct_tracer* T, const RoundStreams& rs, int njobs, size_t p
                        int nb, int part, hipStream_t st) {
   if (T->precision == CT_PRECISION_F32) {
     RoundLaunch R{T->P32, T->jobsDev, njobs, paths, pix, nb, T->stack, T->block,
-                  T->persistBlocks, T->counting, mark_cb, T, rs};
+                  traversal_blocks(T), T->counting, mark_cb, T, rs};
     return launch_part32(R, MULTI, part, st);
   }
   const PathSoA& P = T->P;
@@ -1267,7 +1278,7 @@ static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t s
     // settings, and read the job table from device memory at replay
     std::vector<char> key(sizeof(size_t) * 2 + sizeof(int) * 10);
     size_t kk[2] = {paths, pix};
-    int ki[10] = {nb, (int)R.size(), T->walk, T->shadowWalk, T->stack, (int)T->persistBlocks,
+    int ki[10] = {nb, (int)R.size(), T->walk, T->shadowWalk, T->stack, (int)traversal_blocks(T),
                   T->counting, multi, T->block,
                   T->precision * 64 + (T->overlap ? 32 : 0) + rs.groups};
     memcpy(key.data(), kk, sizeof kk);
@@ -1493,6 +1504,7 @@ int ct_tracer_set_schedule(ct_tracer* T, int32_t persistent, int32_t blocks) {
   if (persistent && (blocks < 1 || blocks > 65535))
     return set_error(CT_EINVAL, "ct_tracer_set_schedule: blocks must be in [1, 65535]");
   T->persistBlocks = persistent ? (unsigned)blocks : 0u;
+  T->blocksAuto = false;
   return CT_OK;
 }
 
@@ -1518,8 +1530,9 @@ int ct_tracer_get_config(const ct_tracer* T, ct_tracer_config* out) {
   out->walk = T->walk;
   out->shadowWalk = T->shadowWalk;
   out->stackDepth = T->stack;
-  out->persistent = T->persistBlocks ? 1 : 0;
-  out->blocks = T->persistBlocks ? (int32_t)T->persistBlocks : 768;
+  const unsigned blocks = traversal_blocks(T);
+  out->persistent = blocks ? 1 : 0;
+  out->blocks = blocks ? (int32_t)blocks : (T->precision == CT_PRECISION_F32 ? 768 : 640);
   out->workgroup = T->block;
   out->pathGroups = T->groups;
   return CT_OK;
@@ -1529,6 +1542,7 @@ int ct_tracer_set_config(ct_tracer* T, const ct_tracer_config* c) {
   if (!T || !c) return set_error(CT_EINVAL, "ct_tracer_set_config: null argument");
   ct_tracer_config old;
   ct_tracer_get_config(T, &old);
+  const bool oldAuto = T->blocksAuto;
   int rc = ct_tracer_set_walk(T, c->walk);
   if (!rc) rc = ct_tracer_set_traversal(T, c->shadowWalk, c->stackDepth);
   if (!rc) rc = ct_tracer_set_schedule(T, c->persistent, c->blocks);
@@ -1543,6 +1557,7 @@ int ct_tracer_set_config(ct_tracer* T, const ct_tracer_config* c) {
     T->shadowWalk = old.shadowWalk;
     T->stack = old.stackDepth;
     T->persistBlocks = old.persistent ? (unsigned)old.blocks : 0u;
+    T->blocksAuto = oldAuto;
     T->block = old.workgroup;
     T->maxPaths = old.maxPaths;
     T->groups = old.pathGroups;

@@ -216,7 +216,8 @@ int ct_tracer_set_walk(ct_tracer *tracer, int32_t walk);
 int ct_tracer_set_traversal(ct_tracer *tracer, int32_t shadowWalk, int32_t stackDepth);
 /* Launch schedule of the traversal kernels: persistent = 1 runs `blocks`
  * workgroups that pull rays from the queue as lanes free up;
- * 0 launches one lane per path.  Results do not depend on it. */
+ * 0 launches one lane per path.  Results do not depend on it.  Until it is
+ * set, persistent launches of 640 workgroups (binary64) or 768 (FP32). */
 int ct_tracer_set_schedule(ct_tracer *tracer, int32_t persistent, int32_t blocks);
 /* accumulatePass (tracer.ts:361-395): adds one sample per pixel into the
  * float32 sum d_accum (width*height*3) with frameIndex as the RNG frame. */
