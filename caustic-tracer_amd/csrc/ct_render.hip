@@ -257,12 +257,11 @@ __device__ __forceinline__ double dielectric_reflectance(double cosI, double eta
   return schlick(cosI, etaI, etaT);
 }
 
-// One bounce of radianceInto for every live path.  Four waves per SIMD (a
-// 128-VGPR cap; the uncapped kernel takes 151 VGPRs and three waves) let the
-// shading launches share CUs with the traversal launches running beside them:
-// +2% on the 1080p soup frame with two path groups, despite a few spills.
+// One bounce of radianceInto for every live path.  (151 VGPRs, three waves
+// per SIMD; a 128-VGPR cap for four waves spills and measures the same on the
+// 1080p soup frame.)
 template <bool MULTI>
-__global__ void __launch_bounds__(RBLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(RBLOCK)
 k_shade(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__ qin,
         const uint32_t* __restrict__ countIn, uint32_t* __restrict__ qout,
         uint32_t* __restrict__ countOut, uint32_t* __restrict__ shadowCount, int bounce) {
