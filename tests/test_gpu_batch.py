@@ -166,6 +166,23 @@ def test_set_config_is_all_or_nothing(ct):
     assert tr.get_config() == before
 
 
+def test_traversal_grid_follows_precision_until_set(ct):
+    """The persistent traversal launches default to 640 workgroups in binary64
+    and 768 in FP32; an explicit schedule sticks across precision changes."""
+    tr = ct.Tracer()
+    cfg = tr.get_config()
+    assert cfg["persistent"] == 1 and cfg["blocks"] == 640
+    tr.set_precision(ct.PRECISION_F32)
+    assert tr.get_config()["blocks"] == 768
+    tr.set_schedule(True, 512)
+    tr.set_precision(ct.PRECISION_F64)
+    assert tr.get_config()["blocks"] == 512
+    tr.set_schedule(False, 1)
+    assert tr.get_config()["persistent"] == 0
+    tr.set_config(dict(tr.get_config(), persistent=1, blocks=700))
+    assert tr.get_config()["blocks"] == 700
+
+
 def test_autotune_picks_a_setting_and_caches_it(ct, tmp_path):
     jobs = _family_jobs(ct)
     ct.Tracer().render_batch(jobs)
