@@ -254,9 +254,12 @@ def plain_form(ct, args):
     # HIP events bracket every launch on its own stream during the timed steps;
     # the last step's events give the per-launch device time of each kernel
     tr.set_profiling(True)
+    lat = []  # render_image returns once its round is done: per-call wall = frame latency
     t = time.perf_counter()
     for i in range(args.steps):
+        t1 = time.perf_counter()
         tr.render_image(ds, frame, opts, out)
+        lat.append(time.perf_counter() - t1)
     ct.sync()
     total = time.perf_counter() - t
     st = tr.stats()
@@ -311,6 +314,9 @@ def plain_form(ct, args):
                    "triangles": args.tris, "width": w, "height": h, "spp": spp,
                    "max_bounces": depth, "parallelism": "single GPU"},
         "spp_mpix_per_s": round(sppmpix, 3),
+        "frame_latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                             "p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                             "frames": len(lat)},
         "rays_per_step": rays, "bvh_build_ms": round(build_ms, 2),
         "closest_hit_queries": chq,
         "roofline": roof, "cpu_baseline": cpu,
