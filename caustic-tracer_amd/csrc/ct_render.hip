@@ -860,6 +860,7 @@ struct ct_tracer {
   uint32_t lastExtendLaunches = 0;
   std::vector<hipEvent_t> events;  // (begin, end) pairs per launch when profiling
   std::vector<int> eventKind;      // 0 extend, 1 shade, 2 shadow
+  std::vector<hipEvent_t> eventPool;  // recorded before, reused by later calls
   hipStream_t lastStream = nullptr;
 };
 
@@ -970,8 +971,12 @@ Frame ct::make_frame(const double* f) {
   return F;
 }
 
+// Returns the last call's events to the pool: a profiled launch then costs
+// one hipEventRecord on each side, not an event creation (the timed steps of a
+// benchmark run profiled).  A call's rounds end with a stream synchronisation,
+// so its events are complete before the next call records them again.
 static void clear_events(ct_tracer* T) {
-  for (hipEvent_t e : T->events) (void)hipEventDestroy(e);
+  T->eventPool.insert(T->eventPool.end(), T->events.begin(), T->events.end());
   T->events.clear();
   T->eventKind.clear();
 }
@@ -979,7 +984,12 @@ static void clear_events(ct_tracer* T) {
 static void mark(ct_tracer* T, hipStream_t st, int kind, bool begin) {
   if (!T->profiling || T->graph) return;
   hipEvent_t e;
-  if (hipEventCreate(&e) != hipSuccess) return;
+  if (!T->eventPool.empty()) {
+    e = T->eventPool.back();
+    T->eventPool.pop_back();
+  } else if (hipEventCreate(&e) != hipSuccess) {
+    return;
+  }
   (void)hipEventRecord(e, st);
   T->events.push_back(e);
   if (begin) T->eventKind.push_back(kind);
@@ -1458,6 +1468,8 @@ int ct_tracer_create(ct_tracer** out) {
 int ct_tracer_destroy(ct_tracer* T) {
   if (!T) return CT_OK;
   clear_events(T);
+  for (hipEvent_t e : T->eventPool) (void)hipEventDestroy(e);
+  T->eventPool.clear();
   hipError_t e = hipSuccess;
   drop_graphs(T);
   if (T->buf) e = hipFree(T->buf);
