@@ -843,7 +843,10 @@ struct ct_tracer {
   unsigned persistBlocks = 768;
   bool blocksAuto = true;
   int block = 256;                   // lanes per workgroup of the traversal launches
-  size_t maxPaths = size_t(1) << 25; // paths in flight per round (~190 B of state each)
+  // paths in flight per round (~190 B of state each, allocated as rounds need
+  // them): one round of a 1024^2 x 64 spp image is 2.4% (binary64) to 5.7%
+  // (FP32) faster than two
+  size_t maxPaths = size_t(1) << 26;
   bool graph = false;                // capture rounds into a hipGraph and replay them
   hipStream_t ownStream = nullptr;   // capture stream
   bool overlap = true;               // NEE launches on side streams (SideStream)

@@ -235,7 +235,8 @@ int ct_render_image(ct_tracer *tracer, const ct_scene *scene, const double *h_fr
  * the reference API: it serves the batching and serving forms. */
 int ct_render_batch(ct_tracer *tracer, const ct_job *jobs, uint32_t n, void *stream);
 int ct_accumulate_batch(ct_tracer *tracer, const ct_pass_job *jobs, uint32_t n, void *stream);
-/* Paths in flight per round (path state is about 190 B per path; default 2^25). */
+/* Paths in flight per round (path state is about 190 B per path, allocated as
+ * rounds need it; default 2^26). */
 int ct_tracer_set_capacity(ct_tracer *tracer, uint64_t maxPaths);
 /* Capture each round's launches into a hipGraph and replay it while the
  * round's shape (paths, pixels, jobs, bounces, settings) repeats. */
@@ -310,7 +311,7 @@ int ct_tracer_autotune(ct_tracer *tracer, const ct_job *jobs, uint32_t n, uint32
 
 typedef struct ct_server_opts {
   uint64_t hbmBudget;   /* bytes of progressive buffers resident at once (0: 4 GiB) */
-  uint64_t maxPaths;    /* paths per launch round (0: the tracer default, 2^25) */
+  uint64_t maxPaths;    /* paths per launch round (0: the tracer default, 2^26) */
   uint32_t maxActive;   /* admitted jobs at most (0: 1024) */
   uint32_t slicePasses; /* passes one job adds per round at most (0: as many as fit) */
   int32_t policy;       /* CT_SERVE_FIFO or CT_SERVE_SRPT */
