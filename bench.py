@@ -128,7 +128,7 @@ def main():
     ap.add_argument("--tune-trials", type=int, default=16, help="batch: autotuner trials")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: requests per second")
     ap.add_argument("--duration", type=float, default=2.0, help="serve: seconds of arrivals")
-    ap.add_argument("--max-paths", type=int, default=1 << 25, help="serve: paths per round")
+    ap.add_argument("--max-paths", type=int, default=1 << 26, help="serve: paths per round")
     ap.add_argument("--policy", choices=["fifo", "srpt"], default="fifo")
     ap.add_argument("--hbm-budget-gib", type=float, default=8.0)
     args = ap.parse_args()
