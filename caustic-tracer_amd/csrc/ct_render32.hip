@@ -810,9 +810,9 @@ k_shade32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restri
 }
 
 // Persistent closest hit over the quantized nodes, organised like the
-// binary64 wave walk: lanes park leaves until every lane of the wave holds
-// one, then the parked leaves are tested together; lanes refill from the
-// queue as their rays finish.
+// binary64 wave walk: lanes park leaves (up to PARK_SLOTS) until every lane of
+// the wave holds PARK_LEAVES of them, then the parked leaves are tested
+// together; lanes refill from the queue as their rays finish.
 template <int STACK, bool COUNT, bool MULTI, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
 k_extend32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__ q,
