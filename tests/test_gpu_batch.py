@@ -204,3 +204,36 @@ def test_autotune_picks_a_setting_and_caches_it(ct, tmp_path):
     assert tr2.get_config() == res["config"]
     for r, g in zip(ref, _images(jobs)):
         assert np.array_equal(r, g)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_presets_in_one_batch(ct, O, precision):
+    """All five presets of scenes.ts in one render_batch (spheres, the mesh,
+    lens cameras and scenes without a sky share the multi-scene launches):
+    each image equals the same job rendered alone, and in binary64 the
+    oracle's within 1e-4."""
+    sizes = [(48, 32, 3, 5, 8), (40, 40, 2, 6, 6), (32, 24, 4, 7, 8), (56, 32, 2, 8, 4),
+             (40, 30, 3, 9, 12)]
+    jobs = []
+    for p, (w, h, spp, seed, depth) in zip(ct.presets.SCENE_PRESETS, sizes):
+        sc = p["scene"]
+        jobs.append({"scene": O.pack_scene(sc), "frame": O.camera_frame(sc["camera"], w / h),
+                     "width": w, "height": h, "spp": spp, "seed": seed, "maxBounces": depth})
+    tr = ct.Tracer()
+    tr.set_precision(ct.PRECISION_F32 if precision == "f32" else ct.PRECISION_F64)
+    for j in jobs:
+        j["out"] = _alloc(ct, j)
+    tr.render_batch(jobs)
+    assert tr.stats()["rounds"] == 1
+    alone_tr = ct.Tracer()
+    alone_tr.set_precision(ct.PRECISION_F32 if precision == "f32" else ct.PRECISION_F64)
+    for j in jobs:
+        m = j["out"].numpy()
+        d = _alloc(ct, j)
+        alone_tr.render_image(j["scene"], j["frame"], j, d)
+        assert np.array_equal(d.numpy().view(np.uint32), m.view(np.uint32))
+        assert np.isfinite(m).all() and m.max() > 0
+        if precision == "f64":
+            o = O.render_image(j["scene"], j["frame"], j["width"], j["height"], j["spp"],
+                               seed=j["seed"], max_bounces=j["maxBounces"])
+            assert rel_err(m, o).max() <= 1e-4
