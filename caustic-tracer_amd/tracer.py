@@ -277,8 +277,13 @@ class Tracer:
     def set_traversal(self, shadow_walk=WALK_WIDE, stack=24):
         check(lib().ct_tracer_set_traversal(self.handle, shadow_walk, stack), "set_traversal")
 
-    def set_schedule(self, persistent=True, blocks=768):
-        check(lib().ct_tracer_set_schedule(self.handle, int(bool(persistent)), blocks), "schedule")
+    def set_schedule(self, persistent=True, blocks=None):
+        """Persistent traversal launches of `blocks` workgroups (default: the
+        tracer's current count), or one lane per path when not persistent."""
+        if blocks is None:
+            blocks = self.get_config()["blocks"]
+        check(lib().ct_tracer_set_schedule(self.handle, int(bool(persistent)), int(blocks)),
+              "schedule")
 
     def set_profiling(self, on):
         check(lib().ct_tracer_set_profiling(self.handle, int(bool(on))), "profiling")
