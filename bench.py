@@ -418,10 +418,10 @@ def batch_form(ct, args):
     log(f"plain sequential: {plain_s * 1e3:.1f} ms, {samples / plain_s / 1e6:.1f} spp*Mpix/s")
 
     tr = ct.Tracer()
+    tr.set_graph(True)  # tuned as it runs: replayed graphs
     tune = tr.autotune(jobs, max_trials=args.tune_trials)
     log(f"autotune: {tune['trials']} trials, {tune['initialMs']:.1f} -> {tune['ms']:.1f} ms, "
         f"config {tune['config']}")
-    tr.set_graph(True)
     for _ in range(args.warmup):
         tr.render_batch(jobs)
     ct.sync()

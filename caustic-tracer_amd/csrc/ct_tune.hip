@@ -184,7 +184,11 @@ extern "C" int ct_tracer_autotune(ct_tracer* T, const ct_job* jobs, uint32_t n, 
     if (same(cand, cur) || trials >= maxTrials) return CT_OK;
     if (ct_tracer_set_config(T, &cand) != CT_OK) return CT_OK;  // not a valid setting: skip
     double m = 0;
+    // the first call under a setting may grow the path state or capture the
+    // round's graphs (in graph mode): it is run untimed
     int r = time_batch(T, jobs, n, st, m);
+    if (r) return r;
+    r = time_batch(T, jobs, n, st, m);
     if (r) return r;
     trials++;
     if (m < best * 0.99) {  // 1% margin against timing noise
