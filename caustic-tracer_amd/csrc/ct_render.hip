@@ -22,6 +22,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <limits>
 #include <type_traits>
 #include <vector>
 
@@ -811,9 +812,9 @@ k_finalize(PathSoA P, const JobDev* __restrict__ jobs, int njobs, size_t totalPi
   }
 }
 
-__global__ void k_zero(float* p, size_t n) {
+__global__ void k_fill(float* p, size_t n, float v) {
   size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.0f;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
 }  // namespace ct
@@ -1444,12 +1445,18 @@ int ct::run_plan(ct_tracer* T, std::vector<PlanJob> jobs, hipStream_t st) {
 
 int ct::zero_floats(float* p, size_t n, hipStream_t st) {
   if (n == 0) return CT_OK;
-  hipLaunchKernelGGL(k_zero, dim3(grid_for(n, RBLOCK, 256 * 16)), dim3(RBLOCK), 0, st, p, n);
-  CT_LAUNCH_CHECK("k_zero");
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n, RBLOCK, 256 * 16)), dim3(RBLOCK), 0, st, p, n, 0.0f);
+  CT_LAUNCH_CHECK("k_fill");
   return CT_OK;
 }
 
 size_t ct::tracer_capacity(const ct_tracer* T) { return T->maxPaths; }
+
+// Value a render's image starts from: zero, or with no samples NaN, which is
+// what renderImage leaves (tracer.ts:351-353: the zero sums times 1/0).
+static float image_start(uint32_t spp) {
+  return spp ? 0.0f : std::numeric_limits<float>::quiet_NaN();
+}
 
 static int check_job(const ct_scene* s, const double* frame, uint32_t w, uint32_t h,
                      int32_t maxBounces, const void* out) {
@@ -1622,8 +1629,9 @@ int ct_render_image(ct_tracer* T, const ct_scene* s, const double* frame16,
   hipStream_t st = (hipStream_t)stream;
   size_t n = (size_t)opts->width * opts->height * 3;
   if (n) {
-    hipLaunchKernelGGL(k_zero, dim3(grid_for(n, RBLOCK, 256 * 16)), dim3(RBLOCK), 0, st, out, n);
-    CT_LAUNCH_CHECK("k_zero");
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n, RBLOCK, 256 * 16)), dim3(RBLOCK), 0, st, out, n,
+                       image_start(opts->spp));
+    CT_LAUNCH_CHECK("k_fill");
   }
   PlanJob j{s, make_frame(frame16),
             {opts->maxBounces, opts->russianRoulette != 0, opts->nee != 0},
@@ -1642,8 +1650,9 @@ int ct_render_batch(ct_tracer* T, const ct_job* jobs, uint32_t n, void* stream) 
     if (rc) return rc;
     size_t m = (size_t)J.width * J.height * 3;
     if (m) {
-      hipLaunchKernelGGL(k_zero, dim3(grid_for(m, RBLOCK, 256 * 16)), dim3(RBLOCK), 0, st, J.d_out, m);
-      CT_LAUNCH_CHECK("k_zero");
+      hipLaunchKernelGGL(k_fill, dim3(grid_for(m, RBLOCK, 256 * 16)), dim3(RBLOCK), 0, st, J.d_out,
+                         m, image_start(J.spp));
+      CT_LAUNCH_CHECK("k_fill");
     }
     plan.push_back({J.scene, make_frame(J.frame),
                     {J.maxBounces, J.russianRoulette != 0, J.nee != 0}, J.width, J.height,

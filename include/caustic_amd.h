@@ -225,7 +225,8 @@ int ct_accumulate_pass(ct_tracer *tracer, const ct_scene *scene, const double *h
                        uint32_t width, uint32_t height, uint32_t frameIndex,
                        const ct_trace_opts *opts, float *d_accum, void *stream);
 /* renderImage (tracer.ts:348-355): spp passes with frameIndex = s + 7919*seed,
- * then d_out *= 1/spp.  d_out is width*height*3 float32 (overwritten). */
+ * then d_out *= 1/spp.  d_out is width*height*3 float32 (overwritten); with
+ * spp 0 every value is NaN, as the reference's zero sums times 1/0. */
 int ct_render_image(ct_tracer *tracer, const ct_scene *scene, const double *h_frame16,
                     const ct_render_opts *opts, float *d_out, void *stream);
 /* Several scenes / cameras / sizes / depths traced in shared launches: the

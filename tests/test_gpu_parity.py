@@ -363,3 +363,24 @@ def test_presets_gpu_bvh_and_render(ct, O, preset):
           f"exact pixels {(g == o).mean():.4f}")
     assert np.isfinite(g).all() and g.max() > 0
     assert err.max() <= REL_TOL
+
+
+@pytest.mark.parametrize("w,h,spp,depth", [(1, 1, 1, 0), (3, 1, 2, 1), (1, 5, 3, 0), (7, 3, 1, 2),
+                                           (2, 2, 0, 3)])
+def test_render_small_shapes_and_depths(ct, O, w, h, spp, depth):
+    """Single-pixel, single-row and single-column images, zero and one bounce,
+    and zero samples (the reference leaves NaN: zero sums times 1/0): the GPU
+    image equals the oracle's, NaN for NaN, within the per-pixel tolerance."""
+    sc = scenes.cornell_box()
+    ps = O.pack_scene(sc)
+    frame = O.camera_frame(sc["camera"], w / h)
+    g = ct.renderImage(ps, frame, {"width": w, "height": h, "spp": spp, "seed": 5,
+                                   "maxBounces": depth})
+    o = O.render_image(ps, frame, w, h, spp, seed=5, max_bounces=depth)
+    assert g.shape == o.shape == (w * h * 3,)
+    assert np.array_equal(np.isnan(g), np.isnan(o))
+    if spp == 0:
+        assert np.isnan(g).all()
+    else:
+        assert np.isfinite(g).all()
+        assert rel_err(g, o).max() <= REL_TOL
