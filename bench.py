@@ -38,6 +38,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = "Mrays/s & spp·Mpix/s at equal RMS error vs plain; p50/p99 frame latency"
 HBM_PEAK_GBS = 8000.0
+# VALU lane-op peak: 256 CUs x 64 lanes x 2.4 GHz, one non-packed lane-op per lane per cycle
+VALU_PEAK_TLOPS = 39.3
 
 
 def load_package():
@@ -187,10 +189,17 @@ def extend_roofline(cnt, prof, kernel, precision="f64", quantized=True):
                  ray_b * cnt["extensionRays"])
     ext_s = prof["extendMs"] / 1e3
     achieved = ext_bytes / ext_s / 1e9 if ext_s > 0 else 0.0
+    # VALU model of the same launches: 16 lane-ops per box test and 50 per
+    # primitive test (spheres counted as triangles) over the same time
+    lane_ops = 16 * cnt["nodeVisits"] + 50 * (cnt["triTests"] + cnt["sphereTests"])
+    valu = lane_ops / ext_s / 1e12 if ext_s > 0 else 0.0
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
             "bytes_per_launch": ext_bytes // launches,
-            "avg_launch_ms": round(prof["extendMs"] / launches, 4)}
+            "avg_launch_ms": round(prof["extendMs"] / launches, 4),
+            "valu_model": {"achieved": round(valu, 3), "peak": VALU_PEAK_TLOPS,
+                           "unit": "T lane-ops/s", "frac": round(valu / VALU_PEAK_TLOPS, 4),
+                           "lane_ops_per_launch": lane_ops // launches}}
 
 
 def traversal_bytes(cnt, precision, quantized):
