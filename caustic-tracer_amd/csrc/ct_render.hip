@@ -538,7 +538,7 @@ __device__ __forceinline__ void walk_pop(int& cur, int& sp, const int* __restric
 // loop, exact mirror), CT_WALK_ORDERED (binary nodes, binary64 slab tests) or
 // CT_WALK_WIDE (the quantized 4-wide nodes walked in FP32 with the binary64
 // ray's rounding folded into the plane slack); the ordered forms park leaves
-// (up to PARK_SLOTS) until every lane of the wave holds PARK_LEAVES of them
+// (up to PARK_SLOTS64) until every lane of the wave holds PARK_LEAVES of them
 // and keep one-byte or bf16 lower bounds of deferred entries.  Primitive tests are binary64 in every walk, so
 // the closest hit is the reference's.
 // The 4-wide walk's LDS stack (4-byte code + 1-byte entry per slot, 30 KB per
@@ -576,7 +576,7 @@ __global__ void __launch_bounds__(BLOCK)
   Hit h;
   h.t = 0; h.u = 0; h.v = 0; h.prim = -1;
   int cur = -1, sp = 0;
-  Parked pk;
+  Parked<PARK_SLOTS64> pk;
   pk.n = 0;
   for (;;) {
     if (!exhausted) {
@@ -609,7 +609,7 @@ __global__ void __launch_bounds__(BLOCK)
       if (WALK != CT_WALK_STACKLESS) {
         while (cur != -1) {
           if (cur <= -2) {
-            if (pk.n == PARK_SLOTS) break;
+            if (pk.n == PARK_SLOTS64) break;
             park(pk, -2 - cur);
             cur = -1;
           } else if (!walk_step<WIDE, true, SD, COUNT, TOP>(S, Q, ix, iy, iz, oix, oiy, oiz, h.t,
@@ -623,7 +623,7 @@ __global__ void __launch_bounds__(BLOCK)
           if (__all(pk.n >= PARK_LEAVES || cur == -1)) break;
         }
 #pragma unroll
-        for (int k = 0; k < PARK_SLOTS; k++)  // constant slots: the words stay in registers
+        for (int k = 0; k < PARK_SLOTS64; k++)  // constant slots: the words stay in registers
           if (k < pk.n) test_leaf<COUNT>(S, pk.w[k], ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
         pk.n = 0;
         if (overflow) {
@@ -698,7 +698,7 @@ k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
   double oix = 0, oiy = 0, oiz = 0;
   qw::QRay Q = qw::qray(0, 0, 0, 1, 1, 1);
   int cur = -1, sp = 0;
-  Parked pk;
+  Parked<SHADOW_PARK_SLOTS> pk;
   pk.n = 0;
   for (;;) {
     if (!exhausted) {

@@ -839,7 +839,7 @@ k_extend32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
   Hit h;
   h.t = 0; h.u = 0; h.v = 0; h.prim = -1;
   int cur = -1, sp = 0;
-  Parked pk;
+  Parked<PARK_SLOTS> pk;
   pk.n = 0;
   for (;;) {
     if (!exhausted) {
@@ -928,7 +928,7 @@ k_shadow32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
   Ray R = make_ray(0, 0, 0, 1, 1, 1);
   float tMax = 0, tCull = 0;
   int cur = -1, sp = 0;
-  Parked pk;
+  Parked<SHADOW_PARK_SLOTS32> pk;
   pk.n = 0;
   for (;;) {
     if (!exhausted) {

@@ -104,7 +104,7 @@ k_closest_wide(DevScene S, const double* __restrict__ rays, uint32_t n, double t
   Hit h;
   h.t = 0; h.u = 0; h.v = 0; h.prim = -1;
   int cur = -1, sp = 0;
-  Parked pk;
+  Parked<PARK_SLOTS> pk;
   pk.n = 0;
   for (;;) {
     if (!have && next < n) {

@@ -142,7 +142,11 @@ __device__ __forceinline__ uint32_t refill(bool need, uint32_t n, uint32_t* fetc
 // walk's boxes and returns the same minimum.  Measured on the 1080p soup frame
 // (binary64 / FP32 Mrays/s): exit 3 with 3 slots 1036 / 1115, exit 3 with 5
 // slots 1065 / 1160, exit 2 with 5 slots 1083 / 1214, exit 1 with 4 slots
-// 1131 / 1236 (3 slots: 1128 / 1230; 2 slots: 1051 / 1140).
+// 1131 / 1236 (3 slots: 1128 / 1230; 2 slots: 1051 / 1140).  Since the
+// traversal grids were retuned, the binary64 extension kernel does best with
+// 3 slots (1166 against 1152 with 4), the FP32 one and the closest-hit
+// queries with 4 (3 slots: 1230 against 1238, queries 872 against 885; 5
+// slots: queries 862; exit 2: queries 842).
 #ifndef CT_PARK_LEAVES
 #define CT_PARK_LEAVES 1
 #endif
@@ -150,8 +154,12 @@ constexpr int PARK_LEAVES = CT_PARK_LEAVES;
 #ifndef CT_PARK_SLOTS
 #define CT_PARK_SLOTS 4
 #endif
-constexpr int PARK_SLOTS = CT_PARK_SLOTS;
-static_assert(PARK_SLOTS >= PARK_LEAVES, "parking slots");
+constexpr int PARK_SLOTS = CT_PARK_SLOTS;  // FP32 extension rays, closest-hit queries
+#ifndef CT_PARK_SLOTS64
+#define CT_PARK_SLOTS64 3
+#endif
+constexpr int PARK_SLOTS64 = CT_PARK_SLOTS64;  // binary64 extension rays
+static_assert(PARK_SLOTS >= PARK_LEAVES && PARK_SLOTS64 >= PARK_LEAVES, "parking slots");
 // The same for any-hit (NEE) rays, whose first blocking primitive ends the
 // ray; their slots differ between the binary64 and the FP32 kernels (with
 // exit 1: 3 and 4 slots; more slots cost the binary64 kernel 3-4%).
@@ -168,18 +176,20 @@ constexpr int SHADOW_PARK_LEAVES = CT_SHADOW_PARK_LEAVES;
 constexpr int SHADOW_PARK_SLOTS = CT_SHADOW_PARK_SLOTS;
 constexpr int SHADOW_PARK_SLOTS32 = CT_SHADOW_PARK_SLOTS32;
 static_assert(SHADOW_PARK_LEAVES >= 1 && SHADOW_PARK_LEAVES <= SHADOW_PARK_SLOTS &&
-                  SHADOW_PARK_LEAVES <= SHADOW_PARK_SLOTS32 && SHADOW_PARK_SLOTS <= PARK_SLOTS &&
-                  SHADOW_PARK_SLOTS32 <= PARK_SLOTS,
+                  SHADOW_PARK_LEAVES <= SHADOW_PARK_SLOTS32,
               "parking slots");
 
+// A lane's parked leaves; N slots, held in registers.
+template <int N>
 struct Parked {
-  int w[PARK_SLOTS];  // leaf words
+  int w[N];  // leaf words
   int n;
 };
 
-__device__ __forceinline__ void park(Parked& p, int word) {
+template <int N>
+__device__ __forceinline__ void park(Parked<N>& p, int word) {
 #pragma unroll
-  for (int k = 0; k < PARK_SLOTS; k++)
+  for (int k = 0; k < N; k++)
     if (p.n == k) p.w[k] = word;
   p.n++;
 }
