@@ -635,6 +635,8 @@ def precision_form(ct, args):
     for prec in ("f64", "f32"):
         tr = ct.Tracer()
         tr.set_precision(ct.PRECISION_F32 if prec == "f32" else ct.PRECISION_F64)
+        if args.persist_blocks is not None:
+            tr.set_schedule(args.persist_blocks > 0, max(1, args.persist_blocks))
         for _ in range(args.warmup):
             tr.render_image(ds, frame, opts, out)
         ct.sync()
