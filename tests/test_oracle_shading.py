@@ -166,3 +166,19 @@ def test_rng_known_answers(O):
     b = O.pcg_hash(9 ^ 0x9E3779B9)
     sd = O.pcg_hash((a ^ ((b * 0x85EBCA6B) & 0xFFFFFFFF)) & 0xFFFFFFFF)
     assert O.seed_for(3, 9) == (sd if sd else 0x1234567)
+
+
+def test_render_image_sample_count_edges(O):
+    """renderImage (tracer.ts:348-355) divides the sums by spp: one sample
+    leaves the accumulatePass sum unchanged, and zero samples leave NaN
+    everywhere (zero sums times 1/0), which the GPU path reproduces."""
+    sc = scenes.cornell_box()
+    ps = O.pack_scene(sc)
+    w, h = 6, 4
+    frame = O.camera_frame(sc["camera"], w / h)
+    one = O.render_image(ps, frame, w, h, 1, seed=3, max_bounces=3)
+    acc = np.zeros(w * h * 3, np.float32)
+    O.accumulate_pass(ps, frame, w, h, 3 * 7919, acc, max_bounces=3)
+    assert np.array_equal(one.view(np.uint32), acc.view(np.uint32))
+    none = O.render_image(ps, frame, w, h, 0, seed=3, max_bounces=3)
+    assert none.shape == (w * h * 3,) and np.isnan(none).all()
