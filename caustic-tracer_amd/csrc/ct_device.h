@@ -150,6 +150,11 @@ __device__ __forceinline__ PrimGeom load_prim(const DevScene& S, int p) {
   g.a = gload(r);
   g.b = gload(r + 1);
   g.c = gload(r + 2);
+  // The third vertex is needed by triangles only, so the compiler would sink
+  // its load into the triangle branch, issued only after the type test has
+  // waited on the first two: a second memory round trip per triangle.  An
+  // empty statement that takes its registers keeps the three loads together.
+  __asm__ volatile("" ::"v"(g.c.x), "v"(g.c.y), "v"(g.c.z));
   return g;
 }
 
