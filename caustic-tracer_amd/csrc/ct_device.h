@@ -158,6 +158,43 @@ __device__ __forceinline__ PrimGeom load_prim(const DevScene& S, int p) {
   return g;
 }
 
+// What shading reads of the primitive a path hit: the record's first two
+// float4 (a sphere's centre and radius), its stored normal (triangles) and
+// its object index, all loads issued together (the normal would otherwise
+// wait behind the type test, and the object index behind the normal).
+struct HitGeom {
+  float4 a, b, n;
+  uint32_t obj;
+};
+
+// A sampled light's whole record (a, b, c: sphere or triangle; n: stored
+// normal) and its object index, all loads issued together.
+struct LightGeom {
+  float4 a, b, c, n;
+  uint32_t obj;
+};
+__device__ __forceinline__ LightGeom load_light(const DevScene& S, int prim) {
+  const float4* r = S.prims + (size_t)prim * 4;
+  LightGeom g;
+  g.a = gload(r);
+  g.b = gload(r + 1);
+  g.c = gload(r + 2);
+  g.n = gload(r + 3);
+  g.obj = gload(S.primObject + prim);
+  __asm__ volatile("" ::"v"(g.a.w), "v"(g.b.x), "v"(g.c.x), "v"(g.n.x), "v"(g.obj));
+  return g;
+}
+__device__ __forceinline__ HitGeom load_hit(const DevScene& S, int prim) {
+  const float4* r = S.prims + (size_t)prim * 4;
+  HitGeom g;
+  g.a = gload(r);
+  g.b = gload(r + 1);
+  g.n = gload(r + 3);
+  g.obj = gload(S.primObject + prim);
+  __asm__ volatile("" ::"v"(g.a.w), "v"(g.b.x), "v"(g.n.x), "v"(g.obj));
+  return g;
+}
+
 // Math.max(0, x): +0 for x <= 0 (including -0), NaN stays NaN.
 __device__ __forceinline__ double max0(double x) { return x > 0 ? x : (x != x ? x : 0.0); }
 

@@ -159,8 +159,8 @@ __device__ bool sample_light(const DevScene& S, double px, double py, double pz,
   int lightCount = S.lightCount;
   if (lightCount == 0) return false;
   int prim = gload(S.lights + rng.pick(lightCount));
-  PrimGeom g = load_prim(S, prim);
-  float4 em = gload(S.mats + (size_t)gload(S.primObject + prim) * 4 + 3);
+  const LightGeom g = load_light(S, prim);
+  float4 em = gload(S.mats + (size_t)g.obj * 4 + 3);
   L.e[0] = em.x; L.e[1] = em.y; L.e[2] = em.z;
   if (g.a.w == 0.0f) {
     double cx = (double)g.a.x - px, cy = (double)g.a.y - py, cz = (double)g.a.z - pz;
@@ -224,7 +224,7 @@ __device__ bool sample_light(const DevScene& S, double px, double py, double pz,
   if (dist2 <= 0) return false;
   double dist = sqrt(dist2);
   double wx = tx / dist, wy = ty / dist, wz = tz / dist;
-  float4 nrm = gload(S.prims + (size_t)prim * 4 + 3);
+  const float4 nrm = g.n;
   double cosL = fabs((double)nrm.x * wx + (double)nrm.y * wy + (double)nrm.z * wz);
   if (cosL < 1e-6) return false;
   L.w[0] = wx; L.w[1] = wy; L.w[2] = wz;
@@ -306,7 +306,7 @@ k_shade(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__
         px = ox + dx * t;
         py = oy + dy * t;
         pz = oz + dz * t;
-        PrimGeom g = load_prim(S, prim);
+        const HitGeom g = load_hit(S, prim);
         double nx, ny, nz;
         if (g.a.w == 0.0f) {
           double invR = 1 / (double)g.b.x;
@@ -314,16 +314,15 @@ k_shade(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__
           ny = (py - (double)g.a.y) * invR;
           nz = (pz - (double)g.a.z) * invR;
         } else {
-          float4 nrm = gload(S.prims + (size_t)prim * 4 + 3);
-          nx = nrm.x;
-          ny = nrm.y;
-          nz = nrm.z;
+          nx = g.n.x;
+          ny = g.n.y;
+          nz = g.n.z;
         }
         bool frontFace = dx * nx + dy * ny + dz * nz < 0;
         double sx = frontFace ? nx : -nx;
         double sy = frontFace ? ny : -ny;
         double sz = frontFace ? nz : -nz;
-        size_t m = (size_t)gload(S.primObject + prim) * 4;
+        size_t m = (size_t)g.obj * 4;
         float4 m0 = gload(S.mats + m), m1 = gload(S.mats + m + 1), m3 = gload(S.mats + m + 3);
         double er = m3.x, eg = m3.y, eb = m3.z;
         if (er > 0 || eg > 0 || eb > 0) {

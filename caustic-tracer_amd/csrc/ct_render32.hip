@@ -471,8 +471,8 @@ __device__ bool sample_light(const DevScene& S, float px, float py, float pz, Rn
   int lightCount = S.lightCount;
   if (lightCount == 0) return false;
   int prim = gload(S.lights + pick(rng, lightCount));
-  PrimGeom g = load_prim(S, prim);
-  float4 em = gload(S.mats + (size_t)gload(S.primObject + prim) * 4 + 3);
+  const LightGeom g = load_light(S, prim);
+  float4 em = gload(S.mats + (size_t)g.obj * 4 + 3);
   L.e[0] = em.x; L.e[1] = em.y; L.e[2] = em.z;
   if (g.a.w == 0.0f) {
     float cx = g.a.x - px, cy = g.a.y - py, cz = g.a.z - pz;
@@ -538,7 +538,7 @@ __device__ bool sample_light(const DevScene& S, float px, float py, float pz, Rn
   if (dist2 <= 0) return false;
   float dist = sqrtf(dist2);
   float wx = tx / dist, wy = ty / dist, wz = tz / dist;
-  float4 nrm = gload(S.prims + (size_t)prim * 4 + 3);
+  const float4 nrm = g.n;
   float cosL = fabsf(nrm.x * wx + nrm.y * wy + nrm.z * wz);
   if (cosL < 1e-6f) return false;
   L.w[0] = wx; L.w[1] = wy; L.w[2] = wz;
@@ -617,7 +617,7 @@ k_shade32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restri
         px = ox + dx * t;
         py = oy + dy * t;
         pz = oz + dz * t;
-        PrimGeom g = load_prim(S, prim);
+        const HitGeom g = load_hit(S, prim);
         float nx, ny, nz;
         if (g.a.w == 0.0f) {
           float invR = 1.0f / g.b.x;
@@ -625,16 +625,15 @@ k_shade32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restri
           ny = (py - g.a.y) * invR;
           nz = (pz - g.a.z) * invR;
         } else {
-          float4 nrm = gload(S.prims + (size_t)prim * 4 + 3);
-          nx = nrm.x;
-          ny = nrm.y;
-          nz = nrm.z;
+          nx = g.n.x;
+          ny = g.n.y;
+          nz = g.n.z;
         }
         bool frontFace = dx * nx + dy * ny + dz * nz < 0;
         float sx = frontFace ? nx : -nx;
         float sy = frontFace ? ny : -ny;
         float sz = frontFace ? nz : -nz;
-        size_t m = (size_t)gload(S.primObject + prim) * 4;
+        size_t m = (size_t)g.obj * 4;
         float4 m0 = gload(S.mats + m), m1 = gload(S.mats + m + 1), m3 = gload(S.mats + m + 3);
         float er = m3.x, eg = m3.y, eb = m3.z;
         if (er > 0 || eg > 0 || eb > 0) {
