@@ -809,7 +809,7 @@ k_shade32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restri
 }
 
 // Persistent closest hit over the quantized nodes, organised like the
-// binary64 wave walk: lanes park leaves (up to PARK_SLOTS) until every lane of
+// binary64 wave walk: lanes park leaves (up to PARK_SLOTS32) until every lane of
 // the wave holds PARK_LEAVES of them, then the parked leaves are tested
 // together; lanes refill from the queue as their rays finish.
 template <int STACK, bool COUNT, bool MULTI, int BLOCK>
@@ -838,7 +838,7 @@ k_extend32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
   Hit h;
   h.t = 0; h.u = 0; h.v = 0; h.prim = -1;
   int cur = -1, sp = 0;
-  Parked<PARK_SLOTS> pk;
+  Parked<PARK_SLOTS32> pk;
   pk.n = 0;
   for (;;) {
     if (!exhausted) {
@@ -859,7 +859,7 @@ k_extend32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
       bool overflow = false;
       while (cur != -1) {
         if (cur <= -2) {
-          if (pk.n == PARK_SLOTS) break;
+          if (pk.n == PARK_SLOTS32) break;
           park(pk, -2 - cur);
           cur = -1;
         } else if (!qwide_step<true, STACK, COUNT, TOP>(S, R, h.t * CULL_SLACK, cur, sp, sNode,
@@ -880,7 +880,7 @@ k_extend32(PathSoA32 P, const JobDev* __restrict__ jobs, const uint32_t* __restr
         if (__all(pk.n >= PARK_LEAVES || cur == -1)) break;
       }
 #pragma unroll
-      for (int k = 0; k < PARK_SLOTS; k++)  // constant slots: the words stay in registers
+      for (int k = 0; k < PARK_SLOTS32; k++)  // constant slots: the words stay in registers
         if (k < pk.n) test_leaf<COUNT>(S, pk.w[k], R, T_MIN_F, h, cnt);
       pk.n = 0;
       if (overflow) {

@@ -144,9 +144,10 @@ __device__ __forceinline__ uint32_t refill(bool need, uint32_t n, uint32_t* fetc
 // slots 1065 / 1160, exit 2 with 5 slots 1083 / 1214, exit 1 with 4 slots
 // 1131 / 1236 (3 slots: 1128 / 1230; 2 slots: 1051 / 1140).  Since the
 // traversal grids were retuned, the binary64 extension kernel does best with
-// 3 slots (1166 against 1152 with 4), the FP32 one and the closest-hit
-// queries with 4 (3 slots: 1230 against 1238, queries 872 against 885; 5
-// slots: queries 862; exit 2: queries 842).
+// 3 slots (1166 against 1152 with 4), the closest-hit queries with 4 (3
+// slots: 872 against 885; 5 slots: 862; exit 2: 842).  Once a triangle's
+// loads were issued together the FP32 extension kernel moved to 5 slots
+// (1350 against 1339 with 4; the binary64 one stays at 3: 1240 with 4).
 #ifndef CT_PARK_LEAVES
 #define CT_PARK_LEAVES 1
 #endif
@@ -154,12 +155,18 @@ constexpr int PARK_LEAVES = CT_PARK_LEAVES;
 #ifndef CT_PARK_SLOTS
 #define CT_PARK_SLOTS 4
 #endif
-constexpr int PARK_SLOTS = CT_PARK_SLOTS;  // FP32 extension rays, closest-hit queries
+constexpr int PARK_SLOTS = CT_PARK_SLOTS;  // closest-hit queries
 #ifndef CT_PARK_SLOTS64
 #define CT_PARK_SLOTS64 3
 #endif
 constexpr int PARK_SLOTS64 = CT_PARK_SLOTS64;  // binary64 extension rays
-static_assert(PARK_SLOTS >= PARK_LEAVES && PARK_SLOTS64 >= PARK_LEAVES, "parking slots");
+#ifndef CT_PARK_SLOTS32
+#define CT_PARK_SLOTS32 5
+#endif
+constexpr int PARK_SLOTS32 = CT_PARK_SLOTS32;  // FP32 extension rays
+static_assert(PARK_SLOTS >= PARK_LEAVES && PARK_SLOTS64 >= PARK_LEAVES &&
+                  PARK_SLOTS32 >= PARK_LEAVES,
+              "parking slots");
 // The same for any-hit (NEE) rays, whose first blocking primitive ends the
 // ray; their slots differ between the binary64 and the FP32 kernels (with
 // exit 1, binary64 with 2 / 3 / 4 slots: 1158 / 1160 / 1144; FP32 with
