@@ -147,7 +147,8 @@ __device__ __forceinline__ uint32_t refill(bool need, uint32_t n, uint32_t* fetc
 // 3 slots (1166 against 1152 with 4), the closest-hit queries with 4 (3
 // slots: 872 against 885; 5 slots: 862; exit 2: 842).  Once a triangle's
 // loads were issued together the FP32 extension kernel moved to 5 slots
-// (1350 against 1339 with 4; the binary64 one stays at 3: 1240 with 4).
+// (1350 against 1339 with 4; the binary64 one stays at 3: 1240 with 4; the
+// queries stay at 4: 3 / 4 / 5 slots 928 / 937 / 913).
 #ifndef CT_PARK_LEAVES
 #define CT_PARK_LEAVES 1
 #endif
