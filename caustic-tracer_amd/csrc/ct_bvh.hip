@@ -19,7 +19,8 @@
 //              L-goods stay, the k-th R-good fills the k-th L-bad's slot, the
 //              k-th L-bad lands at end-2-b_{k-1}, R-bads move one slot inward,
 //              up to the meeting point K = max{k : a_k + b_k + 2 <= n}
-//   median     stable bitonic sort by (centroid, position) in LDS
+//   median     stable sort by (centroid, position): bitonic in LDS up to 2048
+//              primitives, a tiled merge sort over several workgroups above
 //   children   child records appended in parent order (exclusive scan)
 // After the last level, subtree sizes (bottom-up) give DFS indices and skip
 // links (top-down), and each record is written as one 8-float node.
@@ -36,7 +37,8 @@ constexpr int BB = 256;          // block size of position kernels
 constexpr int CHUNK = BB * 8;    // positions per block in the aggregated kernels
 constexpr int LSLOTS = 4;        // nodes per block aggregated in LDS
 constexpr int MAX_BINS = 32;
-constexpr int MEDIAN_MAX = 2048;  // largest all-degenerate node the LDS sort takes
+constexpr int MEDIAN_MAX = 2048;  // largest all-degenerate node the LDS sort takes;
+                                  // larger ones go through k_msort_tiles/k_msort_merge
 
 enum : int32_t { KIND_LEAF = 0, KIND_SAH = 1, KIND_MEDIAN = 2 };
 
@@ -618,6 +620,7 @@ __global__ void __launch_bounds__(BB) k_median(Build b, uint32_t lvStart, uint32
     uint32_t rec = lvStart + j;
     uint32_t start = b.recStart[rec], end = b.recEnd[rec];
     uint32_t n = end - start;
+    if (n > (uint32_t)MEDIAN_MAX) continue;  // sorted by the merge kernels
     int axis = b.axis[j];
     uint32_t P2 = 1;
     while (P2 < n) P2 <<= 1;
@@ -654,6 +657,98 @@ __global__ void __launch_bounds__(BB) k_median(Build b, uint32_t lvStart, uint32
     if (threadIdx.x == 0) b.mid[j] = (start + end) >> 1;
     __syncthreads();
   }
+}
+
+// Stable sort of a large all-degenerate node (more than MEDIAN_MAX primitives):
+// pairs (centroid, position) with position the index in the node's range, so
+// every pair is distinct and "a before b" is (ka < kb) || (!(kb < ka) && pa < pb),
+// the order of the LDS sort and of a stable sort by centroid (-0 and +0 compare
+// equal, as in the reference's comparator).  First every tile of MEDIAN_MAX
+// positions is sorted in LDS, then runs of width w, 2w, ... are merged: an
+// element's place is its index in its own run plus its rank in the partner
+// run, found by binary search.  Runs cover contiguous position ranges, so a
+// left-run element precedes every right-run element of equal centroid.
+__device__ __forceinline__ bool pair_before(double ka, uint32_t pa, double kb, uint32_t pb) {
+  return (ka < kb) || (!(kb < ka) && pa < pb);
+}
+
+__global__ void __launch_bounds__(BB) k_msort_tiles(Build b, uint32_t start, uint32_t n,
+                                                    int axis, double* __restrict__ key,
+                                                    uint32_t* __restrict__ pos) {
+  __shared__ double sKey[MEDIAN_MAX];
+  __shared__ uint32_t sPos[MEDIAN_MAX];
+  for (uint32_t t0 = blockIdx.x * (uint32_t)MEDIAN_MAX; t0 < n;
+       t0 += gridDim.x * (uint32_t)MEDIAN_MAX) {
+    uint32_t len = min((uint32_t)MEDIAN_MAX, n - t0);
+    for (uint32_t t = threadIdx.x; t < (uint32_t)MEDIAN_MAX; t += BB) {
+      if (t < len) {
+        sKey[t] = b.bcen[3 * (size_t)b.order[start + t0 + t] + axis];
+        sPos[t] = t0 + t;
+      } else {
+        sKey[t] = __builtin_inf();
+        sPos[t] = 0xffffffffu;
+      }
+    }
+    __syncthreads();
+    for (uint32_t k = 2; k <= (uint32_t)MEDIAN_MAX; k <<= 1) {
+      for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+        for (uint32_t t = threadIdx.x; t < (uint32_t)MEDIAN_MAX; t += BB) {
+          uint32_t u = t ^ jj;
+          if (u > t) {
+            double ka = sKey[t], kb = sKey[u];
+            uint32_t pa = sPos[t], pb = sPos[u];
+            bool aFirst = pair_before(ka, pa, kb, pb);
+            bool up = (t & k) == 0;
+            if (up != aFirst) {
+              sKey[t] = kb; sKey[u] = ka;
+              sPos[t] = pb; sPos[u] = pa;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (uint32_t t = threadIdx.x; t < len; t += BB) {
+      key[t0 + t] = sKey[t];
+      pos[t0 + t] = sPos[t];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_msort_merge(uint32_t n, uint32_t w, const double* __restrict__ keyIn,
+                              const uint32_t* __restrict__ posIn, double* __restrict__ keyOut,
+                              uint32_t* __restrict__ posOut) {
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint32_t run = i / w;
+    uint32_t own0 = run * w;
+    uint32_t part0 = (run ^ 1u) * w;
+    double k = keyIn[i];
+    uint32_t p = posIn[i];
+    uint32_t rank = 0;
+    if (part0 < n) {
+      uint32_t lo = 0, hi = min(w, n - part0);  // partners in [part0, part0 + hi)
+      while (lo < hi) {                         // count partners that come before (k, p)
+        uint32_t mid = (lo + hi) >> 1;
+        if (pair_before(keyIn[part0 + mid], posIn[part0 + mid], k, p)) lo = mid + 1;
+        else hi = mid;
+      }
+      rank = lo;
+    }
+    uint32_t dst = min(own0, part0) + (i - own0) + rank;
+    keyOut[dst] = k;
+    posOut[dst] = p;
+  }
+}
+
+__global__ void k_msort_emit(Build b, uint32_t j, uint32_t start, uint32_t end,
+                             const uint32_t* __restrict__ pos) {
+  uint32_t n = end - start;
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride)
+    b.orderNext[start + t] = b.order[start + pos[t]];
+  if (blockIdx.x == 0 && threadIdx.x == 0) b.mid[j] = (start + end) >> 1;
 }
 
 __global__ void k_split_flags(Build b, uint32_t m) {
@@ -899,13 +994,34 @@ extern "C" int ct_bvh_build_ex(const double* d_min, const double* d_max, const d
       if (e == hipSuccess) e = hipMemcpyAsync(en0.data(), b.recEnd + lvStart, m * 4, hipMemcpyDeviceToHost, st);
       if (e == hipSuccess) e = hipStreamSynchronize(st);
       if (e != hipSuccess) return fail(check_hip(e, "ct_bvh_build: median check"));
-      for (uint32_t j = 0; j < m; j++)
-        if (kinds[j] == KIND_MEDIAN && en0[j] - st0[j] > (uint32_t)MEDIAN_MAX)
-          return fail(set_error(CT_ELIMIT,
-                                "ct_bvh_build: more than 2048 primitives with coincident "
-                                "centroids in one node"));
       unsigned gMed = (unsigned)std::min<uint32_t>(m, 4096u);
       BVH_LAUNCH(k_median, gMed, BB, b, lvStart, m);
+      // nodes above the LDS sort's size: tiled merge sort, one node at a time,
+      // in the level's key buffer (free once k_setup has read it; 96 B per
+      // position, the sort takes 24)
+      double* keyA = (double*)b.keys;
+      double* keyB = keyA + N;
+      uint32_t* posA = (uint32_t*)(keyB + N);
+      uint32_t* posB = posA + N;
+      for (uint32_t j = 0; j < m; j++) {
+        uint32_t len = en0[j] - st0[j];
+        if (kinds[j] != KIND_MEDIAN || len <= (uint32_t)MEDIAN_MAX) continue;
+        int32_t ax = 0;
+        hipError_t ea = hipMemcpyAsync(&ax, b.axis + j, 4, hipMemcpyDeviceToHost, st);
+        if (ea == hipSuccess) ea = hipStreamSynchronize(st);
+        if (ea != hipSuccess) return fail(check_hip(ea, "ct_bvh_build: median axis"));
+        unsigned tiles = (len + MEDIAN_MAX - 1) / MEDIAN_MAX;
+        BVH_LAUNCH(k_msort_tiles, std::min(tiles, 4096u), BB, b, st0[j], len, (int)ax, keyA,
+                   posA);
+        double *ki = keyA, *ko = keyB;
+        uint32_t *pi = posA, *po = posB;
+        for (uint32_t w = MEDIAN_MAX; w < len; w *= 2) {
+          BVH_LAUNCH(k_msort_merge, grid_for(len, BB), BB, len, w, ki, pi, ko, po);
+          std::swap(ki, ko);
+          std::swap(pi, po);
+        }
+        BVH_LAUNCH(k_msort_emit, grid_for(len, BB), BB, b, j, st0[j], en0[j], pi);
+      }
     }
     BVH_LAUNCH(k_split_flags, gM, BB, b, m);
     rc = scan_u32(b.flag, b.flagScan, m, b.scanTmp, st);

@@ -35,6 +35,7 @@ struct SJob {
   uint32_t done = 0;      // passes added
   uint32_t inFlight = 0;  // passes in the round being traced
   int state = CT_JOB_QUEUED;
+  bool pending = false;   // admitted; the worker is allocating and clearing its buffer
   bool release = false;   // free once the round in flight ends
   bool charged = false;   // counted in the server's HBM in use
   double submit = 0, admit = -1, finish = -1;
@@ -108,6 +109,7 @@ void serve_loop(ct_server* S) {
         break;
       S->queue.pop_front();
       j->state = CT_JOB_RUNNING;
+      j->pending = true;  // no client touches the buffer until it is allocated and zeroed
       j->admit = S->now();
       S->st.hbmInUse += j->bytes;
       j->charged = true;
@@ -163,6 +165,15 @@ void serve_loop(ct_server* S) {
       if (S->opts.precision == CT_PRECISION_F32)
         j->comp = j->accum + (size_t)j->desc.width * j->desc.height * 3;
     }
+    // the clears complete before any client may read a fresh buffer (reads go
+    // through another stream, which is not ordered after this one)
+    if (!fresh.empty()) {
+      hipError_t e = hipStreamSynchronize(S->stream);
+      if (e != hipSuccess)
+        for (SJob* j : fresh)
+          if (j->error.empty()) j->error = std::string("clearing a progressive buffer: ") +
+                                           hipGetErrorString(e);
+    }
     std::vector<PlanJob> run;
     for (size_t k = 0; k < picked.size(); k++) {
       if (!picked[k]->accum || !picked[k]->error.empty()) continue;
@@ -187,9 +198,28 @@ void serve_loop(ct_server* S) {
       S->st.pixelSamples += samples;
     }
     double now = S->now();
+    // fresh jobs this round did not reach: their buffers are ready now (before the
+    // picked jobs, which may be freed below)
+    for (SJob* j : fresh) {
+      if (std::find(picked.begin(), picked.end(), j) != picked.end()) continue;
+      j->pending = false;
+      if (!j->error.empty() || !j->accum) {
+        if (j->error.empty()) j->error = "no progressive buffer";
+        j->state = CT_JOB_FAILED;
+        j->finish = now;
+        S->st.failed++;
+        drop_active(S, j);
+      }
+      if (j->release) {
+        if (j->state == CT_JOB_RUNNING) drop_active(S, j);
+        S->jobs.erase(j->id);
+        free_job(S, j);
+      }
+    }
     for (SJob* j : picked) {
       uint32_t p = j->inFlight;
       j->inFlight = 0;
+      j->pending = false;
       bool failed = !j->error.empty() || !j->accum || rc;
       if (failed) {
         if (j->error.empty()) j->error = err.empty() ? "render round failed" : err;
@@ -368,10 +398,10 @@ int ct_job_read(ct_server* S, uint64_t id, float* dst, int32_t dstOnDevice) {
   std::unique_lock<std::mutex> lk(S->mu);
   SJob* j = find(S, id);
   if (!j) return set_error(CT_EINVAL, "ct_job_read: unknown job");
-  // never read a buffer while a round writes it
+  // never read a buffer while a round writes it or before it is cleared
   S->cvDone.wait(lk, [&] {
     SJob* x = find(S, id);
-    return !x || x->inFlight == 0;
+    return !x || (x->inFlight == 0 && !x->pending);
   });
   j = find(S, id);
   if (!j) return set_error(CT_EINVAL, "ct_job_read: job released while reading");
@@ -391,7 +421,7 @@ int ct_job_read_rgba(ct_server* S, uint64_t id, double exposureEv, int32_t mode,
   if (!j) return set_error(CT_EINVAL, "ct_job_read_rgba: unknown job");
   S->cvDone.wait(lk, [&] {
     SJob* x = find(S, id);
-    return !x || x->inFlight == 0;
+    return !x || (x->inFlight == 0 && !x->pending);
   });
   j = find(S, id);
   if (!j) return set_error(CT_EINVAL, "ct_job_read_rgba: job released while reading");
@@ -424,7 +454,7 @@ int ct_job_release(ct_server* S, uint64_t id) {
       S->queue.erase(std::remove(S->queue.begin(), S->queue.end(), j), S->queue.end());
       S->jobs.erase(id);
       delete j;  // no buffer yet
-    } else if (j->inFlight) {
+    } else if (j->inFlight || j->pending) {
       j->release = true;  // the worker frees it after the round
     } else {
       if (j->state == CT_JOB_RUNNING) drop_active(S, j);

@@ -210,6 +210,47 @@ def test_gpu_bvh_build_soup_bit_exact(ct, O, n):
     assert np.array_equal(go, oo)
 
 
+def _degenerate_bounds(case):
+    """PrimBounds whose nodes fall back to medianSplit (bvh.ts:284-296) with more
+    primitives than one workgroup sorts in LDS."""
+    rs = np.random.default_rng(31)
+    if case == "coincident5000":  # one centroid: the stable sort keeps position order
+        n = 5000
+        cen = np.tile([1.0, 2.0, 3.0], (n, 1))
+    elif case == "ties_x6000":  # centroid x in 1e-16 steps (extent <= 1e-12), many ties
+        n = 6000
+        cen = np.zeros((n, 3))
+        cen[:, 0] = 0.25 + rs.integers(0, 9000, n) * 1e-16
+        cen[:, 1] = -1.0 + rs.integers(0, 3, n) * 1e-16
+        cen[:, 2] = 4.0
+    elif case == "ties_y4100":  # the widest degenerate axis is y
+        n = 4100
+        cen = np.zeros((n, 3))
+        cen[:, 0] = 3.0 + rs.integers(0, 2, n) * 1e-16
+        cen[:, 1] = rs.integers(-4000, 4000, n) * 1e-16
+        cen[:, 2] = -7.0 + rs.integers(0, 5, n) * 1e-17
+    else:  # "mixed": a SAH top over a 3000-primitive cluster and a spread cloud
+        n = 6000
+        cen = np.zeros((n, 3))
+        cen[:3000] = [0.5, 0.5, 0.5]
+        cen[3000:] = rs.uniform(-10, 10, (3000, 3))
+        cen = cen[rs.permutation(n)]
+    half = rs.uniform(0.01, 1.0, (n, 3))
+    return cen - half, cen + half, cen
+
+
+@pytest.mark.parametrize("case", ["coincident5000", "ties_x6000", "ties_y4100", "mixed"])
+def test_gpu_bvh_build_large_median_nodes(ct, O, case):
+    bmin, bmax, bcen = _degenerate_bounds(case)
+    on, onc, oo = O.build_bvh(bmin.ravel(), bmax.ravel(), bcen.ravel(), 12)
+    rep = O.validate_bvh(on, onc, oo, len(bcen))
+    assert rep["errors"] == 0
+    gn, gnc, go = ct.buildBvh(bmin, bmax, bcen, 12)
+    assert gnc == onc
+    assert np.array_equal(gn.view(np.uint32), on.view(np.uint32))
+    assert np.array_equal(go, oo)
+
+
 def test_pack_scene_matches_oracle(ct, O):
     sc = scenes.random_scene(O.Rng(1234), 1000, 2000)
     g = ct.packScene(sc)

@@ -160,3 +160,42 @@ def test_alone_equals_in_mix_fp32(ct):
         out = ct.DeviceBuffer.empty((r["width"] * r["height"] * 3,), np.float32)
         tr.render_image(r["scene"], r["frame"], r, out)
         assert np.array_equal(out.numpy().view(np.uint32), img.view(np.uint32))
+
+
+def test_read_of_a_job_queued_behind_a_round_filling_job(ct):
+    """A job admitted while another one takes every path of the round gets its
+    buffer allocated and cleared by the worker outside the lock; a read in that
+    window must wait for the clear.  Every read returns the sum of the passes
+    added so far (zeros before the first), never uninitialised memory."""
+    r = _requests(ct)[0]
+    w, h, spp, seed = 32, 24, 3, 11
+    opts = {"width": w, "height": h, "spp": spp, "seed": seed, "maxBounces": 6}
+    prefix = [np.zeros(w * h * 3, np.float32)]
+    acc = np.zeros(w * h * 3, np.float32)
+    for s in range(spp - 1):
+        ct.accumulatePass(r["scene"], r["frame"], w, h, s + 7919 * seed, opts, acc)
+        prefix.append(acc.copy())
+    final = ct.renderImage(r["scene"], r["frame"], opts)
+    valid = [p.view(np.uint32) for p in prefix] + [final.view(np.uint32)]
+    # one pass of the big job is the whole round: the small job waits admitted
+    with ct.Server(max_paths=128 * 128) as srv:
+        big = srv.submit(r["scene"], r["frame"], 128, 128, 24, seed=1)
+        small = srv.submit(r["scene"], r["frame"], w, h, spp, seed=seed, maxBounces=6)
+        t0 = time.time()
+        while srv.poll(small)["state"] == ct.serve.QUEUED and time.time() - t0 < 60:
+            time.sleep(0.0002)
+        seen = set()
+        for _ in range(200):
+            img = srv.read(small, w, h).view(np.uint32)
+            k = [i for i, v in enumerate(valid) if np.array_equal(v, img)]
+            assert k, "a read returned neither a pass sum nor the final image"
+            seen.add(k[0])
+            if srv.poll(small)["state"] == ct.serve.DONE:
+                break
+        assert srv.wait(small, timeout=120)["state"] == ct.serve.DONE
+        img = srv.read(small, w, h)
+        assert np.array_equal(img.view(np.uint32), valid[-1])
+        assert srv.wait(big, timeout=120)["state"] == ct.serve.DONE
+        srv.release(small)
+        srv.release(big)
+        assert _drained(srv)
