@@ -119,6 +119,7 @@ SIGNATURES = {
     "ct_tracer_set_graph": [vp, i32],
     "ct_bvh_build": [vp, vp, vp, u32, i32, vp, C.POINTER(u32), vp, vp],
     "ct_bvh_build_ex": [vp, vp, vp, u32, i32, vp, C.POINTER(u32), vp, vp, vp],
+    "ct_bvh_build_noalloc": [vp, vp, vp, u32, i32, vp, C.POINTER(u32), vp, vp, vp],
     "ct_tracer_set_workgroup": [vp, i32],
     "ct_tracer_set_precision": [vp, i32],
     "ct_closest_hit32": [vp, vp, u32, C.c_float, vp, i32, vp, vp, vp, vp, vp],

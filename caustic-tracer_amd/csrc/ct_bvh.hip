@@ -26,7 +26,10 @@
 // links (top-down), and each record is written as one 8-float node.
 #include <string.h>
 
+#include <stdio.h>
+
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "ct_internal.h"
@@ -861,15 +864,107 @@ extern "C" int ct_bvh_build(const double* d_min, const double* d_max, const doub
                          stream);
 }
 
+namespace {
+
+constexpr int SCRATCH_SECTIONS = 30;
+
+// Device bytes of the build's working memory for n primitives and `bins` buckets.
+size_t build_scratch_bytes(size_t N, int bins, size_t* offsets) {
+  const size_t NR = 2 * N, M = N, NS = N / 5 + 1;
+  const size_t sizes[] = {N * 4, N * 4, N * 4, (N + 1) * 4, N * 4, N * 4,
+                          NR * 4, NR * 4, NR * 4, NR * 4, NR * 48, NR * 4, NR * 4,
+                          M * 96, M * 4, M * 4, M * 4, M * 8, M * 8, M * 24, M * 24,
+                          M * 4, (M + 1) * 4, (M + 1) * 4, M * 4, M * 4,
+                          NS * 3 * (size_t)bins * 4, NS * 3 * (size_t)bins * 48, 16,
+                          scan_tmp_entries(N + 1) * 4};
+  static_assert(sizeof(sizes) / sizeof(sizes[0]) == SCRATCH_SECTIONS, "one offset per section");
+  size_t sz = 0, k = 0;
+  for (size_t b : sizes) {
+    if (offsets) offsets[k] = sz;
+    k++;
+    sz += al(b);
+  }
+  return sz;
+}
+
+// The build proper.  `mem` is working memory of build_scratch_bytes(n, bins)
+// bytes (uninitialised: every kernel clears or writes what it reads), or null
+// to allocate it on the stream and free it at the end.
+int bvh_build(const double* d_min, const double* d_max, const double* d_cen, uint32_t n,
+              int32_t bins, float* d_nodes, uint32_t* h_nodeCount, uint32_t* d_order,
+              uint32_t* d_leafWords, hipStream_t st, char* mem);
+
+}  // namespace
+
 extern "C" int ct_bvh_build_ex(const double* d_min, const double* d_max, const double* d_cen,
                                uint32_t n, int32_t bins, float* d_nodes, uint32_t* h_nodeCount,
                                uint32_t* d_order, uint32_t* d_leafWords, void* stream) {
+  return bvh_build(d_min, d_max, d_cen, n, bins, d_nodes, h_nodeCount, d_order, d_leafWords,
+                   (hipStream_t)stream, nullptr);
+}
+
+// Working memory of ct_bvh_build_noalloc: one array per device, held until the
+// process ends.  Host code never passes `scratch` itself to a kernel (in host
+// code the name is a stand-in, not the device address): the address comes from
+// hipGetSymbolAddress, once per device.
+static __device__ __align__(256) char scratch[CT_BVH_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratchMu;
+char* g_scratchPtr[MAX_DEVICES] = {};
+
+int scratch_for_current_device(char** out) {
+  int dev = 0;
+  CT_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES)
+    return set_error(CT_ELIMIT, "ct_bvh_build_noalloc: device index beyond 63");
+  std::lock_guard<std::mutex> lk(g_scratchMu);
+  if (!g_scratchPtr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return check_hip(e, "ct_bvh_build_noalloc: hipGetSymbolAddress");
+    g_scratchPtr[dev] = (char*)p;
+  }
+  *out = g_scratchPtr[dev];
+  return CT_OK;
+}
+}  // namespace
+
+extern "C" int ct_bvh_build_noalloc(const double* d_min, const double* d_max,
+                                    const double* d_cen, uint32_t n, int32_t bins,
+                                    float* d_nodes, uint32_t* h_nodeCount, uint32_t* d_order,
+                                    uint32_t* d_leafWords, void* stream) {
+  if (bins < 2 || bins > MAX_BINS)
+    return set_error(CT_EINVAL, "ct_bvh_build_noalloc: bins must be in [2, 32]");
+  size_t need = n ? build_scratch_bytes(n, bins, nullptr) : 0;
+  if (need > (size_t)CT_BVH_NOALLOC_SCRATCH_BYTES) {
+    char msg[200];
+    snprintf(msg, sizeof msg,
+             "ct_bvh_build_noalloc: %u primitives at %d bins need %zu bytes of working "
+             "memory; the static buffer holds %zu",
+             n, bins, need, (size_t)CT_BVH_NOALLOC_SCRATCH_BYTES);
+    return set_error(CT_ELIMIT, msg);
+  }
+  char* mem = nullptr;
+  if (n) {
+    int rc = scratch_for_current_device(&mem);
+    if (rc) return rc;
+  }
+  return bvh_build(d_min, d_max, d_cen, n, bins, d_nodes, h_nodeCount, d_order, d_leafWords,
+                   (hipStream_t)stream, mem);
+}
+
+namespace {
+
+int bvh_build(const double* d_min, const double* d_max, const double* d_cen, uint32_t n,
+              int32_t bins, float* d_nodes, uint32_t* h_nodeCount, uint32_t* d_order,
+              uint32_t* d_leafWords, hipStream_t st, char* scratchMem) {
   if (!d_nodes || !h_nodeCount || (n && (!d_min || !d_max || !d_cen || !d_order)))
     return set_error(CT_EINVAL, "ct_bvh_build: null argument");
   if (bins < 2 || bins > MAX_BINS)
     return set_error(CT_EINVAL, "ct_bvh_build: bins must be in [2, 32]");
   if (n > (1u << 30)) return set_error(CT_ELIMIT, "ct_bvh_build: more than 2^30 primitives");
-  hipStream_t st = (hipStream_t)stream;
   if (n == 0) {
     // a single empty leaf (bvh.ts:93-100)
     float node[8] = {0, 0, 0, 1, 0, 0, 0, 0};
@@ -881,25 +976,20 @@ extern "C" int ct_bvh_build_ex(const double* d_min, const double* d_max, const d
     *h_nodeCount = 1;
     return CT_OK;
   }
-  const size_t N = n, NR = 2 * N;           // records: 2n-1 at most
-  const size_t M = N;                       // active nodes per level: at most n
-  const size_t NS = N / 5 + 1;              // SAH nodes per level: each holds >= 5 prims
-  const size_t scanTmp = scan_tmp_entries(N + 1);
-  size_t sz = 0;
-  auto reserve = [&](size_t bytes) { size_t o = sz; sz += al(bytes); return o; };
-  size_t oOrderB = reserve(N * 4), oNodeOf = reserve(N * 4), oGood = reserve(N * 4),
-         oGoodScan = reserve((N + 1) * 4), oA = reserve(N * 4), oB = reserve(N * 4),
-         oRecStart = reserve(NR * 4), oRecEnd = reserve(NR * 4), oRecLeft = reserve(NR * 4),
-         oRecRight = reserve(NR * 4), oRecBox = reserve(NR * 48), oRecSize = reserve(NR * 4),
-         oRecIdx = reserve(NR * 4), oKeys = reserve(M * 96), oKind = reserve(M * 4),
-         oAxis = reserve(M * 4), oSplit = reserve(M * 4), oOrigin = reserve(M * 8),
-         oScale = reserve(M * 8), oCmin3 = reserve(M * 24), oTob3 = reserve(M * 24),
-         oFlag = reserve(M * 4), oFlagScan = reserve((M + 1) * 4), oBinSlot = reserve((M + 1) * 4),
-         oMid = reserve(M * 4), oMeetX = reserve(M * 4),
-         oBinCount = reserve(NS * 3 * bins * 4), oBinKey = reserve(NS * 3 * bins * 48),
-         oLevelInfo = reserve(16), oScanTmp = reserve(scanTmp * 4);
-  char* mem = nullptr;
-  CT_HIP(hipMallocAsync((void**)&mem, sz, st));
+  const size_t N = n, NR = 2 * N;  // records: 2n-1 at most
+  size_t off[SCRATCH_SECTIONS];
+  const size_t sz = build_scratch_bytes(N, bins, off);
+  const bool owned = scratchMem == nullptr;
+  char* mem = scratchMem;
+  if (owned) CT_HIP(hipMallocAsync((void**)&mem, sz, st));
+  const size_t oOrderB = off[0], oNodeOf = off[1], oGood = off[2], oGoodScan = off[3],
+               oA = off[4], oB = off[5], oRecStart = off[6], oRecEnd = off[7],
+               oRecLeft = off[8], oRecRight = off[9], oRecBox = off[10], oRecSize = off[11],
+               oRecIdx = off[12], oKeys = off[13], oKind = off[14], oAxis = off[15],
+               oSplit = off[16], oOrigin = off[17], oScale = off[18], oCmin3 = off[19],
+               oTob3 = off[20], oFlag = off[21], oFlagScan = off[22], oBinSlot = off[23],
+               oMid = off[24], oMeetX = off[25], oBinCount = off[26], oBinKey = off[27],
+               oLevelInfo = off[28], oScanTmp = off[29];
   Build b;
   b.bmin = d_min;
   b.bmax = d_max;
@@ -941,7 +1031,7 @@ extern "C" int ct_bvh_build_ex(const double* d_min, const double* d_max, const d
   int rc = CT_OK;
   std::vector<std::pair<uint32_t, uint32_t>> levels;
   auto fail = [&](int code) {
-    (void)hipFreeAsync(mem, st);
+    if (owned) (void)hipFreeAsync(mem, st);
     (void)hipStreamSynchronize(st);
     return code;
   };
@@ -1052,9 +1142,11 @@ extern "C" int ct_bvh_build_ex(const double* d_min, const double* d_max, const d
                levels[L].first, levels[L].second);
   BVH_LAUNCH(k_emit, grid_for(nrec, BB), BB, b, nrec, d_nodes, d_leafWords);
 #undef BVH_LAUNCH
-  hipError_t e = hipFreeAsync(mem, st);
+  hipError_t e = owned ? hipFreeAsync(mem, st) : hipSuccess;
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return check_hip(e, "ct_bvh_build: finish");
   *h_nodeCount = nrec;
   return CT_OK;
 }
+
+}  // namespace

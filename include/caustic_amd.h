@@ -387,6 +387,20 @@ int ct_bvh_build_ex(const double *d_min, const double *d_max, const double *d_ce
                     uint32_t count, int32_t bins, float *d_nodes, uint32_t *h_nodeCount,
                     uint32_t *d_order, uint32_t *d_leafWords, void *stream);
 
+/* ct_bvh_build_ex without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, serving loops that must not call
+ * hipMalloc, harnesses that forbid it).  No counterpart in the reference.
+ * The build's working memory is a static device array of
+ * CT_BVH_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends: at 12
+ * bins that is room for about 91,000 primitives (the call returns CT_ELIMIT
+ * with the bytes it needs when count is larger).  Every call on one GPU
+ * shares that array: calls that overlap on one GPU race, so the caller
+ * serialises them (the call synchronises its stream before returning). */
+#define CT_BVH_NOALLOC_SCRATCH_BYTES (64u << 20)
+int ct_bvh_build_noalloc(const double *d_min, const double *d_max, const double *d_centroid,
+                         uint32_t count, int32_t bins, float *d_nodes, uint32_t *h_nodeCount,
+                         uint32_t *d_order, uint32_t *d_leafWords, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
