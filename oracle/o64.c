@@ -22,6 +22,14 @@ static const double PI_D = 3.141592653589793;       /* Math.PI */
 #define INV_PI (1 / PI_D)                            /* materials.ts:10 */
 #define SHADOW_SHORTEN (1 - 1e-3)                    /* tracer.ts:36 */
 
+/* Planted mistakes, for the tests that show the oracle's pins have teeth: a
+ * build with -DO64_PLANT=k carries mistake k (the numbers are listed in
+ * o64.h); the default build (0) carries none. */
+#ifndef O64_PLANT
+#define O64_PLANT 0
+#endif
+#define PLANTED(k) (O64_PLANT == (k))
+
 /* ======================================================================
  * rng.ts
  * ====================================================================== */
@@ -208,7 +216,8 @@ static int sah_split(Builder *B, int start, int end, double cMinX, double cMinY,
   double bestCost = INFINITY;
   int bestAxis = -1, bestSplit = -1;
   double bestOrigin = 0, bestScale = 0;
-  for (int axis = 0; axis < 3; axis++) {
+  for (int axisStep = 0; axisStep < 3; axisStep++) {
+    int axis = PLANTED(O64_PLANT_AXIS_REVERSED) ? 2 - axisStep : axisStep;
     double cmin = axis == 0 ? cMinX : axis == 1 ? cMinY : cMinZ;
     double cmax = axis == 0 ? cMaxX : axis == 1 ? cMaxY : cMaxZ;
     double extent = cmax - cmin;
@@ -249,8 +258,9 @@ static int sah_split(Builder *B, int start, int end, double cMinX, double cMinY,
         if (B->bucketMax[bo + 2] > rz1) rz1 = B->bucketMax[bo + 2];
         rc += B->bucketCount[s];
       }
-      B->rightCount[s - 1] = rc;
-      B->rightArea[s - 1] = rc > 0 ? surface_area(rx1 - rx0, ry1 - ry0, rz1 - rz0) : 0;
+      int slot = PLANTED(O64_PLANT_SWEEP_SHIFT) ? (s < NB - 1 ? s : s - 1) : s - 1;
+      B->rightCount[slot] = rc;
+      B->rightArea[slot] = rc > 0 ? surface_area(rx1 - rx0, ry1 - ry0, rz1 - rz0) : 0;
     }
     double lx0 = INFINITY, ly0 = INFINITY, lz0 = INFINITY;
     double lx1 = -INFINITY, ly1 = -INFINITY, lz1 = -INFINITY;
@@ -269,7 +279,7 @@ static int sah_split(Builder *B, int start, int end, double cMinX, double cMinY,
       int rcs = B->rightCount[s];
       if (lc == 0 || rcs == 0) continue;
       double cost = lc * surface_area(lx1 - lx0, ly1 - ly0, lz1 - lz0) + rcs * B->rightArea[s];
-      if (cost < bestCost) {
+      if (PLANTED(O64_PLANT_SAH_LE) ? cost <= bestCost : cost < bestCost) {
         bestCost = cost;
         bestAxis = axis;
         bestSplit = s;
@@ -288,7 +298,7 @@ static int sah_split(Builder *B, int start, int end, double cMinX, double cMinY,
     if (fb >= NB) b = NB - 1;
     else if (fb < 0) b = 0;
     else b = (int)fb;
-    if (b <= bestSplit) {
+    if (PLANTED(O64_PLANT_PARTITION_LT) ? b < bestSplit : b <= bestSplit) {
       lo++;
     } else {
       uint32_t tmp = order[lo];
@@ -314,7 +324,7 @@ static void stable_sort_range(Builder *B, int start, int end, int axis) {
       while (i < mid && j < hi) {
         double ki = B->cen[a[i] * 3 + axis], kj = B->cen[a[j] * 3 + axis];
         /* comparator a-b: take right only when strictly smaller */
-        if (kj - ki < 0) tmp[k++] = a[j++];
+        if (PLANTED(O64_PLANT_MEDIAN_UNSTABLE) ? kj - ki <= 0 : kj - ki < 0) tmp[k++] = a[j++];
         else tmp[k++] = a[i++];
       }
       while (i < mid) tmp[k++] = a[i++];
@@ -329,6 +339,7 @@ static int median_split(Builder *B, int start, int end, double cMinX, double cMi
                         double cMaxX, double cMaxY, double cMaxZ) {
   double ex = cMaxX - cMinX, ey = cMaxY - cMinY, ez = cMaxZ - cMinZ;
   int axis = ex >= ey && ex >= ez ? 0 : ey >= ez ? 1 : 2;
+  if (PLANTED(O64_PLANT_MEDIAN_AXIS_LAST)) axis = ez >= ey && ez >= ex ? 2 : ey >= ex ? 1 : 0;
   stable_sort_range(B, start, end, axis);
   return (start + end) >> 1;
 }
@@ -359,14 +370,15 @@ static void build_node(Builder *B, int start, int end) {
     if (cy > cMaxY) cMaxY = cy;
     if (cz > cMaxZ) cMaxZ = cz;
   }
-  nd[0] = (float)(bMinX - fabs(bMinX) * 1.2e-7 - 1e-9);
-  nd[1] = (float)(bMinY - fabs(bMinY) * 1.2e-7 - 1e-9);
-  nd[2] = (float)(bMinZ - fabs(bMinZ) * 1.2e-7 - 1e-9);
-  nd[4] = (float)(bMaxX + fabs(bMaxX) * 1.2e-7 + 1e-9);
-  nd[5] = (float)(bMaxY + fabs(bMaxY) * 1.2e-7 + 1e-9);
-  nd[6] = (float)(bMaxZ + fabs(bMaxZ) * 1.2e-7 + 1e-9);
+  const double nudgeSign = PLANTED(O64_PLANT_NUDGE_INWARD) ? -1 : 1;
+  nd[0] = (float)(bMinX - nudgeSign * (fabs(bMinX) * 1.2e-7) - nudgeSign * 1e-9);
+  nd[1] = (float)(bMinY - nudgeSign * (fabs(bMinY) * 1.2e-7) - nudgeSign * 1e-9);
+  nd[2] = (float)(bMinZ - nudgeSign * (fabs(bMinZ) * 1.2e-7) - nudgeSign * 1e-9);
+  nd[4] = (float)(bMaxX + nudgeSign * (fabs(bMaxX) * 1.2e-7) + nudgeSign * 1e-9);
+  nd[5] = (float)(bMaxY + nudgeSign * (fabs(bMaxY) * 1.2e-7) + nudgeSign * 1e-9);
+  nd[6] = (float)(bMaxZ + nudgeSign * (fabs(bMaxZ) * 1.2e-7) + nudgeSign * 1e-9);
   int count = end - start;
-  if (count <= MAX_LEAF_PRIMS) {
+  if (count <= (PLANTED(O64_PLANT_LEAF3) ? 3 : MAX_LEAF_PRIMS)) {
     nd[3] = (float)(nodeIndex + 1);
     nd[7] = (float)((double)start * 8 + count); /* encodeLeaf bvh.ts:60-62 */
     return;
@@ -1073,7 +1085,8 @@ void o64_generate_ray(const double *f, double sx, double sy, double lx, double l
     focus[k] = f[k] + (f[3 + k] * px + f[6 + k] * py + f[9 + k]) * fd;
     o3[k] = f[k] + (f[3 + k] * (lx * lr) + f[6 + k] * (ly * lr));
   }
-  for (int k = 0; k < 3; k++) dir[k] = focus[k] - o3[k];
+  for (int k = 0; k < 3; k++)
+    dir[k] = PLANTED(O64_PLANT_LENS_NO_REFOCUS) ? focus[k] - f[k] : focus[k] - o3[k];
   normalize3(dir);
   d3[0] = dir[0];
   d3[1] = dir[1];
@@ -1126,8 +1139,11 @@ static int sample_light(const O64Scene *s, double px, double py, double pz, O64R
     double r = PRIMF(prim, 4);
     double d2 = cx * cx + cy * cy + cz * cz;
     if (d2 > r * r) {
-      double cosMax = sqrt(js_max2(0, 1 - (r * r) / d2));
-      double cosT = 1 - o64_rng_next(rng) * (1 - cosMax);
+      double cosMax = PLANTED(O64_PLANT_CONE_NO_SQRT) ? js_max2(0, 1 - (r * r) / d2)
+                                                      : sqrt(js_max2(0, 1 - (r * r) / d2));
+      double cosT = PLANTED(O64_PLANT_CONE_ANGLE_UNIFORM)
+                        ? cos(o64_rng_next(rng) * acos(cosMax))
+                        : 1 - o64_rng_next(rng) * (1 - cosMax);
       double sinT = sqrt(js_max2(0, 1 - cosT * cosT));
       double phi = TWO_PI * o64_rng_next(rng);
       double invD = 1 / sqrt(d2);
@@ -1151,7 +1167,8 @@ static int sample_light(const O64Scene *s, double px, double py, double pz, O64R
       out[1] = wy;
       out[2] = wz;
       out[3] = dist;
-      out[4] = 1 / (TWO_PI * (1 - cosMax) * lightCount);
+      out[4] = 1 / ((PLANTED(O64_PLANT_CONE_PDF_2X) ? 2 * TWO_PI : TWO_PI) * (1 - cosMax) *
+                    lightCount);
       return 1;
     }
     double u1 = o64_rng_next(rng);
@@ -1169,7 +1186,8 @@ static int sample_light(const O64Scene *s, double px, double py, double pz, O64R
     out[1] = wy;
     out[2] = wz;
     out[3] = dist;
-    out[4] = dist2 / (PRIMF(prim, 11) * cosL * lightCount);
+    out[4] = dist2 / ((PLANTED(O64_PLANT_INSIDE_HALF_AREA) ? 0.5 * PRIMF(prim, 11) : PRIMF(prim, 11)) *
+                      cosL * lightCount);
     return 1;
   }
   double u1 = o64_rng_next(rng);
@@ -1215,7 +1233,7 @@ void o64_radiance(const O64Scene *s, double ox, double oy, double oz, double dx,
     if (rc) rc->extension++;
     if (!o64_intersect_closest(s, o, d, T_MIN, INFINITY, &hit, NULL)) {
       if (s->skyKind == 1) {
-        double t = 0.5 * (dy + 1);
+        double t = PLANTED(O64_PLANT_SKY_FLIP) ? 0.5 * (1 - dy) : 0.5 * (dy + 1);
         Lr += br * (s->horizon[0] + (s->zenith[0] - s->horizon[0]) * t) * s->skyStrength;
         Lg += bg * (s->horizon[1] + (s->zenith[1] - s->horizon[1]) * t) * s->skyStrength;
         Lb += bb * (s->horizon[2] + (s->zenith[2] - s->horizon[2]) * t) * s->skyStrength;
@@ -1336,6 +1354,40 @@ void o64_radiance(const O64Scene *s, double ox, double oy, double oz, double dx,
   out[2] = Lb;
 }
 
+/* The camera ray of one path (tracer.ts:373-383): pixel jitter, then the lens
+ * draws when lensRadius > 0, then generateRay.  For the lens and sky pins. */
+void o64_camera_sample(const double *frame, int width, int height, int x, int y,
+                       uint32_t frameIndex, double *o3, double *d3) {
+  uint32_t pixel = (uint32_t)y * (uint32_t)width + (uint32_t)x;
+  O64Rng rng;
+  rng.state = o64_seed_for(pixel, frameIndex);
+  double sx = (x + o64_rng_next(&rng)) / width;
+  double sy = 1 - (y + o64_rng_next(&rng)) / height;
+  double lx = 0, ly = 0;
+  if (frame[14] > 0) {
+    double r = PLANTED(O64_PLANT_LENS_LINEAR_R) ? o64_rng_next(&rng) : sqrt(o64_rng_next(&rng));
+    double theta = TWO_PI * o64_rng_next(&rng);
+    lx = r * cos(theta);
+    ly = r * sin(theta);
+  }
+  o64_generate_ray(frame, sx, sy, lx, ly, o3, d3);
+}
+
+/* sampleLightInto / lightPdfScalar (tracer.ts:61-180) from a given RNG state,
+ * for the light-sampling pins.  *state is advanced as the draws consume it. */
+int o64_sample_light(const O64Scene *s, double px, double py, double pz, uint32_t *state,
+                     double *out8) {
+  O64Rng rng;
+  rng.state = *state;
+  int ok = sample_light(s, px, py, pz, &rng, out8);
+  *state = rng.state;
+  return ok;
+}
+double o64_light_pdf(const O64Scene *s, double px, double py, double pz, double hx, double hy,
+                     double hz, double hnx, double hny, double hnz, int prim) {
+  return light_pdf(s, px, py, pz, hx, hy, hz, hnx, hny, hnz, prim);
+}
+
 /* Camera ray + radiance of one path: tracer.ts:373-385 */
 static void pixel_sample(const O64Scene *s, const double *frame, int width, int height, int x,
                          int y, uint32_t frameIndex, const O64TraceOpts *opts, double *c,
@@ -1347,7 +1399,7 @@ static void pixel_sample(const O64Scene *s, const double *frame, int width, int 
   double sy = 1 - (y + o64_rng_next(&rng)) / height;
   double lx = 0, ly = 0;
   if (frame[14] > 0) {
-    double r = sqrt(o64_rng_next(&rng));
+    double r = PLANTED(O64_PLANT_LENS_LINEAR_R) ? o64_rng_next(&rng) : sqrt(o64_rng_next(&rng));
     double theta = TWO_PI * o64_rng_next(&rng);
     lx = r * cos(theta);
     ly = r * sin(theta);

@@ -177,6 +177,33 @@ double o64_linear_to_srgb(double x);
 void o64_tonemap_buffer(const float *accum, int width, int height, double samples,
                         double exposureEv, int mode, uint8_t *rgba);
 
+/* Planted-mistake numbers (-DO64_PLANT=k; tests/test_oracle_pins.py builds
+ * each variant and shows that a pin fails on it). */
+#define O64_PLANT_SAH_LE 1              /* cost <= best: the last of equal-cost splits */
+#define O64_PLANT_AXIS_REVERSED 2       /* axes tried z, y, x */
+#define O64_PLANT_NUDGE_INWARD 3        /* node boxes nudged inward */
+#define O64_PLANT_PARTITION_LT 4        /* bucket < bestSplit goes left */
+#define O64_PLANT_MEDIAN_UNSTABLE 5     /* equal centroids reordered */
+#define O64_PLANT_MEDIAN_AXIS_LAST 6    /* median axis ties prefer z */
+#define O64_PLANT_SWEEP_SHIFT 7         /* suffix-sweep counts stored one slot off */
+#define O64_PLANT_LEAF3 8               /* leaves of at most three primitives */
+#define O64_PLANT_CONE_NO_SQRT 20       /* cone half-angle cosine without the square root */
+#define O64_PLANT_CONE_ANGLE_UNIFORM 21 /* cone sampled uniform in angle, not in cosine */
+#define O64_PLANT_CONE_PDF_2X 22        /* cone pdf over twice the solid angle */
+#define O64_PLANT_INSIDE_HALF_AREA 23   /* inside-sphere pdf with half the area */
+#define O64_PLANT_LENS_LINEAR_R 30      /* lens radius square root dropped */
+#define O64_PLANT_LENS_NO_REFOCUS 31    /* lens rays aimed from the lens centre */
+#define O64_PLANT_SKY_FLIP 40           /* sky gradient upside down */
+
+/* Test exports: the camera ray of one path (tracer.ts:373-383), and light
+ * sampling from a given RNG state (tracer.ts:61-180). */
+void o64_camera_sample(const double *frame, int width, int height, int x, int y,
+                       uint32_t frameIndex, double *o3, double *d3);
+int o64_sample_light(const O64Scene *s, double px, double py, double pz, uint32_t *state,
+                     double *out8);
+double o64_light_pdf(const O64Scene *s, double px, double py, double pz, double hx, double hy,
+                     double hz, double hnx, double hny, double hnz, int prim);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,52 @@ def lib():
     return _lib
 
 
+# Planted-mistake variants (o64.h, O64_PLANT_*): the same source built with
+# -DO64_PLANT=k, used only by the tests that show each pin catches a mistake.
+PLANTS = {"sah_le": 1, "axis_reversed": 2, "nudge_inward": 3, "partition_lt": 4,
+          "median_unstable": 5, "median_axis_last": 6, "sweep_shift": 7, "leaf3": 8,
+          "cone_no_sqrt": 20, "cone_angle_uniform": 21, "cone_pdf_2x": 22,
+          "inside_half_area": 23, "lens_linear_r": 30, "lens_no_refocus": 31, "sky_flip": 40}
+_CFLAGS = ["-O2", "-fPIC", "-std=c11", "-ffp-contract=off", "-fno-fast-math",
+           "-fexcess-precision=standard"]
+
+
+def build_planted(names, outdir):
+    """Builds liboracle_<name>.so for each planted mistake into outdir (in
+    parallel); returns {name: path}."""
+    import subprocess
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(name):
+        out = os.path.join(outdir, f"liboracle_{name}.so")
+        subprocess.run(["gcc", *_CFLAGS, f"-DO64_PLANT={PLANTS[name]}", "-shared", "-o", out,
+                        os.path.join(_HERE, "o64.c"), "-lm"], check=True)
+        return name, out
+
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        return dict(ex.map(one, names))
+
+
+class using_library:
+    """Context in which every oracle function calls the library at `path`."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        global _lib
+        self.saved = lib()
+        L = C.CDLL(self.path)
+        _declare(L)
+        _lib = L
+        return L
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
+        return False
+
+
 def _p(t):
     return C.POINTER(t)
 
@@ -120,6 +166,13 @@ def _declare(L):
         getattr(L, name).restype = d
         getattr(L, name).argtypes = [d]
     L.o64_tonemap_buffer.argtypes = [vp, C.c_int, C.c_int, d, d, C.c_int, vp]
+    L.o64_camera_sample.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, u32, vp, vp]
+    L.o64_sample_light.restype = C.c_int
+    L.o64_sample_light.argtypes = [_p(O64Scene), d, d, d, _p(u32), vp]
+    L.o64_light_pdf.restype = d
+    L.o64_light_pdf.argtypes = [_p(O64Scene), d, d, d, d, d, d, d, d, d, C.c_int]
+    L.o64_radiance.argtypes = [_p(O64Scene), d, d, d, d, d, d, _p(O64Rng), _p(O64TraceOpts), vp,
+                               vp]
 
 
 def _ptr(a):
@@ -462,8 +515,43 @@ def generate_ray(frame, sx, sy, lx=0.0, ly=0.0):
     return o, d
 
 
+def camera_sample(frame, width, height, x, y, frame_index):
+    """The camera ray of the path at pixel (x, y) in pass frame_index
+    (tracer.ts:373-383: jitter, lens draws, generateRay)."""
+    o = np.zeros(3)
+    d = np.zeros(3)
+    lib().o64_camera_sample(_ptr(np.ascontiguousarray(frame, np.float64)), width, height, x, y,
+                            frame_index & 0xFFFFFFFF, _ptr(o), _ptr(d))
+    return o, d
+
+
 # ---------------------------------------------------------------------------
 # tracer.ts
+
+def sample_light(ps, p, state):
+    """sampleLightInto (tracer.ts:93-180) at point p from RNG state `state`.
+    Returns (ok, [dir3, dist, pdf, emission3], state after the draws)."""
+    st = C.c_uint32(state & 0xFFFFFFFF)
+    out = np.zeros(8)
+    ok = lib().o64_sample_light(C.byref(ps.view), p[0], p[1], p[2], C.byref(st), _ptr(out))
+    return bool(ok), out, st.value
+
+
+def light_pdf(ps, p, h, hn, prim):
+    """lightPdfScalar (tracer.ts:61-90)."""
+    return lib().o64_light_pdf(C.byref(ps.view), p[0], p[1], p[2], h[0], h[1], h[2], hn[0],
+                               hn[1], hn[2], int(prim))
+
+
+def radiance(ps, o, d, state, max_bounces=8, russian_roulette=True, nee=True):
+    """radianceInto (tracer.ts:191-328) of one ray with RNG state `state`."""
+    rng = O64Rng(state & 0xFFFFFFFF)
+    out = np.zeros(3)
+    opts = _opts(max_bounces, russian_roulette, nee)
+    lib().o64_radiance(C.byref(ps.view), o[0], o[1], o[2], d[0], d[1], d[2], C.byref(rng),
+                       C.byref(opts), _ptr(out), None)
+    return out
+
 
 def _opts(max_bounces=8, russian_roulette=True, nee=True):
     return O64TraceOpts(int(max_bounces), int(bool(russian_roulette)), int(bool(nee)))

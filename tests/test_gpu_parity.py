@@ -189,6 +189,22 @@ def test_gpu_bvh_build_bit_exact(ct, O, case, bins):
     assert np.array_equal(go, oo)
 
 
+def test_gpu_bvh_build_hand_derived_fixtures(ct):
+    """The GPU builder against node arrays derived by hand from bvh.ts's rules
+    (tests/golden/make_bvh_fixtures.py): equal-cost splits, axis ties, the
+    partition's reordering and the median fallback."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "bvh_fixtures.json")) as f:
+        fixtures = json.load(f)
+    for fx in fixtures:
+        b = [np.array(fx[k]).reshape(-1, 3) for k in ("bmin", "bmax", "bcen")]
+        gn, gnc, go = ct.buildBvh(*b, fx["bins"])
+        assert gnc == fx["nodeCount"], fx["name"]
+        assert np.array_equal(gn.view(np.uint32), np.array(fx["nodes_u32"], np.uint32)), fx["name"]
+        assert list(go) == fx["order"], fx["name"]
+
+
 def test_gpu_bvh_build_empty(ct):
     z = np.zeros((0, 3))
     nodes, nc, order = ct.buildBvh(z, z, z)
