@@ -299,16 +299,73 @@ __device__ __forceinline__ double safe_inv(double d) { return 1 / (d == 0 ? 1e-2
 struct Hit {
   double t, u, v;
   int prim;
+  bool tie;  // a primitive was hit at exactly t (TIES_DETECT)
+};
+
+// How a closest-hit walk settles primitives hit at exactly the same t.
+enum TieMode : int {
+  TIES_FIRST = 0,    // the first tested stays (intersectClosestStackless: its DFS
+                     // order makes that the lower packed index)
+  TIES_DETECT = 1,   // mark the tie in Hit::tie; settle_ties re-walks the ray
+  TIES_ORDERED = 2,  // the one intersectClosest reaches first (ordered_first)
 };
 
 struct TraversalCounts {
   uint32_t nodes, tris, spheres, fallbacks;
 };
 
-// Leaf primitive loop shared by both closest-hit walks (intersect.ts:155-193,
+// Which of two primitives hit at the same t the reference's ordered walk
+// (intersect.ts:202-282) reaches first.  Its leaves are visited depth first,
+// at each interior node the child with the smaller slab entry first (tl <= tr:
+// left), and on an exact tie the primitive found first stays (strict '<'), so
+// the winner lies under the child taken first at the two primitives' lowest
+// common ancestor -- or, in one leaf, is the lower packed index.  Subtrees
+// hold contiguous packed ranges (DFS pre-order), so the ancestor is found by
+// descending with the first primitive of each right subtree (the leaf at the
+// end of its left spine).  Both children of that ancestor contain a hit at
+// the tied t, so both entries are finite and the reference's comparison is
+// made on them, with its own slab arithmetic.  Called only on exact ties.
+__device__ __attribute__((noinline)) bool ordered_first(const DevScene& S, int p, int q,
+                                                        double ox, double oy, double oz,
+                                                        double dx, double dy, double dz) {
+  const double ix = safe_inv(dx), iy = safe_inv(dy), iz = safe_inv(dz);
+  int c = 0;
+  for (;;) {
+    if (gload(S.nodes + 2 * (size_t)c + 1).w >= 0.0f) return p < q;  // one leaf
+    int L = c + 1;
+    float4 llo = gload(S.nodes + 2 * (size_t)L), lhi = gload(S.nodes + 2 * (size_t)L + 1);
+    int R = (int)llo.w;
+    int x = R;
+    while (gload(S.nodes + 2 * (size_t)x + 1).w < 0.0f) x++;  // left spine of R
+    int mid = (int)(gload(S.leafWords + x) >> 3);
+    bool pl = p < mid, ql = q < mid;
+    if (pl == ql) {
+      c = pl ? L : R;
+      continue;
+    }
+    float4 rlo = gload(S.nodes + 2 * (size_t)R), rhi = gload(S.nodes + 2 * (size_t)R + 1);
+    double tl = ray_aabb(ox, oy, oz, ix, iy, iz, llo, lhi, __builtin_inf());
+    double tr = ray_aabb(ox, oy, oz, ix, iy, iz, rlo, rhi, __builtin_inf());
+    return pl == (tl <= tr);
+  }
+}
+
+// The next binary64 above a positive t (+inf stays): hits up to and including
+// the current closest t come back from the primitive tests, so exact ties can
+// be resolved.
+__device__ __forceinline__ double tie_bound(double t) {
+  return t < __builtin_inf() ? __longlong_as_double(__double_as_longlong(t) + 1) : t;
+}
+
+// Leaf primitive loop shared by the closest-hit walks (intersect.ts:155-193,
 // 324-351): strict '<' keeps the first prim tested on exact ties; a sphere hit
-// leaves u, v as the last triangle hit set them.
-template <bool COUNT>
+// leaves u, v as the last triangle hit set them.  Walks that stand for
+// intersectClosest but do not share its visiting order (the 4-wide and parked
+// walks, the FMA-slab binary walk) run TIES_DETECT: a primitive hit at exactly
+// the current t only sets Hit::tie, and settle_ties then walks that ray again
+// with TIES_ORDERED, where such a tie goes to the primitive intersectClosest
+// reaches first (ordered_first) whatever the order of the tests.
+template <bool COUNT, int TIES = TIES_DETECT>
 __device__ __forceinline__ void test_leaf(const DevScene& S, int leafWord, double ox, double oy,
                                           double oz, double dx, double dy, double dz,
                                           double tMin, Hit& h, TraversalCounts& cnt) {
@@ -317,20 +374,28 @@ __device__ __forceinline__ void test_leaf(const DevScene& S, int leafWord, doubl
   int start = li >> 3;
   for (int p = start; p < start + count; p++) {
     PrimGeom g = load_prim(S, p);
+    const double tMax = TIES != TIES_FIRST ? tie_bound(h.t) : h.t;
+    double t, u = 0, v = 0;
+    bool hit;
     if (g.a.w == 0.0f) {  // PRIM_SPHERE
       if (COUNT) cnt.spheres++;
-      double t = ray_sphere(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, tMin, h.t);
-      if (t > 0) {
-        h.t = t;
-        h.prim = p;
-      }
+      t = ray_sphere(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, tMin, tMax);
+      hit = t > 0;
     } else {
       if (COUNT) cnt.tris++;
-      double t, u, v;
-      if (ray_triangle(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, g.b.y, g.b.z, g.c.x,
-                       g.c.y, g.c.z, tMin, h.t, t, u, v)) {
-        h.t = t;
-        h.prim = p;
+      hit = ray_triangle(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, g.b.y, g.b.z, g.c.x,
+                         g.c.y, g.c.z, tMin, tMax, t, u, v);
+    }
+    if (!hit) continue;
+    bool take = TIES == TIES_FIRST || t < h.t;
+    if (!take && h.prim >= 0) {  // t == h.t: an exact tie with the current hit
+      if (TIES == TIES_DETECT) h.tie = true;
+      if (TIES == TIES_ORDERED) take = ordered_first(S, p, h.prim, ox, oy, oz, dx, dy, dz);
+    }
+    if (take) {
+      h.t = t;
+      h.prim = p;
+      if (g.a.w != 0.0f) {
         h.u = u;
         h.v = v;
       }
@@ -340,7 +405,10 @@ __device__ __forceinline__ void test_leaf(const DevScene& S, int leafWord, doubl
 
 // Stackless skip-link closest hit (intersect.ts:290-365): node hit -> node+1,
 // miss -> skip.  h.t enters as tMax and leaves as the closest t (prim -1 = miss).
-template <bool COUNT>
+// TIES: TIES_FIRST for intersectClosestStackless itself, whose DFS order
+// gives ties to the lower packed index; TIES_DETECT as the fallback of a walk
+// that stands for intersectClosest; TIES_ORDERED inside settle_ties.
+template <bool COUNT, int TIES = TIES_DETECT>
 __device__ __forceinline__ void closest_stackless(const DevScene& S, double ox, double oy,
                                                   double oz, double dx, double dy, double dz,
                                                   double tMin, Hit& h, TraversalCounts& cnt) {
@@ -357,9 +425,27 @@ __device__ __forceinline__ void closest_stackless(const DevScene& S, double ox, 
       continue;
     }
     if (hi.w >= 0.0f)
-      test_leaf<COUNT>(S, (int)gload(S.leafWords + node), ox, oy, oz, dx, dy, dz, tMin, h, cnt);
+      test_leaf<COUNT, TIES>(S, (int)gload(S.leafWords + node), ox, oy, oz, dx, dy, dz, tMin, h,
+                             cnt);
     node++;
   }
+}
+
+// A ray of a TIES_DETECT walk that met an exact tie is walked again from
+// (tMin, tMax) with the ties settled in intersectClosest's order; rays
+// without one keep their hit.  Exact ties need shared edges or vertices hit
+// at bit-identical t, so the second walk is rare.
+template <bool COUNT>
+__device__ __forceinline__ void settle_ties(const DevScene& S, double ox, double oy, double oz,
+                                            double dx, double dy, double dz, double tMin,
+                                            double tMax, Hit& h, TraversalCounts& cnt) {
+  if (!h.tie) return;
+  h.t = tMax;
+  h.u = 0;
+  h.v = 0;
+  h.prim = -1;
+  h.tie = false;
+  closest_stackless<COUNT, TIES_ORDERED>(S, ox, oy, oz, dx, dy, dz, tMin, h, cnt);
 }
 
 // Stackless any-hit on (tMin, tMax) (intersect.ts:368-420).

@@ -573,7 +573,7 @@ __global__ void __launch_bounds__(BLOCK)
   double oix = 0, oiy = 0, oiz = 0;
   qw::QRay Q = qw::qray(0, 0, 0, 1, 1, 1);
   Hit h;
-  h.t = 0; h.u = 0; h.v = 0; h.prim = -1;
+  h.t = 0; h.u = 0; h.v = 0; h.prim = -1; h.tie = false;
   int cur = -1, sp = 0;
   Parked<PARK_SLOTS64> pk;
   pk.n = 0;
@@ -585,7 +585,7 @@ __global__ void __launch_bounds__(BLOCK)
         if (MULTI) S = jobs[P.jobOf[pid]].S;
         ox = P.ox[pid]; oy = P.oy[pid]; oz = P.oz[pid];
         dx = P.dx[pid]; dy = P.dy[pid]; dz = P.dz[pid];
-        h.t = __builtin_inf(); h.u = 0; h.v = 0; h.prim = -1;
+        h.t = __builtin_inf(); h.u = 0; h.v = 0; h.prim = -1; h.tie = false;
         have = true;
         if (WALK == CT_WALK_STACKLESS) {
           closest_stackless<COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
@@ -633,6 +633,7 @@ __global__ void __launch_bounds__(BLOCK)
         }
       }
       if (cur == -1) {
+        settle_ties<COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, __builtin_inf(), h, cnt);
         P.hitPrim[pid] = h.prim;
         P.hitT[pid] = h.t;
         have = false;

@@ -41,16 +41,20 @@ k_closest(DevScene S, const double* __restrict__ rays, uint32_t n, double tMin,
     double ox, oy, oz, dx, dy, dz;
     load_ray(rays, i, ox, oy, oz, dx, dy, dz);
     Hit h;
-    h.t = tMax ? tMax[i] : __builtin_inf();
+    const double t0 = tMax ? tMax[i] : __builtin_inf();
+    h.t = t0;
     h.u = 0;
     h.v = 0;
     h.prim = -1;
-    if (WALK == CT_WALK_ORDERED)
+    h.tie = false;
+    if (WALK == CT_WALK_ORDERED) {
       closest_wave<ORDERED_STACK, COUNT>(S, ox, oy, oz, dx, dy, dz, tMin, h,
                                          sNode + threadIdx.x, sEntry + threadIdx.x,
                                          QUERY_BLOCK, cnt);
-    else
-      closest_stackless<COUNT>(S, ox, oy, oz, dx, dy, dz, tMin, h, cnt);
+      settle_ties<COUNT>(S, ox, oy, oz, dx, dy, dz, tMin, t0, h, cnt);
+    } else {  // intersectClosestStackless itself: ties to the lower packed index
+      closest_stackless<COUNT, TIES_FIRST>(S, ox, oy, oz, dx, dy, dz, tMin, h, cnt);
+    }
     if (prim) prim[i] = h.prim;
     if (tOut) tOut[i] = h.prim < 0 ? __builtin_inf() : h.t;
     if (uOut) uOut[i] = h.u;
@@ -102,7 +106,7 @@ k_closest_wide(DevScene S, const double* __restrict__ rays, uint32_t n, double t
   double ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0;
   qw::QRay Q = qw::qray(0, 0, 0, 1, 1, 1);
   Hit h;
-  h.t = 0; h.u = 0; h.v = 0; h.prim = -1;
+  h.t = 0; h.u = 0; h.v = 0; h.prim = -1; h.tie = false;
   int cur = -1, sp = 0;
   Parked<PARK_SLOTS> pk;
   pk.n = 0;
@@ -112,7 +116,7 @@ k_closest_wide(DevScene S, const double* __restrict__ rays, uint32_t n, double t
       next += stride;
       load_ray(rays, i, ox, oy, oz, dx, dy, dz);
       h.t = tMax ? tMax[i] : __builtin_inf();
-      h.u = 0; h.v = 0; h.prim = -1;
+      h.u = 0; h.v = 0; h.prim = -1; h.tie = false;
       Q = qw::qray64(S, ox, oy, oz, dx, dy, dz);
       sp = 0;
       pk.n = 0;
@@ -147,6 +151,8 @@ k_closest_wide(DevScene S, const double* __restrict__ rays, uint32_t n, double t
         sp = 0;
       }
       if (cur == -1) {
+        settle_ties<COUNT>(S, ox, oy, oz, dx, dy, dz, tMin, tMax ? tMax[i] : __builtin_inf(), h,
+                           cnt);
         if (prim) prim[i] = h.prim;
         if (tOut) tOut[i] = h.prim < 0 ? __builtin_inf() : h.t;
         if (uOut) uOut[i] = h.u;

@@ -148,6 +148,45 @@ def test_wide_walk_adversarial_rays(ct, O, off):
     assert (o["prim"] >= 0).mean() > 0.7
 
 
+def _tie_rays(ps, seed, per):
+    """Rays aimed at the vertices, edge midpoints and an edge quarter point of
+    every triangle: where meshes and quads share edges and vertices, several
+    primitives are hit at bit-identical t."""
+    rs = np.random.default_rng(seed)
+    prims = ps.prims.reshape(-1, 16)
+    rays = []
+    for p in np.flatnonzero(prims[:, 3] == 1):
+        a, b, c = (prims[p, k:k + 3].astype(np.float64) for k in (0, 4, 8))
+        for tgt in (a, b, c, 0.5 * (a + b), 0.5 * (b + c), 0.5 * (a + c), a + 0.25 * (b - a)):
+            for _ in range(per):
+                o = tgt + rs.uniform(-6, 6, 3)
+                d = tgt - o
+                rays.append(np.concatenate([o, d / np.linalg.norm(d)]))
+    return np.array(rays)
+
+
+@pytest.mark.parametrize("preset", ["cornell", "mirror"])
+def test_exact_ties_follow_the_reference_walks(ct, O, preset):
+    """On exact-t ties the ordered walk (intersect.ts:202-282) keeps the
+    primitive of the leaf it reaches first (nearer child first, tl <= tr to the
+    left) and the stackless walk (intersect.ts:290-365) the lowest packed
+    index.  The GPU's ordered and 4-wide walks must return the former, its
+    stackless walk the latter."""
+    sc = ct.presetById(preset)["scene"]
+    ps = O.pack_scene(sc)
+    rays = _tie_rays(ps, 3, 12 if preset == "cornell" else 2)
+    ordered = O.closest_batch(ps, rays, walk=O.WALK_ORDERED)
+    stackless = O.closest_batch(ps, rays, walk=O.WALK_STACKLESS)
+    ties = int((ordered["prim"] != stackless["prim"]).sum())
+    assert ties > 0  # the ray set does produce ties
+    for walk, ref in ((ct.WALK_ORDERED, ordered), (ct.WALK_WIDE, ordered),
+                      (ct.WALK_STACKLESS, stackless)):
+        g = ct.closest_hit(ps, rays, walk=walk)
+        bad = np.flatnonzero(g["prim"] != ref["prim"])
+        assert len(bad) == 0, (walk, ties, bad[:8], g["prim"][bad[:8]], ref["prim"][bad[:8]])
+        assert np.array_equal(g["t"], ref["t"])
+
+
 def test_empty_and_coincident(ct, O):
     empty = O.pack_scene({"objects": [], "sky": {"kind": "none"}})
     g = ct.closest_hit(empty, np.array([[0, 0, 0, 0, 0, 1.0]]))
