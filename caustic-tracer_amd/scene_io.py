@@ -38,8 +38,34 @@ def _fmt(x):
     return np.format_float_positional(float(x), trim="-") if "e" in r else r
 
 
+def _js_string(v):
+    """String(v) as JS prints a JSON value (null, true, numbers, arrays, objects)."""
+    if v is None:
+        return "null"
+    if isinstance(v, bool):
+        return "true" if v else "false"
+    if isinstance(v, (int, float)):
+        try:
+            return _fmt(float(v))
+        except OverflowError:
+            return "Infinity" if v > 0 else "-Infinity"
+    if isinstance(v, list):
+        return ",".join("" if x is None else _js_string(x) for x in v)
+    if isinstance(v, dict):
+        return "[object Object]"
+    return str(v)
+
+
 def _num(v, path, lo=-math.inf, hi=math.inf):
-    if isinstance(v, bool) or not isinstance(v, (int, float)) or not math.isfinite(v):
+    # JSON numbers are binary64 in JS: an integer literal beyond the double
+    # range is Infinity there, so it fails as non-finite here too
+    if isinstance(v, bool) or not isinstance(v, (int, float)):
+        _fail(f"{path} must be a finite number")
+    try:
+        v = float(v)
+    except OverflowError:
+        _fail(f"{path} must be a finite number")
+    if not math.isfinite(v):
         _fail(f"{path} must be a finite number")
     if v < lo or v > hi:
         _fail(f"{path} must be within [{_fmt(lo)}, {_fmt(hi)}]")
@@ -135,8 +161,11 @@ def _camera(v, path):
             "focusDistance": _num(v.get("focusDistance"), f"{path}.focusDistance", 1e-3, 1e6)}
 
 
+_MISSING = object()  # a key that is absent, as opposed to present with null
+
+
 def _sky(v, path):
-    if v is None:
+    if v is _MISSING:
         return {"kind": "none"}
     if not _is_obj(v):
         _fail(f"{path} must be an object")
@@ -154,7 +183,7 @@ def parseScene(doc):
     if not _is_obj(doc):
         _fail("scene must be a JSON object")
     if "format" in doc and doc["format"] != SCENE_FORMAT:
-        _fail(f'unknown format "{doc["format"]}"')
+        _fail(f'unknown format "{_js_string(doc["format"])}"')
     name = "Imported scene" if "name" not in doc else _str(doc["name"], "name")
     raw = doc.get("objects")
     if not isinstance(raw, list):
@@ -175,7 +204,7 @@ def parseScene(doc):
                         "shape": _shape(r.get("shape"), f"objects[{i}].shape"),
                         "material": _material(r.get("material"), f"objects[{i}].material")})
     return {"name": name, "objects": objects, "camera": _camera(doc.get("camera"), "camera"),
-            "sky": _sky(doc.get("sky"), "sky")}
+            "sky": _sky(doc.get("sky", _MISSING), "sky")}
 
 
 def parseSceneJSON(text):

@@ -461,6 +461,32 @@ def test_presets_gpu_bvh_and_render(ct, O, preset):
     assert err.max() <= REL_TOL
 
 
+@pytest.mark.parametrize("preset", ["cornell", "glass", "night", "dof", "mirror"])
+def test_ingested_scene_renders_like_the_oracle(ct, O, preset):
+    """Scene ingestion end to end: the preset written as a caustic-scene
+    document (sceneToJSON, serialize.ts:128-139), parsed back (parseSceneJSON,
+    serialize.ts:172-179), packed with the GPU builder and rendered on the GPU
+    -- arrays identical to the oracle's pack of the original preset, image
+    within 1e-4 of the oracle's render of it."""
+    sc = next(p for p in ct.presets.SCENE_PRESETS if p["id"] == preset)["scene"]
+    doc = ct.sceneToJSON(sc)
+    parsed = ct.parseSceneJSON(doc)
+    g_ps = ct.packScene(parsed)
+    o_ps = O.pack_scene(sc)
+    for k in ("prims", "nodes", "mats"):
+        assert np.array_equal(getattr(g_ps, k).view(np.uint32), getattr(o_ps, k).view(np.uint32)), k
+    assert np.array_equal(g_ps.lights, o_ps.lights)
+    assert np.array_equal(g_ps.primObject, o_ps.primObject)
+    w, h, spp = 40, 30, 2
+    fr = ct.cameraFrame(parsed["camera"], w / h)
+    assert np.array_equal(fr, O.camera_frame(sc["camera"], w / h))
+    g = ct.renderImage(g_ps, fr, {"width": w, "height": h, "spp": spp, "seed": 11,
+                                  "maxBounces": 6})
+    o = O.render_image(o_ps, fr, w, h, spp, seed=11, max_bounces=6)
+    assert np.isfinite(g).all() and g.max() > 0
+    assert rel_err(g, o).max() <= REL_TOL
+
+
 @pytest.mark.parametrize("w,h,spp,depth", [(1, 1, 1, 0), (3, 1, 2, 1), (1, 5, 3, 0), (7, 3, 1, 2),
                                            (2, 2, 0, 3)])
 def test_render_small_shapes_and_depths(ct, O, w, h, spp, depth):

@@ -103,6 +103,14 @@ def json_mod_dump(sc):
     (lambda d: d["objects"][0].__setitem__("shape", {"kind": "mesh", "vertices": [{"x": 0, "y": 0, "z": 0}], "indices": [0, 0]}),
      "objects[0].shape.indices length must be a multiple of 3"),
     (lambda d: d.__setitem__("sky", {"kind": "hdr"}), "sky.kind must be none | gradient"),
+    # a present null is not a missing key (serialize.ts:118: only undefined means none)
+    (lambda d: d.__setitem__("sky", None), "sky must be an object"),
+    (lambda d: d.__setitem__("format", None), 'unknown format "null"'),
+    (lambda d: d.__setitem__("format", 2), 'unknown format "2"'),
+    (lambda d: d.__setitem__("format", [1, "a"]), 'unknown format "1,a"'),
+    (lambda d: d["objects"][0].__setitem__("name", None), "objects[0].name must be a string"),
+    (lambda d: d["objects"][0]["material"].__setitem__("emission", None),
+     "objects[0].material.emission must be an object {r, g, b}"),
 ])
 def test_parse_errors(ct, mutate, msg):
     d = _doc()
@@ -124,3 +132,18 @@ def test_parse_defaults_and_dedup(ct):
     assert sc["objects"][2]["name"] == "Object 3"
     with pytest.raises(ct.SceneParseError):
         ct.parseSceneJSON("{not json")
+
+
+def test_parse_number_edges(ct):
+    """JSON numbers are binary64 in the reference: an integer literal too large
+    for a double is Infinity (not finite), 1e999 likewise, and a large exact
+    integer is range-checked after rounding."""
+    text = scenes_json()
+    for lit in ("1" + "0" * 400, "-" + "9" * 330, "1e999"):
+        d = text.replace('"radius":', '"radius":' + lit + ', "_r":', 1)
+        with pytest.raises(ct.SceneParseError) as e:
+            ct.parseSceneJSON(d)
+        assert str(e.value).endswith("shape.radius must be a finite number"), str(e.value)
+    d = json.loads(text)
+    d["camera"]["yaw"] = 2 ** 70
+    assert ct.parseScene(d)["camera"]["yaw"] == float(2 ** 70)
