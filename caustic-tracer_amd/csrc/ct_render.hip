@@ -28,6 +28,7 @@
 
 #include "ct_engine.h"
 #include "ct_qwalk.h"
+#include "ct_jsmath.h"
 #include "ct_wavefront.h"
 
 namespace ct {
@@ -61,8 +62,8 @@ k_generate(PathSoA P, const JobDev* __restrict__ jobs, int njobs, size_t base, s
     if (F.lensRadius > 0) {
       double r = sqrt(rng.next());
       double theta = TWO_PI * rng.next();
-      lx = r * cos(theta);
-      ly = r * sin(theta);
+      lx = r * jsm::cos(theta);
+      ly = r * jsm::sin(theta);
     }
     double px = (2 * sx - 1) * F.tanHalfFov * F.aspect;
     double py = (2 * sy - 1) * F.tanHalfFov;
@@ -143,8 +144,8 @@ __device__ __forceinline__ void unit_vector(double u1, double u2, double* out) {
   double z = 1 - 2 * u1;
   double r = sqrt(z * z < 1 ? 1 - z * z : 0);
   double phi = TWO_PI * u2;
-  out[0] = r * cos(phi);
-  out[1] = r * sin(phi);
+  out[0] = r * jsm::cos(phi);
+  out[1] = r * jsm::sin(phi);
   out[2] = z;
 }
 
@@ -175,7 +176,7 @@ __device__ bool sample_light(const DevScene& S, double px, double py, double pz,
       double ax = cx * invD, ay = cy * invD, az = cz * invD;
       double b[6];
       basis(ax, ay, az, b);
-      double sx = sinT * cos(phi), sy = sinT * sin(phi);
+      double sx = sinT * jsm::cos(phi), sy = sinT * jsm::sin(phi);
       double wx = b[0] * sx + b[3] * sy + ax * cosT;
       double wy = b[1] * sx + b[4] * sy + ay * cosT;
       double wz = b[2] * sx + b[5] * sy + az * cosT;
@@ -375,8 +376,8 @@ k_shade(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__
           double u2 = rng.next();
           double r = sqrt(u1);
           double phi = TWO_PI * u2;
-          double lx = r * cos(phi);
-          double ly = r * sin(phi);
+          double lx = r * jsm::cos(phi);
+          double ly = r * jsm::sin(phi);
           double lz = sqrt(u1 < 1 ? 1 - u1 : 0);
           double b[6];
           basis(sx, sy, sz, b);
@@ -404,7 +405,7 @@ k_shade(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__
           if (rough > 0) {
             double uv[3];
             unit_vector(u1, u2, uv);
-            double rad = cbrt(u3) * rough;
+            double rad = jsm::cbrt(u3) * rough;
             rx += uv[0] * rad;
             ry += uv[1] * rad;
             rz += uv[2] * rad;

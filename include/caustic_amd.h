@@ -155,9 +155,9 @@ int ct_scene_create_device(const float *d_prims, uint32_t primCount, const float
 /* Layout extension for scenes of 2^21 primitives or more, where the float32
  * leaf words of pack.ts (start*8 + count, bvh.ts:60-62) stop being exact:
  * h_leafWords holds the exact word of every node as uint32 (0xffffffff on
- * interior nodes; ct_bvh_build_ex writes it).  The 4-wide walks read only
- * these; the binary walks of the reference layout return CT_ELIMIT on a scene
- * whose float32 words are inexact. */
+ * interior nodes; ct_bvh_build_ex writes it).  Every walk decodes leaves from
+ * these words (the float32 ones only tell leaves from interior nodes), so all
+ * walks return the reference's hits on such scenes as well. */
 int ct_scene_create_ex(const float *h_prims, uint32_t primCount, const float *h_nodes,
                        uint32_t nodeCount, const float *h_mats, uint32_t matCount,
                        const int32_t *h_lights, uint32_t lightCount,

@@ -5,6 +5,7 @@
  * writes it; every Float32Array store is a (float) conversion.
  */
 #include "o64.h"
+#include "jsmath.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -21,6 +22,10 @@ static const double PI_D = 3.141592653589793;       /* Math.PI */
 #define TWO_PI (2 * PI_D)                            /* materials.ts:11, tracer.ts:37 */
 #define INV_PI (1 / PI_D)                            /* materials.ts:10 */
 #define SHADOW_SHORTEN (1 - 1e-3)                    /* tracer.ts:36 */
+
+/* Math.sin/cos/cbrt in draws (materials.ts, tracer.ts) are the JS engine's:
+ * js_sin/js_cos/js_cbrt (jsmath.h).  The camera frame (camera.ts:25-51, host
+ * side on both the oracle and the product) keeps the C library's. */
 
 /* Planted mistakes, for the tests that show the oracle's pins have teeth: a
  * build with -DO64_PLANT=k carries mistake k (the numbers are listed in
@@ -194,6 +199,7 @@ typedef struct {
   const double *min, *max, *cen;
   uint32_t *order;
   float *nodes;
+  uint32_t *words; /* exact leaf words, or NULL */
   int nodeCount;
   int bins;
   /* scratch (bvh.ts:79-83) */
@@ -381,26 +387,30 @@ static void build_node(Builder *B, int start, int end) {
   if (count <= (PLANTED(O64_PLANT_LEAF3) ? 3 : MAX_LEAF_PRIMS)) {
     nd[3] = (float)(nodeIndex + 1);
     nd[7] = (float)((double)start * 8 + count); /* encodeLeaf bvh.ts:60-62 */
+    if (B->words) B->words[nodeIndex] = (uint32_t)start * 8u + (uint32_t)count;
     return;
   }
   int mid = sah_split(B, start, end, cMinX, cMinY, cMinZ, cMaxX, cMaxY, cMaxZ);
   if (mid <= start || mid >= end)
     mid = median_split(B, start, end, cMinX, cMinY, cMinZ, cMaxX, cMaxY, cMaxZ);
   nd[7] = -1;
+  if (B->words) B->words[nodeIndex] = 0xffffffffu;
   build_node(B, start, mid);
   build_node(B, mid, end);
   B->nodes[(size_t)nodeIndex * NODE_STRIDE + 3] = (float)B->nodeCount;
 }
 
 /* bvh.ts:92-109, 299-301 */
-int o64_build_bvh(int n, const double *bmin, const double *bmax, const double *bcen, int bins,
-                  float *nodes, uint32_t *order) {
+int o64_build_bvh_ex(int n, const double *bmin, const double *bmax, const double *bcen, int bins,
+                     float *nodes, uint32_t *order, uint32_t *leafWords) {
   Builder B;
   memset(&B, 0, sizeof B);
   B.min = bmin; B.max = bmax; B.cen = bcen;
   B.order = order;
   B.nodes = nodes;
+  B.words = leafWords;
   B.bins = bins;
+  if (n == 0 && leafWords) leafWords[0] = 0;
   int maxNodes = n > 0 ? 2 * n - 1 : 1;
   memset(nodes, 0, sizeof(float) * (size_t)maxNodes * NODE_STRIDE);
   for (int i = 0; i < n; i++) order[i] = (uint32_t)i;
@@ -421,6 +431,11 @@ int o64_build_bvh(int n, const double *bmin, const double *bmax, const double *b
   return B.nodeCount;
 }
 
+int o64_build_bvh(int n, const double *bmin, const double *bmax, const double *bcen, int bins,
+                  float *nodes, uint32_t *order) {
+  return o64_build_bvh_ex(n, bmin, bmax, bcen, bins, nodes, order, NULL);
+}
+
 int o64_partition_flags(int n, const uint8_t *good, int32_t *perm) {
   /* perm[i] = original index found at position i after bvh.ts:262-278 */
   for (int i = 0; i < n; i++) perm[i] = i;
@@ -438,10 +453,11 @@ int o64_partition_flags(int n, const uint8_t *good, int32_t *perm) {
   return lo;
 }
 
-/* pack.ts:195-223 */
-int o64_pack_scene(int n, const int32_t *kind, const double *geom, const int32_t *obj,
-                   const uint8_t *emits, int bins, float *prims, uint32_t *primObject,
-                   float *nodes, uint32_t *order, int32_t *lights, int *lightCount) {
+/* pack.ts:195-223; leafWords (or NULL) receives the exact leaf words. */
+int o64_pack_scene_ex(int n, const int32_t *kind, const double *geom, const int32_t *obj,
+                      const uint8_t *emits, int bins, float *prims, uint32_t *primObject,
+                      float *nodes, uint32_t *order, int32_t *lights, int *lightCount,
+                      uint32_t *leafWords) {
   int cap = n > 0 ? n : 1;
   float *raw = (float *)calloc((size_t)cap * PRIM_STRIDE, sizeof(float));
   double *bmin = (double *)calloc((size_t)cap * 3, sizeof(double));
@@ -449,7 +465,7 @@ int o64_pack_scene(int n, const int32_t *kind, const double *geom, const int32_t
   double *bcen = (double *)calloc((size_t)cap * 3, sizeof(double));
   write_prims(n, kind, geom, obj, raw);
   o64_prim_bounds(n, kind, geom, bmin, bmax, bcen);
-  int nodeCount = o64_build_bvh(n, bmin, bmax, bcen, bins, nodes, order);
+  int nodeCount = o64_build_bvh_ex(n, bmin, bmax, bcen, bins, nodes, order, leafWords);
   memset(prims, 0, sizeof(float) * (size_t)cap * PRIM_STRIDE);
   memset(primObject, 0, sizeof(uint32_t) * (size_t)cap);
   for (int i = 0; i < n; i++) {
@@ -466,9 +482,17 @@ int o64_pack_scene(int n, const int32_t *kind, const double *geom, const int32_t
   return nodeCount;
 }
 
+int o64_pack_scene(int n, const int32_t *kind, const double *geom, const int32_t *obj,
+                   const uint8_t *emits, int bins, float *prims, uint32_t *primObject,
+                   float *nodes, uint32_t *order, int32_t *lights, int *lightCount) {
+  return o64_pack_scene_ex(n, kind, geom, obj, emits, bins, prims, primObject, nodes, order,
+                           lights, lightCount, NULL);
+}
+
 /* bvh.ts:319-381 */
 typedef struct {
   const float *n;
+  const uint32_t *words; /* exact leaf words, or NULL */
   int nodeCount;
   const uint32_t *order;
   int primCount;
@@ -484,7 +508,8 @@ static int vwalk(VState *S, int index, int d) {
   }
   const float *nb = S->n + (size_t)index * NODE_STRIDE;
   double skip = nb[3];
-  double leaf = nb[7];
+  double leaf = S->words ? (S->words[index] == 0xffffffffu ? -1.0 : (double)S->words[index])
+                         : nb[7];
   if (leaf >= 0) {
     double start = floor(leaf / 8);
     double count = leaf - floor(leaf / 8) * 8;
@@ -520,11 +545,12 @@ static int vwalk(VState *S, int index, int d) {
   return afterRight;
 }
 
-int o64_validate_bvh(const float *nodes, int nodeCount, const uint32_t *order, int primCount,
-                     int32_t *stats) {
+int o64_validate_bvh_ex(const float *nodes, int nodeCount, const uint32_t *order, int primCount,
+                        const uint32_t *leafWords, int32_t *stats) {
   VState S;
   memset(&S, 0, sizeof S);
   S.n = nodes;
+  S.words = leafWords;
   S.nodeCount = nodeCount;
   S.order = order;
   S.primCount = primCount;
@@ -543,6 +569,11 @@ int o64_validate_bvh(const float *nodes, int nodeCount, const uint32_t *order, i
     stats[2] = S.depth;
   }
   return S.errors;
+}
+
+int o64_validate_bvh(const float *nodes, int nodeCount, const uint32_t *order, int primCount,
+                     int32_t *stats) {
+  return o64_validate_bvh_ex(nodes, nodeCount, order, primCount, NULL, stats);
 }
 
 /* ======================================================================
@@ -631,6 +662,14 @@ static double safe_inv(double d) { return 1 / (d == 0 ? 1e-20 : d); }
 #define NODEF(i, k) ((double)nodes[(size_t)(i) * NODE_STRIDE + (k)])
 #define PRIMF(p, k) ((double)prims[(size_t)(p) * PRIM_STRIDE + (k)])
 
+/* A node's leaf word: the float32 one of the reference layout, or the exact
+ * one of the layout extension (interior nodes: -1 either way). */
+static double node_leaf(const O64Scene *s, int node) {
+  if (s->leafWords)
+    return s->leafWords[node] == 0xffffffffu ? -1.0 : (double)s->leafWords[node];
+  return (double)s->nodes[(size_t)node * NODE_STRIDE + 7];
+}
+
 /* Shared leaf test: intersect.ts:155-193 (ordered) and 328-350 (stackless). */
 static double test_leaf(const float *prims, double leaf, const double *o, const double *d,
                         double tMin, double closest, int32_t *bestPrim, double *bestU,
@@ -691,7 +730,7 @@ int o64_intersect_closest(const O64Scene *s, const double *o, const double *d, d
   int sp = 0;
   int node = 0;
   for (;;) {
-    double leaf = NODEF(node, 7);
+    double leaf = node_leaf(s, node);
     if (leaf >= 0) {
       if (cnt) cnt->leaves++;
       closest = test_leaf(prims, leaf, o, d, tMin, closest, &bestPrim, &bestU, &bestV, cnt);
@@ -764,7 +803,7 @@ int o64_intersect_stackless(const O64Scene *s, const double *o, const double *d,
       node = (int)NODEF(node, 3);
       continue;
     }
-    double leaf = NODEF(node, 7);
+    double leaf = node_leaf(s, node);
     if (leaf >= 0) {
       if (cnt) cnt->leaves++;
       closest = test_leaf(prims, leaf, o, d, tMin, closest, &bestPrim, &bestU, &bestV, cnt);
@@ -799,7 +838,7 @@ int o64_occluded(const O64Scene *s, const double *o, const double *d, double tMi
       node = (int)NODEF(node, 3);
       continue;
     }
-    double leaf = NODEF(node, 7);
+    double leaf = node_leaf(s, node);
     if (leaf >= 0) {
       if (cnt) cnt->leaves++;
       int li = (int)leaf;
@@ -924,8 +963,8 @@ void o64_basis(double nx, double ny, double nz, double *out) {
 void o64_cosine_hemisphere(double nx, double ny, double nz, double u1, double u2, double *out) {
   double r = sqrt(u1);
   double phi = TWO_PI * u2;
-  double x = r * cos(phi);
-  double y = r * sin(phi);
+  double x = r * js_cos(phi);
+  double y = r * js_sin(phi);
   double z = sqrt(u1 < 1 ? 1 - u1 : 0);
   double b[6];
   o64_basis(nx, ny, nz, b);
@@ -943,8 +982,8 @@ void o64_unit_vector(double u1, double u2, double *out) {
   double z = 1 - 2 * u1;
   double r = sqrt(z * z < 1 ? 1 - z * z : 0);
   double phi = TWO_PI * u2;
-  out[0] = r * cos(phi);
-  out[1] = r * sin(phi);
+  out[0] = r * js_cos(phi);
+  out[1] = r * js_sin(phi);
   out[2] = z;
 }
 
@@ -975,7 +1014,7 @@ void o64_metal(double dx, double dy, double dz, double nx, double ny, double nz,
   double rz = dz - k * nz;
   if (roughness > 0) {
     o64_unit_vector(u1, u2, out);
-    double rad = cbrt(u3) * roughness;
+    double rad = js_cbrt(u3) * roughness;
     rx += out[0] * rad;
     ry += out[1] * rad;
     rz += out[2] * rad;
@@ -1150,7 +1189,7 @@ static int sample_light(const O64Scene *s, double px, double py, double pz, O64R
       double ax = cx * invD, ay = cy * invD, az = cz * invD;
       double b[6];
       o64_basis(ax, ay, az, b);
-      double sx = sinT * cos(phi), sy = sinT * sin(phi);
+      double sx = sinT * js_cos(phi), sy = sinT * js_sin(phi);
       double wx = b[0] * sx + b[3] * sy + ax * cosT;
       double wy = b[1] * sx + b[4] * sy + ay * cosT;
       double wz = b[2] * sx + b[5] * sy + az * cosT;
@@ -1354,6 +1393,12 @@ void o64_radiance(const O64Scene *s, double ox, double oy, double oz, double dx,
   out[2] = Lb;
 }
 
+/* The JS engine's Math.sin/cos/cbrt as the oracle computes them (jsmath.h),
+ * exported for the tests that pin them against the engine's own outputs. */
+double o64_js_sin(double x) { return js_sin(x); }
+double o64_js_cos(double x) { return js_cos(x); }
+double o64_js_cbrt(double x) { return js_cbrt(x); }
+
 /* The camera ray of one path (tracer.ts:373-383): pixel jitter, then the lens
  * draws when lensRadius > 0, then generateRay.  For the lens and sky pins. */
 void o64_camera_sample(const double *frame, int width, int height, int x, int y,
@@ -1367,8 +1412,8 @@ void o64_camera_sample(const double *frame, int width, int height, int x, int y,
   if (frame[14] > 0) {
     double r = PLANTED(O64_PLANT_LENS_LINEAR_R) ? o64_rng_next(&rng) : sqrt(o64_rng_next(&rng));
     double theta = TWO_PI * o64_rng_next(&rng);
-    lx = r * cos(theta);
-    ly = r * sin(theta);
+    lx = r * js_cos(theta);
+    ly = r * js_sin(theta);
   }
   o64_generate_ray(frame, sx, sy, lx, ly, o3, d3);
 }
@@ -1401,8 +1446,8 @@ static void pixel_sample(const O64Scene *s, const double *frame, int width, int 
   if (frame[14] > 0) {
     double r = PLANTED(O64_PLANT_LENS_LINEAR_R) ? o64_rng_next(&rng) : sqrt(o64_rng_next(&rng));
     double theta = TWO_PI * o64_rng_next(&rng);
-    lx = r * cos(theta);
-    ly = r * sin(theta);
+    lx = r * js_cos(theta);
+    ly = r * js_sin(theta);
   }
   double o[3], d[3];
   o64_generate_ray(frame, sx, sy, lx, ly, o, d);

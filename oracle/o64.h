@@ -84,6 +84,18 @@ int o64_build_bvh(int n, const double *bmin, const double *bmax, const double *b
 int o64_validate_bvh(const float *nodes, int nodeCount, const uint32_t *order, int primCount,
                      int32_t *stats);
 
+/* o64_pack_scene plus the exact leaf words (see O64Scene). */
+int o64_pack_scene_ex(int n, const int32_t *kind, const double *geom, const int32_t *obj,
+                      const uint8_t *emits, int bins, float *prims, uint32_t *primObject,
+                      float *nodes, uint32_t *order, int32_t *lights, int *lightCount,
+                      uint32_t *leafWords);
+/* o64_build_bvh plus the exact leaf word of every node (see O64Scene), and
+ * validateBvh reading those words instead of the float32 ones. */
+int o64_build_bvh_ex(int n, const double *bmin, const double *bmax, const double *bcen, int bins,
+                     float *nodes, uint32_t *order, uint32_t *leafWords);
+int o64_validate_bvh_ex(const float *nodes, int nodeCount, const uint32_t *order, int primCount,
+                        const uint32_t *leafWords, int32_t *stats);
+
 /* Two-pointer partition exactly as bvh.ts:262-278, on a flag vector (tests). */
 int o64_partition_flags(int n, const uint8_t *good, int32_t *perm);
 
@@ -96,6 +108,11 @@ typedef struct {
   const uint32_t *primObject;
   int skyKind;                 /* 0 none, 1 gradient */
   double zenith[3], horizon[3], skyStrength;
+  /* Layout extension: the exact leaf word start*8+count of every node
+   * (0xffffffff on interior nodes), for scenes of 2^21 primitives or more
+   * where the float32 words of bvh.ts:60-62 stop being exact.  NULL: the
+   * walks read the float32 words, as the reference does. */
+  const uint32_t *leafWords;
 } O64Scene;
 
 /* ---- intersect.ts ----------------------------------------------------- */
@@ -195,6 +212,9 @@ void o64_tonemap_buffer(const float *accum, int width, int height, double sample
 #define O64_PLANT_LENS_NO_REFOCUS 31    /* lens rays aimed from the lens centre */
 #define O64_PLANT_SKY_FLIP 40           /* sky gradient upside down */
 
+double o64_js_sin(double x);
+double o64_js_cos(double x);
+double o64_js_cbrt(double x);
 /* Test exports: the camera ray of one path (tracer.ts:373-383), and light
  * sampling from a given RNG state (tracer.ts:61-180). */
 void o64_camera_sample(const double *frame, int width, int height, int x, int y,

@@ -38,6 +38,7 @@ _lib = None
 
 PRIM_STRIDE = 16
 MAT_STRIDE = 16
+EXACT_LEAF_LIMIT = 1 << 21  # float32 leaf words start*8+count are exact below this
 NODE_STRIDE = 8
 T_MIN = 1e-4
 
@@ -111,6 +112,7 @@ class O64Scene(C.Structure):
         ("primObject", _p(C.c_uint32)),
         ("skyKind", C.c_int),
         ("zenith", C.c_double * 3), ("horizon", C.c_double * 3), ("skyStrength", C.c_double),
+        ("leafWords", _p(C.c_uint32)),
     ]
 
 
@@ -134,10 +136,17 @@ def _declare(L):
     L.o64_pack_scene.restype = C.c_int
     L.o64_pack_scene.argtypes = [C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp, vp, _p(C.c_int)]
     L.o64_prim_bounds.argtypes = [C.c_int, vp, vp, vp, vp, vp]
+    L.o64_pack_scene_ex.restype = C.c_int
+    L.o64_pack_scene_ex.argtypes = [C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp, vp,
+                                    _p(C.c_int), vp]
     L.o64_build_bvh.restype = C.c_int
     L.o64_build_bvh.argtypes = [C.c_int, vp, vp, vp, C.c_int, vp, vp]
+    L.o64_build_bvh_ex.restype = C.c_int
+    L.o64_build_bvh_ex.argtypes = [C.c_int, vp, vp, vp, C.c_int, vp, vp, vp]
     L.o64_validate_bvh.restype = C.c_int
     L.o64_validate_bvh.argtypes = [vp, C.c_int, vp, C.c_int, vp]
+    L.o64_validate_bvh_ex.restype = C.c_int
+    L.o64_validate_bvh_ex.argtypes = [vp, C.c_int, vp, C.c_int, vp, vp]
     L.o64_partition_flags.restype = C.c_int
     L.o64_partition_flags.argtypes = [C.c_int, vp, vp]
     L.o64_closest_batch.argtypes = [_p(O64Scene), C.c_int, vp, d, vp, C.c_int, vp, vp, vp, vp, vp]
@@ -166,6 +175,9 @@ def _declare(L):
         getattr(L, name).restype = d
         getattr(L, name).argtypes = [d]
     L.o64_tonemap_buffer.argtypes = [vp, C.c_int, C.c_int, d, d, C.c_int, vp]
+    for name in ("o64_js_sin", "o64_js_cos", "o64_js_cbrt"):
+        getattr(L, name).restype = d
+        getattr(L, name).argtypes = [d]
     L.o64_camera_sample.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, u32, vp, vp]
     L.o64_sample_light.restype = C.c_int
     L.o64_sample_light.argtypes = [_p(O64Scene), d, d, d, _p(u32), vp]
@@ -327,8 +339,10 @@ def material_record(m):
 class Packed:
     """The PackedScene of pack.ts:46-59 plus the oracle's view struct."""
 
-    def __init__(self, prims, nodes, nodeCount, mats, lights, primObject, order, sky):
+    def __init__(self, prims, nodes, nodeCount, mats, lights, primObject, order, sky,
+                 leafWords=None):
         self.prims = prims
+        self.leafWords = leafWords
         self.primCount = len(order)
         self.nodes = nodes
         self.nodeCount = nodeCount
@@ -355,6 +369,8 @@ class Packed:
             s.skyStrength = sky["strength"]
         else:
             s.skyKind = 0
+        if leafWords is not None:  # the exact-leaf-word layout extension
+            s.leafWords = leafWords.ctypes.data_as(_p(C.c_uint32))
         self.view = s
 
 
@@ -375,12 +391,17 @@ def pack_scene(scene, bins=12):
     order = np.zeros(cap, np.uint32)
     lights = np.zeros(cap, np.int32)
     nl = C.c_int(0)
-    nc = lib().o64_pack_scene(n, _ptr(kind), _ptr(geom), _ptr(obj), _ptr(emits), bins,
-                              _ptr(prims), _ptr(primObject), _ptr(nodes), _ptr(order),
-                              _ptr(lights), C.byref(nl))
+    # from 2^21 primitives the float32 leaf words (bvh.ts:60-62) are inexact:
+    # the exact words travel beside the nodes, as in the product's packScene
+    words = np.zeros(max(1, 2 * n - 1), np.uint32) if n >= EXACT_LEAF_LIMIT else None
+    nc = lib().o64_pack_scene_ex(n, _ptr(kind), _ptr(geom), _ptr(obj), _ptr(emits), bins,
+                                 _ptr(prims), _ptr(primObject), _ptr(nodes), _ptr(order),
+                                 _ptr(lights), C.byref(nl), _ptr(words) if words is not None
+                                 else None)
     return Packed(prims, nodes[: nc * NODE_STRIDE].copy(), nc, mats,
                   lights[: nl.value].copy(), primObject, order[:n].copy(),
-                  scene.get("sky", {"kind": "none"}))
+                  scene.get("sky", {"kind": "none"}),
+                  words[:nc].copy() if words is not None else None)
 
 
 def prim_bounds(kind, geom):
@@ -394,23 +415,28 @@ def prim_bounds(kind, geom):
     return bmin[: 3 * n], bmax[: 3 * n], bcen[: 3 * n]
 
 
-def build_bvh(bmin, bmax, bcen, bins=12):
-    """buildBvh (bvh.ts:299) on PrimBounds arrays. Returns (nodes, nodeCount, order)."""
+def build_bvh(bmin, bmax, bcen, bins=12, leaf_words=False):
+    """buildBvh (bvh.ts:299) on PrimBounds arrays. Returns (nodes, nodeCount,
+    order), plus the exact leaf word of every node when leaf_words is set."""
     n = len(bmin) // 3
     nodes = np.zeros(max(1, 2 * n - 1) * NODE_STRIDE, np.float32)
     order = np.zeros(max(1, n), np.uint32)
-    nc = lib().o64_build_bvh(n, _ptr(np.ascontiguousarray(bmin, np.float64)),
-                             _ptr(np.ascontiguousarray(bmax, np.float64)),
-                             _ptr(np.ascontiguousarray(bcen, np.float64)), bins, _ptr(nodes),
-                             _ptr(order))
+    words = np.zeros(max(1, 2 * n - 1), np.uint32) if leaf_words else None
+    nc = lib().o64_build_bvh_ex(n, _ptr(np.ascontiguousarray(bmin, np.float64)),
+                                _ptr(np.ascontiguousarray(bmax, np.float64)),
+                                _ptr(np.ascontiguousarray(bcen, np.float64)), bins, _ptr(nodes),
+                                _ptr(order), _ptr(words) if leaf_words else None)
+    if leaf_words:
+        return nodes[: nc * NODE_STRIDE].copy(), nc, order[:n].copy(), words[:nc].copy()
     return nodes[: nc * NODE_STRIDE].copy(), nc, order[:n].copy()
 
 
-def validate_bvh(nodes, nodeCount, order, primCount):
+def validate_bvh(nodes, nodeCount, order, primCount, leafWords=None):
     stats = np.zeros(3, np.int32)
-    errs = lib().o64_validate_bvh(_ptr(np.ascontiguousarray(nodes, np.float32)), nodeCount,
-                                  _ptr(np.ascontiguousarray(order, np.uint32)), primCount,
-                                  _ptr(stats))
+    words = None if leafWords is None else np.ascontiguousarray(leafWords, np.uint32)
+    errs = lib().o64_validate_bvh_ex(_ptr(np.ascontiguousarray(nodes, np.float32)), nodeCount,
+                                     _ptr(np.ascontiguousarray(order, np.uint32)), primCount,
+                                     _ptr(words) if words is not None else None, _ptr(stats))
     return {"errors": errs, "maxLeafSize": int(stats[0]), "leafCount": int(stats[1]),
             "depth": int(stats[2])}
 
@@ -513,6 +539,20 @@ def generate_ray(frame, sx, sy, lx=0.0, ly=0.0):
     lib().o64_generate_ray(_ptr(np.ascontiguousarray(frame, np.float64)), sx, sy, lx, ly, _ptr(o),
                            _ptr(d))
     return o, d
+
+
+def js_sin(x):
+    """Math.sin as the JS engine computes it (fdlibm 5.3, jsmath.h)."""
+    return lib().o64_js_sin(x)
+
+
+def js_cos(x):
+    return lib().o64_js_cos(x)
+
+
+def js_cbrt(x):
+    """Math.cbrt as the JS engine computes it (FreeBSD msun s_cbrt, jsmath.h)."""
+    return lib().o64_js_cbrt(x)
 
 
 def camera_sample(frame, width, height, x, y, frame_index):

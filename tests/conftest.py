@@ -45,4 +45,4 @@ def ct():
 def oracle_packed(O, ps):
     """The oracle's view of a product PackedScene (same arrays)."""
     return O.Packed(ps.prims, ps.nodes, ps.nodeCount, ps.mats, ps.lights, ps.primObject, ps.order,
-                    ps.sky)
+                    ps.sky, getattr(ps, "leafWords", None))

@@ -461,6 +461,23 @@ def test_presets_gpu_bvh_and_render(ct, O, preset):
     assert err.max() <= REL_TOL
 
 
+@pytest.mark.parametrize("seed", [11, 12, 13, 14])
+def test_mirror_paths_bit_identical(ct, O, seed):
+    """Chains of rough-metal bounces (Mirror Room: cbrt-scaled fuzz, cosine and
+    sphere directions from sin/cos of 2*pi*u) -- the GPU's Math.sin/cos/cbrt
+    are the engine's, bit for bit, so every pixel equals the oracle's exactly,
+    not only within 1e-4."""
+    sc = next(p for p in ct.presets.SCENE_PRESETS if p["id"] == "mirror")["scene"]
+    ps = O.pack_scene(sc)
+    w, h, spp = 40, 30, 2
+    fr = O.camera_frame(sc["camera"], w / h)
+    g = ct.renderImage(ps, fr, {"width": w, "height": h, "spp": spp, "seed": seed,
+                                "maxBounces": 6})
+    o = O.render_image(ps, fr, w, h, spp, seed=seed, max_bounces=6)
+    bad = np.flatnonzero(g.view(np.uint32) != o.view(np.uint32))
+    assert len(bad) == 0, (bad[:6] // 3, g[bad[:6]], o[bad[:6]])
+
+
 @pytest.mark.parametrize("preset", ["cornell", "glass", "night", "dof", "mirror"])
 def test_ingested_scene_renders_like_the_oracle(ct, O, preset):
     """Scene ingestion end to end: the preset written as a caustic-scene
