@@ -70,52 +70,113 @@ def cpu_model():
     return None
 
 
-def cpu_baseline(ct, ps, frame, w, h, spp, depth, budget_s=12.0):
-    """Oracle (CPU restatement of tracer.ts) on a band of rows of the same frame,
-    in the protocol of scripts/bench.ts:59-127: a warm-up pass, then three timed
-    trials; every trial is reported and the best is the value."""
+def host_cpus():
+    """Cores this process may use: the affinity mask, capped by a cgroup CPU
+    quota when one is set (a container's share of a larger host)."""
+    n = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = n
+    quota = None
+    for path, parse in (("/sys/fs/cgroup/cpu.max", lambda t: t.split()),
+                        ("/sys/fs/cgroup/cpu/cpu.cfs_quota_us", None)):
+        try:
+            with open(path) as f:
+                text = f.read().strip()
+            if parse:
+                q, per = parse(text)
+                if q != "max":
+                    quota = int(q) / int(per)
+            else:
+                q = int(text)
+                with open("/sys/fs/cgroup/cpu/cpu.cfs_period_us") as f:
+                    per = int(f.read().strip())
+                if q > 0:
+                    quota = q / per
+            break
+        except (OSError, ValueError):
+            continue
+    allowed = aff if quota is None else max(1, min(aff, int(quota)))
+    return {"cpu_count": n, "affinity": aff, "cgroup_quota": quota, "allowed": allowed}
+
+
+def _band_rates(O, view, frame, w, h, spp, depth, seed, threads, rows, acc, trials=3):
+    """Trials of renderImage's passes (frame index s + 7919*seed) over a band."""
+    rates, rays = [], 0
+    for _ in range(trials):
+        acc[rows[0] * w * 3:rows[1] * w * 3] = 0
+        t = time.perf_counter()
+        rays = 0
+        for s in range(spp):
+            c = O.accumulate_pass(view, frame, w, h, s + 7919 * seed, acc, max_bounces=depth,
+                                  threads=threads, rows=rows)
+            rays += int(c[0]) + int(c[1])
+        rates.append(rays / (time.perf_counter() - t) / 1e6)
+    return rates, rays
+
+
+def cpu_baseline(ct, ps, frame, w, h, spp, depth, seed, budget_s=12.0):
+    """Oracle (CPU restatement of tracer.ts) on bands of rows of the same frame
+    -- same camera, seeds, depth and passes -- in the protocol of
+    scripts/bench.ts:59-131: a warm-up pass, then three timed trials; every
+    trial is reported and the best is the value.  Once on every core the
+    process may use, once on one thread (comparable with bench.ts's own
+    single-threaded figures).  Also returns the band's image (renderImage's
+    average) for the parity check of the timed frame."""
     sys.path.insert(0, ROOT)
     from oracle import oracle as O  # the checker, timed as the CPU baseline
 
     import numpy as np
 
     view = O.Packed(ps.prims, ps.nodes, ps.nodeCount, ps.mats, ps.lights, ps.primObject,
-                    ps.order, ps.sky)
-    threads = max(1, min(16, os.cpu_count() or 1))
+                    ps.order, ps.sky, getattr(ps, "leafWords", None))
+    cpus = host_cpus()
+    threads = cpus["allowed"]
     acc = np.zeros(w * h * 3, np.float32)
-    y0 = h // 2 - threads // 2
-    # probe: one pass over `threads` rows
+    # single thread: probe one row, then a band of about a second per trial
+    y1 = h // 2
     t = time.perf_counter()
-    c = O.accumulate_pass(view, frame, w, h, 0, acc, max_bounces=depth, threads=threads,
-                          rows=(y0, y0 + threads))
+    O.accumulate_pass(view, frame, w, h, 7919 * seed, acc, max_bounces=depth, threads=1,
+                      rows=(y1, y1 + 1))
+    probe1 = time.perf_counter() - t
+    rows1 = max(1, min(h, int(1.0 / max(probe1 * spp, 1e-4))))
+    r1, rays1 = _band_rates(O, view, frame, w, h, spp, depth, seed, 1, (y1, y1 + rows1), acc)
+    # all allowed cores: warm-up pass over `threads` rows, then the band
+    y0 = max(0, h // 2 - threads // 2)
+    t = time.perf_counter()
+    O.accumulate_pass(view, frame, w, h, 7919 * seed, acc, max_bounces=depth, threads=threads,
+                      rows=(y0, min(h, y0 + threads)))
     probe = time.perf_counter() - t  # wall seconds for one pass over `threads` rows
     trials = 3
     rows = threads * max(1, int(budget_s / trials / max(probe * spp, 1e-3)))
     rows = min(rows, h)
     y0 = max(0, (h - rows) // 2)  # a band centred on the frame, the whole frame if affordable
-    rates = []
-    for _ in range(trials):
-        acc[:] = 0
-        t = time.perf_counter()
-        rays = 0
-        for s in range(spp):
-            c = O.accumulate_pass(view, frame, w, h, s, acc, max_bounces=depth, threads=threads,
-                                  rows=(y0, y0 + rows))
-            rays += int(c[0]) + int(c[1])
-        dt = time.perf_counter() - t
-        rates.append(rays / dt / 1e6)
+    acc[:] = 0
+    rates, rays = _band_rates(O, view, frame, w, h, spp, depth, seed, threads, (y0, y0 + rows),
+                              acc, trials)
     best = max(rates)
+    band = acc[y0 * w * 3:(y0 + rows) * w * 3].astype(np.float64) * (1.0 / spp)
     return {"value": best, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "cpu_model": cpu_model(), "trials": [round(r, 4) for r in rates],
+            "cpu_model": cpu_model(), "host": cpus, "trials": [round(r, 4) for r in rates],
+            "single_thread": {"value": round(max(r1), 4), "unit": "Mrays/s",
+                              "trials": [round(r, 4) for r in r1],
+                              "sample": f"rows {y1}..{y1 + rows1 - 1}, {spp} passes "
+                                        f"({rays1} rays) per trial"},
             "sample": f"rows {y0}..{y0 + rows - 1} of the {w}x{h} frame, {spp} passes "
                       f"({rays} rays) per trial, 3 trials after a warm-up pass, best "
-                      f"{rows * w * spp * best / rays:.4f} spp*Mpix/s"}
+                      f"{rows * w * spp * best / rays:.4f} spp*Mpix/s"}, \
+        ((y0, y0 + rows), band.astype(np.float32))
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--form", choices=["plain", "batch", "serve", "precision", "copy"],
+    ap.add_argument("--form", choices=["plain", "batch", "serve", "precision", "copy", "traffic"],
                     default="plain")
+    ap.add_argument("--pmc-dir", default=None,
+                    help="traffic: directory holding the rocprofv3 --pmc outputs in fetch/, "
+                         "write/ (the plain form's runs) and copy_fetch/, copy_write/ (the "
+                         "copy form's); writes the traffic profile bench.py reports")
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="copy: bytes moved by the calibration copy kernel (MiB)")
     ap.add_argument("--ref-spp", type=int, default=4096,
@@ -145,6 +206,9 @@ def main():
                     help="plain: path groups per round on their own streams (0: the tracer's "
                          "default)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-isolated", action="store_true",
+                    help="plain: skip the frame traced with every launch alone (profiling runs "
+                         "that count the timed launches only)")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--scenes", type=int, default=0, help="batch/serve: scene count (0: 64/128)")
     ap.add_argument("--tune-trials", type=int, default=16, help="batch: autotuner trials")
@@ -161,6 +225,9 @@ def main():
 
     if args.form == "copy":
         print(json.dumps(copy_form(args)), flush=True)
+        return
+    if args.form == "traffic":
+        print(json.dumps(traffic_form(args)), flush=True)
         return
     ct = load_package()
     ct.init(0)
@@ -218,18 +285,47 @@ def traversal_bytes(cnt, precision, quantized):
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "extend_traffic_soup1m_1080p_4spp.json")
 
 
-def measured_traffic(args):
-    """HBM bytes per extend launch for the default workload, from the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE runs (scaled by the copy-kernel
-    calibration); None for other workloads or settings."""
-    default = (args.tris == 1_000_000 and args.width == 1920 and args.height == 1080 and
-               args.spp == 4 and args.depth == 6 and args.walk == "wide" and args.stack == 24)
-    if not default or not os.path.exists(TRAFFIC_PROFILE):
-        return None
+def kernel_source_digest():
+    """SHA-1 over the HIP sources and headers the library is built from (and
+    the build flags), so a traffic profile can name the kernels it measured."""
+    import glob
+    import hashlib
+    h = hashlib.sha1()
+    files = sorted(glob.glob(os.path.join(ROOT, "caustic-tracer_amd", "csrc", "*")) +
+                   [os.path.join(ROOT, "include", "caustic_amd.h"),
+                    os.path.join(ROOT, "caustic-tracer_amd", "build.py")])
+    for fn in files:
+        if os.path.isfile(fn):
+            h.update(os.path.basename(fn).encode())
+            with open(fn, "rb") as f:
+                h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def workload_settings(args, cfg):
+    """What a traffic profile must match besides the kernel sources."""
+    return {"tris": args.tris, "width": args.width, "height": args.height, "spp": args.spp,
+            "depth": args.depth, "precision": args.precision, "walk": args.walk,
+            "shadow_walk": args.shadow_walk, "stack": args.stack, "blocks": cfg["blocks"],
+            "workgroup": cfg["workgroup"], "pathGroups": cfg["pathGroups"]}
+
+
+def measured_traffic(args, cfg):
+    """HBM bytes per extend launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs (scaled by the copy-kernel calibration), when the profile
+    was taken of these kernel sources under these settings; else None and why."""
+    if not os.path.exists(TRAFFIC_PROFILE):
+        return None, "no traffic profile"
     with open(TRAFFIC_PROFILE) as f:
         prof = json.load(f)
     entry = prof.get(args.precision)
-    return int(entry["traffic_bytes_per_launch"]) if entry else None
+    if not entry:
+        return None, f"no {args.precision} entry in the traffic profile"
+    if entry.get("source_digest") != kernel_source_digest():
+        return None, "traffic profile taken of other kernel sources"
+    if entry.get("settings") != workload_settings(args, cfg):
+        return None, "traffic profile taken under other settings"
+    return int(entry["traffic_bytes_per_launch"]), None
 
 
 def plain_form(ct, args):
@@ -308,14 +404,22 @@ def plain_form(ct, args):
     tr.set_counting(False)
     kname = (f"k_extend32<{args.stack}>" if args.precision == "f32"
              else f"k_extend<{args.walk},{args.stack}>")
+    cfg = tr.get_config()
     roof = extend_roofline(cnt, prof, kname, args.precision, quantized=args.walk == "wide")
-    roof["traffic"] = measured_traffic(args)
+    roof["traffic"], why = measured_traffic(args, cfg)
+    if why:
+        roof["traffic_note"] = why
     # launches of both path groups and the NEE launches run side by side, so a
     # launch's duration is a shared-chip figure; beside it, the algorithmic
     # traversal bytes of a whole step (extension + NEE rays) over the step time
     chip = traversal_bytes(cnt, args.precision, args.walk == "wide") / (ms_step / 1e3) / 1e9
     roof["concurrent_launches"] = True
     roof["step_traversal_gbs"] = round(chip, 1)
+    # the same frame with every launch alone on the chip (one path group, NEE
+    # launches in order on its stream): a single-kernel figure whose summed
+    # launch times fit inside the frame's own time
+    if not args.no_isolated:
+        isolated_roofline(ct, tr, ds, frame, opts, out, cfg, roof, prof)
     log(f"extend: {prof['extendLaunches']} launches, {prof['extendMs']:.2f} ms total, "
         f"{cnt['nodeVisits'] / max(1, cnt['extensionRays']):.1f} nodes/ray, "
         f"{cnt['triTests'] / max(1, cnt['extensionRays']):.1f} tris/ray, {roof['achieved']:.0f} GB/s, "
@@ -325,10 +429,25 @@ def plain_form(ct, args):
         f"{cnt['shadowStackFallbacks']} fallbacks)")
     chq = closest_hit_queries(ct, ds, ps, args.rays) if args.rays > 0 else None
     cpu = None
+    parity = None
     if not args.no_cpu:
         log("CPU baseline (oracle on host cores)...")
-        cpu = cpu_baseline(ct, ps, frame, w, h, spp, depth, args.cpu_budget)
-        log(f"CPU baseline {cpu['value']:.3f} Mrays/s on {cpu['cores']} threads")
+        cpu, (rows, band) = cpu_baseline(ct, ps, frame, w, h, spp, depth, opts["seed"],
+                                         args.cpu_budget)
+        log(f"CPU baseline {cpu['value']:.3f} Mrays/s on {cpu['cores']} threads "
+            f"({cpu['host']}), one thread {cpu['single_thread']['value']:.3f} Mrays/s")
+        # the timed frame against the oracle's rendering of the same rows
+        g = out.numpy()[rows[0] * w * 3:rows[1] * w * 3].astype(np.float64)
+        o = band.astype(np.float64)
+        den = np.maximum(np.abs(g), np.abs(o))
+        rel = np.where(den > 0, np.abs(g - o) / np.where(den > 0, den, 1), 0.0)
+        px = rel.reshape(-1, 3).max(1)
+        parity = {"reference": "oracle", "rows": [rows[0], rows[1]],
+                  "pixels": int(px.size), "max_rel": float(rel.max()),
+                  "pixels_over_1e-4": int((px > 1e-4).sum()),
+                  "exact_fraction": round(float((g == o).mean()), 6)}
+        log(f"parity of the timed frame, rows {rows[0]}..{rows[1] - 1}: max rel "
+            f"{parity['max_rel']:.2e}, {parity['pixels_over_1e-4']} of {px.size} pixels over 1e-4")
     res = {
         "metric": METRIC, "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
@@ -348,24 +467,53 @@ def plain_form(ct, args):
                              "frames": len(lat)},
         "rays_per_step": rays, "bvh_build_ms": round(build_ms, 2),
         "closest_hit_queries": chq,
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "parity": parity,
     }
     return res
 
 
+def isolated_roofline(ct, tr, ds, frame, opts, out, cfg, roof, prof):
+    """The frame again with one path group and the NEE launches in order, so
+    each extend launch runs alone: its average duration and the algorithmic
+    bytes over it, added to roof["isolated"]."""
+    tr.set_config(dict(cfg, pathGroups=1))
+    tr.set_overlap(False)
+    tr.set_profiling(True)
+    ct.sync()
+    t = time.perf_counter()
+    tr.render_image(ds, frame, opts, out)
+    ct.sync()
+    iso_frame_ms = (time.perf_counter() - t) * 1e3
+    iso = tr.stats()
+    tr.set_profiling(False)
+    tr.set_overlap(True)
+    tr.set_config(cfg)
+    ext_total = roof["bytes_per_launch"] * max(1, prof["extendLaunches"])
+    iso_launches = max(1, iso["extendLaunches"])
+    iso_gbs = ext_total / (iso["extendMs"] / 1e3) / 1e9 if iso["extendMs"] > 0 else 0.0
+    roof["isolated"] = {
+        "avg_launch_ms": round(iso["extendMs"] / iso_launches, 4),
+        "launches": iso["extendLaunches"], "bytes_per_launch": ext_total // iso_launches,
+        "achieved": round(iso_gbs, 1), "frac": round(iso_gbs / HBM_PEAK_GBS, 4),
+        "extend_ms_total": round(iso["extendMs"], 3), "frame_ms": round(iso_frame_ms, 3),
+        "setting": "one path group, NEE launches in order: each launch runs alone"}
+    log(f"isolated extend launches: {iso['extendLaunches']} x "
+        f"{iso['extendMs'] / iso_launches:.3f} ms = {iso['extendMs']:.2f} ms of a "
+        f"{iso_frame_ms:.2f} ms frame, {iso_gbs:.0f} GB/s")
+
+
 def closest_hit_queries(ct, ds, ps, n):
-    """Closest-hit-only pass on n random rays over the same scene, in the shape
-    of bvh.test.ts:94-99 (origins in [-14,14]^3 aimed at points of [-8,8]^3,
-    unit binary64 directions): throughput of the 4-wide walk, whether every
-    ray's primitive and t equal the stackless walk's, and a sample checked
-    against the oracle's ordered walk."""
+    """Closest-hit-only pass on n random rays over the same scene, generated as
+    bvh.test.ts:89-99 does (Rng(99): origins range(-14,14), targets
+    range(-8,8), directions normalised by Math.hypot in binary64): throughput
+    of the 4-wide walk, whether every ray's primitive and t equal the stackless
+    walk's, and a sample checked against the oracle's ordered walk."""
     import numpy as np
 
     sys.path.insert(0, ROOT)
     from oracle import oracle as O  # the checker
 
-    g = np.random.default_rng(99)
-    rays = ct.synth.random_rays(g.random(6 * n), n)
+    rays = ct.synth.bvh_test_rays(n, seed=99)
     d_rays = ct.DeviceBuffer.from_array(rays)
     prim = ct.DeviceBuffer.empty((n,), np.int32)
     tt = ct.DeviceBuffer.empty((n,), np.float64)
@@ -604,6 +752,81 @@ def copy_form(args):
     dt = time.perf_counter() - t
     return {"form": "copy", "bytes_read": n, "bytes_written": n, "launches": 4,
             "gbs": round(2 * n / dt / 1e9, 1)}
+
+
+def _pmc_per_dispatch(directory, counter):
+    """{kernel name: [value per dispatch]} of one counter from the
+    *counter_collection.csv files rocprofv3 wrote under directory."""
+    import csv
+    import glob
+    out = {}
+    for fn in glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True):
+        with open(fn) as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") != counter:
+                    continue
+                name = row.get("Kernel_Name", "")
+                out.setdefault(name, {}).setdefault(row.get("Dispatch_Id"), 0.0)
+                out[name][row.get("Dispatch_Id")] += float(row["Counter_Value"])
+    return {k: list(v.values()) for k, v in out.items()}
+
+
+def traffic_form(args):
+    """Calibrated HBM bytes per launch of the timed extend kernel, from the
+    rocprofv3 FETCH_SIZE and WRITE_SIZE runs of the plain form and of the copy
+    form (a copy of known size fixes how many bytes a counter unit is), stored
+    with the kernel-source digest and the settings of the runs."""
+    d = args.pmc_dir
+    fetch = _pmc_per_dispatch(os.path.join(d, "fetch"), "FETCH_SIZE")
+    write = _pmc_per_dispatch(os.path.join(d, "write"), "WRITE_SIZE")
+    cf = _pmc_per_dispatch(os.path.join(d, "copy_fetch"), "FETCH_SIZE")
+    cw = _pmc_per_dispatch(os.path.join(d, "copy_write"), "WRITE_SIZE")
+
+    def pick(table, pattern):
+        hits = {k: v for k, v in table.items() if pattern(k)}
+        if len(hits) != 1:
+            raise SystemExit(f"traffic: expected one kernel, found {sorted(hits)}")
+        return next(iter(hits.items()))
+
+    # the copy kernel moves copy_mib MiB each way per launch (its largest dispatches)
+    ck, cfv = pick(cf, lambda k: "copy" in k.lower())
+    _, cwv = pick(cw, lambda k: k == ck)
+    nbytes = args.copy_mib << 20
+    fetch_factor = nbytes / (max(cfv) * 1024)
+    write_factor = nbytes / (max(cwv) * 1024)
+    ext = "k_extend32<" if args.precision == "f32" else "k_extend<"
+    timed = lambda k: ext in k and ", false, false," in k
+    kf, fv = pick(fetch, timed)
+    _, wv = pick(write, lambda k: k == kf)
+    fb = sum(fv) / len(fv) * 1024 * fetch_factor
+    wb = sum(wv) / len(wv) * 1024 * write_factor
+    ct = load_package()
+    ct.init(0)
+    tr = ct.Tracer()
+    tr.set_precision(ct.PRECISION_F32 if args.precision == "f32" else ct.PRECISION_F64)
+    tr.set_workgroup(args.workgroup)
+    if args.persist_blocks is not None:
+        tr.set_schedule(args.persist_blocks > 0, max(1, args.persist_blocks))
+    cfg = tr.get_config()
+    if args.path_groups:
+        cfg["pathGroups"] = args.path_groups
+    prof = {}
+    if os.path.exists(TRAFFIC_PROFILE):
+        with open(TRAFFIC_PROFILE) as f:
+            prof = json.load(f)
+    prof["workload"] = (f"triangle soup {args.tris} tris + 64 emissive, {args.width}x"
+                        f"{args.height}, {args.spp} spp, maxBounces {args.depth}")
+    prof["copy_calibration"] = {"kernel": ck, "bytes_copied": nbytes,
+                                "FETCH_SIZE_KiB": max(cfv), "WRITE_SIZE_KiB": max(cwv),
+                                "fetch_factor": fetch_factor, "write_factor": write_factor}
+    prof[args.precision] = {"kernel": kf.split("(")[0], "dispatches": len(fv),
+                            "fetch_bytes_per_launch": fb, "write_bytes_per_launch": wb,
+                            "traffic_bytes_per_launch": fb + wb,
+                            "source_digest": kernel_source_digest(),
+                            "settings": workload_settings(args, cfg)}
+    with open(TRAFFIC_PROFILE, "w") as f:
+        json.dump(prof, f, indent=1)
+    return {"form": "traffic", "precision": args.precision, **prof[args.precision]}
 
 
 def precision_form(ct, args):

@@ -201,6 +201,9 @@ __device__ __forceinline__ double max0(double x) { return x > 0 ? x : (x != x ? 
 // ---------------------------------------------------------------------------
 // Intersection (intersect.ts:20-123), binary64 over float32 inputs
 
+// TIE: a miss whose root equals tMax (a hit the strict bound rejects only
+// because it ties the current closest one) returns -2 instead of -1.
+template <bool TIE = false>
 __device__ __forceinline__ double ray_sphere(double ox, double oy, double oz, double dx,
                                              double dy, double dz, double cx, double cy,
                                              double cz, double r, double tMin, double tMax) {
@@ -213,8 +216,9 @@ __device__ __forceinline__ double ray_sphere(double ox, double oy, double oz, do
   double s = sqrt(disc);
   double t = (-halfB - s) / a;
   if (t <= tMin || t >= tMax) {
+    bool tied = TIE && t == tMax;
     t = (-halfB + s) / a;
-    if (t <= tMin || t >= tMax) return -1;
+    if (t <= tMin || t >= tMax) return (tied || (TIE && t == tMax)) ? -2 : -1;
   }
   return t;
 }
@@ -226,12 +230,14 @@ __device__ __forceinline__ double ray_sphere(double ox, double oy, double oz, do
 // of 1e-14 relative and 1e-250 absolute keep rounding and underflow out of
 // the decision), or likewise for v and u + v, i.e. only when the reference's
 // own tests would reject it; otherwise the reference's computation runs.
-__device__ __forceinline__ bool ray_triangle(double ox, double oy, double oz, double dx,
-                                             double dy, double dz, double ax, double ay,
-                                             double az, double bx, double by, double bz,
-                                             double cx, double cy, double cz, double tMin,
-                                             double tMax, double& tOut, double& uOut,
-                                             double& vOut) {
+// TIE: returns 2 instead of 0 for a hit rejected only because t == tMax.
+template <bool TIE = false>
+__device__ __forceinline__ int ray_triangle(double ox, double oy, double oz, double dx,
+                                            double dy, double dz, double ax, double ay,
+                                            double az, double bx, double by, double bz,
+                                            double cx, double cy, double cz, double tMin,
+                                            double tMax, double& tOut, double& uOut,
+                                            double& vOut) {
   double e1x = bx - ax, e1y = by - ay, e1z = bz - az;
   double e2x = cx - ax, e2y = cy - ay, e2z = cz - az;
   double px = dy * e2z - dz * e2y;
@@ -257,11 +263,11 @@ __device__ __forceinline__ bool ray_triangle(double ox, double oy, double oz, do
   double v = vn * invDet;
   if (v < 0 || u + v > 1) return false;
   double t = (e2x * qx + e2y * qy + e2z * qz) * invDet;
-  if (t <= tMin || t >= tMax) return false;
+  if (t <= tMin || t >= tMax) return (TIE && t == tMax) ? 2 : 0;
   tOut = t;
   uOut = u;
   vOut = v;
-  return true;
+  return 1;
 }
 
 // Slab test; returns +inf on a miss (entry before tMax required), else max(tNear,0).
@@ -325,7 +331,7 @@ struct TraversalCounts {
 // end of its left spine).  Both children of that ancestor contain a hit at
 // the tied t, so both entries are finite and the reference's comparison is
 // made on them, with its own slab arithmetic.  Called only on exact ties.
-__device__ __attribute__((noinline)) bool ordered_first(const DevScene& S, int p, int q,
+__device__ __forceinline__ bool ordered_first(const DevScene& S, int p, int q,
                                                         double ox, double oy, double oz,
                                                         double dx, double dy, double dz) {
   const double ix = safe_inv(dx), iy = safe_inv(dy), iz = safe_inv(dz);
@@ -361,9 +367,11 @@ __device__ __forceinline__ double tie_bound(double t) {
 // 324-351): strict '<' keeps the first prim tested on exact ties; a sphere hit
 // leaves u, v as the last triangle hit set them.  Walks that stand for
 // intersectClosest but do not share its visiting order (the 4-wide and parked
-// walks, the FMA-slab binary walk) run TIES_DETECT: a primitive hit at exactly
-// the current t only sets Hit::tie, and settle_ties then walks that ray again
-// with TIES_ORDERED, where such a tie goes to the primitive intersectClosest
+// walks, the FMA-slab binary walk) run TIES_DETECT: a primitive whose hit the
+// strict bound rejects only because it equals the current t sets Hit::tie
+// (a hit exactly at a query's own tMax sets it too: the second walk then
+// finds nothing more), and settle_ties walks that ray again with
+// TIES_ORDERED, where such a tie goes to the primitive intersectClosest
 // reaches first (ordered_first) whatever the order of the tests.
 template <bool COUNT, int TIES = TIES_DETECT>
 __device__ __forceinline__ void test_leaf(const DevScene& S, int leafWord, double ox, double oy,
@@ -374,24 +382,28 @@ __device__ __forceinline__ void test_leaf(const DevScene& S, int leafWord, doubl
   int start = li >> 3;
   for (int p = start; p < start + count; p++) {
     PrimGeom g = load_prim(S, p);
-    const double tMax = TIES != TIES_FIRST ? tie_bound(h.t) : h.t;
+    // TIES_ORDERED sees hits up to and including h.t; the others keep the
+    // strict bound, TIES_DETECT noting a primitive rejected only by a tie
+    const double tMax = TIES == TIES_ORDERED ? tie_bound(h.t) : h.t;
+    constexpr bool DET = TIES == TIES_DETECT;
     double t, u = 0, v = 0;
     bool hit;
     if (g.a.w == 0.0f) {  // PRIM_SPHERE
       if (COUNT) cnt.spheres++;
-      t = ray_sphere(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, tMin, tMax);
+      t = ray_sphere<DET>(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, tMin, tMax);
       hit = t > 0;
+      if (DET && t == -2) h.tie = true;
     } else {
       if (COUNT) cnt.tris++;
-      hit = ray_triangle(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, g.b.y, g.b.z, g.c.x,
-                         g.c.y, g.c.z, tMin, tMax, t, u, v);
+      int r = ray_triangle<DET>(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, g.b.y, g.b.z,
+                                g.c.x, g.c.y, g.c.z, tMin, tMax, t, u, v);
+      hit = r == 1;
+      if (DET && r == 2) h.tie = true;
     }
     if (!hit) continue;
-    bool take = TIES == TIES_FIRST || t < h.t;
-    if (!take && h.prim >= 0) {  // t == h.t: an exact tie with the current hit
-      if (TIES == TIES_DETECT) h.tie = true;
-      if (TIES == TIES_ORDERED) take = ordered_first(S, p, h.prim, ox, oy, oz, dx, dy, dz);
-    }
+    bool take = TIES != TIES_ORDERED || t < h.t;
+    if (!take && h.prim >= 0)  // t == h.t: an exact tie with the current hit
+      take = ordered_first(S, p, h.prim, ox, oy, oz, dx, dy, dz);
     if (take) {
       h.t = t;
       h.prim = p;

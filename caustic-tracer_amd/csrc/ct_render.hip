@@ -1603,6 +1603,13 @@ int ct_tracer_set_graph(ct_tracer* T, int32_t enable) {
   return CT_OK;
 }
 
+int ct_tracer_set_overlap(ct_tracer* T, int32_t enable) {
+  if (!T) return set_error(CT_EINVAL, "null tracer");
+  T->overlap = enable != 0;
+  drop_graphs(T);  // a captured round keeps the streams it was captured with
+  return CT_OK;
+}
+
 int ct_tracer_set_walk(ct_tracer* T, int32_t walk) {
   if (!T) return set_error(CT_EINVAL, "null tracer");
   if (walk != CT_WALK_ORDERED && walk != CT_WALK_STACKLESS && walk != CT_WALK_WIDE)

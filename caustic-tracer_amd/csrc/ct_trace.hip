@@ -82,6 +82,24 @@ k_occluded(DevScene S, const double* __restrict__ rays, uint32_t n, double tMin,
 
 constexpr int WIDE_STACK = 24;
 
+// u, v of the closest triangle hit, recomputed once after the walk: the same
+// test on the same operands gives the same bits, and the walk does not carry
+// two doubles per lane through every step.  A sphere hit gets 0, 0 (the
+// reference leaves whatever the last triangle hit of its own visiting order
+// set, intersect.ts:173-176).
+__device__ __forceinline__ void hit_uv(const DevScene& S, int p, double ox, double oy, double oz,
+                                       double dx, double dy, double dz, double tMin, double& u,
+                                       double& v) {
+  u = 0;
+  v = 0;
+  if (p < 0) return;
+  PrimGeom g = load_prim(S, p);
+  double t;
+  if (g.a.w != 0.0f)
+    ray_triangle(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, g.b.y, g.b.z, g.c.x, g.c.y,
+                 g.c.z, tMin, __builtin_inf(), t, u, v);
+}
+
 // Closest hit over the derived 4-wide nodes, organised like the path
 // tracer's extend kernel: each lane walks its own rays (a fixed sequence
 // i, i + stride, ...), parks leaves (up to PARK_SLOTS) until the wave's lanes
@@ -98,9 +116,11 @@ k_closest_wide(DevScene S, const double* __restrict__ rays, uint32_t n, double t
   int* sNode = sNodeAll + threadIdx.x;
   qw::Entry* sEntry = sEntryAll + threadIdx.x;
   const int lanes = QUERY_BLOCK;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  size_t next = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t i = 0;
+  // 32-bit ray indices (n is): `next` advances by the grid's lane count and
+  // stops at n without wrapping
+  const uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t next = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = 0;
   TraversalCounts cnt = {0, 0, 0, 0};
   bool have = false;
   double ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0;
@@ -113,7 +133,7 @@ k_closest_wide(DevScene S, const double* __restrict__ rays, uint32_t n, double t
   for (;;) {
     if (!have && next < n) {
       i = next;
-      next += stride;
+      next = n - next > stride ? next + stride : n;
       load_ray(rays, i, ox, oy, oz, dx, dy, dz);
       h.t = tMax ? tMax[i] : __builtin_inf();
       h.u = 0; h.v = 0; h.prim = -1; h.tie = false;
@@ -155,8 +175,12 @@ k_closest_wide(DevScene S, const double* __restrict__ rays, uint32_t n, double t
                            cnt);
         if (prim) prim[i] = h.prim;
         if (tOut) tOut[i] = h.prim < 0 ? __builtin_inf() : h.t;
-        if (uOut) uOut[i] = h.u;
-        if (vOut) vOut[i] = h.v;
+        if (uOut || vOut) {
+          double u, v;
+          hit_uv(S, h.prim, ox, oy, oz, dx, dy, dz, tMin, u, v);
+          if (uOut) uOut[i] = u;
+          if (vOut) vOut[i] = v;
+        }
         have = false;
       }
     }

@@ -42,6 +42,97 @@ class Rng:
         return lo + (hi - lo) * self.next()
 
 
+def _xorshift_step(x):
+    x ^= (x << 13) & M32
+    x ^= x >> 17
+    x ^= (x << 5) & M32
+    return x & M32
+
+
+def _apply_map(cols, v):
+    """A GF(2)-linear map of 32-bit words (cols[j] = image of bit j) applied to v."""
+    r = 0
+    j = 0
+    while v:
+        if v & 1:
+            r ^= cols[j]
+        v >>= 1
+        j += 1
+    return r
+
+
+def _jump_tables(cols):
+    """Byte tables of a linear map: map(v) = t0[v & 255] ^ t1[..] ^ t2[..] ^ t3[v >> 24]."""
+    t = np.zeros((4, 256), np.uint32)
+    for k in range(4):
+        for b in range(256):
+            t[k, b] = _apply_map(cols, b << (8 * k))
+    return t
+
+
+def rng_u32_stream(seed, n, block=4096):
+    """The first n outputs of Rng(seed).nextU32() (rng.ts:28-50) as uint32.
+
+    xorshift32 is linear over GF(2): with T the one-step map, output k + m is
+    T^m of output k.  The first `block` outputs are stepped one by one, then the
+    filled prefix is doubled by applying T^filled to it (byte lookup tables),
+    so n outputs cost a few table passes over the array."""
+    out = np.empty(max(n, 0), np.uint32)
+    if n <= 0:
+        return out
+    x = Rng(seed).state
+    first = min(n, block)
+    for i in range(first):
+        x = _xorshift_step(x)
+        out[i] = x
+    filled = first
+    cols = [_xorshift_step(1 << j) for j in range(32)]  # T
+    power = [c for c in cols]
+    # T^block by repeated squaring
+    acc = [1 << j for j in range(32)]
+    e, base = block, power
+    while e:
+        if e & 1:
+            acc = [_apply_map(base, c) for c in acc]
+        base = [_apply_map(base, c) for c in base]
+        e >>= 1
+    jump = acc  # T^filled while filled == block
+    while filled < n:
+        t = _jump_tables(jump)
+        take = min(filled, n - filled)
+        src = out[:take]
+        out[filled:filled + take] = (t[0][src & 0xFF] ^ t[1][(src >> 8) & 0xFF] ^
+                                     t[2][(src >> 16) & 0xFF] ^ t[3][src >> 24])
+        filled += take
+        jump = [_apply_map(jump, c) for c in jump]  # T^(2 filled)
+    return out
+
+
+def rng_stream(seed, n):
+    """The first n outputs of Rng(seed).next() (rng.ts:52-54), binary64."""
+    return rng_u32_stream(seed, n).astype(np.float64) / 4294967296.0
+
+
+def hypot3(x, y, z):
+    """Math.hypot(x, y, z) elementwise over arrays, in the engine's order (the
+    vector form of _hypot)."""
+    a = [np.abs(np.asarray(v, np.float64)) for v in (x, y, z)]
+    m = np.maximum(np.maximum(a[0], a[1]), a[2])
+    safe = np.where(m > 0, m, 1.0)
+    s = np.zeros_like(m)
+    comp = np.zeros_like(m)
+    with np.errstate(invalid="ignore"):  # inf / inf rows are replaced below
+        for v in a:
+            q = v / safe
+            summand = q * q - comp
+            prelim = s + summand
+            comp = (prelim - s) - summand
+            s = prelim
+        r = np.sqrt(s) * m
+    r = np.where(m == 0, 0.0, r)
+    return np.where(np.isinf(m), np.inf, r)
+
+
 def _hypot(*xs):
     """Math.hypot in V8's order: scaled by the largest magnitude, Kahan-summed."""
     a = [abs(float(x)) for x in xs]

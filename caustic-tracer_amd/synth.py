@@ -215,6 +215,20 @@ def caustic_scene(n_spheres=256, sphere_bands=65, sphere_segments=128, plane_row
     return {"name": "caustics", "objects": objects, "camera": camera, "sky": {"kind": "none"}}
 
 
+def bvh_test_rays(n, seed=99, lo=-14.0, hi=14.0, tlo=-8.0, thi=8.0):
+    """The closest-hit ray set of bvh.test.ts:89-99 at any size: per ray, from
+    Rng(seed), origin = range(lo, hi) x3, target = range(tlo, thi) x3,
+    direction (target - origin) / Math.hypot(...) in binary64."""
+    from .presets import hypot3, rng_stream
+
+    d = rng_stream(seed, 6 * n).reshape(n, 6)
+    o = lo + (hi - lo) * d[:, 0:3]
+    t = tlo + (thi - tlo) * d[:, 3:6]
+    v = t - o
+    ln = hypot3(v[:, 0], v[:, 1], v[:, 2])
+    return np.concatenate([o, v / ln[:, None]], 1)
+
+
 def random_rays(rng_stream, n, lo=-14.0, hi=14.0, tlo=-8.0, thi=8.0):
     """Rays in the shape of bvh.test.ts:94-99 from a stream of uniforms in [0,1):
     origins in [lo,hi]^3 aimed at points of [tlo,thi]^3, unit binary64 directions."""

@@ -316,6 +316,10 @@ class Tracer:
     def set_graph(self, on):
         check(lib().ct_tracer_set_graph(self.handle, int(bool(on))), "set_graph")
 
+    def set_overlap(self, on):
+        """NEE launches beside the next extension launch (True) or in order."""
+        check(lib().ct_tracer_set_overlap(self.handle, int(bool(on))), "set_overlap")
+
     def set_precision(self, precision):
         """PRECISION_F64 (reference arithmetic) or PRECISION_F32 (precision form)."""
         check(lib().ct_tracer_set_precision(self.handle, int(precision)), "set_precision")

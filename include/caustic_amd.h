@@ -242,6 +242,11 @@ int ct_tracer_set_capacity(ct_tracer *tracer, uint64_t maxPaths);
 /* Capture each round's launches into a hipGraph and replay it while the
  * round's shape (paths, pixels, jobs, bounces, settings) repeats. */
 int ct_tracer_set_graph(ct_tracer *tracer, int32_t enable);
+/* NEE launches of each path group on a side stream beside the next bounce's
+ * extension launch (1, default) or in order on the group's stream (0).  With
+ * one path group and no overlap every launch of a round runs alone, which
+ * per-launch profiling uses; results never depend on it. */
+int ct_tracer_set_overlap(ct_tracer *tracer, int32_t enable);
 /* CT_PRECISION_F64 or CT_PRECISION_F32 (which always walks the quantized
  * 4-wide nodes; the walk settings then only pick the stack depth). */
 int ct_tracer_set_precision(ct_tracer *tracer, int32_t precision);

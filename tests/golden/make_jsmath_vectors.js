@@ -1,5 +1,6 @@
 // Writes jsmath_vectors.json: inputs and outputs of the JS engine's Math.sin,
-// Math.cos and Math.cbrt, as 16-hex-digit binary64 bit patterns, on the
+// Math.cos, Math.cbrt and Math.hypot (three arguments, as bvh.test.ts:97 and
+// scripts/bench.ts:86 call it), as 16-hex-digit binary64 bit patterns, on the
 // arguments the tracer passes them (2*PI*u and u with u = k / 2^32, the draws
 // of rng.ts:52-54) and on a spread of other arguments.  Run with node.
 'use strict'
@@ -38,11 +39,24 @@ for (const x of [0, -0, 5e-324, 1e-310, -1e-310, 1e-300, 8, -27, 0.001, 1e300, -
   roots.push(x)
 for (let i = 0; i < 600; i++) roots.push((nextU32() / 4294967296 - 0.5) * Math.pow(10, (nextU32() % 600) - 300))
 
+const hyp = []
+for (let i = 0; i < 1500; i++) {
+  const e = (nextU32() % 40) - 20
+  hyp.push([0, 1, 2].map(() => (nextU32() / 4294967296 - 0.5) * Math.pow(2, e + (nextU32() % 5))))
+}
+hyp.push([0, 0, 0], [-0, 3, 4], [1e300, 1e300, 1e-300], [5e-324, 0, 0], [3, 4, 12], [Infinity, 1, 2], [1, -Infinity, 0])
+for (let i = 0; i < 500; i++) {  // the shape of the ray set: targets minus origins
+  const o = [0, 1, 2].map(() => -14 + 28 * (nextU32() / 4294967296))
+  const t = [0, 1, 2].map(() => -8 + 16 * (nextU32() / 4294967296))
+  hyp.push([t[0] - o[0], t[1] - o[1], t[2] - o[2]])
+}
+
 const out = {
   engine: 'node ' + process.version + ' (V8 ' + process.versions.v8 + ')',
   sin: trig.map((x) => [hex(x), hex(Math.sin(x))]),
   cos: trig.map((x) => [hex(x), hex(Math.cos(x))]),
   cbrt: roots.map((x) => [hex(x), hex(Math.cbrt(x))]),
+  hypot3: hyp.map((v) => [hex(v[0]), hex(v[1]), hex(v[2]), hex(Math.hypot(v[0], v[1], v[2]))]),
 }
 fs.writeFileSync(path.join(__dirname, 'jsmath_vectors.json'), JSON.stringify(out))
-console.log('wrote', out.sin.length, out.cos.length, out.cbrt.length, 'vectors from', out.engine)
+console.log('wrote', out.sin.length, out.cos.length, out.cbrt.length, out.hypot3.length, 'vectors from', out.engine)

@@ -192,8 +192,8 @@ def night_emissives():
 
 def js_hypot(*xs):
     """Math.hypot as V8 evaluates it: scale by the largest magnitude, then a
-    Kahan-compensated sum of squares (last-ulp parity with V8 is unpinned; it
-    only shapes test inputs)."""
+    Kahan-compensated sum of squares (bit-exact against the engine's outputs in
+    tests/golden/jsmath_vectors.json)."""
     a = [abs(x) for x in xs]
     m = max(a) if a else 0.0
     if m == math.inf:

@@ -41,3 +41,19 @@ def test_c_library_differs_from_the_engine():
     for name, fn in (("sin", math.sin), ("cos", math.cos)):
         miss = sum(_bits(fn(_d(x))) != y for x, y in v[name])
         assert miss > 20, name  # the vectors separate the two implementations
+
+
+def test_hypot_restatements_bit_exact(ct):
+    """Math.hypot(x, y, z) (the ray sets' normalisation, bvh.test.ts:97): the
+    scalar restatements of the tests and the presets and the vector form that
+    generates the 16M-ray set all equal the engine's results."""
+    import numpy as np
+
+    import scenes
+    vec = _vectors()["hypot3"]
+    xs = np.array([[_d(a), _d(b), _d(c)] for a, b, c, _ in vec])
+    want = [r for _, _, _, r in vec]
+    v = ct.presets.hypot3(xs[:, 0], xs[:, 1], xs[:, 2])
+    assert [_bits(float(x)) for x in v] == want
+    assert [_bits(scenes.js_hypot(*row)) for row in xs] == want
+    assert [_bits(ct.presets._hypot(*row)) for row in xs] == want
