@@ -181,6 +181,9 @@ def main():
                     help="copy: bytes moved by the calibration copy kernel (MiB)")
     ap.add_argument("--ref-spp", type=int, default=4096,
                     help="precision: spp of the binary64 reference image")
+    ap.add_argument("--ref-cpu-pixels", type=int, default=16384,
+                    help="precision: pixels of the FP64 CPU reference (the oracle on host cores, "
+                         "--ref-spp samples each; 0: none)")
     ap.add_argument("--precision", choices=["f64", "f32"], default="f64",
                     help="f64: the reference's binary64 arithmetic; f32: the precision form "
                          "(quantized nodes, FP32 intersection and shading)")
@@ -854,7 +857,32 @@ def precision_form(ct, args):
     log(f"binary64 reference: {args.ref_spp} spp in {ref_s:.1f} s, mean {ref.mean():.5f}")
     del ref_tr
 
+    # the FP64 CPU reference: the oracle at --ref-spp on pixels spread over the
+    # image, on every core the process may use
+    cpu_ref = None
+    if args.ref_cpu_pixels > 0:
+        sys.path.insert(0, ROOT)
+        from oracle import oracle as O  # the checker, as the FP64 CPU reference
+
+        view = O.Packed(ps.prims, ps.nodes, ps.nodeCount, ps.mats, ps.lights, ps.primObject,
+                        ps.order, ps.sky, ps.leafWords)
+        px = np.sort(np.random.default_rng(5).choice(w * h, args.ref_cpu_pixels, replace=False))
+        threads = host_cpus()["allowed"]
+        t = time.perf_counter()
+        cref = O.render_pixels(view, frame, w, h, px, args.ref_spp, seed=1000,
+                               max_bounces=depth, threads=threads).astype(np.float64)
+        cpu_s = time.perf_counter() - t
+        g = ref.reshape(-1, 3)[px]
+        den = np.maximum(np.abs(g), np.abs(cref))
+        rel = np.where(den > 0, np.abs(g - cref) / np.where(den > 0, den, 1), 0.0)
+        cpu_ref = {"pixels": int(len(px)), "spp": args.ref_spp, "seconds": round(cpu_s, 2),
+                   "threads": threads, "gpu_reference_max_rel": float(rel.max()),
+                   "px": px, "values": cref}
+        log(f"FP64 CPU reference: {len(px)} pixels x {args.ref_spp} spp on {threads} threads in "
+            f"{cpu_s:.1f} s; the GPU binary64 reference there: max rel {rel.max():.2e}")
+
     res = {}
+    imgs = {}
     for prec in ("f64", "f32"):
         tr = ct.Tracer()
         tr.set_precision(ct.PRECISION_F32 if prec == "f32" else ct.PRECISION_F64)
@@ -870,6 +898,7 @@ def precision_form(ct, args):
         total = time.perf_counter() - t
         st = tr.stats()
         img = out.numpy().astype(np.float64)
+        imgs[prec] = img
         rms = float(np.sqrt(np.mean((img - ref) ** 2)))
         rays = st["extensionRays"] + st["shadowRays"]
         r = {"ms_per_step": total / args.steps * 1e3, "mrays": rays * args.steps / total / 1e6,
@@ -886,12 +915,31 @@ def precision_form(ct, args):
         r["roofline"] = extend_roofline(cnt, prof, "k_extend32<24>" if prec == "f32" else
                                         "k_extend<wide,24>", prec)
         r["fallbacks"] = cnt["stackFallbacks"]
+        if cpu_ref:
+            r["rms_cpu"] = float(np.sqrt(np.mean((img.reshape(-1, 3)[cpu_ref["px"]] -
+                                                  cpu_ref["values"]) ** 2)))
         log(f"{prec}: {r['ms_per_step']:.1f} ms/step, {r['mrays']:.0f} Mrays/s, "
             f"{r['sppmpix']:.1f} spp*Mpix/s, RMS vs reference {rms:.6f} "
             f"(relative {r['rel_rms']:.4f}), extend {prof['extendMs']:.1f} ms, "
             f"{cnt['nodeVisits'] / max(1, cnt['extensionRays']):.1f} nodes/ray")
         res[prec] = r
     f32, f64 = res["f32"], res["f64"]
+    # arithmetic apart from sampling noise: the same seed in both precisions,
+    # beside two binary64 seeds
+    tr = ct.Tracer()
+    tr.render_image(ds, frame, dict(opts, seed=2), out)
+    ct.sync()
+    seed2 = out.numpy().astype(np.float64)
+    same_seed = float(np.sqrt(np.mean((imgs["f32"] - imgs["f64"]) ** 2)))
+    seed_noise = float(np.sqrt(np.mean((imgs["f64"] - seed2) ** 2)))
+    log(f"RMS FP32 vs binary64 at the same seed {same_seed:.3e}; binary64 seed 1 vs 2 "
+        f"{seed_noise:.3e}")
+    cpu_line = None
+    if cpu_ref:
+        cpu_line = {"pixels": cpu_ref["pixels"], "spp": cpu_ref["spp"],
+                    "f32": f32["rms_cpu"], "f64": f64["rms_cpu"],
+                    "gpu_binary64_reference_max_rel": cpu_ref["gpu_reference_max_rel"],
+                    "cpu_seconds": cpu_ref["seconds"], "threads": cpu_ref["threads"]}
     return {
         "metric": METRIC, "value": round(f32["mrays"], 2), "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(f32["ms_per_step"], 3),
@@ -905,13 +953,20 @@ def precision_form(ct, args):
                    "max_bounces": depth, "parallelism": "single GPU"},
         "spp_mpix_per_s": round(f32["sppmpix"], 3),
         "rms_vs_reference": {"f32": f32["rms"], "f64": f64["rms"], "reference_spp": args.ref_spp,
+                             "reference": "GPU binary64 render (equal per pixel to the oracle's)",
                              "f32_over_f64": round(f32["rms"] / f64["rms"], 4)},
+        "rms_vs_fp64_cpu": cpu_line,
+        "rms_same_seed_f32_vs_f64": same_seed, "rms_seed_noise_f64": seed_noise,
         "binary64_form": {"ms_per_step": round(f64["ms_per_step"], 3),
                           "mrays_per_s": round(f64["mrays"], 2),
                           "spp_mpix_per_s": round(f64["sppmpix"], 3),
                           "roofline": f64["roofline"]},
         "speedup_vs_binary64_at_equal_spp": round(f64["ms_per_step"] / f32["ms_per_step"], 3),
-        "roofline": f32["roofline"], "cpu_baseline": None,
+        "roofline": f32["roofline"],
+        "cpu_baseline": None if not cpu_ref else {
+            "value": round(cpu_ref["pixels"] * cpu_ref["spp"] / cpu_ref["seconds"] / 1e6, 4),
+            "unit": "spp*Mpix/s", "cores": cpu_ref["threads"], "kind": "port",
+            "sample": f"the FP64 CPU reference: {cpu_ref['pixels']} pixels x {cpu_ref['spp']} spp"},
     }
 
 

@@ -1486,6 +1486,26 @@ void o64_pass_radiance(const O64Scene *s, const double *frame, int width, int he
                    out + ((size_t)y * width + x) * 3, NULL);
 }
 
+/* renderImage (tracer.ts:348-355) of single pixels: each listed pixel's spp
+ * passes (frame index s + 7919*seed) summed in float32 in pass order, then
+ * times 1/spp -- the values renderImage leaves there.  out: 3 per pixel. */
+void o64_render_pixels(const O64Scene *s, const double *frame, int width, int height,
+                       const uint32_t *pixels, int count, int spp, uint32_t seed,
+                       const O64TraceOpts *opts, float *out) {
+  double inv = 1.0 / spp;
+  for (int k = 0; k < count; k++) {
+    int x = (int)(pixels[k] % (uint32_t)width), y = (int)(pixels[k] / (uint32_t)width);
+    float acc[3] = {0, 0, 0};
+    for (int sIdx = 0; sIdx < spp; sIdx++) {
+      double c[3];
+      pixel_sample(s, frame, width, height, x, y, (uint32_t)sIdx + seed * 7919u, opts, c, NULL);
+      if (isfinite(c[0]) && isfinite(c[1]) && isfinite(c[2]))
+        for (int j = 0; j < 3; j++) acc[j] = (float)((double)acc[j] + c[j]);
+    }
+    for (int j = 0; j < 3; j++) out[3 * k + j] = (float)((double)acc[j] * inv);
+  }
+}
+
 /* tracer.ts:348-355 */
 void o64_render_image(const O64Scene *s, const double *frame, int width, int height, int spp,
                       uint32_t seed, const O64TraceOpts *opts, float *out) {

@@ -215,6 +215,10 @@ void o64_tonemap_buffer(const float *accum, int width, int height, double sample
 double o64_js_sin(double x);
 double o64_js_cos(double x);
 double o64_js_cbrt(double x);
+/* renderImage's values at a list of pixels (y*width + x). */
+void o64_render_pixels(const O64Scene *s, const double *frame, int width, int height,
+                       const uint32_t *pixels, int count, int spp, uint32_t seed,
+                       const O64TraceOpts *opts, float *out);
 /* Test exports: the camera ray of one path (tracer.ts:373-383), and light
  * sampling from a given RNG state (tracer.ts:61-180). */
 void o64_camera_sample(const double *frame, int width, int height, int x, int y,

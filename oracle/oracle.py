@@ -178,6 +178,8 @@ def _declare(L):
     for name in ("o64_js_sin", "o64_js_cos", "o64_js_cbrt"):
         getattr(L, name).restype = d
         getattr(L, name).argtypes = [d]
+    L.o64_render_pixels.argtypes = [_p(O64Scene), vp, C.c_int, C.c_int, vp, C.c_int, C.c_int,
+                                    u32, _p(O64TraceOpts), vp]
     L.o64_camera_sample.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, u32, vp, vp]
     L.o64_sample_light.restype = C.c_int
     L.o64_sample_light.argtypes = [_p(O64Scene), d, d, d, _p(u32), vp]
@@ -647,6 +649,33 @@ def render_image(ps, frame, width, height, spp, seed=0, max_bounces=8, russian_r
     opts = _opts(max_bounces, russian_roulette, nee)
     lib().o64_render_image(C.byref(ps.view), _ptr(np.ascontiguousarray(frame, np.float64)),
                            width, height, spp, seed & 0xFFFFFFFF, C.byref(opts), _ptr(out))
+    return out
+
+
+def render_pixels(ps, frame, width, height, pixels, spp, seed=0, max_bounces=8,
+                  russian_roulette=True, nee=True, threads=1):
+    """renderImage's values (count, 3) at pixel indices y*width + x, the list
+    split over threads."""
+    pixels = np.ascontiguousarray(pixels, np.uint32)
+    out = np.zeros((len(pixels), 3), np.float32)
+    opts = _opts(max_bounces, russian_roulette, nee)
+    fr = np.ascontiguousarray(frame, np.float64)
+    chunks = np.array_split(np.arange(len(pixels)), max(1, threads))
+
+    def work(idx):
+        if len(idx) == 0:
+            return
+        sub = np.ascontiguousarray(pixels[idx])
+        res = np.zeros((len(idx), 3), np.float32)
+        lib().o64_render_pixels(C.byref(ps.view), _ptr(fr), width, height, _ptr(sub), len(idx),
+                                spp, seed & 0xFFFFFFFF, C.byref(opts), _ptr(res))
+        out[idx] = res
+
+    ts = [threading.Thread(target=work, args=(c,)) for c in chunks]
+    for th in ts:
+        th.start()
+    for th in ts:
+        th.join()
     return out
 
 

@@ -182,3 +182,18 @@ def test_render_image_sample_count_edges(O):
     assert np.array_equal(one.view(np.uint32), acc.view(np.uint32))
     none = O.render_image(ps, frame, w, h, 0, seed=3, max_bounces=3)
     assert none.shape == (w * h * 3,) and np.isnan(none).all()
+
+
+def test_render_pixels_equal_render_image(O):
+    """renderImage's values at listed pixels (the CPU reference of the
+    precision form) equal those of the whole image, bit for bit."""
+    import numpy as np
+
+    import scenes
+    sc = scenes.cornell_box()
+    ps = O.pack_scene(sc)
+    fr = O.camera_frame(sc["camera"], 1.5)
+    img = O.render_image(ps, fr, 24, 16, 5, seed=3).reshape(-1, 3)
+    px = np.array([0, 5, 17, 100, 383, 200])
+    got = O.render_pixels(ps, fr, 24, 16, px, 5, seed=3, threads=3)
+    assert np.array_equal(got.view(np.uint32), img[px].view(np.uint32))
