@@ -9,6 +9,7 @@
  */
 const path = require('path')
 const addon = require(path.join(__dirname, 'caustic_amd.node'))
+const pack = require(path.join(__dirname, 'pack.js'))
 
 const WALK_STACKLESS = 0
 const WALK_ORDERED = 1
@@ -37,6 +38,24 @@ function frameArray(f) {
 /** buildBvh (bvh.ts:299) on the GPU; same {nodes, nodeCount, order}. */
 function buildBvh(pb, bins) {
   return addon.buildBvh(pb.min, pb.max, pb.centroid, bins === undefined ? 12 : bins)
+}
+
+/** packScene (pack.ts:195) with the BVH built on the GPU. */
+function packScene(scene, bins) {
+  return pack.packScene(scene, buildBvh, bins)
+}
+
+// renderImage and accumulatePass take a Scene or a PackedScene (tracer.ts:349
+// packs a Scene on demand); a packed Scene is kept per Scene object.
+const packedScenes = new WeakMap()
+function packedOf(sceneOrPacked) {
+  if ('primCount' in sceneOrPacked) return sceneOrPacked
+  let ps = packedScenes.get(sceneOrPacked)
+  if (ps === undefined) {
+    ps = packScene(sceneOrPacked)
+    packedScenes.set(sceneOrPacked, ps)
+  }
+  return ps
 }
 
 function single(ps, walk, ox, oy, oz, dx, dy, dz, tMin, tMax, hit) {
@@ -80,14 +99,14 @@ function occludedBatch(ps, rays, tMin, tMax) {
 
 /** accumulatePass (tracer.ts:361): adds one sample per pixel into accum. */
 function accumulatePass(ps, frame, width, height, frameIndex, opts, accum) {
-  addon.accumulatePass(deviceScene(ps), frameArray(frame), width, height, frameIndex,
+  addon.accumulatePass(deviceScene(packedOf(ps)), frameArray(frame), width, height, frameIndex,
     opts.maxBounces, opts.russianRoulette ? 1 : 0, opts.nee ? 1 : 0, accum)
 }
 
-/** renderImage (tracer.ts:348): linear RGB averaged over opts.spp. */
-function renderImage(ps, frame, opts) {
+/** renderImage (tracer.ts:348): linear RGB averaged over opts.spp; scene is a Scene or a PackedScene. */
+function renderImage(scene, frame, opts) {
   const seed = opts.seed === undefined ? 0 : opts.seed
-  return addon.renderImage(deviceScene(ps), frameArray(frame), opts.width, opts.height, opts.spp,
+  return addon.renderImage(deviceScene(packedOf(scene)), frameArray(frame), opts.width, opts.height, opts.spp,
     seed, opts.maxBounces, opts.russianRoulette ? 1 : 0, opts.nee ? 1 : 0)
 }
 
@@ -150,7 +169,7 @@ class Server {
 }
 
 module.exports = {
-  WALK_STACKLESS, WALK_ORDERED, T_MIN, buildBvh, intersectClosest, intersectClosestStackless,
+  WALK_STACKLESS, WALK_ORDERED, T_MIN, buildBvh, packScene, intersectClosest, intersectClosestStackless,
   occluded, closestHitBatch, occludedBatch, accumulatePass, renderImage, frameArray,
   setPrecision, Server, tonemapBuffer, pick,
 }

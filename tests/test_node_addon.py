@@ -25,7 +25,7 @@ def test_addon_loads_and_exports_reference_names():
                          capture_output=True, text=True, timeout=60)
     assert out.returncode == 0, out.stderr
     names = set(out.stdout.strip().split(","))
-    for n in ("buildBvh", "intersectClosest", "intersectClosestStackless", "occluded",
+    for n in ("buildBvh", "packScene", "intersectClosest", "intersectClosestStackless", "occluded",
               "accumulatePass", "renderImage", "setPrecision", "Server", "tonemapBuffer", "pick"):
         assert n in names
 
@@ -100,3 +100,33 @@ def test_node_serving_jobs_match_oracle(O, tmp_path):
         den = np.maximum(np.abs(g), np.abs(o)).astype(np.float64)
         err = np.where(den > 0, np.abs(g - o) / np.where(den > 0, den, 1), 0)
         assert err.max() <= 1e-4
+
+
+@needs_node
+@pytest.mark.gpu
+@pytest.mark.parametrize("preset", ["cornell", "mirror"])
+def test_node_pack_scene_and_render_a_scene(ct, O, preset, tmp_path):
+    """The Node mirror's packScene (pack.ts:195-223, BVH on the GPU) gives the
+    oracle's arrays, and renderImage takes a Scene as tracer.ts:349 does:
+    the image equals that of the PackedScene and the oracle's."""
+    sc = next(p for p in ct.presets.SCENE_PRESETS if p["id"] == preset)["scene"]
+    (tmp_path / "scene.json").write_text(ct.sceneToJSON(sc))
+    w, h, spp = 40, 30, 2
+    fr = O.camera_frame(sc["camera"], w / h)
+    fr.tofile(tmp_path / "frame.bin")
+    script = os.path.join(ROOT, "tests", "node", "pack_render.js")
+    r = subprocess.run([NODE, script, str(tmp_path), str(w), str(h), str(spp), "7"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    ps = O.pack_scene(sc)
+    meta = json.loads((tmp_path / "meta.json").read_text())
+    assert meta["primCount"] == ps.primCount and meta["nodeCount"] == ps.nodeCount
+    for name, arr in (("prims", ps.prims), ("nodes", ps.nodes), ("mats", ps.mats),
+                      ("lights", ps.lights), ("primObject", ps.primObject)):
+        got = np.fromfile(tmp_path / f"{name}.bin", arr.dtype)
+        assert np.array_equal(got.view(np.uint32), arr.view(np.uint32)), name
+    a = np.fromfile(tmp_path / "img_scene.bin", np.float32)
+    b = np.fromfile(tmp_path / "img_packed.bin", np.float32)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    o = O.render_image(ps, fr, w, h, spp, seed=7, max_bounces=6)
+    assert np.array_equal(a.view(np.uint32), o.view(np.uint32))
