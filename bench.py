@@ -217,8 +217,12 @@ def main():
     ap.add_argument("--tune-trials", type=int, default=16, help="batch: autotuner trials")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: requests per second")
     ap.add_argument("--duration", type=float, default=2.0, help="serve: seconds of arrivals")
+    ap.add_argument("--rates", type=lambda v: [float(x) for x in v.split(",") if x], default=[],
+                    help="serve: comma-separated rates for a latency-vs-rate curve after the "
+                         "main stream (each over --curve-duration seconds, no plain comparison)")
+    ap.add_argument("--curve-duration", type=float, default=10.0)
     ap.add_argument("--max-paths", type=int, default=1 << 26, help="serve: paths per round")
-    ap.add_argument("--policy", choices=["fifo", "srpt"], default="fifo")
+    ap.add_argument("--policy", choices=["fifo", "srpt", "scene"], default="scene")
     ap.add_argument("--hbm-budget-gib", type=float, default=8.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -682,16 +686,30 @@ def serve_form(ct, args):
     ref = {}
     t = time.perf_counter()
     rays_plain = 0
+    per_req = np.zeros(len(reqs))  # seconds from launch to counters read (synchronous)
+    req_rays = np.zeros(len(reqs))
+    last = t
     for k, r in enumerate(reqs):
+        t1 = time.perf_counter()
         plain.render_image(dss[r["sceneIndex"]], frames[k], r, out)
         st = plain.stats()
-        rays_plain += st["extensionRays"] + st["shadowRays"]
+        per_req[k] = time.perf_counter() - t1
+        req_rays[k] = st["extensionRays"] + st["shadowRays"]
+        rays_plain += req_rays[k]
         if k in keep:
             ref[k] = _digest(out.numpy()[:r["width"] * r["height"] * 3])
+        if time.perf_counter() - last > 10:
+            last = time.perf_counter()
+            log(f"plain: {k + 1}/{len(reqs)} requests in {last - t:.1f} s")
     ct.sync()
     plain_s = time.perf_counter() - t
     del plain, out
     log(f"plain one-at-a-time: {plain_s:.2f} s, {samples / plain_s / 1e6:.1f} spp*Mpix/s")
+    fixed = _fixed_cost(reqs, per_req, req_rays, plain_s)
+    log(f"plain fixed cost per request {fixed['fixed_ms']:.2f} ms + {fixed['ns_per_ray']:.3f} "
+        f"ns/ray; requests under 2^20 samples: {fixed['small']['requests']} of {len(reqs)}, "
+        f"{fixed['small']['time_frac']:.3f} of the plain time for "
+        f"{fixed['small']['sample_frac']:.4f} of the samples")
 
     srv = ct.Server(hbm_budget=int(args.hbm_budget_gib * (1 << 30)), max_paths=args.max_paths,
                     policy=args.policy)
@@ -714,6 +732,9 @@ def serve_form(ct, args):
     log(f"served: {wall:.2f} s wall, {sppmpix:.1f} spp*Mpix/s, {mrays:.0f} Mrays/s, p50 "
         f"{p50 * 1e3:.0f} ms, p99 {p99 * 1e3:.0f} ms, GPU busy {st['busySec'] / wall:.2f}, "
         f"{st['rounds'] - base['rounds']} rounds, same as alone: {same}")
+    curve = []
+    for rate in args.rates:
+        curve.append(_serve_point(ct, args, dss, rate))
     return {
         "metric": METRIC, "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1,
         "steps": 1, "warmup": 1, "ms_per_step": round(wall * 1e3, 3),
@@ -733,8 +754,63 @@ def serve_form(ct, args):
                                 "spp_mpix_per_s": round(samples / plain_s / 1e6, 3),
                                 "mrays_per_s": round(rays_plain / plain_s / 1e6, 2)},
         "speedup_vs_plain": round(sppmpix / (samples / plain_s / 1e6), 3),
+        "plain_fixed_cost": fixed, "submit_lag_ms_max": round(res["submit_lag_max"] * 1e3, 2),
+        "latency_vs_rate": curve,
         "same_as_alone": same, "roofline": None, "cpu_baseline": None,
     }
+
+
+def _fixed_cost(reqs, per_req, req_rays, plain_s):
+    """Least-squares fit of the plain per-request time to a + b * rays: a is
+    the fixed cost of a request (launches, tails, the host round trip), b the
+    marginal cost of a ray; and the share of the plain time the small requests
+    (under 2^20 pixel samples) take."""
+    import numpy as np
+
+    A = np.stack([np.ones_like(req_rays), req_rays], 1)
+    (a, b), *_ = np.linalg.lstsq(A, per_req, rcond=None)
+    samp = np.array([r["width"] * r["height"] * r["spp"] for r in reqs], np.float64)
+    small = samp < (1 << 20)
+    return {"fixed_ms": round(float(a) * 1e3, 3), "ns_per_ray": round(float(b) * 1e9, 4),
+            "fixed_frac_of_plain": round(float(a) * len(reqs) / float(plain_s), 4),
+            "median_request_ms": round(float(np.median(per_req)) * 1e3, 3),
+            "small": {"requests": int(small.sum()),
+                      "time_frac": round(float(per_req[small].sum() / per_req.sum()), 4),
+                      "sample_frac": round(float(samp[small].sum() / samp.sum()), 5),
+                      "median_ms": round(float(np.median(per_req[small])) * 1e3, 3)
+                      if small.any() else None}}
+
+
+def _serve_point(ct, args, dss, rate):
+    """One point of the latency-vs-rate curve: the same generator at `rate`
+    requests per second for `duration` seconds, served by a fresh server."""
+    reqs = ct.synth.request_stream(len(dss), rate=rate, duration=args.curve_duration, seed=6)
+    frames = [ct.cameraFrame(r["camera"], 1.0) for r in reqs]
+    samples = sum(r["width"] * r["height"] * r["spp"] for r in reqs)
+    srv = ct.Server(hbm_budget=int(args.hbm_budget_gib * (1 << 30)), max_paths=args.max_paths,
+                    policy=args.policy)
+    wid = srv.submit(dss[0], frames[0], 64, 64, 1)
+    srv.wait(wid)
+    srv.release(wid)
+    base = srv.stats()
+    res = ct.serve.replay(srv, dss, frames, reqs, log=log)
+    st = res["stats"]
+    srv.close()
+    lat = res["latency"]
+    pix = st["pixelSamples"] - base["pixelSamples"]
+    pt = {"rate": rate, "requests": len(reqs), "wall_s": round(res["wall"], 3),
+          "offered_spp_mpix_per_s": round(samples / args.curve_duration / 1e6, 2),
+          "spp_mpix_per_s": round(pix / res["wall"] / 1e6, 2),
+          "p50_ms": round(ct.serve.percentile(lat, 50) * 1e3, 2),
+          "p99_ms": round(ct.serve.percentile(lat, 99) * 1e3, 2),
+          "max_ms": round(max(lat) * 1e3, 2),
+          "gpu_busy_frac": round((st["busySec"] - base["busySec"]) / res["wall"], 4),
+          "rounds": st["rounds"] - base["rounds"],
+          "submit_lag_ms_max": round(res["submit_lag_max"] * 1e3, 2)}
+    log(f"rate {rate}/s: {pt['requests']} requests, p50 {pt['p50_ms']} ms, p99 {pt['p99_ms']} "
+        f"ms, {pt['spp_mpix_per_s']} spp*Mpix/s of {pt['offered_spp_mpix_per_s']} offered, "
+        f"GPU busy {pt['gpu_busy_frac']}")
+    return pt
 
 
 def copy_form(args):

@@ -127,9 +127,17 @@ void serve_loop(ct_server* S) {
     }
     // one round: passes from every admitted job, up to the path capacity
     std::vector<SJob*> order = S->active;
-    if (S->opts.policy == CT_SERVE_SRPT)
+    if (S->opts.policy == CT_SERVE_SRPT) {
       std::stable_sort(order.begin(), order.end(),
                        [](const SJob* a, const SJob* b) { return a->remaining() < b->remaining(); });
+    } else if (S->opts.policy == CT_SERVE_SCENE) {
+      // scenes ranked by their oldest admitted job; each scene's jobs together
+      std::unordered_map<const ct_scene*, size_t> rank;
+      for (SJob* j : order) rank.emplace(j->desc.scene, rank.size());
+      std::stable_sort(order.begin(), order.end(), [&](const SJob* a, const SJob* b) {
+        return rank[a->desc.scene] < rank[b->desc.scene];
+      });
+    }
     size_t budget = capacity;
     std::vector<SJob*> picked;
     std::vector<PlanJob> plan;
@@ -272,7 +280,7 @@ int ct_server_create(const ct_server_opts* opts, ct_server** out) {
   if (opts) o = *opts;
   if (!o.hbmBudget) o.hbmBudget = uint64_t(4) << 30;
   if (!o.maxActive) o.maxActive = 1024;
-  if (o.policy != CT_SERVE_FIFO && o.policy != CT_SERVE_SRPT)
+  if (o.policy != CT_SERVE_FIFO && o.policy != CT_SERVE_SRPT && o.policy != CT_SERVE_SCENE)
     return set_error(CT_EINVAL, "ct_server_create: unknown policy");
   if (o.precision != CT_PRECISION_F64 && o.precision != CT_PRECISION_F32)
     return set_error(CT_EINVAL, "ct_server_create: unknown precision");

@@ -17,7 +17,7 @@ from ._capi import DeviceBuffer, check, lib
 from .tracer import Tracer, device_scene
 
 QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
-POLICIES = {"fifo": 0, "srpt": 1}
+POLICIES = {"fifo": 0, "srpt": 1, "scene": 2}
 
 
 def _status(s):
@@ -122,18 +122,21 @@ def replay(server, scenes, frames, requests, keep=None, poll_s=0.0005, log=None)
     scenes[i] / frames[k]: the resident scene of request k's sceneIndex and its
     camera frame.  keep: optional set of request indices whose final images are
     read back before release (returned in `images`).  Returns per-request
-    latency (done - submit, server clock), the wall time of the whole replay
-    and the server's counters."""
+    latency (done - submit, server clock), the wall time of the whole replay,
+    the server's counters and the largest delay between a request's arrival
+    time and its submission (`submit_lag_max`, seconds)."""
     pending = {}
     lat = [None] * len(requests)
     images = {}
     t0 = time.perf_counter()
     k = 0
     last_log = t0
+    lag = 0.0
     while k < len(requests) or pending:
         now = time.perf_counter() - t0
         while k < len(requests) and requests[k]["at"] <= now:
             r = requests[k]
+            lag = max(lag, now - r["at"])
             jid = server.submit(scenes[r["sceneIndex"]], frames[k], r["width"], r["height"],
                                 r["spp"], seed=r["seed"], maxBounces=r["maxBounces"],
                                 russianRoulette=r.get("russianRoulette", True),
@@ -162,4 +165,5 @@ def replay(server, scenes, frames, requests, keep=None, poll_s=0.0005, log=None)
         if not done:
             time.sleep(poll_s)
     wall = time.perf_counter() - t0
-    return {"latency": lat, "wall": wall, "stats": server.stats(), "images": images}
+    return {"latency": lat, "wall": wall, "stats": server.stats(), "images": images,
+            "submit_lag_max": lag}

@@ -314,13 +314,16 @@ int ct_tracer_autotune(ct_tracer *tracer, const ct_job *jobs, uint32_t n, uint32
 
 #define CT_SERVE_FIFO 0 /* rounds filled in admission order */
 #define CT_SERVE_SRPT 1 /* rounds filled shortest remaining work first */
+#define CT_SERVE_SCENE 2 /* admission order, each scene's jobs together: a round
+                            takes the oldest job's scene's jobs, then the next
+                            scene's, so it touches fewer scenes' trees */
 
 typedef struct ct_server_opts {
   uint64_t hbmBudget;   /* bytes of progressive buffers resident at once (0: 4 GiB) */
   uint64_t maxPaths;    /* paths per launch round (0: the tracer default, 2^26) */
   uint32_t maxActive;   /* admitted jobs at most (0: 1024) */
   uint32_t slicePasses; /* passes one job adds per round at most (0: as many as fit) */
-  int32_t policy;       /* CT_SERVE_FIFO or CT_SERVE_SRPT */
+  int32_t policy;       /* CT_SERVE_FIFO, CT_SERVE_SRPT or CT_SERVE_SCENE */
   int32_t graph;        /* replay rounds of a repeated shape as hipGraphs */
   int32_t precision;    /* CT_PRECISION_F64 or CT_PRECISION_F32 (compensation terms
                            double each job's buffer footprint) */

@@ -52,6 +52,7 @@ def rel_err(g, o):
     {},                                            # one round takes everything
     {"max_paths": 4096, "slice_passes": 1},         # many rounds, jobs interleaved pass by pass
     {"max_paths": 8192, "policy": "srpt"},
+    {"max_paths": 8192, "policy": "scene"},          # each scene's jobs together
     {"hbm_budget": 64 * 48 * 12 + 1, "max_paths": 16384},  # HBM admits about one job at a time
     {"max_active": 2, "max_paths": 8192, "graph": True}])
 def test_alone_equals_in_mix(ct, O, opts):
