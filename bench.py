@@ -116,14 +116,36 @@ def _band_rates(O, view, frame, w, h, spp, depth, seed, threads, rows, acc, tria
     return rates, rays
 
 
+def _js_rates(O, view, frame, w, h, spp, depth, seed, threads, rows, trials):
+    """Trials of renderImage's passes over a band by the JS restatement of the
+    reference tracer on `threads` Node worker threads (after a warm-up row per
+    thread).  Returns (Mrays/s per trial, rays per trial, report, sums)."""
+    rep, sums = O.render_band_js(view, frame, w, h, spp, seed, rows, threads, trials,
+                                 max_bounces=depth)
+    rays = [t["extension"] + t["shadow"] for t in rep["trials"]]
+    rates = [r / t["seconds"] / 1e6 for r, t in zip(rays, rep["trials"])]
+    return rates, rays[-1], rep, sums
+
+
+def _band_size(probe_s, per, budget_s, trials, h):
+    """Rows (a multiple of `per`) for trials of about budget_s/trials seconds,
+    from the seconds one pass set over `per` rows took."""
+    return min(h, per * max(1, int(budget_s / trials / max(probe_s, 1e-3))))
+
+
 def cpu_baseline(ct, ps, frame, w, h, spp, depth, seed, budget_s=12.0):
-    """Oracle (CPU restatement of tracer.ts) on bands of rows of the same frame
-    -- same camera, seeds, depth and passes -- in the protocol of
-    scripts/bench.ts:59-131: a warm-up pass, then three timed trials; every
-    trial is reported and the best is the value.  Once on every core the
-    process may use, once on one thread (comparable with bench.ts's own
-    single-threaded figures).  Also returns the band's image (renderImage's
-    average) for the parity check of the timed frame."""
+    """The reference tracer's algorithm on the host's cores, over a band of rows
+    of the same frame -- same camera, seeds, depth and passes -- in the protocol
+    of scripts/bench.ts:59-131: a warm-up, then three timed trials; every trial
+    is reported and the best is the value.
+
+    The value is the JS restatement of tracer.ts (oracle/js/trace.js: the
+    reference's arithmetic in the JS engine, Math functions included) on Node
+    worker threads, one per core the process may use, with a one-thread figure
+    beside it; the C oracle's figures on the same cores follow for comparison.
+    Without a node executable the C oracle is the value.  Also returns the
+    band's image (renderImage's average; the JS and C restatements agree bit
+    for bit) for the parity check of the timed frame."""
     sys.path.insert(0, ROOT)
     from oracle import oracle as O  # the checker, timed as the CPU baseline
 
@@ -133,40 +155,85 @@ def cpu_baseline(ct, ps, frame, w, h, spp, depth, seed, budget_s=12.0):
                     ps.order, ps.sky, getattr(ps, "leafWords", None))
     cpus = host_cpus()
     threads = cpus["allowed"]
+    trials = 3
+    mid = h // 2
+    out = {"unit": "Mrays/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
+           "host": cpus}
     acc = np.zeros(w * h * 3, np.float32)
-    # single thread: probe one row, then a band of about a second per trial
-    y1 = h // 2
+
+    def c_single(rows1):
+        return _band_rates(O, view, frame, w, h, spp, depth, seed, 1, (mid, mid + rows1), acc)
+
+    node = O.node_binary()
+    if node is not None:
+        # one thread: probe one row, then a band of about a second per trial
+        _, _, rep, _ = _js_rates(O, view, frame, w, h, spp, depth, seed, 1, (mid, mid + 1), 1)
+        rows1 = _band_size(rep["trials"][0]["seconds"], 1, 1.0 * trials, trials, h)
+        r1, rays1, _, _ = _js_rates(O, view, frame, w, h, spp, depth, seed, 1,
+                                    (mid, mid + rows1), trials)
+        # every allowed core: probe one row per thread, then the band
+        y0 = max(0, mid - threads // 2)
+        _, _, rep, _ = _js_rates(O, view, frame, w, h, spp, depth, seed, threads,
+                                 (y0, min(h, y0 + threads)), 1)
+        rows = _band_size(rep["trials"][0]["seconds"], threads, budget_s, trials, h)
+        y0 = max(0, (h - rows) // 2)
+        rates, rays, rep, sums = _js_rates(O, view, frame, w, h, spp, depth, seed, threads,
+                                           (y0, y0 + rows), trials)
+        best = max(rates)
+        band = sums[y0 * w * 3:(y0 + rows) * w * 3].astype(np.float64) * (1.0 / spp)
+        out.update({
+            "value": best, "impl": f"JS restatement of tracer.ts on Node {rep['node']} worker "
+                                   f"threads (oracle/js)",
+            "trials": [round(r, 4) for r in rates],
+            "single_thread": {"value": round(max(r1), 4), "unit": "Mrays/s",
+                              "trials": [round(r, 4) for r in r1],
+                              "sample": f"rows {mid}..{mid + rows1 - 1}, {spp} passes "
+                                        f"({rays1} rays) per trial"},
+            "sample": f"rows {y0}..{y0 + rows - 1} of the {w}x{h} frame, {spp} passes "
+                      f"({rays} rays) per trial, {trials} trials after a warm-up row per "
+                      f"thread, best {rows * w * spp * best / rays:.4f} spp*Mpix/s"})
+        # the C oracle on the same cores, one trial over a band of a third of the budget
+        t = time.perf_counter()
+        O.accumulate_pass(view, frame, w, h, 7919 * seed, acc, max_bounces=depth,
+                          threads=threads, rows=(mid, min(h, mid + threads)))
+        crow = _band_size((time.perf_counter() - t) * spp, threads, budget_s / 3, 1, h)
+        cy0 = max(0, (h - crow) // 2)
+        cr, crays = _band_rates(O, view, frame, w, h, spp, depth, seed, threads,
+                                (cy0, cy0 + crow), acc, 1)
+        c1, _ = c_single(max(1, rows1 // 2))
+        out["c_oracle"] = {"value": round(cr[0], 4), "cores": threads,
+                           "single_thread": round(c1[0], 4),
+                           "sample": f"rows {cy0}..{cy0 + crow - 1}, {spp} passes ({crays} "
+                                     f"rays), one trial"}
+        return out, ((y0, y0 + rows), band.astype(np.float32))
+    # no node: the C oracle is the value
     t = time.perf_counter()
     O.accumulate_pass(view, frame, w, h, 7919 * seed, acc, max_bounces=depth, threads=1,
-                      rows=(y1, y1 + 1))
-    probe1 = time.perf_counter() - t
-    rows1 = max(1, min(h, int(1.0 / max(probe1 * spp, 1e-4))))
-    r1, rays1 = _band_rates(O, view, frame, w, h, spp, depth, seed, 1, (y1, y1 + rows1), acc)
-    # all allowed cores: warm-up pass over `threads` rows, then the band
-    y0 = max(0, h // 2 - threads // 2)
+                      rows=(mid, mid + 1))
+    rows1 = _band_size((time.perf_counter() - t) * spp, 1, 1.0 * trials, trials, h)
+    r1, rays1 = c_single(rows1)
+    y0 = max(0, mid - threads // 2)
     t = time.perf_counter()
     O.accumulate_pass(view, frame, w, h, 7919 * seed, acc, max_bounces=depth, threads=threads,
                       rows=(y0, min(h, y0 + threads)))
-    probe = time.perf_counter() - t  # wall seconds for one pass over `threads` rows
-    trials = 3
-    rows = threads * max(1, int(budget_s / trials / max(probe * spp, 1e-3)))
-    rows = min(rows, h)
-    y0 = max(0, (h - rows) // 2)  # a band centred on the frame, the whole frame if affordable
+    rows = _band_size((time.perf_counter() - t) * spp, threads, budget_s, trials, h)
+    y0 = max(0, (h - rows) // 2)
     acc[:] = 0
     rates, rays = _band_rates(O, view, frame, w, h, spp, depth, seed, threads, (y0, y0 + rows),
                               acc, trials)
     best = max(rates)
     band = acc[y0 * w * 3:(y0 + rows) * w * 3].astype(np.float64) * (1.0 / spp)
-    return {"value": best, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "cpu_model": cpu_model(), "host": cpus, "trials": [round(r, 4) for r in rates],
-            "single_thread": {"value": round(max(r1), 4), "unit": "Mrays/s",
-                              "trials": [round(r, 4) for r in r1],
-                              "sample": f"rows {y1}..{y1 + rows1 - 1}, {spp} passes "
-                                        f"({rays1} rays) per trial"},
-            "sample": f"rows {y0}..{y0 + rows - 1} of the {w}x{h} frame, {spp} passes "
-                      f"({rays} rays) per trial, 3 trials after a warm-up pass, best "
-                      f"{rows * w * spp * best / rays:.4f} spp*Mpix/s"}, \
-        ((y0, y0 + rows), band.astype(np.float32))
+    out.update({
+        "value": best, "impl": "C oracle (no node executable)",
+        "trials": [round(r, 4) for r in rates],
+        "single_thread": {"value": round(max(r1), 4), "unit": "Mrays/s",
+                          "trials": [round(r, 4) for r in r1],
+                          "sample": f"rows {mid}..{mid + rows1 - 1}, {spp} passes "
+                                    f"({rays1} rays) per trial"},
+        "sample": f"rows {y0}..{y0 + rows - 1} of the {w}x{h} frame, {spp} passes ({rays} "
+                  f"rays) per trial, {trials} trials after a warm-up pass, best "
+                  f"{rows * w * spp * best / rays:.4f} spp*Mpix/s"})
+    return out, ((y0, y0 + rows), band.astype(np.float32))
 
 
 def main():
@@ -438,11 +505,12 @@ def plain_form(ct, args):
     cpu = None
     parity = None
     if not args.no_cpu:
-        log("CPU baseline (oracle on host cores)...")
+        log("CPU baseline (the reference tracer restated in JS, Node worker threads)...")
         cpu, (rows, band) = cpu_baseline(ct, ps, frame, w, h, spp, depth, opts["seed"],
                                          args.cpu_budget)
         log(f"CPU baseline {cpu['value']:.3f} Mrays/s on {cpu['cores']} threads "
-            f"({cpu['host']}), one thread {cpu['single_thread']['value']:.3f} Mrays/s")
+            f"({cpu['host']}), one thread {cpu['single_thread']['value']:.3f} Mrays/s "
+            f"[{cpu['impl']}]; C oracle {cpu.get('c_oracle')}")
         # the timed frame against the oracle's rendering of the same rows
         g = out.numpy()[rows[0] * w * 3:rows[1] * w * 3].astype(np.float64)
         o = band.astype(np.float64)
@@ -638,8 +706,8 @@ def batch_form(ct, args):
     if not args.no_cpu:
         j = max(jobs, key=lambda x: x["width"])
         ps = packed[j["sceneIndex"]]
-        cpu = cpu_baseline(ct, ps, j["frame"], j["width"], j["height"], j["spp"], j["maxBounces"],
-                           args.cpu_budget)
+        cpu, _ = cpu_baseline(ct, ps, j["frame"], j["width"], j["height"], j["spp"],
+                              j["maxBounces"], j.get("seed", 0), args.cpu_budget)
     return {
         "metric": METRIC, "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),

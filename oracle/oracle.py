@@ -682,6 +682,67 @@ def render_pixels(ps, frame, width, height, pixels, spp, seed=0, max_bounces=8,
 # ---------------------------------------------------------------------------
 # tonemap.ts
 
+JS_BENCH = os.path.join(_HERE, "js", "cpu_bench.js")
+
+
+def node_binary():
+    """Path of the `node` executable, or None."""
+    import shutil
+    return shutil.which("node")
+
+
+def render_band_js(ps, frame, width, height, spp, seed, rows, threads, trials=1,
+                   max_bounces=8, russian_roulette=True, nee=True, workdir=None, timeout=None):
+    """renderImage's passes over rows [y0, y1) by the JS restatement of the
+    reference tracer (oracle/js/trace.js) on `threads` Node worker threads,
+    `trials` timed runs after a warm-up.  Returns (report, sums): the report
+    of cpu_bench.js (seconds and rays per trial, thread count, CPU model) and
+    the float32 per-pixel sums of the band's passes (w*h*3, zero outside the
+    band) -- not yet times 1/spp."""
+    import json
+    import subprocess
+    import tempfile
+
+    node = node_binary()
+    if node is None:
+        raise RuntimeError("render_band_js: no node executable")
+    own = workdir is None
+    d = tempfile.mkdtemp(prefix="ct_js_") if own else workdir
+    try:
+        arrays = {"prims": ("f32", ps.prims), "nodes": ("f32", ps.nodes),
+                  "mats": ("f32", ps.mats), "lights": ("i32", ps.lights),
+                  "primObject": ("u32", ps.primObject)}
+        if ps.leafWords is not None:
+            arrays["leafWords"] = ("u32", ps.leafWords)
+        dtypes = {"f32": np.float32, "i32": np.int32, "u32": np.uint32}
+        for name, (t, a) in arrays.items():
+            np.ascontiguousarray(a, dtypes[t]).tofile(os.path.join(d, f"{name}.{t}"))
+        sky = ps.sky if ps.sky.get("kind") == "gradient" else {"kind": "none"}
+        if sky.get("kind") == "gradient":
+            sky = {"kind": "gradient", "zenith": [float(x) for x in sky["zenith"]],
+                   "horizon": [float(x) for x in sky["horizon"]],
+                   "strength": float(sky["strength"])}
+        job = {"arrays": {k: v[0] for k, v in arrays.items()}, "nodeCount": int(ps.nodeCount),
+               "lightCount": int(len(ps.lights)), "sky": sky,
+               "frame": [float(x) for x in frame], "width": int(width), "height": int(height),
+               "spp": int(spp), "seed": int(seed) & 0xFFFFFFFF, "maxBounces": int(max_bounces),
+               "russianRoulette": bool(russian_roulette), "nee": bool(nee),
+               "rows": [int(rows[0]), int(rows[1])], "threads": int(threads),
+               "trials": int(trials)}
+        with open(os.path.join(d, "job.json"), "w") as f:
+            json.dump(job, f)
+        r = subprocess.run([node, JS_BENCH, d], capture_output=True, text=True, timeout=timeout)
+        if r.returncode != 0:
+            raise RuntimeError(f"cpu_bench.js failed ({r.returncode}): {r.stderr[-2000:]}")
+        report = json.loads(r.stdout.strip().splitlines()[-1])
+        sums = np.fromfile(os.path.join(d, "accum.f32"), np.float32)
+        return report, sums
+    finally:
+        if own:
+            import shutil
+            shutil.rmtree(d, ignore_errors=True)
+
+
 def aces(x):
     return lib().o64_aces(x)
 
