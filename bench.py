@@ -517,7 +517,9 @@ def plain_form(ct, args):
         den = np.maximum(np.abs(g), np.abs(o))
         rel = np.where(den > 0, np.abs(g - o) / np.where(den > 0, den, 1), 0.0)
         px = rel.reshape(-1, 3).max(1)
-        parity = {"reference": "oracle", "rows": [rows[0], rows[1]],
+        parity = {"reference": "oracle" if cpu["impl"].startswith("C") else
+                  "oracle's JS restatement (equal to the C oracle bit for bit)",
+                  "rows": [rows[0], rows[1]],
                   "pixels": int(px.size), "max_rel": float(rel.max()),
                   "pixels_over_1e-4": int((px > 1e-4).sum()),
                   "exact_fraction": round(float((g == o).mean()), 6)}
