@@ -42,6 +42,13 @@ constexpr int LSLOTS = 4;        // nodes per block aggregated in LDS
 constexpr int MAX_BINS = 32;
 constexpr int MEDIAN_MAX = 2048;  // largest all-degenerate node the LDS sort takes;
                                   // larger ones go through k_msort_tiles/k_msort_merge
+// Nodes of up to this many positions are reduced by one wave each
+// (k_bounds_small, k_bin_small); the chunked kernels take the larger ones,
+// of which a chunk then touches at most three, all within its LDS slots.
+constexpr uint32_t SMALL_NODE = 2048;
+// Bounds of nodes of up to this many positions are reduced by one lane each.
+constexpr uint32_t TINY_NODE = 16;
+constexpr int WAVES = BB / 64;
 
 enum : int32_t { KIND_LEAF = 0, KIND_SAH = 1, KIND_MEDIAN = 2 };
 
@@ -97,7 +104,8 @@ struct Build {
   // bins per SAH node: 3 * bins * (count + 6 keys)
   uint32_t* binCount;
   unsigned long long* binKey;
-  uint32_t* levelInfo;       // [0] nSplit [1] nMedian [2] nSah
+  uint32_t* levelInfo;       // [0] splits [1] median nodes [2] inconsistent partitions
+                             // [3] median nodes above the LDS sort
   uint32_t* scanTmp;         // block sums for scans
 };
 
@@ -212,10 +220,32 @@ __global__ void k_level_init(Build b, uint32_t m) {
     for (int k = 0; k < 12; k++)
       b.keys[12 * (size_t)j + k] = (k < 3 || (k >= 6 && k < 9)) ? KEY_POS_INF : KEY_NEG_INF;
   }
-  if (blockIdx.x == 0 && threadIdx.x < 2) b.levelInfo[threadIdx.x] = 0;
+  // [0] splits, [1] median nodes, [3] median nodes above the LDS sort; [2]
+  // (inconsistent partitions) accumulates over the levels
+  if (blockIdx.x == 0 && (threadIdx.x < 2 || threadIdx.x == 3)) b.levelInfo[threadIdx.x] = 0;
 }
 
-// First active local node index in this block's chunk (or m if none).
+__device__ __forceinline__ bool small_node(const Build& b, uint32_t rec) {
+  return b.recEnd[rec] - b.recStart[rec] <= SMALL_NODE;
+}
+
+__device__ __forceinline__ unsigned long long wave_min(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_xor(v, off);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ unsigned long long wave_max(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_xor(v, off);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// First active local node index in this block's chunk that the chunked
+// kernels reduce (not a small node), or m if none.
 __device__ uint32_t block_first_node(Build& b, uint32_t c0, uint32_t lvStart, uint32_t m,
                                      uint32_t* sFirst) {
   if (threadIdx.x == 0) *sFirst = m;
@@ -223,7 +253,7 @@ __device__ uint32_t block_first_node(Build& b, uint32_t c0, uint32_t lvStart, ui
   uint32_t mine = m;
   for (uint32_t i = c0 + threadIdx.x; i < c0 + CHUNK && i < b.n; i += BB) {
     int32_t r = b.nodeOf[i];
-    if (r >= 0) {
+    if (r >= 0 && !small_node(b, (uint32_t)r)) {
       mine = (uint32_t)r - lvStart;
       break;
     }
@@ -245,7 +275,7 @@ __global__ void __launch_bounds__(BB) k_bounds(Build b, uint32_t lvStart, uint32
   uint32_t j0 = block_first_node(b, c0, lvStart, m, &sFirst);
   for (uint32_t i = c0 + threadIdx.x; i < c0 + CHUNK && i < b.n; i += BB) {
     int32_t r = b.nodeOf[i];
-    if (r < 0) continue;
+    if (r < 0 || small_node(b, (uint32_t)r)) continue;
     uint32_t j = (uint32_t)r - lvStart;
     size_t o = 3 * (size_t)b.order[i];
     unsigned long long v[12];
@@ -284,6 +314,66 @@ __global__ void __launch_bounds__(BB) k_bounds(Build b, uint32_t lvStart, uint32
     unsigned long long* g = b.keys + 12 * (size_t)j + f;
     if (isMin) atomicMin(g, x);
     else atomicMax(g, x);
+  }
+}
+
+// Bounds of the tiny nodes: one lane per node, stored without atomics (the
+// chunked k_bounds skips every small node).
+__global__ void k_bounds_tiny(Build b, uint32_t lvStart, uint32_t m) {
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += stride) {
+    const uint32_t s = b.recStart[lvStart + j], e = b.recEnd[lvStart + j];
+    if (e - s > TINY_NODE) continue;
+    unsigned long long v[12];
+    for (int k = 0; k < 12; k++)
+      v[k] = (k < 3 || (k >= 6 && k < 9)) ? KEY_POS_INF : KEY_NEG_INF;
+    for (uint32_t i = s; i < e; i++) {
+      size_t o = 3 * (size_t)b.order[i];
+      for (int k = 0; k < 3; k++) {
+        unsigned long long lo = dkey(b.bmin[o + k]), hi = dkey(b.bmax[o + k]);
+        unsigned long long c = dkey(b.bcen[o + k]);
+        v[k] = lo < v[k] ? lo : v[k];
+        v[3 + k] = hi > v[3 + k] ? hi : v[3 + k];
+        v[6 + k] = c < v[6 + k] ? c : v[6 + k];
+        v[9 + k] = c > v[9 + k] ? c : v[9 + k];
+      }
+    }
+    for (int k = 0; k < 12; k++) b.keys[12 * (size_t)j + k] = v[k];
+  }
+}
+
+// Bounds of the small nodes above TINY_NODE: one wave per node (wave w takes
+// nodes w, w + waves, ...), keys reduced in registers and across the wave,
+// stored without atomics.
+__global__ void __launch_bounds__(BB) k_bounds_small(Build b, uint32_t lvStart, uint32_t m) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t j = wave; j < m; j += nwaves) {  // wave-uniform
+    const uint32_t s = b.recStart[lvStart + j], e = b.recEnd[lvStart + j];
+    if (e - s <= TINY_NODE || e - s > SMALL_NODE) continue;
+    unsigned long long v[12];
+    for (int k = 0; k < 12; k++)
+      v[k] = (k < 3 || (k >= 6 && k < 9)) ? KEY_POS_INF : KEY_NEG_INF;
+    for (uint32_t i = s + lane; i < e; i += 64) {
+      size_t o = 3 * (size_t)b.order[i];
+      for (int k = 0; k < 3; k++) {
+        unsigned long long lo = dkey(b.bmin[o + k]), hi = dkey(b.bmax[o + k]);
+        unsigned long long c = dkey(b.bcen[o + k]);
+        v[k] = lo < v[k] ? lo : v[k];
+        v[3 + k] = hi > v[3 + k] ? hi : v[3 + k];
+        v[6 + k] = c < v[6 + k] ? c : v[6 + k];
+        v[9 + k] = c > v[9 + k] ? c : v[9 + k];
+      }
+    }
+    for (int k = 0; k < 3; k++) {
+      v[k] = wave_min(v[k]);
+      v[3 + k] = wave_max(v[3 + k]);
+      v[6 + k] = wave_min(v[6 + k]);
+      v[9 + k] = wave_max(v[9 + k]);
+    }
+    if (lane == 0)
+      for (int k = 0; k < 12; k++) b.keys[12 * (size_t)j + k] = v[k];
   }
 }
 
@@ -327,11 +417,11 @@ __global__ void k_setup(Build b, uint32_t lvStart, uint32_t m) {
   }
 }
 
-__global__ void k_bin_init(Build b, uint32_t m) {
+__global__ void k_bin_init(Build b, uint32_t lvStart, uint32_t m) {
   uint32_t stride = gridDim.x * blockDim.x;
   const int per = 3 * b.bins;
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += stride) {
-    if (b.kind[j] != KIND_SAH) continue;
+    if (b.kind[j] != KIND_SAH || small_node(b, lvStart + j)) continue;
     size_t s = (size_t)b.binSlot[j] * per;
     for (int q = 0; q < per; q++) {
       b.binCount[s + q] = 0;
@@ -370,7 +460,7 @@ __global__ void __launch_bounds__(BB) k_bin(Build b, uint32_t lvStart, uint32_t 
     int32_t r = b.nodeOf[i];
     if (r < 0) continue;
     uint32_t j = (uint32_t)r - lvStart;
-    if (b.kind[j] != KIND_SAH) continue;
+    if (b.kind[j] != KIND_SAH || small_node(b, (uint32_t)r)) continue;
     size_t o = 3 * (size_t)b.order[i];
     unsigned long long mn[3], mx[3];
     for (int k = 0; k < 3; k++) {
@@ -419,69 +509,94 @@ __device__ __forceinline__ double surface_area(double dx, double dy, double dz) 
   return 2 * (dx * dy + dy * dz + dz * dx);  // bvh.ts:70
 }
 
-// Suffix/prefix sweeps and the strict-'<' argmin (bvh.ts:213-257).
-__global__ void k_sweep(Build b, uint32_t m) {
-  uint32_t stride = gridDim.x * blockDim.x;
+// Suffix/prefix sweeps of one axis of SAH node j (bvh.ts:213-257): the first
+// split of strictly least cost, or split -1 when no split has both sides
+// non-empty.  cnt/key: the node's 3*bins bucket counts and 6 keys per bucket.
+__device__ void sweep_axis(const Build& b, uint32_t j, int a, const uint32_t* cnt0,
+                           const unsigned long long* key0, double& bestCost, int& bestSplit) {
   const int bins = b.bins;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += stride) {
-    if (b.kind[j] != KIND_SAH) continue;
-    size_t base = (size_t)b.binSlot[j] * 3 * bins;
-    double bestCost = __builtin_inf();
-    int bestAxis = -1, bestSplit = -1;
-    double bestOrigin = 0, bestScale = 0;
-    double rightArea[MAX_BINS];
-    uint32_t rightCount[MAX_BINS];
-    for (int a = 0; a < 3; a++) {
-      double tob = b.tob3[3 * (size_t)j + a];
-      if (tob == 0) continue;
-      const uint32_t* cnt = b.binCount + base + a * bins;
-      const unsigned long long* key = b.binKey + 6 * (base + a * bins);
-      double rx0 = __builtin_inf(), ry0 = rx0, rz0 = rx0;
-      double rx1 = -__builtin_inf(), ry1 = rx1, rz1 = rx1;
-      uint32_t rc = 0;
-      for (int s = bins - 1; s >= 1; s--) {
-        if (cnt[s] > 0) {
-          double v;
-          v = dval(key[6 * s + 0]); if (v < rx0) rx0 = v;
-          v = dval(key[6 * s + 1]); if (v < ry0) ry0 = v;
-          v = dval(key[6 * s + 2]); if (v < rz0) rz0 = v;
-          v = dval(key[6 * s + 3]); if (v > rx1) rx1 = v;
-          v = dval(key[6 * s + 4]); if (v > ry1) ry1 = v;
-          v = dval(key[6 * s + 5]); if (v > rz1) rz1 = v;
-          rc += cnt[s];
-        }
-        rightCount[s - 1] = rc;
-        rightArea[s - 1] = rc > 0 ? surface_area(rx1 - rx0, ry1 - ry0, rz1 - rz0) : 0;
-      }
-      double lx0 = __builtin_inf(), ly0 = lx0, lz0 = lx0;
-      double lx1 = -__builtin_inf(), ly1 = lx1, lz1 = lx1;
-      uint32_t lc = 0;
-      for (int s = 0; s < bins - 1; s++) {
-        if (cnt[s] > 0) {
-          double v;
-          v = dval(key[6 * s + 0]); if (v < lx0) lx0 = v;
-          v = dval(key[6 * s + 1]); if (v < ly0) ly0 = v;
-          v = dval(key[6 * s + 2]); if (v < lz0) lz0 = v;
-          v = dval(key[6 * s + 3]); if (v > lx1) lx1 = v;
-          v = dval(key[6 * s + 4]); if (v > ly1) ly1 = v;
-          v = dval(key[6 * s + 5]); if (v > lz1) lz1 = v;
-          lc += cnt[s];
-        }
-        uint32_t rcs = rightCount[s];
-        if (lc == 0 || rcs == 0) continue;
-        double cost = (double)lc * surface_area(lx1 - lx0, ly1 - ly0, lz1 - lz0) +
-                      (double)rcs * rightArea[s];
-        if (cost < bestCost) {
-          bestCost = cost;
-          bestAxis = a;
-          bestSplit = s;
-          bestOrigin = b.cmin3[3 * (size_t)j + a];
-          bestScale = tob;
-        }
-      }
+  bestCost = __builtin_inf();
+  bestSplit = -1;
+  if (b.tob3[3 * (size_t)j + a] == 0) return;
+  double rightArea[MAX_BINS];
+  uint32_t rightCount[MAX_BINS];
+  const uint32_t* cnt = cnt0 + a * bins;
+  const unsigned long long* key = key0 + 6 * (a * bins);
+  double rx0 = __builtin_inf(), ry0 = rx0, rz0 = rx0;
+  double rx1 = -__builtin_inf(), ry1 = rx1, rz1 = rx1;
+  uint32_t rc = 0;
+  for (int s = bins - 1; s >= 1; s--) {
+    if (cnt[s] > 0) {
+      double v;
+      v = dval(key[6 * s + 0]); if (v < rx0) rx0 = v;
+      v = dval(key[6 * s + 1]); if (v < ry0) ry0 = v;
+      v = dval(key[6 * s + 2]); if (v < rz0) rz0 = v;
+      v = dval(key[6 * s + 3]); if (v > rx1) rx1 = v;
+      v = dval(key[6 * s + 4]); if (v > ry1) ry1 = v;
+      v = dval(key[6 * s + 5]); if (v > rz1) rz1 = v;
+      rc += cnt[s];
+    }
+    rightCount[s - 1] = rc;
+    rightArea[s - 1] = rc > 0 ? surface_area(rx1 - rx0, ry1 - ry0, rz1 - rz0) : 0;
+  }
+  double lx0 = __builtin_inf(), ly0 = lx0, lz0 = lx0;
+  double lx1 = -__builtin_inf(), ly1 = lx1, lz1 = lx1;
+  uint32_t lc = 0;
+  for (int s = 0; s < bins - 1; s++) {
+    if (cnt[s] > 0) {
+      double v;
+      v = dval(key[6 * s + 0]); if (v < lx0) lx0 = v;
+      v = dval(key[6 * s + 1]); if (v < ly0) ly0 = v;
+      v = dval(key[6 * s + 2]); if (v < lz0) lz0 = v;
+      v = dval(key[6 * s + 3]); if (v > lx1) lx1 = v;
+      v = dval(key[6 * s + 4]); if (v > ly1) ly1 = v;
+      v = dval(key[6 * s + 5]); if (v > lz1) lz1 = v;
+      lc += cnt[s];
+    }
+    uint32_t rcs = rightCount[s];
+    if (lc == 0 || rcs == 0) continue;
+    double cost = (double)lc * surface_area(lx1 - lx0, ly1 - ly0, lz1 - lz0) +
+                  (double)rcs * rightArea[s];
+    if (cost < bestCost) {
+      bestCost = cost;
+      bestSplit = s;
+    }
+  }
+}
+
+// The sweeps of every SAH node, four lanes per node (lane a < 3 sweeps axis a),
+// and the choice over the axes in the reference's order: a later axis wins
+// only with a strictly lower cost (bvh.ts:249-256); no split on any axis
+// falls back to the median split.
+__global__ void k_sweep(Build b, uint32_t m) {
+  const uint32_t stride = gridDim.x * blockDim.x;  // a multiple of 4
+  for (uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x; tid < 4 * m; tid += stride) {
+    // the four lanes of a group share j and run the same iterations
+    const uint32_t j = tid >> 2;
+    const int a = (int)(tid & 3);
+    const bool sah = b.kind[j] == KIND_SAH;
+    double cost = __builtin_inf();
+    int split = -1;
+    if (sah && a < 3) {
+      size_t base = (size_t)b.binSlot[j] * 3 * b.bins;
+      sweep_axis(b, j, a, b.binCount + base, b.binKey + 6 * base, cost, split);
+    }
+    double c1 = __shfl_down(cost, 1, 4), c2 = __shfl_down(cost, 2, 4);
+    int s1 = __shfl_down(split, 1, 4), s2 = __shfl_down(split, 2, 4);
+    if (!sah || a != 0) continue;
+    int bestAxis = split >= 0 ? 0 : -1, bestSplit = split;
+    double bestCost = cost;
+    if (s1 >= 0 && c1 < bestCost) {
+      bestCost = c1;
+      bestAxis = 1;
+      bestSplit = s1;
+    }
+    if (s2 >= 0 && c2 < bestCost) {
+      bestCost = c2;
+      bestAxis = 2;
+      bestSplit = s2;
     }
     if (bestAxis < 0) {
-      // no split with both sides non-empty: fall back to the median split
       const unsigned long long* kk = b.keys + 12 * (size_t)j;
       double ex = dval(kk[9]) - dval(kk[6]);
       double ey = dval(kk[10]) - dval(kk[7]);
@@ -492,9 +607,74 @@ __global__ void k_sweep(Build b, uint32_t m) {
     } else {
       b.axis[j] = bestAxis;
       b.split[j] = bestSplit;
-      b.origin[j] = bestOrigin;
-      b.scale[j] = bestScale;
+      b.origin[j] = b.cmin3[3 * (size_t)j + bestAxis];
+      b.scale[j] = b.tob3[3 * (size_t)j + bestAxis];
     }
+  }
+}
+
+// Completes this wave's LDS traffic before the wave reads what it wrote (the
+// slice is the wave's own: no other wave touches it).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Buckets of the small SAH nodes: one wave per node (wave w takes nodes w,
+// w + waves, ...), the histogram in the wave's own LDS slice, then stored
+// without atomics (the chunked k_bin skips these nodes, and k_bin_init leaves
+// their buckets to this kernel).
+__global__ void __launch_bounds__(BB) k_bin_small(Build b, uint32_t lvStart, uint32_t m) {
+  __shared__ uint32_t sCount[WAVES * 3 * MAX_BINS];
+  __shared__ unsigned long long sKey[WAVES * 3 * MAX_BINS * 6];
+  const int bins = b.bins;
+  const int per = 3 * bins;
+  const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  uint32_t* cnt = sCount + w * 3 * MAX_BINS;
+  unsigned long long* key = sKey + (size_t)w * 3 * MAX_BINS * 6;
+  for (uint32_t j = wave; j < m; j += nwaves) {  // wave-uniform
+    const uint32_t s = b.recStart[lvStart + j], e = b.recEnd[lvStart + j];
+    if (e - s > SMALL_NODE || b.kind[j] != KIND_SAH) continue;
+    for (int q = (int)lane; q < per; q += 64) {
+      cnt[q] = 0;
+      for (int k = 0; k < 3; k++) {
+        key[6 * q + k] = KEY_POS_INF;
+        key[6 * q + 3 + k] = KEY_NEG_INF;
+      }
+    }
+    wave_lds_sync();
+    double cmin[3], tob[3];
+    for (int a = 0; a < 3; a++) {
+      cmin[a] = b.cmin3[3 * (size_t)j + a];
+      tob[a] = b.tob3[3 * (size_t)j + a];
+    }
+    for (uint32_t i = s + lane; i < e; i += 64) {
+      size_t o = 3 * (size_t)b.order[i];
+      unsigned long long mn[3], mx[3];
+      for (int k = 0; k < 3; k++) {
+        mn[k] = dkey(b.bmin[o + k]);
+        mx[k] = dkey(b.bmax[o + k]);
+      }
+      for (int a = 0; a < 3; a++) {
+        if (tob[a] == 0) continue;
+        int q = a * bins + bucket_of(b.bcen[o + a], cmin[a], tob[a], bins);
+        atomicAdd(cnt + q, 1u);
+        for (int k = 0; k < 3; k++) {
+          atomicMin(key + 6 * q + k, mn[k]);
+          atomicMax(key + 6 * q + 3 + k, mx[k]);
+        }
+      }
+    }
+    wave_lds_sync();
+    const size_t gs = (size_t)b.binSlot[j] * per;
+    for (int q = (int)lane; q < per; q += 64) {
+      b.binCount[gs + q] = cnt[q];
+      for (int k = 0; k < 6; k++) b.binKey[6 * (gs + q) + k] = key[6 * q + k];
+    }
+    wave_lds_sync();
   }
 }
 
@@ -623,7 +803,13 @@ __global__ void __launch_bounds__(BB) k_median(Build b, uint32_t lvStart, uint32
     uint32_t rec = lvStart + j;
     uint32_t start = b.recStart[rec], end = b.recEnd[rec];
     uint32_t n = end - start;
-    if (n > (uint32_t)MEDIAN_MAX) continue;  // sorted by the merge kernels
+    if (n > (uint32_t)MEDIAN_MAX) {  // sorted by the merge kernels, after the level
+      if (threadIdx.x == 0) {
+        b.mid[j] = (start + end) >> 1;
+        atomicAdd(b.levelInfo + 3, 1u);
+      }
+      continue;
+    }
     int axis = b.axis[j];
     uint32_t P2 = 1;
     while (P2 < n) P2 <<= 1;
@@ -1055,28 +1241,46 @@ int bvh_build(const double* d_min, const double* d_max, const double* d_cen, uin
     levels.push_back({lvStart, lvEnd});
     unsigned gM = grid_for(m, BB);
     BVH_LAUNCH(k_level_init, gM, BB, b, m);
+    // waves of the per-node kernels: one per node, at most 32768
+    const unsigned gW = (unsigned)((std::min<uint32_t>(m, 32768u) + WAVES - 1) / WAVES);
     BVH_LAUNCH(k_bounds, gChunks, BB, b, lvStart, m);
+    BVH_LAUNCH(k_bounds_tiny, gM, BB, b, lvStart, m);
+    BVH_LAUNCH(k_bounds_small, gW, BB, b, lvStart, m);
     BVH_LAUNCH(k_setup, gM, BB, b, lvStart, m);
     rc = scan_u32(b.flag, b.binSlot, m, b.scanTmp, st);
     if (rc) return fail(rc);
-    BVH_LAUNCH(k_bin_init, gM, BB, b, m);
+    BVH_LAUNCH(k_bin_init, gM, BB, b, lvStart, m);
     BVH_LAUNCH(k_bin, gChunks, BB, b, lvStart, m);
-    BVH_LAUNCH(k_sweep, gM, BB, b, m);
+    BVH_LAUNCH(k_bin_small, gW, BB, b, lvStart, m);
+    BVH_LAUNCH(k_sweep, grid_for(4 * (size_t)m, BB), BB, b, m);
     BVH_LAUNCH(k_flags, gN, BB, b, lvStart);
     rc = scan_u32(b.good, b.goodScan, n, b.scanTmp, st);
     if (rc) return fail(rc);
     BVH_LAUNCH(k_ab, gN, BB, b, lvStart);
     BVH_LAUNCH(k_meet, gM, BB, b, lvStart, m);
     BVH_LAUNCH(k_scatter, gN, BB, b, lvStart);
-    // median nodes are rare; find out whether this level has any
-    uint32_t info[3] = {0, 0, 0};
+    // median nodes up to MEDIAN_MAX positions are sorted here (the kernel
+    // passes over a level without any quickly); it sets every median node's
+    // split position, so the children can be laid out before larger ones are
+    // sorted
+    BVH_LAUNCH(k_median, std::min<uint32_t>(m, 4096u), BB, b, lvStart, m);
+    BVH_LAUNCH(k_split_flags, gM, BB, b, m);
+    rc = scan_u32(b.flag, b.flagScan, m, b.scanTmp, st);
+    if (rc) return fail(rc);
+    BVH_LAUNCH(k_children, gM, BB, b, lvStart, lvEnd, m);
+    BVH_LAUNCH(k_node_of, gN, BB, b, lvStart);
+    // one host round trip per level: the split count (the next level's size)
+    // and whether any median node is too large for the LDS sort
+    uint32_t info[4] = {0, 0, 0, 0};
     {
       hipError_t e = hipMemcpyAsync(info, b.levelInfo, sizeof info, hipMemcpyDeviceToHost, st);
       if (e == hipSuccess) e = hipStreamSynchronize(st);
       if (e != hipSuccess) return fail(check_hip(e, "ct_bvh_build: level info"));
     }
-    if (info[1]) {
-      // every median node must fit the LDS sort
+    if (info[3]) {
+      // nodes above the LDS sort's size: tiled merge sort, one node at a time,
+      // in the level's key buffer (free once k_setup has read it; 96 B per
+      // position, the sort takes 24), into the next level's order
       std::vector<int32_t> kinds(m);
       std::vector<uint32_t> st0(m), en0(m);
       hipError_t e = hipMemcpyAsync(kinds.data(), b.kind, m * 4, hipMemcpyDeviceToHost, st);
@@ -1084,11 +1288,6 @@ int bvh_build(const double* d_min, const double* d_max, const double* d_cen, uin
       if (e == hipSuccess) e = hipMemcpyAsync(en0.data(), b.recEnd + lvStart, m * 4, hipMemcpyDeviceToHost, st);
       if (e == hipSuccess) e = hipStreamSynchronize(st);
       if (e != hipSuccess) return fail(check_hip(e, "ct_bvh_build: median check"));
-      unsigned gMed = (unsigned)std::min<uint32_t>(m, 4096u);
-      BVH_LAUNCH(k_median, gMed, BB, b, lvStart, m);
-      // nodes above the LDS sort's size: tiled merge sort, one node at a time,
-      // in the level's key buffer (free once k_setup has read it; 96 B per
-      // position, the sort takes 24)
       double* keyA = (double*)b.keys;
       double* keyB = keyA + N;
       uint32_t* posA = (uint32_t*)(keyB + N);
@@ -1113,17 +1312,7 @@ int bvh_build(const double* d_min, const double* d_max, const double* d_cen, uin
         BVH_LAUNCH(k_msort_emit, grid_for(len, BB), BB, b, j, st0[j], en0[j], pi);
       }
     }
-    BVH_LAUNCH(k_split_flags, gM, BB, b, m);
-    rc = scan_u32(b.flag, b.flagScan, m, b.scanTmp, st);
-    if (rc) return fail(rc);
-    BVH_LAUNCH(k_children, gM, BB, b, lvStart, lvEnd, m);
-    BVH_LAUNCH(k_node_of, gN, BB, b, lvStart);
     std::swap(b.order, b.orderNext);
-    {
-      hipError_t e = hipMemcpyAsync(info, b.levelInfo, sizeof info, hipMemcpyDeviceToHost, st);
-      if (e == hipSuccess) e = hipStreamSynchronize(st);
-      if (e != hipSuccess) return fail(check_hip(e, "ct_bvh_build: level info"));
-    }
     if (info[2]) return fail(set_error(CT_EDEVICE, "ct_bvh_build: inconsistent partition"));
     uint32_t nSplit = info[0];
     lvStart = lvEnd;
