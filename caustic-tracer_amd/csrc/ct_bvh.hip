@@ -69,6 +69,9 @@ struct Build {
   const double* bmin;
   const double* bmax;
   const double* bcen;
+  // the inputs gathered per primitive, 9 doubles: bmin.xyz bmax.xyz bcen.xyz
+  // (one primitive's bounds in one or two cache lines for the per-level gathers)
+  double* box;
   uint32_t n;
   int bins;
   // position arrays
@@ -211,6 +214,12 @@ __global__ void k_init_positions(Build b) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += stride) {
     b.order[i] = i;
     b.nodeOf[i] = 0;
+    double* x = b.box + 9 * (size_t)i;
+    for (int k = 0; k < 3; k++) {
+      x[k] = b.bmin[3 * (size_t)i + k];
+      x[3 + k] = b.bmax[3 * (size_t)i + k];
+      x[6 + k] = b.bcen[3 * (size_t)i + k];
+    }
   }
 }
 
@@ -277,12 +286,12 @@ __global__ void __launch_bounds__(BB) k_bounds(Build b, uint32_t lvStart, uint32
     int32_t r = b.nodeOf[i];
     if (r < 0 || small_node(b, (uint32_t)r)) continue;
     uint32_t j = (uint32_t)r - lvStart;
-    size_t o = 3 * (size_t)b.order[i];
+    const double* x = b.box + 9 * (size_t)b.order[i];
     unsigned long long v[12];
     for (int k = 0; k < 3; k++) {
-      v[k] = dkey(b.bmin[o + k]);
-      v[3 + k] = dkey(b.bmax[o + k]);
-      v[6 + k] = dkey(b.bcen[o + k]);
+      v[k] = dkey(x[k]);
+      v[3 + k] = dkey(x[3 + k]);
+      v[6 + k] = dkey(x[6 + k]);
       v[9 + k] = v[6 + k];
     }
     if (j - j0 < (uint32_t)LSLOTS) {
@@ -328,10 +337,10 @@ __global__ void k_bounds_tiny(Build b, uint32_t lvStart, uint32_t m) {
     for (int k = 0; k < 12; k++)
       v[k] = (k < 3 || (k >= 6 && k < 9)) ? KEY_POS_INF : KEY_NEG_INF;
     for (uint32_t i = s; i < e; i++) {
-      size_t o = 3 * (size_t)b.order[i];
+      const double* x = b.box + 9 * (size_t)b.order[i];
       for (int k = 0; k < 3; k++) {
-        unsigned long long lo = dkey(b.bmin[o + k]), hi = dkey(b.bmax[o + k]);
-        unsigned long long c = dkey(b.bcen[o + k]);
+        unsigned long long lo = dkey(x[k]), hi = dkey(x[3 + k]);
+        unsigned long long c = dkey(x[6 + k]);
         v[k] = lo < v[k] ? lo : v[k];
         v[3 + k] = hi > v[3 + k] ? hi : v[3 + k];
         v[6 + k] = c < v[6 + k] ? c : v[6 + k];
@@ -356,10 +365,10 @@ __global__ void __launch_bounds__(BB) k_bounds_small(Build b, uint32_t lvStart, 
     for (int k = 0; k < 12; k++)
       v[k] = (k < 3 || (k >= 6 && k < 9)) ? KEY_POS_INF : KEY_NEG_INF;
     for (uint32_t i = s + lane; i < e; i += 64) {
-      size_t o = 3 * (size_t)b.order[i];
+      const double* x = b.box + 9 * (size_t)b.order[i];
       for (int k = 0; k < 3; k++) {
-        unsigned long long lo = dkey(b.bmin[o + k]), hi = dkey(b.bmax[o + k]);
-        unsigned long long c = dkey(b.bcen[o + k]);
+        unsigned long long lo = dkey(x[k]), hi = dkey(x[3 + k]);
+        unsigned long long c = dkey(x[6 + k]);
         v[k] = lo < v[k] ? lo : v[k];
         v[3 + k] = hi > v[3 + k] ? hi : v[3 + k];
         v[6 + k] = c < v[6 + k] ? c : v[6 + k];
@@ -461,18 +470,18 @@ __global__ void __launch_bounds__(BB) k_bin(Build b, uint32_t lvStart, uint32_t 
     if (r < 0) continue;
     uint32_t j = (uint32_t)r - lvStart;
     if (b.kind[j] != KIND_SAH || small_node(b, (uint32_t)r)) continue;
-    size_t o = 3 * (size_t)b.order[i];
+    const double* x = b.box + 9 * (size_t)b.order[i];
     unsigned long long mn[3], mx[3];
     for (int k = 0; k < 3; k++) {
-      mn[k] = dkey(b.bmin[o + k]);
-      mx[k] = dkey(b.bmax[o + k]);
+      mn[k] = dkey(x[k]);
+      mx[k] = dkey(x[3 + k]);
     }
     bool local = j - j0 < (uint32_t)LSLOTS;
     size_t gs = (size_t)b.binSlot[j] * per;
     for (int a = 0; a < 3; a++) {
       double tob = b.tob3[3 * (size_t)j + a];
       if (tob == 0) continue;
-      int bk = bucket_of(b.bcen[o + a], b.cmin3[3 * (size_t)j + a], tob, bins);
+      int bk = bucket_of(x[6 + a], b.cmin3[3 * (size_t)j + a], tob, bins);
       if (local) {
         int q = (int)(j - j0) * per + a * bins + bk;
         atomicAdd(sCount + q, 1u);
@@ -652,15 +661,15 @@ __global__ void __launch_bounds__(BB) k_bin_small(Build b, uint32_t lvStart, uin
       tob[a] = b.tob3[3 * (size_t)j + a];
     }
     for (uint32_t i = s + lane; i < e; i += 64) {
-      size_t o = 3 * (size_t)b.order[i];
+      const double* x = b.box + 9 * (size_t)b.order[i];
       unsigned long long mn[3], mx[3];
       for (int k = 0; k < 3; k++) {
-        mn[k] = dkey(b.bmin[o + k]);
-        mx[k] = dkey(b.bmax[o + k]);
+        mn[k] = dkey(x[k]);
+        mx[k] = dkey(x[3 + k]);
       }
       for (int a = 0; a < 3; a++) {
         if (tob[a] == 0) continue;
-        int q = a * bins + bucket_of(b.bcen[o + a], cmin[a], tob[a], bins);
+        int q = a * bins + bucket_of(x[6 + a], cmin[a], tob[a], bins);
         atomicAdd(cnt + q, 1u);
         for (int k = 0; k < 3; k++) {
           atomicMin(key + 6 * q + k, mn[k]);
@@ -687,7 +696,7 @@ __global__ void k_flags(Build b, uint32_t lvStart) {
     if (r >= 0) {
       uint32_t j = (uint32_t)r - lvStart;
       if (b.kind[j] == KIND_SAH) {
-        double c = b.bcen[3 * (size_t)b.order[i] + b.axis[j]];
+        double c = b.box[9 * (size_t)b.order[i] + 6 + b.axis[j]];
         int bk = bucket_of(c, b.origin[j], b.scale[j], b.bins);
         g = bk <= b.split[j] ? 1u : 0u;
       }
@@ -815,7 +824,7 @@ __global__ void __launch_bounds__(BB) k_median(Build b, uint32_t lvStart, uint32
     while (P2 < n) P2 <<= 1;
     for (uint32_t t = threadIdx.x; t < P2; t += BB) {
       if (t < n) {
-        sKey[t] = b.bcen[3 * (size_t)b.order[start + t] + axis];
+        sKey[t] = b.box[9 * (size_t)b.order[start + t] + 6 + axis];
         sPos[t] = t;
       } else {
         sKey[t] = __builtin_inf();
@@ -871,7 +880,7 @@ __global__ void __launch_bounds__(BB) k_msort_tiles(Build b, uint32_t start, uin
     uint32_t len = min((uint32_t)MEDIAN_MAX, n - t0);
     for (uint32_t t = threadIdx.x; t < (uint32_t)MEDIAN_MAX; t += BB) {
       if (t < len) {
-        sKey[t] = b.bcen[3 * (size_t)b.order[start + t0 + t] + axis];
+        sKey[t] = b.box[9 * (size_t)b.order[start + t0 + t] + 6 + axis];
         sPos[t] = t0 + t;
       } else {
         sKey[t] = __builtin_inf();
@@ -1052,7 +1061,7 @@ extern "C" int ct_bvh_build(const double* d_min, const double* d_max, const doub
 
 namespace {
 
-constexpr int SCRATCH_SECTIONS = 30;
+constexpr int SCRATCH_SECTIONS = 31;
 
 // Device bytes of the build's working memory for n primitives and `bins` buckets.
 size_t build_scratch_bytes(size_t N, int bins, size_t* offsets) {
@@ -1062,7 +1071,7 @@ size_t build_scratch_bytes(size_t N, int bins, size_t* offsets) {
                           M * 96, M * 4, M * 4, M * 4, M * 8, M * 8, M * 24, M * 24,
                           M * 4, (M + 1) * 4, (M + 1) * 4, M * 4, M * 4,
                           NS * 3 * (size_t)bins * 4, NS * 3 * (size_t)bins * 48, 16,
-                          scan_tmp_entries(N + 1) * 4};
+                          scan_tmp_entries(N + 1) * 4, N * 72};
   static_assert(sizeof(sizes) / sizeof(sizes[0]) == SCRATCH_SECTIONS, "one offset per section");
   size_t sz = 0, k = 0;
   for (size_t b : sizes) {
@@ -1175,7 +1184,7 @@ int bvh_build(const double* d_min, const double* d_max, const double* d_cen, uin
                oSplit = off[16], oOrigin = off[17], oScale = off[18], oCmin3 = off[19],
                oTob3 = off[20], oFlag = off[21], oFlagScan = off[22], oBinSlot = off[23],
                oMid = off[24], oMeetX = off[25], oBinCount = off[26], oBinKey = off[27],
-               oLevelInfo = off[28], oScanTmp = off[29];
+               oLevelInfo = off[28], oScanTmp = off[29], oBox = off[30];
   Build b;
   b.bmin = d_min;
   b.bmax = d_max;
@@ -1213,6 +1222,7 @@ int bvh_build(const double* d_min, const double* d_max, const double* d_cen, uin
   b.binKey = (unsigned long long*)(mem + oBinKey);
   b.levelInfo = (uint32_t*)(mem + oLevelInfo);
   b.scanTmp = (uint32_t*)(mem + oScanTmp);
+  b.box = (double*)(mem + oBox);
 
   int rc = CT_OK;
   std::vector<std::pair<uint32_t, uint32_t>> levels;

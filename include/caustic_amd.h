@@ -400,7 +400,7 @@ int ct_bvh_build_ex(const double *d_min, const double *d_max, const double *d_ce
  * hipMalloc, harnesses that forbid it).  No counterpart in the reference.
  * The build's working memory is a static device array of
  * CT_BVH_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends: at 12
- * bins that is room for about 91,000 primitives (the call returns CT_ELIMIT
+ * bins that is room for about 83,000 primitives (the call returns CT_ELIMIT
  * with the bytes it needs when count is larger).  Every call on one GPU
  * shares that array: calls that overlap on one GPU race, so the caller
  * serialises them (the call synchronises its stream before returning). */
