@@ -145,13 +145,19 @@ k_scan_tile(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_
     if (i < n) out[i] = run;
     run += v[k];
   }
-  if (threadIdx.x == SCAN_BLOCK - 1) tileSums[blockIdx.x] = s[SCAN_BLOCK - 1];
+  if (threadIdx.x == SCAN_BLOCK - 1) {
+    tileSums[blockIdx.x] = s[SCAN_BLOCK - 1];
+    if (gridDim.x == 1) out[n] = s[SCAN_BLOCK - 1];  // one tile: its sum is the total
+  }
 }
 
+// tileOffsets: the exclusive scan of the tile sums, gridDim.x + 1 entries (the
+// last is the total, written to out[n])
 __global__ void __launch_bounds__(SCAN_BLOCK)
 k_scan_add(uint32_t* __restrict__ out, uint32_t n, const uint32_t* __restrict__ tileOffsets) {
   uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
   uint32_t add = tileOffsets[blockIdx.x];
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = tileOffsets[gridDim.x];
   for (int k = 0; k < SCAN_ITEMS; k++) {
     uint32_t i = base + k;
     if (i < n) out[i] += add;
@@ -159,11 +165,6 @@ k_scan_add(uint32_t* __restrict__ out, uint32_t n, const uint32_t* __restrict__ 
 }
 
 __global__ void k_store_u32(uint32_t* p, uint32_t v) { *p = v; }
-
-__global__ void k_scan_total(uint32_t* out, uint32_t n, const uint32_t* in) {
-  // out[n] = out[n-1] + in[n-1]  (n >= 1)
-  out[n] = out[n - 1] + in[n - 1];
-}
 
 // out[0..n] = exclusive scan of in[0..n-1]; tmp needs room for the tile sums of
 // every recursion level (about n / 1000 entries).
@@ -184,8 +185,6 @@ static int scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp
     hipLaunchKernelGGL(k_scan_add, dim3(tiles), dim3(SCAN_BLOCK), 0, st, out, n, offs);
     CT_LAUNCH_CHECK("k_scan_add");
   }
-  hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(1), 0, st, out, n, in);
-  CT_LAUNCH_CHECK("k_scan_total");
   return CT_OK;
 }
 
