@@ -523,3 +523,40 @@ def test_render_small_shapes_and_depths(ct, O, w, h, spp, depth):
     else:
         assert np.isfinite(g).all()
         assert rel_err(g, o).max() <= REL_TOL
+
+
+def test_deepest_paths_and_the_depth_limit(ct, O):
+    """maxBounces 254, the deepest the per-bounce counters hold: without Russian
+    roulette every path in the closed furnace bounces 254 times, and the image
+    equals the oracle's; 255 is refused with CT_ELIMIT (tracer.ts:209 loops to
+    any depth), never traced to a different depth."""
+    sc = scenes.furnace()
+    ps = O.pack_scene(sc)
+    fr = O.camera_frame(sc["camera"], 4 / 3)
+    opts = {"width": 4, "height": 3, "spp": 2, "seed": 3, "maxBounces": 254,
+            "russianRoulette": False, "nee": False}
+    g = ct.renderImage(ps, fr, opts)
+    o = O.render_image(ps, fr, 4, 3, 2, seed=3, max_bounces=254, russian_roulette=False,
+                       nee=False)
+    # every path sees the shell's emission at each of its 255 hits: sum of 0.8^k
+    assert np.allclose(g, 5 * (1 - 0.8 ** 255), rtol=1e-6)
+    assert rel_err(g, o).max() <= REL_TOL
+    with pytest.raises(Exception, match="maxBounces"):
+        ct.renderImage(ps, fr, dict(opts, maxBounces=255))
+
+
+@pytest.mark.parametrize("sky", [{"kind": "none"},
+                                 {"kind": "gradient", "zenith": (0.3, 0.5, 0.9),
+                                  "horizon": (1.0, 0.9, 0.8), "strength": 1.5}],
+                         ids=["black", "gradient"])
+def test_render_empty_scene(ct, O, sky):
+    """No primitives (bvh.ts:93-100's single empty leaf): every camera ray
+    misses, the image is the sky of tracer.ts:211-216 or black, as the oracle's."""
+    sc = {"objects": [], "sky": sky, "camera": scenes.cornell_box()["camera"]}
+    ps = O.pack_scene(sc)
+    fr = O.camera_frame(sc["camera"], 1.5)
+    g = ct.renderImage(ps, fr, {"width": 12, "height": 8, "spp": 3, "seed": 2, "maxBounces": 4})
+    o = O.render_image(ps, fr, 12, 8, 3, seed=2, max_bounces=4)
+    assert np.array_equal(g, o)
+    assert (g > 0).all() if sky["kind"] == "gradient" else (g == 0).all()
+    assert not ct.occluded(ps, np.array([[0, 0, 0, 0, 1.0, 0]]), np.array([10.0]))[0]
