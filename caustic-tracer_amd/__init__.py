@@ -8,7 +8,9 @@ HIP kernels through libcaustic_amd.so; importing works without a GPU, and any
 call that needs the library fails loudly when it is missing.
 """
 from . import display, presets, scene_io, serve, synth
-from .presets import SCENE_PRESETS, presetById
+from .presets import SCENE_PRESETS, Rng, presetById
+from .presets import pcg_hash as pcgHash
+from .presets import seed_for as seedFor
 from .scene_io import SceneParseError, parseScene, parseSceneJSON, sceneToJSON
 from .display import decodePPM, encodePPM, tonemapBuffer
 from ._capi import CTError, DeviceBuffer, init, sync
@@ -17,7 +19,7 @@ from .tracer import (PRECISION_F32, PRECISION_F64, T_MIN, WALK_BRUTE, WALK_ORDER
                      WALK_STACKLESS, WALK_WIDE, DeviceScene, Tracer, accumulatePass, cameraFrame,
                      closest_hit, closest_hit32, closest_hit32_counts, closest_hit_counts,
                      image_sum_f64, intersectClosest, occluded, occluded32, pick, primaryRay,
-                     renderImage)
+                     radiance_batch, radianceInto, renderImage)
 from .serve import Server
 
 __all__ = [
@@ -27,5 +29,5 @@ __all__ = [
     "PRECISION_F64", "PRECISION_F32", "closest_hit32", "closest_hit32_counts", "occluded32",
     "image_sum_f64", "pick", "primaryRay", "DeviceScene", "Tracer",
     "accumulatePass", "cameraFrame", "closest_hit", "closest_hit_counts", "intersectClosest",
-    "occluded", "renderImage",
+    "occluded", "radianceInto", "radiance_batch", "renderImage", "Rng", "pcgHash", "seedFor",
 ]

@@ -112,6 +112,7 @@ SIGNATURES = {
     "ct_tracer_set_schedule": [vp, i32, i32],
     "ct_accumulate_pass": [vp, vp, vp, u32, u32, u32, C.POINTER(TraceOpts), vp, vp],
     "ct_render_image": [vp, vp, vp, C.POINTER(RenderOpts), vp, vp],
+    "ct_radiance": [vp, vp, vp, u32, vp, C.POINTER(TraceOpts), vp, vp],
     "ct_tracer_stats": [vp, C.POINTER(RenderStats)],
     "ct_render_batch": [vp, vp, u32, vp],
     "ct_accumulate_batch": [vp, vp, u32, vp],

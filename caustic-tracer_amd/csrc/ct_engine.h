@@ -35,6 +35,10 @@ struct PlanJob {
   // by the caller across calls, or null for per-call terms folded into
   // `accum` when the call's last pass is added
   float* comp = nullptr;
+  // radianceInto jobs: rays, RNG states in and out, radiance out (JobDev)
+  const double* rays = nullptr;
+  uint32_t* rngIO = nullptr;
+  double* radiance = nullptr;
 };
 
 Frame make_frame(const double* frame16);

@@ -55,6 +55,19 @@ k_generate(PathSoA P, const JobDev* __restrict__ jobs, int njobs, size_t base, s
     uint32_t pixel = (uint32_t)(local - (size_t)pass * npix);
     uint32_t y = pixel / width, x = pixel - y * width;
     Rng rng;
+    if (J.rays) {  // radianceInto: the caller's ray and RNG state (tracer.ts:191-198)
+      const double* r = J.rays + 6 * (size_t)pixel;
+      P.ox[pid] = r[0]; P.oy[pid] = r[1]; P.oz[pid] = r[2];
+      P.dx[pid] = r[3]; P.dy[pid] = r[4]; P.dz[pid] = r[5];
+      P.br[pid] = 1; P.bg[pid] = 1; P.bb[pid] = 1;
+      P.Lr[pid] = 0; P.Lg[pid] = 0; P.Lb[pid] = 0;
+      P.prevPdf[pid] = 0;
+      P.specular[pid] = 1;
+      P.rng[pid] = J.rngIO[pixel];
+      if (MULTI) P.jobOf[pid] = (uint32_t)j;
+      P.q0[i] = (uint32_t)pid;
+      continue;
+    }
     rng.s = seed_for(pixel, J.frameBase + pass);
     double sx = (x + rng.next()) / width;
     double sy = 1 - (y + rng.next()) / height;
@@ -472,6 +485,8 @@ k_shade(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__
       P.ox[pid] = px;
       P.oy[pid] = py;
       P.oz[pid] = pz;
+      // kept for ended paths too: radianceInto hands the state back
+      P.rng[pid] = rng.s;
       if (cont && bounce < opts.maxBounces) {
         alive = true;
         P.dx[pid] = ndx;
@@ -482,7 +497,6 @@ k_shade(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__
         P.bb[pid] = bb;
         P.prevPdf[pid] = prevPdf;
         P.specular[pid] = specularBounce ? 1 : 0;
-        P.rng[pid] = rng.s;
       }
     }
     wave_push(alive, pid, qout, countOut);
@@ -791,6 +805,14 @@ k_finalize(PathSoA P, const JobDev* __restrict__ jobs, int njobs, size_t totalPi
   for (size_t gi = (size_t)blockIdx.x * blockDim.x + threadIdx.x; gi < totalPix; gi += stride) {
     const JobDev& J = jobs[job_of<MULTI, true>(jobs, njobs, gi)];
     uint32_t pix = (uint32_t)(gi - J.pixelBase);
+    if (J.rays) {  // radianceInto: the path's radiance as it is, and its RNG state
+      const size_t pid = J.pathBase + pix;
+      J.radiance[3 * (size_t)pix] = P.Lr[pid];
+      J.radiance[3 * (size_t)pix + 1] = P.Lg[pid];
+      J.radiance[3 * (size_t)pix + 2] = P.Lb[pid];
+      J.rngIO[pix] = P.rng[pid];
+      continue;
+    }
     float* acc = J.accum + 3 * (size_t)pix;
     float a0 = acc[0], a1 = acc[1], a2 = acc[2];
     for (uint32_t s = 0; s < J.passes; s++) {
@@ -1068,6 +1090,9 @@ struct RoundJob {
   double scale;
   float* comp = nullptr;  // FP32 compensation terms
   int fold = 0;
+  const double* rays = nullptr;  // radianceInto jobs (JobDev)
+  uint32_t* rngIO = nullptr;
+  double* radiance = nullptr;
 };
 
 static void mark_cb(void* ctx, hipStream_t st, int kind, bool begin) {
@@ -1276,6 +1301,9 @@ static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t s
     d.accum = j.accum;
     d.comp = j.comp;
     d.fold = j.fold;
+    d.rays = j.rays;
+    d.rngIO = j.rngIO;
+    d.radiance = j.radiance;
     pb += (size_t)d.npix * j.passes;
     xb += d.npix;
   }
@@ -1430,6 +1458,9 @@ int ct::run_plan(ct_tracer* T, std::vector<PlanJob> jobs, hipStream_t st) {
       r.scale = j.scale;
       r.comp = comp[k];
       r.fold = (comp[k] && !j.comp && done[k] + p == j.count) ? 1 : 0;
+      r.rays = j.rays;
+      r.rngIO = j.rngIO;
+      r.radiance = j.radiance;
       R.push_back(r);
       done[k] += p;
       budget -= p * npix;
@@ -1646,6 +1677,30 @@ int ct_render_image(ct_tracer* T, const ct_scene* s, const double* frame16,
             opts->width, opts->height, opts->seed * 7919u, 0, opts->spp, out, true,
             opts->spp ? 1.0 / opts->spp : 1.0};
   return run_plan(T, {j}, st);
+}
+
+int ct_radiance(ct_tracer* T, const ct_scene* s, const double* rays, uint32_t n,
+                uint32_t* rngStates, const ct_trace_opts* opts, double* out, void* stream) {
+  if (!T || !s || !opts || (n && (!rays || !rngStates || !out)))
+    return set_error(CT_EINVAL, "ct_radiance: null argument");
+  if (opts->maxBounces < 0 || opts->maxBounces + 2 > MAX_BOUNCE_SLOTS)
+    return set_error(CT_ELIMIT, "ct_radiance: maxBounces must be in [0, 254]");
+  if (T->precision != CT_PRECISION_F64)
+    return set_error(CT_EINVAL, "ct_radiance: radianceInto is traced in binary64 only");
+  if (n == 0) return CT_OK;
+  // rays beyond the path capacity go in further jobs of the same plan
+  std::vector<PlanJob> plan;
+  const size_t cap = T->maxPaths;
+  for (size_t k = 0; k < n; k += cap) {
+    uint32_t m = (uint32_t)std::min<size_t>(cap, n - k);
+    PlanJob j{s, Frame{}, {opts->maxBounces, opts->russianRoulette != 0, opts->nee != 0},
+              m, 1, 0, 0, 1, nullptr, false, 1.0};
+    j.rays = rays + 6 * k;
+    j.rngIO = rngStates + k;
+    j.radiance = out + 3 * k;
+    plan.push_back(j);
+  }
+  return run_plan(T, plan, (hipStream_t)stream);
 }
 
 int ct_render_batch(ct_tracer* T, const ct_job* jobs, uint32_t n, void* stream) {

@@ -25,6 +25,12 @@ struct JobDev {
   float* accum;        // width*height*3 float32 sum
   float* comp;         // width*height*3 compensation terms (FP32 precision form)
   int fold;            // add the compensation into the sum after this round (FP32)
+  // radianceInto jobs (binary64): the paths start from these rays (6 doubles
+  // each, one pass, width = ray count) with these RNG states, and end with
+  // their radiance (3 doubles each) and RNG states written back; accum unused
+  const double* rays;
+  uint32_t* rngIO;
+  double* radiance;
 };
 
 struct PathSoA {

@@ -20,12 +20,27 @@ def pcg_hash(x):
     return ((word >> 22) ^ word) & M32
 
 
+def seed_for(pixel, frame):
+    """seedFor (rng.ts:18-23)."""
+    a = pcg_hash(pixel & M32)
+    b = pcg_hash((frame & M32) ^ 0x9E3779B9)
+    s = pcg_hash(a ^ ((b * 0x85EBCA6B) & M32))
+    return 0x1234567 if s == 0 else s
+
+
 class Rng:
     """Rng (rng.ts:28-64): PCG-seeded xorshift32."""
 
     def __init__(self, seed):
         s = pcg_hash(seed & M32)
         self.state = 0x9E3779B9 if s == 0 else s
+
+    @staticmethod
+    def fromPixel(pixel, frame):
+        """Rng.fromPixel (rng.ts:33-37): the state is seedFor(pixel, frame)."""
+        r = Rng(0)
+        r.state = seed_for(pixel, frame)
+        return r
 
     def next_u32(self):
         x = self.state
@@ -40,6 +55,12 @@ class Rng:
 
     def range(self, lo, hi):
         return lo + (hi - lo) * self.next()
+
+    def int(self, n):
+        """rng.ts:62-64: an integer in [0, n)."""
+        return min(n - 1, int(self.next() * n // 1))
+
+    nextU32 = next_u32
 
 
 def _xorshift_step(x):

@@ -301,6 +301,15 @@ class Tracer:
                                        frame_index & 0xFFFFFFFF, C.byref(o), d_accum.ptr, stream),
               "ct_accumulate_pass")
 
+    def radiance(self, ps, d_rays, n, d_rng, opts, d_out, stream=None):
+        """radianceInto (tracer.ts:191-328) of n rays on the device: d_rays
+        (n*6 float64), d_rng (n uint32 RNG states, advanced in place), d_out
+        (n*3 float64)."""
+        o = A.TraceOpts(int(opts.get("maxBounces", 8)), int(bool(opts.get("russianRoulette", True))),
+                        int(bool(opts.get("nee", True))))
+        check(lib().ct_radiance(self.handle, device_scene(ps).handle, d_rays.ptr, int(n),
+                                d_rng.ptr, C.byref(o), d_out.ptr, stream), "ct_radiance")
+
     def render_image(self, ps, frame, opts, d_out, stream=None):
         o = A.RenderOpts(opts["width"], opts["height"], opts["spp"], opts.get("seed", 0) & 0xFFFFFFFF,
                          int(opts.get("maxBounces", 8)), int(bool(opts.get("russianRoulette", True))),
@@ -428,6 +437,34 @@ def accumulatePass(ps, frame, width, height, frameIndex, opts, accum):
     d = DeviceBuffer.from_array(np.ascontiguousarray(accum, np.float32))
     _tracer().accumulate_pass(ps, frame, width, height, frameIndex, opts, d)
     accum[...] = d.numpy().reshape(accum.shape)
+
+
+def radiance_batch(ps, rays, rng_states, opts):
+    """radianceInto (tracer.ts:191-328) of many rays: rays (n, 6) origin and
+    direction, rng_states (n,) the xorshift32 states of their Rng objects.
+    Returns (radiance (n, 3) float64, the states after the calls)."""
+    rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
+    n = len(rays)
+    st = np.ascontiguousarray(rng_states, np.uint32).reshape(n)
+    if n == 0:
+        return np.zeros((0, 3)), st.copy()
+    d_rays = DeviceBuffer.from_array(rays)
+    d_rng = DeviceBuffer.from_array(st)
+    d_out = DeviceBuffer.empty((n * 3,), np.float64)
+    _tracer().radiance(ps, d_rays, n, d_rng, opts, d_out)
+    return d_out.numpy().reshape(n, 3), d_rng.numpy()
+
+
+def radianceInto(ps, ox, oy, oz, dx, dy, dz, rng, opts, out=None):
+    """radianceInto (tracer.ts:191-328): the radiance along one ray, drawing
+    from `rng` (an Rng; its state advances as the reference's does), into
+    `out` (3 floats, a new array when omitted), which is returned."""
+    L, st = radiance_batch(ps, [[ox, oy, oz, dx, dy, dz]], [rng.state], opts)
+    rng.state = int(st[0])
+    if out is None:
+        out = np.zeros(3)
+    out[0], out[1], out[2] = L[0]
+    return out
 
 
 def renderImage(ps, frame, opts):

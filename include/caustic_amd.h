@@ -229,6 +229,15 @@ int ct_accumulate_pass(ct_tracer *tracer, const ct_scene *scene, const double *h
  * spp 0 every value is NaN, as the reference's zero sums times 1/0. */
 int ct_render_image(ct_tracer *tracer, const ct_scene *scene, const double *h_frame16,
                     const ct_render_opts *opts, float *d_out, void *stream);
+/* radianceInto (tracer.ts:191-328) for n rays at once: ray i (6 doubles at
+ * d_rays + 6i: origin, direction) is traced with the xorshift32 state
+ * d_rngStates[i] (the Rng passed to radianceInto), through at most
+ * opts->maxBounces bounces; its radiance (3 doubles at d_out + 3i, as the
+ * reference returns it: no finiteness filter) is written and d_rngStates[i]
+ * left as the Rng is after the call.  Binary64 arithmetic only (CT_EINVAL
+ * under CT_PRECISION_F32); maxBounces above 254: CT_ELIMIT. */
+int ct_radiance(ct_tracer *tracer, const ct_scene *scene, const double *d_rays, uint32_t n,
+                uint32_t *d_rngStates, const ct_trace_opts *opts, double *d_out, void *stream);
 /* Several scenes / cameras / sizes / depths traced in shared launches: the
  * paths of all jobs share every extend, shade and shadow launch (rounds of at
  * most the tracer's path capacity).  Each output equals what the job alone

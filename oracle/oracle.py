@@ -585,14 +585,16 @@ def light_pdf(ps, p, h, hn, prim):
                                hn[1], hn[2], int(prim))
 
 
-def radiance(ps, o, d, state, max_bounces=8, russian_roulette=True, nee=True):
-    """radianceInto (tracer.ts:191-328) of one ray with RNG state `state`."""
+def radiance(ps, o, d, state, max_bounces=8, russian_roulette=True, nee=True,
+             with_state=False):
+    """radianceInto (tracer.ts:191-328) of one ray with RNG state `state`; with
+    with_state also the RNG state after the call."""
     rng = O64Rng(state & 0xFFFFFFFF)
     out = np.zeros(3)
     opts = _opts(max_bounces, russian_roulette, nee)
     lib().o64_radiance(C.byref(ps.view), o[0], o[1], o[2], d[0], d[1], d[2], C.byref(rng),
                        C.byref(opts), _ptr(out), None)
-    return out
+    return (out, rng.state) if with_state else out
 
 
 def _opts(max_bounces=8, russian_roulette=True, nee=True):

@@ -560,3 +560,56 @@ def test_render_empty_scene(ct, O, sky):
     assert np.array_equal(g, o)
     assert (g > 0).all() if sky["kind"] == "gradient" else (g == 0).all()
     assert not ct.occluded(ps, np.array([[0, 0, 0, 0, 1.0, 0]]), np.array([10.0]))[0]
+
+
+@pytest.mark.parametrize("preset", ["cornell", "night", "glass", "mirror"])
+def test_radiance_into_matches_oracle(ct, O, preset):
+    """radianceInto (tracer.ts:191-328) of rays with the caller's RNG states:
+    the radiance bit for bit and the RNG states after the calls equal the
+    oracle's, for camera rays with pixel-seeded states and for rays from
+    inside the scene with arbitrary states;
+    also with NEE and Russian roulette off, and for rays past the path
+    capacity (further jobs of one plan)."""
+    sc = ct.presetById(preset)["scene"]
+    ps = O.pack_scene(sc)
+    fr = O.camera_frame(sc["camera"], 1.0)
+    rs = np.random.default_rng(17)
+    rays, states = [], []
+    for k in range(200):
+        x, y = int(rs.integers(0, 64)), int(rs.integers(0, 64))
+        o, d = O.camera_sample(fr, 64, 64, x, y, k)
+        rays.append(np.concatenate([o, d]))
+        states.append(O.seed_for(y * 64 + x, 1000 + k))
+    for k in range(100):
+        o = rs.uniform(-0.5, 0.5, 3)
+        v = rs.normal(size=3)
+        rays.append(np.concatenate([o, v / np.linalg.norm(v)]))
+        states.append(int(rs.integers(1, 1 << 32)))
+    rays = np.array(rays)
+    states = np.array(states, np.uint32)
+    for opts in ({"maxBounces": 8}, {"maxBounces": 12, "nee": False, "russianRoulette": False}):
+        L, st = ct.radiance_batch(ps, rays, states, opts)
+        for i in range(len(rays)):
+            oL, ost = O.radiance(ps, rays[i, :3], rays[i, 3:], int(states[i]),
+                                 max_bounces=opts["maxBounces"],
+                                 russian_roulette=opts.get("russianRoulette", True),
+                                 nee=opts.get("nee", True), with_state=True)
+            assert np.array_equal(L[i], oL), (i, L[i], oL)
+            assert st[i] == ost
+    # the single-ray mirror updates the caller's Rng as the reference does
+    rng = ct.Rng.fromPixel(5, 9)
+    out = ct.radianceInto(ps, *rays[0], rng, {"maxBounces": 8})
+    oL, ost = O.radiance(ps, rays[0, :3], rays[0, 3:], int(ct.seedFor(5, 9)), with_state=True)
+    assert np.array_equal(out, oL) and rng.state == ost
+    # more rays than one round holds: a tracer of small capacity runs several jobs
+    big = np.tile(rays, (9, 1))[:2500]
+    big_states = (np.tile(states, 9)[:2500] + np.arange(2500, dtype=np.uint32)) | 1
+    tr = ct.Tracer()
+    tr.set_capacity(1024)  # three jobs of at most 1024 rays
+    d_rays = ct.DeviceBuffer.from_array(np.ascontiguousarray(big))
+    d_rng = ct.DeviceBuffer.from_array(big_states.copy())
+    d_out = ct.DeviceBuffer.empty((len(big) * 3,), np.float64)
+    tr.radiance(ps, d_rays, len(big), d_rng, {"maxBounces": 8}, d_out)
+    L8, st8 = ct.radiance_batch(ps, big, big_states, {"maxBounces": 8})
+    assert np.array_equal(d_out.numpy().reshape(-1, 3), L8)
+    assert np.array_equal(d_rng.numpy(), st8)
