@@ -11,7 +11,10 @@ and NEE + MIS on.  The BVH is built on the GPU (timed separately).
 Prints one JSON line: Mrays/s (extension + shadow rays per second) as `value`,
 spp*Mpix/s beside it, the roofline of the dominant kernel (closest-hit
 extend) and a CPU baseline: the oracle's restatement of the reference tracer
-on a bounded row sample of the same frame on the host cores.
+on a bounded row sample of the same frame on the host cores.  After the
+frame, the line's "forms" object summarizes shorter runs of the other
+configurations (--forms): the 256x256 Cornell variant with its whole-frame
+parity and CPU rate, the precision, batching and serving forms below.
 
 --form batch: 64 soup scenes of lognormal size (1k-500k triangles, median
 20k), resolutions from {128,256,512,768}^2, 1-32 spp, maxBounces 8, traced
@@ -299,6 +302,12 @@ def main():
     ap.add_argument("--max-paths", type=int, default=1 << 26, help="serve: paths per round")
     ap.add_argument("--policy", choices=["fifo", "srpt", "scene"], default="scene")
     ap.add_argument("--hbm-budget-gib", type=float, default=8.0)
+    ap.add_argument("--forms", type=lambda v: [x for x in v.split(",") if x],
+                    default=["c1", "precision", "batch", "serve"],
+                    help="plain: other configurations measured after the plain frame, summarized "
+                         "in the line's 'forms' object (c1, precision, batch, serve; '' for none)")
+    ap.add_argument("--forms-serve-duration", type=float, default=5.0,
+                    help="plain: seconds of the request stream in the serving summary")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one process, one "
@@ -321,6 +330,8 @@ def main():
         res = serve_form(ct, args)
     else:
         res = plain_form(ct, args)
+        if args.forms:
+            res["forms"] = other_forms(ct, args)
     print(json.dumps(res), flush=True)
 
 
@@ -574,6 +585,121 @@ def plain_form(ct, args):
         "roofline": roof, "cpu_baseline": cpu, "parity": parity,
     }
     return res
+
+
+def c1_form(ct, args):
+    """The small plain workload: the 256x256 Cornell variant, 16 spp,
+    maxBounces 8, 16-bucket SAH tree built on the GPU; the CPU baseline of the
+    headline line on the same frame, and the frame checked against the band of
+    rows (most or all of the frame) that baseline renders with the same
+    per-pixel seeds."""
+    import numpy as np
+
+    sc = ct.synth.cornell_c1()
+    w = h = 256
+    spp, depth = 16, 8
+    ps = ct.packScene(sc, bins=16)
+    ds = ct.DeviceScene(ps)
+    frame = ct.cameraFrame(sc["camera"], 1.0)
+    opts = {"width": w, "height": h, "spp": spp, "seed": 1, "maxBounces": depth,
+            "russianRoulette": True, "nee": True}
+    out = ct.DeviceBuffer.empty((w * h * 3,), np.float32)
+    tr = ct.Tracer()
+    steps = max(args.steps, 20)
+    for _ in range(max(1, args.warmup)):
+        tr.render_image(ds, frame, opts, out)
+    ct.sync()
+    lat = []
+    t = time.perf_counter()
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        tr.render_image(ds, frame, opts, out)
+        lat.append(time.perf_counter() - t1)
+    ct.sync()
+    total = time.perf_counter() - t
+    st = tr.stats()
+    rays = st["extensionRays"] + st["shadowRays"]
+    g = out.numpy().astype(np.float64)
+    res = {"workload": f"Cornell variant (scenes.ts box, light radiance 15, metal roughness 0.1, "
+                       f"checker back wall; {ps.primCount} primitives, 16-bucket SAH), {w}x{h}, "
+                       f"{spp} spp, maxBounces {depth}, RR+NEE",
+           "value": round(rays * steps / total / 1e6, 2), "unit": "Mrays/s", "dtype": "f64",
+           "steps": steps, "ms_per_step": round(total / steps * 1e3, 3),
+           "spp_mpix_per_s": round(w * h * spp * steps / total / 1e6, 3),
+           "frame_latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                                "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
+           "rays_per_step": rays}
+    if not args.no_cpu:
+        # the CPU baseline of the headline line on this frame (the JS restatement on
+        # worker threads, the C oracle beside it); its band, here the whole frame
+        # or most of it, is the oracle's render the GPU frame is checked against
+        cpu, (rows, band) = cpu_baseline(ct, ps, frame, w, h, spp, depth, opts["seed"],
+                                         budget_s=6.0)
+        gb = g[rows[0] * w * 3:rows[1] * w * 3]
+        o = band.astype(np.float64)
+        den = np.maximum(np.abs(gb), np.abs(o))
+        rel = np.where(den > 0, np.abs(gb - o) / np.where(den > 0, den, 1), 0.0)
+        res["parity"] = {"reference": "oracle's render of rows %d..%d" % (rows[0], rows[1] - 1),
+                         "pixels": int(rel.size // 3), "max_rel": float(rel.max()),
+                         "pixels_over_1e-4": int((rel.reshape(-1, 3).max(1) > 1e-4).sum()),
+                         "exact_fraction": round(float((gb == o).mean()), 6)}
+        res["cpu_baseline"] = cpu
+    log(f"c1: {res['ms_per_step']:.2f} ms/step, {res['value']:.0f} Mrays/s, "
+        f"{res['spp_mpix_per_s']:.1f} spp*Mpix/s, parity {res.get('parity')}")
+    return res
+
+
+# fields of each form's own line kept in the default line's "forms" object
+_FORM_FIELDS = {
+    "batch": ["value", "unit", "dtype", "ms_per_step", "spp_mpix_per_s", "rounds",
+              "plain_sequential", "speedup_vs_plain", "identical_to_plain"],
+    "precision": ["value", "unit", "dtype", "ms_per_step", "spp_mpix_per_s", "rms_vs_reference",
+                  "rms_vs_fp64_cpu", "rms_same_seed_f32_vs_f64", "rms_seed_noise_f64",
+                  "binary64_form", "speedup_vs_binary64_at_equal_spp"],
+    "serve": ["value", "unit", "dtype", "ms_per_step", "spp_mpix_per_s", "latency_ms",
+              "offered_spp_mpix_per_s", "gpu_busy_frac", "plain_one_at_a_time",
+              "speedup_vs_plain", "same_as_alone"],
+}
+
+
+def other_forms(ct, args):
+    """The other configurations, each in a shorter run than its own --form
+    line (batching: 3 timed steps, no CPU leg; serving: --forms-serve-duration
+    seconds of the stream), summarized for the default line."""
+    out = {}
+    for name in args.forms:
+        sub = argparse.Namespace(**vars(args))
+        t = time.perf_counter()
+        log(f"form {name}...")
+        try:
+            if name == "c1":
+                r = c1_form(ct, sub)
+            elif name == "batch":
+                sub.steps, sub.warmup, sub.no_cpu = 3, 1, True
+                full = batch_form(ct, sub)
+                r = {"workload": full["config"]["workload"],
+                     **{k: full[k] for k in _FORM_FIELDS[name]},
+                     "autotune": {k: full["autotune"][k] for k in ("trials", "ms", "config")},
+                     "roofline": {k: full["roofline"][k] for k in ("kernel", "achieved", "frac")}}
+            elif name == "precision":
+                sub.steps, sub.warmup = 3, 1
+                full = precision_form(ct, sub)
+                r = {"workload": full["config"]["workload"],
+                     **{k: full[k] for k in _FORM_FIELDS[name]},
+                     "roofline": {k: full["roofline"][k] for k in ("kernel", "achieved", "frac")}}
+            elif name == "serve":
+                sub.duration, sub.rates = args.forms_serve_duration, []
+                full = serve_form(ct, sub)
+                r = {"workload": full["config"]["workload"],
+                     **{k: full[k] for k in _FORM_FIELDS[name]}}
+            else:
+                raise ValueError(f"unknown form {name!r}")
+        except Exception as e:  # one form's failure leaves the others' figures
+            r = {"error": f"{type(e).__name__}: {e}"}
+            log(f"form {name} failed: {r['error']}")
+        r["run_s"] = round(time.perf_counter() - t, 1)
+        out[name] = r
+    return out
 
 
 def isolated_roofline(ct, tr, ds, frame, opts, out, cfg, roof, prof):

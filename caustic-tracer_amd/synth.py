@@ -215,6 +215,27 @@ def caustic_scene(n_spheres=256, sphere_bands=65, sphere_segments=128, plane_row
     return {"name": "caustics", "objects": objects, "camera": camera, "sky": {"kind": "none"}}
 
 
+def cornell_c1():
+    """The small plain-method workload: buildCornellBox (scenes.ts:45-67) with
+    its ceiling light at radiance 15 (white), the metal sphere at roughness 0.1
+    and the back wall checker-modulated (color2 0.25, scale 8); rendered at
+    256x256, 16 spp, maxBounces 8 over a 16-bucket SAH tree."""
+    from .presets import cornell_box
+
+    sc = cornell_box()
+    objects = []
+    for o in sc["objects"]:
+        m = dict(o["material"])
+        if o["name"] == "Ceiling light":
+            m["color"], m["strength"] = (1, 1, 1), 15
+        elif o["name"] == "Metal sphere":
+            m["roughness"] = 0.1
+        elif o["name"] == "Back wall":
+            m["checker"] = {"color2": (0.25, 0.25, 0.25), "scale": 8}
+        objects.append(dict(o, material=m))
+    return dict(sc, objects=objects)
+
+
 def bvh_test_rays(n, seed=99, lo=-14.0, hi=14.0, tlo=-8.0, thi=8.0):
     """The closest-hit ray set of bvh.test.ts:89-99 at any size: per ray, from
     Rng(seed), origin = range(lo, hi) x3, target = range(tlo, thi) x3,

@@ -39,3 +39,17 @@ def test_percentile_nearest_rank(ct):
     assert ct.serve.percentile(xs, 99) == 99
     assert ct.serve.percentile(xs, 100) == 100
     assert ct.serve.percentile([7.0], 99) == 7.0
+
+
+def test_cornell_c1_workload_is_the_test_variant(ct):
+    # the small plain workload the bench times is the 256x256 Cornell variant the
+    # parity tests render against the oracle
+    import scenes
+
+    assert ct.synth.cornell_c1() == scenes.cornell_c1()
+    ref = ct.presets.cornell_box()
+    c1 = {o["name"]: o["material"] for o in ct.synth.cornell_c1()["objects"]}
+    assert c1["Ceiling light"]["strength"] == 15 and ref["objects"][5]["material"]["strength"] == 12
+    assert c1["Metal sphere"]["roughness"] == 0.1
+    assert c1["Back wall"]["checker"] == {"color2": (0.25, 0.25, 0.25), "scale": 8}
+    assert "checker" not in ref["objects"][2]["material"]
