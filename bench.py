@@ -619,6 +619,7 @@ def c1_form(ct, args):
     total = time.perf_counter() - t
     st = tr.stats()
     rays = st["extensionRays"] + st["shadowRays"]
+    per_path = tr.get_path_kernel()[1] > 0
     g = out.numpy().astype(np.float64)
     res = {"workload": f"Cornell variant (scenes.ts box, light radiance 15, metal roughness 0.1, "
                        f"checker back wall; {ps.primCount} primitives, 16-bucket SAH), {w}x{h}, "
@@ -628,7 +629,9 @@ def c1_form(ct, args):
            "spp_mpix_per_s": round(w * h * spp * steps / total / 1e6, 3),
            "frame_latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                                 "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
-           "rays_per_step": rays}
+           "rays_per_step": rays,
+           "launches": "per-path (k_paths, one launch per path group)" if per_path else
+                       "wavefront (extend, shade, NEE launches per bounce)"}
     if not args.no_cpu:
         # the CPU baseline of the headline line on this frame (the JS restatement on
         # worker threads, the C oracle beside it); its band, here the whole frame
@@ -705,31 +708,44 @@ def other_forms(ct, args):
 def isolated_roofline(ct, tr, ds, frame, opts, out, cfg, roof, prof):
     """The frame again with one path group and the NEE launches in order, so
     each extend launch runs alone: its average duration and the algorithmic
-    bytes over it, added to roof["isolated"]."""
-    tr.set_config(dict(cfg, pathGroups=1))
+    bytes over it, added to roof["isolated"].  A launch alone has the chip to
+    itself, so the grid is swept from the shared-chip count up to what five
+    waves per SIMD fill (the occupancy of both precisions' extend kernels,
+    bounded by their LDS stacks), and the fastest is kept."""
+    ext_total = roof["bytes_per_launch"] * max(1, prof["extendLaunches"])
+    full = 256 * 4 * 5 * 64 // cfg["workgroup"]  # CUs x SIMDs x waves x lanes / workgroup
+    grids = sorted({cfg["blocks"], full * 3 // 4, full})
+    sweep = []
     tr.set_overlap(False)
-    tr.set_profiling(True)
-    ct.sync()
-    t = time.perf_counter()
-    tr.render_image(ds, frame, opts, out)
-    ct.sync()
-    iso_frame_ms = (time.perf_counter() - t) * 1e3
-    iso = tr.stats()
-    tr.set_profiling(False)
+    for g in grids:
+        tr.set_config(dict(cfg, pathGroups=1, blocks=g))
+        tr.render_image(ds, frame, opts, out)  # untimed: first use of this setting
+        tr.set_profiling(True)
+        ct.sync()
+        t = time.perf_counter()
+        tr.render_image(ds, frame, opts, out)
+        ct.sync()
+        frame_ms = (time.perf_counter() - t) * 1e3
+        iso = tr.stats()
+        tr.set_profiling(False)
+        sweep.append((iso["extendMs"] / max(1, iso["extendLaunches"]), g, iso, frame_ms))
     tr.set_overlap(True)
     tr.set_config(cfg)
-    ext_total = roof["bytes_per_launch"] * max(1, prof["extendLaunches"])
+    avg, grid, iso, iso_frame_ms = min(sweep, key=lambda x: x[0])
     iso_launches = max(1, iso["extendLaunches"])
     iso_gbs = ext_total / (iso["extendMs"] / 1e3) / 1e9 if iso["extendMs"] > 0 else 0.0
     roof["isolated"] = {
-        "avg_launch_ms": round(iso["extendMs"] / iso_launches, 4),
+        "avg_launch_ms": round(avg, 4),
         "launches": iso["extendLaunches"], "bytes_per_launch": ext_total // iso_launches,
         "achieved": round(iso_gbs, 1), "frac": round(iso_gbs / HBM_PEAK_GBS, 4),
         "extend_ms_total": round(iso["extendMs"], 3), "frame_ms": round(iso_frame_ms, 3),
+        "workgroups": grid,
+        "grid_sweep": [{"workgroups": g, "avg_launch_ms": round(a, 4),
+                        "frame_ms": round(f, 3)} for a, g, _, f in sweep],
         "setting": "one path group, NEE launches in order: each launch runs alone"}
-    log(f"isolated extend launches: {iso['extendLaunches']} x "
-        f"{iso['extendMs'] / iso_launches:.3f} ms = {iso['extendMs']:.2f} ms of a "
-        f"{iso_frame_ms:.2f} ms frame, {iso_gbs:.0f} GB/s")
+    log(f"isolated extend launches: {iso['extendLaunches']} x {avg:.3f} ms = "
+        f"{iso['extendMs']:.2f} ms of a {iso_frame_ms:.2f} ms frame, {iso_gbs:.0f} GB/s, "
+        f"{grid} workgroups (swept {grids})")
 
 
 def closest_hit_queries(ct, ds, ps, n):

@@ -15,7 +15,8 @@ from .scene_io import SceneParseError, parseScene, parseSceneJSON, sceneToJSON
 from .display import decodePPM, encodePPM, tonemapBuffer
 from ._capi import CTError, DeviceBuffer, init, sync
 from .pack import PackedScene, buildBvh, packScene, primBounds
-from .tracer import (PRECISION_F32, PRECISION_F64, T_MIN, WALK_BRUTE, WALK_ORDERED,
+from .tracer import (PATHS_AUTO, PATHS_PER_PATH, PATHS_WAVEFRONT, PRECISION_F32, PRECISION_F64,
+                     T_MIN, WALK_BRUTE, WALK_ORDERED,
                      WALK_STACKLESS, WALK_WIDE, DeviceScene, Tracer, accumulatePass, cameraFrame,
                      closest_hit, closest_hit32, closest_hit32_counts, closest_hit_counts,
                      image_sum_f64, intersectClosest, occluded, occluded32, pick, primaryRay,
@@ -26,7 +27,8 @@ __all__ = [
     "presets", "SCENE_PRESETS", "presetById", "scene_io", "SceneParseError", "parseScene",
     "parseSceneJSON", "sceneToJSON", "display", "tonemapBuffer", "encodePPM", "decodePPM", "serve", "Server", "synth", "CTError", "DeviceBuffer", "init", "sync", "PackedScene", "buildBvh", "packScene",
     "primBounds", "T_MIN", "WALK_ORDERED", "WALK_STACKLESS", "WALK_WIDE", "WALK_BRUTE",
-    "PRECISION_F64", "PRECISION_F32", "closest_hit32", "closest_hit32_counts", "occluded32",
+    "PRECISION_F64", "PRECISION_F32", "PATHS_AUTO", "PATHS_WAVEFRONT", "PATHS_PER_PATH",
+    "closest_hit32", "closest_hit32_counts", "occluded32",
     "image_sum_f64", "pick", "primaryRay", "DeviceScene", "Tracer",
     "accumulatePass", "cameraFrame", "closest_hit", "closest_hit_counts", "intersectClosest",
     "occluded", "radianceInto", "radiance_batch", "renderImage", "Rng", "pcgHash", "seedFor",

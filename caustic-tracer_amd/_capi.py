@@ -119,6 +119,8 @@ SIGNATURES = {
     "ct_tracer_set_capacity": [vp, C.c_uint64],
     "ct_tracer_set_graph": [vp, i32],
     "ct_tracer_set_overlap": [vp, i32],
+    "ct_tracer_set_path_kernel": [vp, i32],
+    "ct_tracer_get_path_kernel": [vp, C.POINTER(i32), C.POINTER(u32)],
     "ct_bvh_build": [vp, vp, vp, u32, i32, vp, C.POINTER(u32), vp, vp],
     "ct_bvh_build_ex": [vp, vp, vp, u32, i32, vp, C.POINTER(u32), vp, vp, vp],
     "ct_bvh_build_noalloc": [vp, vp, vp, u32, i32, vp, C.POINTER(u32), vp, vp, vp],

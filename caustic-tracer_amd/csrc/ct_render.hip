@@ -39,7 +39,69 @@ __global__ void k_clear(uint32_t* ctr, int n, unsigned long long* ctr64, int n64
   if (i < n64) ctr64[i] = 0;
 }
 
-// camera.ts:62-72 generateRay with the pixel/lens draws of tracer.ts:373-384.
+// A path's state between bounces (the locals of radianceInto, tracer.ts:199-208).
+struct PathState {
+  double ox, oy, oz, dx, dy, dz;
+  double br, bg, bb;
+  double Lr, Lg, Lb;
+  double prevPdf;
+  bool specular;
+  Rng rng;
+};
+
+// Path `local` of job J at its start: the caller's ray and RNG state for a
+// radianceInto job, else the camera ray of its pixel and pass
+// (camera.ts:62-72 generateRay with the pixel/lens draws of tracer.ts:373-384).
+__device__ __forceinline__ PathState start_path(const JobDev& J, size_t local) {
+  const Frame& F = J.F;
+  const uint32_t width = J.width, height = J.height, npix = J.npix;
+  uint32_t pass = (uint32_t)(local / npix);
+  uint32_t pixel = (uint32_t)(local - (size_t)pass * npix);
+  PathState p;
+  p.br = 1; p.bg = 1; p.bb = 1;
+  p.Lr = 0; p.Lg = 0; p.Lb = 0;
+  p.prevPdf = 0;
+  p.specular = true;
+  if (J.rays) {  // radianceInto: the caller's ray and RNG state (tracer.ts:191-198)
+    const double* r = J.rays + 6 * (size_t)pixel;
+    p.ox = r[0]; p.oy = r[1]; p.oz = r[2];
+    p.dx = r[3]; p.dy = r[4]; p.dz = r[5];
+    p.rng.s = J.rngIO[pixel];
+    return p;
+  }
+  uint32_t y = pixel / width, x = pixel - y * width;
+  Rng rng;
+  rng.s = seed_for(pixel, J.frameBase + pass);
+  double sx = (x + rng.next()) / width;
+  double sy = 1 - (y + rng.next()) / height;
+  double lx = 0, ly = 0;
+  if (F.lensRadius > 0) {
+    double r = sqrt(rng.next());
+    double theta = TWO_PI * rng.next();
+    lx = r * jsm::cos(theta);
+    ly = r * jsm::sin(theta);
+  }
+  double px = (2 * sx - 1) * F.tanHalfFov * F.aspect;
+  double py = (2 * sy - 1) * F.tanHalfFov;
+  double o[3], d[3];
+  for (int k = 0; k < 3; k++) {
+    double focus = F.o[k] + (F.r[k] * px + F.u[k] * py + F.f[k]) * F.focusDistance;
+    o[k] = F.o[k] + (F.r[k] * (lx * F.lensRadius) + F.u[k] * (ly * F.lensRadius));
+    d[k] = focus - o[k];
+  }
+  double len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  if (len > 0) {
+    double s = 1 / len;
+    d[0] *= s; d[1] *= s; d[2] *= s;
+  } else {
+    d[0] = d[1] = d[2] = 0;
+  }
+  p.ox = o[0]; p.oy = o[1]; p.oz = o[2];
+  p.dx = d[0]; p.dy = d[1]; p.dz = d[2];
+  p.rng = rng;
+  return p;
+}
+
 template <bool MULTI>
 __global__ void __launch_bounds__(RBLOCK)
 k_generate(PathSoA P, const JobDev* __restrict__ jobs, int njobs, size_t base, size_t count) {
@@ -47,59 +109,14 @@ k_generate(PathSoA P, const JobDev* __restrict__ jobs, int njobs, size_t base, s
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
     const size_t pid = base + i;
     int j = job_of<MULTI, false>(jobs, njobs, pid);
-    const JobDev& J = jobs[j];
-    const Frame& F = J.F;
-    const uint32_t width = J.width, height = J.height, npix = J.npix;
-    size_t local = pid - J.pathBase;
-    uint32_t pass = (uint32_t)(local / npix);
-    uint32_t pixel = (uint32_t)(local - (size_t)pass * npix);
-    uint32_t y = pixel / width, x = pixel - y * width;
-    Rng rng;
-    if (J.rays) {  // radianceInto: the caller's ray and RNG state (tracer.ts:191-198)
-      const double* r = J.rays + 6 * (size_t)pixel;
-      P.ox[pid] = r[0]; P.oy[pid] = r[1]; P.oz[pid] = r[2];
-      P.dx[pid] = r[3]; P.dy[pid] = r[4]; P.dz[pid] = r[5];
-      P.br[pid] = 1; P.bg[pid] = 1; P.bb[pid] = 1;
-      P.Lr[pid] = 0; P.Lg[pid] = 0; P.Lb[pid] = 0;
-      P.prevPdf[pid] = 0;
-      P.specular[pid] = 1;
-      P.rng[pid] = J.rngIO[pixel];
-      if (MULTI) P.jobOf[pid] = (uint32_t)j;
-      P.q0[i] = (uint32_t)pid;
-      continue;
-    }
-    rng.s = seed_for(pixel, J.frameBase + pass);
-    double sx = (x + rng.next()) / width;
-    double sy = 1 - (y + rng.next()) / height;
-    double lx = 0, ly = 0;
-    if (F.lensRadius > 0) {
-      double r = sqrt(rng.next());
-      double theta = TWO_PI * rng.next();
-      lx = r * jsm::cos(theta);
-      ly = r * jsm::sin(theta);
-    }
-    double px = (2 * sx - 1) * F.tanHalfFov * F.aspect;
-    double py = (2 * sy - 1) * F.tanHalfFov;
-    double o[3], d[3];
-    for (int k = 0; k < 3; k++) {
-      double focus = F.o[k] + (F.r[k] * px + F.u[k] * py + F.f[k]) * F.focusDistance;
-      o[k] = F.o[k] + (F.r[k] * (lx * F.lensRadius) + F.u[k] * (ly * F.lensRadius));
-      d[k] = focus - o[k];
-    }
-    double len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    if (len > 0) {
-      double s = 1 / len;
-      d[0] *= s; d[1] *= s; d[2] *= s;
-    } else {
-      d[0] = d[1] = d[2] = 0;
-    }
-    P.ox[pid] = o[0]; P.oy[pid] = o[1]; P.oz[pid] = o[2];
-    P.dx[pid] = d[0]; P.dy[pid] = d[1]; P.dz[pid] = d[2];
-    P.br[pid] = 1; P.bg[pid] = 1; P.bb[pid] = 1;
-    P.Lr[pid] = 0; P.Lg[pid] = 0; P.Lb[pid] = 0;
-    P.prevPdf[pid] = 0;
-    P.specular[pid] = 1;
-    P.rng[pid] = rng.s;
+    const PathState p = start_path(jobs[j], pid - jobs[j].pathBase);
+    P.ox[pid] = p.ox; P.oy[pid] = p.oy; P.oz[pid] = p.oz;
+    P.dx[pid] = p.dx; P.dy[pid] = p.dy; P.dz[pid] = p.dz;
+    P.br[pid] = p.br; P.bg[pid] = p.bg; P.bb[pid] = p.bb;
+    P.Lr[pid] = p.Lr; P.Lg[pid] = p.Lg; P.Lb[pid] = p.Lb;
+    P.prevPdf[pid] = p.prevPdf;
+    P.specular[pid] = p.specular ? 1 : 0;
+    P.rng[pid] = p.rng.s;
     if (MULTI) P.jobOf[pid] = (uint32_t)j;
     P.q0[i] = (uint32_t)pid;
   }
@@ -272,8 +289,224 @@ __device__ __forceinline__ double dielectric_reflectance(double cosI, double eta
   return schlick(cosI, etaI, etaT);
 }
 
-// One bounce of radianceInto for every live path.  (151 VGPRs, three waves
-// per SIMD; a 128-VGPR cap for four waves spills and measures the same on the
+// The NEE ray a bounce queues: direction, segment end and the term added to
+// the path's radiance when nothing blocks it (tracer.ts:262-282).
+struct NeeRay {
+  double w[3], tMax, c[3];
+};
+
+// One bounce of radianceInto (tracer.ts:209-312) at the closest hit (prim,
+// tHit) of the path's ray (prim -1: a miss): emission or sky added to p's radiance,
+// the NEE ray set in `nee` (neeOn), the next direction, throughput, pdf and
+// specular flag drawn into p, p's origin moved to the hit point.  Returns
+// whether the path goes on (before the maxBounces test).
+__device__ __forceinline__ bool shade_bounce(const DevScene& S, const TraceOpts& opts,
+                                             PathState& ps, int prim, double tHit, int bounce,
+                                             bool& neeOn, NeeRay& nee) {
+  const bool useNee = opts.nee && S.lightCount > 0;
+  double ox = ps.ox, oy = ps.oy, oz = ps.oz;
+  double dx = ps.dx, dy = ps.dy, dz = ps.dz;
+  double br = ps.br, bg = ps.bg, bb = ps.bb;
+  double Lr = ps.Lr, Lg = ps.Lg, Lb = ps.Lb;
+  double prevPdf = ps.prevPdf;
+  bool specularBounce = ps.specular;
+  Rng rng = ps.rng;
+  neeOn = false;
+  bool cont = true;
+  double ndx = 0, ndy = 0, ndz = 0, px = ox, py = oy, pz = oz;
+  if (prim < 0) {
+    if (S.skyKind == 1) {
+      double t = 0.5 * (dy + 1);
+      Lr += br * (S.horizon[0] + (S.zenith[0] - S.horizon[0]) * t) * S.skyStrength;
+      Lg += bg * (S.horizon[1] + (S.zenith[1] - S.horizon[1]) * t) * S.skyStrength;
+      Lb += bb * (S.horizon[2] + (S.zenith[2] - S.horizon[2]) * t) * S.skyStrength;
+    }
+    cont = false;
+  } else {
+    px = ox + dx * tHit;
+    py = oy + dy * tHit;
+    pz = oz + dz * tHit;
+    const HitGeom g = load_hit(S, prim);
+    double nx, ny, nz;
+    if (g.a.w == 0.0f) {
+      double invR = 1 / (double)g.b.x;
+      nx = (px - (double)g.a.x) * invR;
+      ny = (py - (double)g.a.y) * invR;
+      nz = (pz - (double)g.a.z) * invR;
+    } else {
+      nx = g.n.x;
+      ny = g.n.y;
+      nz = g.n.z;
+    }
+    bool frontFace = dx * nx + dy * ny + dz * nz < 0;
+    double sx = frontFace ? nx : -nx;
+    double sy = frontFace ? ny : -ny;
+    double sz = frontFace ? nz : -nz;
+    size_t m = (size_t)g.obj * 4;
+    float4 m0 = gload(S.mats + m), m1 = gload(S.mats + m + 1), m3 = gload(S.mats + m + 3);
+    double er = m3.x, eg = m3.y, eb = m3.z;
+    if (er > 0 || eg > 0 || eb > 0) {
+      double w = 1;
+      if (useNee && !specularBounce) {
+        double pl = light_pdf(S, ox, oy, oz, px, py, pz, nx, ny, nz, prim);
+        w = power_heuristic(prevPdf, pl);
+      }
+      Lr += br * er * w;
+      Lg += bg * eg * w;
+      Lb += bb * eb * w;
+    }
+    float type = m0.w;
+    if (type == 3.0f) {
+      cont = false;  // MAT_EMISSIVE
+    } else if (type == 0.0f) {  // MAT_LAMBERTIAN
+      double ar = m0.x, ag = m0.y, ab = m0.z;
+      if (m1.z > 0.5f) {
+        double cs = m1.y;
+        double sum = floor(px * cs + 0.5) + floor(py * cs + 0.5) + floor(pz * cs + 0.5);
+        // ((sum % 2) + 2) % 2 != 0 (materials.ts checker parity): sum is an
+        // integer, so sum - 2*floor(sum/2) is exactly 1 when odd, 0 when even
+        // (NaN for NaN or infinities, as the remainders give)
+        if (sum - 2.0 * floor(0.5 * sum) != 0) {  // !checkerPrimary
+          float4 m2 = gload(S.mats + m + 2);
+          ar = m2.x;
+          ag = m2.y;
+          ab = m2.z;
+        }
+      }
+      LightSample ls;
+      if (useNee && sample_light(S, px, py, pz, rng, ls)) {
+        double pdfL = ls.pdf;
+        double cosS = sx * ls.w[0] + sy * ls.w[1] + sz * ls.w[2];
+        if (cosS > 0 && pdfL > 0) {
+          double pdfB = cosS * INV_PI;
+          double k = (cosS * INV_PI * power_heuristic(pdfL, pdfB)) / pdfL;
+          neeOn = true;
+          nee.w[0] = ls.w[0];
+          nee.w[1] = ls.w[1];
+          nee.w[2] = ls.w[2];
+          nee.tMax = ls.dist * SHADOW_SHORTEN;
+          nee.c[0] = br * ar * ls.e[0] * k;
+          nee.c[1] = bg * ag * ls.e[1] * k;
+          nee.c[2] = bb * ab * ls.e[2] * k;
+        }
+      }
+      // cosineHemisphereInto (materials.ts:47-64)
+      double u1 = rng.next();
+      double u2 = rng.next();
+      double r = sqrt(u1);
+      double phi = TWO_PI * u2;
+      double lx = r * jsm::cos(phi);
+      double ly = r * jsm::sin(phi);
+      double lz = sqrt(u1 < 1 ? 1 - u1 : 0);
+      double b[6];
+      basis(sx, sy, sz, b);
+      double ddx = b[0] * lx + b[3] * ly + sx * lz;
+      double ddy = b[1] * lx + b[4] * ly + sy * lz;
+      double ddz = b[2] * lx + b[5] * ly + sz * lz;
+      double inv = 1 / sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+      ndx = ddx * inv;
+      ndy = ddy * inv;
+      ndz = ddz * inv;
+      prevPdf = max0(sx * ndx + sy * ndy + sz * ndz) * INV_PI;
+      br *= ar;
+      bg *= ag;
+      bb *= ab;
+      specularBounce = false;
+    } else if (type == 1.0f) {  // MAT_METAL: metalInto (materials.ts:102-124)
+      double u1 = rng.next();
+      double u2 = rng.next();
+      double u3 = rng.next();
+      double rough = m1.x;
+      double k = 2 * (dx * sx + dy * sy + dz * sz);
+      double rx = dx - k * sx;
+      double ry = dy - k * sy;
+      double rz = dz - k * sz;
+      if (rough > 0) {
+        double uv[3];
+        unit_vector(u1, u2, uv);
+        double rad = jsm::cbrt(u3) * rough;
+        rx += uv[0] * rad;
+        ry += uv[1] * rad;
+        rz += uv[2] * rad;
+      }
+      double inv = 1 / sqrt(rx * rx + ry * ry + rz * rz);
+      ndx = rx * inv;
+      ndy = ry * inv;
+      ndz = rz * inv;
+      if (ndx * sx + ndy * sy + ndz * sz <= 0) {
+        cont = false;
+      } else {
+        br *= (double)m0.x;
+        bg *= (double)m0.y;
+        bb *= (double)m0.z;
+        specularBounce = true;
+      }
+    } else {  // MAT_DIELECTRIC: dielectricInto (materials.ts:131-166)
+      double u = rng.next();
+      double ior = m1.x;
+      double etaI = frontFace ? 1 : ior;
+      double etaT = frontFace ? ior : 1;
+      double eta = etaI / etaT;
+      double cosI = -(dx * sx + dy * sy + dz * sz);
+      if (cosI > 1) cosI = 1;
+      double refl = dielectric_reflectance(cosI, etaI, etaT);
+      double sin2T = eta * eta * (1 - cosI * cosI);
+      if (u < refl || sin2T > 1) {
+        double k = 2 * (dx * sx + dy * sy + dz * sz);
+        ndx = dx - k * sx;
+        ndy = dy - k * sy;
+        ndz = dz - k * sz;
+      } else {
+        double cosT = sqrt(1 - sin2T);
+        double s2 = eta * cosI - cosT;
+        double tx = dx * eta + sx * s2;
+        double ty = dy * eta + sy * s2;
+        double tz = dz * eta + sz * s2;
+        double inv = 1 / sqrt(tx * tx + ty * ty + tz * tz);
+        ndx = tx * inv;
+        ndy = ty * inv;
+        ndz = tz * inv;
+      }
+      br *= (double)m0.x;
+      bg *= (double)m0.y;
+      bb *= (double)m0.z;
+      specularBounce = true;
+    }
+    if (cont && opts.rr && bounce >= 3) {  // tracer.ts:305-312
+      double maxB = br > bg ? (br > bb ? br : bb) : bg > bb ? bg : bb;
+      double pSurvive = maxB > 1 ? 1 : maxB < 0.05 ? 0.05 : maxB;
+      if (rng.next() > pSurvive) {
+        cont = false;
+      } else {
+        br /= pSurvive;
+        bg /= pSurvive;
+        bb /= pSurvive;
+      }
+    }
+  }
+  ps.Lr = Lr;
+  ps.Lg = Lg;
+  ps.Lb = Lb;
+  // the NEE ray starts at the hit point, which is also the next origin
+  ps.ox = px;
+  ps.oy = py;
+  ps.oz = pz;
+  ps.rng = rng;
+  if (cont) {
+    ps.dx = ndx;
+    ps.dy = ndy;
+    ps.dz = ndz;
+    ps.br = br;
+    ps.bg = bg;
+    ps.bb = bb;
+    ps.prevPdf = prevPdf;
+    ps.specular = specularBounce;
+  }
+  return cont;
+}
+
+// One bounce of radianceInto for every live path.  (137 VGPRs, three waves
+// per SIMD; a 128-VGPR cap for four waves spills and measured the same on the
 // 1080p soup frame.)
 template <bool MULTI>
 __global__ void __launch_bounds__(RBLOCK)
@@ -290,226 +523,51 @@ k_shade(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict__
     bool valid = slot < n;
     uint32_t pid = valid ? qin[slot] : 0;
     bool alive = false, shadow = false;
-    double wdir[3] = {0, 0, 0}, sTmax = 0, sc[3] = {0, 0, 0};
+    NeeRay nee;
     if (valid) {
       const JobDev& J = jobs[MULTI ? P.jobOf[pid] : 0];
-      const DevScene& S = J.S;
-      const TraceOpts opts = J.opts;
-      const bool useNee = opts.nee && S.lightCount > 0;
-      double ox = P.ox[pid], oy = P.oy[pid], oz = P.oz[pid];
-      double dx = P.dx[pid], dy = P.dy[pid], dz = P.dz[pid];
-      double br = P.br[pid], bg = P.bg[pid], bb = P.bb[pid];
-      double Lr = P.Lr[pid], Lg = P.Lg[pid], Lb = P.Lb[pid];
-      double prevPdf = P.prevPdf[pid];
-      bool specularBounce = P.specular[pid] != 0;
-      Rng rng;
-      rng.s = P.rng[pid];
-      int prim = P.hitPrim[pid];
-      bool cont = true;
-      double ndx = 0, ndy = 0, ndz = 0, px = ox, py = oy, pz = oz;
-      if (prim < 0) {
-        if (S.skyKind == 1) {
-          double t = 0.5 * (dy + 1);
-          Lr += br * (S.horizon[0] + (S.zenith[0] - S.horizon[0]) * t) * S.skyStrength;
-          Lg += bg * (S.horizon[1] + (S.zenith[1] - S.horizon[1]) * t) * S.skyStrength;
-          Lb += bb * (S.horizon[2] + (S.zenith[2] - S.horizon[2]) * t) * S.skyStrength;
-        }
-        cont = false;
-      } else {
-        double t = P.hitT[pid];
-        px = ox + dx * t;
-        py = oy + dy * t;
-        pz = oz + dz * t;
-        const HitGeom g = load_hit(S, prim);
-        double nx, ny, nz;
-        if (g.a.w == 0.0f) {
-          double invR = 1 / (double)g.b.x;
-          nx = (px - (double)g.a.x) * invR;
-          ny = (py - (double)g.a.y) * invR;
-          nz = (pz - (double)g.a.z) * invR;
-        } else {
-          nx = g.n.x;
-          ny = g.n.y;
-          nz = g.n.z;
-        }
-        bool frontFace = dx * nx + dy * ny + dz * nz < 0;
-        double sx = frontFace ? nx : -nx;
-        double sy = frontFace ? ny : -ny;
-        double sz = frontFace ? nz : -nz;
-        size_t m = (size_t)g.obj * 4;
-        float4 m0 = gload(S.mats + m), m1 = gload(S.mats + m + 1), m3 = gload(S.mats + m + 3);
-        double er = m3.x, eg = m3.y, eb = m3.z;
-        if (er > 0 || eg > 0 || eb > 0) {
-          double w = 1;
-          if (useNee && !specularBounce) {
-            double pl = light_pdf(S, ox, oy, oz, px, py, pz, nx, ny, nz, prim);
-            w = power_heuristic(prevPdf, pl);
-          }
-          Lr += br * er * w;
-          Lg += bg * eg * w;
-          Lb += bb * eb * w;
-        }
-        float type = m0.w;
-        if (type == 3.0f) {
-          cont = false;  // MAT_EMISSIVE
-        } else if (type == 0.0f) {  // MAT_LAMBERTIAN
-          double ar = m0.x, ag = m0.y, ab = m0.z;
-          if (m1.z > 0.5f) {
-            double cs = m1.y;
-            double sum = floor(px * cs + 0.5) + floor(py * cs + 0.5) + floor(pz * cs + 0.5);
-            // ((sum % 2) + 2) % 2 != 0 (materials.ts checker parity): sum is an
-            // integer, so sum - 2*floor(sum/2) is exactly 1 when odd, 0 when even
-            // (NaN for NaN or infinities, as the remainders give)
-            if (sum - 2.0 * floor(0.5 * sum) != 0) {  // !checkerPrimary
-              float4 m2 = gload(S.mats + m + 2);
-              ar = m2.x;
-              ag = m2.y;
-              ab = m2.z;
-            }
-          }
-          LightSample ls;
-          if (useNee && sample_light(S, px, py, pz, rng, ls)) {
-            double pdfL = ls.pdf;
-            double cosS = sx * ls.w[0] + sy * ls.w[1] + sz * ls.w[2];
-            if (cosS > 0 && pdfL > 0) {
-              double pdfB = cosS * INV_PI;
-              double k = (cosS * INV_PI * power_heuristic(pdfL, pdfB)) / pdfL;
-              shadow = true;
-              wdir[0] = ls.w[0];
-              wdir[1] = ls.w[1];
-              wdir[2] = ls.w[2];
-              sTmax = ls.dist * SHADOW_SHORTEN;
-              sc[0] = br * ar * ls.e[0] * k;
-              sc[1] = bg * ag * ls.e[1] * k;
-              sc[2] = bb * ab * ls.e[2] * k;
-            }
-          }
-          // cosineHemisphereInto (materials.ts:47-64)
-          double u1 = rng.next();
-          double u2 = rng.next();
-          double r = sqrt(u1);
-          double phi = TWO_PI * u2;
-          double lx = r * jsm::cos(phi);
-          double ly = r * jsm::sin(phi);
-          double lz = sqrt(u1 < 1 ? 1 - u1 : 0);
-          double b[6];
-          basis(sx, sy, sz, b);
-          double ddx = b[0] * lx + b[3] * ly + sx * lz;
-          double ddy = b[1] * lx + b[4] * ly + sy * lz;
-          double ddz = b[2] * lx + b[5] * ly + sz * lz;
-          double inv = 1 / sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
-          ndx = ddx * inv;
-          ndy = ddy * inv;
-          ndz = ddz * inv;
-          prevPdf = max0(sx * ndx + sy * ndy + sz * ndz) * INV_PI;
-          br *= ar;
-          bg *= ag;
-          bb *= ab;
-          specularBounce = false;
-        } else if (type == 1.0f) {  // MAT_METAL: metalInto (materials.ts:102-124)
-          double u1 = rng.next();
-          double u2 = rng.next();
-          double u3 = rng.next();
-          double rough = m1.x;
-          double k = 2 * (dx * sx + dy * sy + dz * sz);
-          double rx = dx - k * sx;
-          double ry = dy - k * sy;
-          double rz = dz - k * sz;
-          if (rough > 0) {
-            double uv[3];
-            unit_vector(u1, u2, uv);
-            double rad = jsm::cbrt(u3) * rough;
-            rx += uv[0] * rad;
-            ry += uv[1] * rad;
-            rz += uv[2] * rad;
-          }
-          double inv = 1 / sqrt(rx * rx + ry * ry + rz * rz);
-          ndx = rx * inv;
-          ndy = ry * inv;
-          ndz = rz * inv;
-          if (ndx * sx + ndy * sy + ndz * sz <= 0) {
-            cont = false;
-          } else {
-            br *= (double)m0.x;
-            bg *= (double)m0.y;
-            bb *= (double)m0.z;
-            specularBounce = true;
-          }
-        } else {  // MAT_DIELECTRIC: dielectricInto (materials.ts:131-166)
-          double u = rng.next();
-          double ior = m1.x;
-          double etaI = frontFace ? 1 : ior;
-          double etaT = frontFace ? ior : 1;
-          double eta = etaI / etaT;
-          double cosI = -(dx * sx + dy * sy + dz * sz);
-          if (cosI > 1) cosI = 1;
-          double refl = dielectric_reflectance(cosI, etaI, etaT);
-          double sin2T = eta * eta * (1 - cosI * cosI);
-          if (u < refl || sin2T > 1) {
-            double k = 2 * (dx * sx + dy * sy + dz * sz);
-            ndx = dx - k * sx;
-            ndy = dy - k * sy;
-            ndz = dz - k * sz;
-          } else {
-            double cosT = sqrt(1 - sin2T);
-            double s2 = eta * cosI - cosT;
-            double tx = dx * eta + sx * s2;
-            double ty = dy * eta + sy * s2;
-            double tz = dz * eta + sz * s2;
-            double inv = 1 / sqrt(tx * tx + ty * ty + tz * tz);
-            ndx = tx * inv;
-            ndy = ty * inv;
-            ndz = tz * inv;
-          }
-          br *= (double)m0.x;
-          bg *= (double)m0.y;
-          bb *= (double)m0.z;
-          specularBounce = true;
-        }
-        if (cont && opts.rr && bounce >= 3) {  // tracer.ts:305-312
-          double maxB = br > bg ? (br > bb ? br : bb) : bg > bb ? bg : bb;
-          double pSurvive = maxB > 1 ? 1 : maxB < 0.05 ? 0.05 : maxB;
-          if (rng.next() > pSurvive) {
-            cont = false;
-          } else {
-            br /= pSurvive;
-            bg /= pSurvive;
-            bb /= pSurvive;
-          }
-        }
-      }
-      P.Lr[pid] = Lr;
-      P.Lg[pid] = Lg;
-      P.Lb[pid] = Lb;
-      // the shadow ray starts at the hit point, which is also the next origin
-      P.ox[pid] = px;
-      P.oy[pid] = py;
-      P.oz[pid] = pz;
+      PathState ps;
+      ps.ox = P.ox[pid]; ps.oy = P.oy[pid]; ps.oz = P.oz[pid];
+      ps.dx = P.dx[pid]; ps.dy = P.dy[pid]; ps.dz = P.dz[pid];
+      ps.br = P.br[pid]; ps.bg = P.bg[pid]; ps.bb = P.bb[pid];
+      ps.Lr = P.Lr[pid]; ps.Lg = P.Lg[pid]; ps.Lb = P.Lb[pid];
+      ps.prevPdf = P.prevPdf[pid];
+      ps.specular = P.specular[pid] != 0;
+      ps.rng.s = P.rng[pid];
+      const int prim = P.hitPrim[pid];
+      const double t = prim >= 0 ? P.hitT[pid] : 0.0;
+      const bool cont = shade_bounce(J.S, J.opts, ps, prim, t, bounce, shadow, nee);
+      P.Lr[pid] = ps.Lr;
+      P.Lg[pid] = ps.Lg;
+      P.Lb[pid] = ps.Lb;
+      P.ox[pid] = ps.ox;
+      P.oy[pid] = ps.oy;
+      P.oz[pid] = ps.oz;
       // kept for ended paths too: radianceInto hands the state back
-      P.rng[pid] = rng.s;
-      if (cont && bounce < opts.maxBounces) {
+      P.rng[pid] = ps.rng.s;
+      if (cont && bounce < J.opts.maxBounces) {
         alive = true;
-        P.dx[pid] = ndx;
-        P.dy[pid] = ndy;
-        P.dz[pid] = ndz;
-        P.br[pid] = br;
-        P.bg[pid] = bg;
-        P.bb[pid] = bb;
-        P.prevPdf[pid] = prevPdf;
-        P.specular[pid] = specularBounce ? 1 : 0;
+        P.dx[pid] = ps.dx;
+        P.dy[pid] = ps.dy;
+        P.dz[pid] = ps.dz;
+        P.br[pid] = ps.br;
+        P.bg[pid] = ps.bg;
+        P.bb[pid] = ps.bb;
+        P.prevPdf[pid] = ps.prevPdf;
+        P.specular[pid] = ps.specular ? 1 : 0;
       }
     }
     wave_push(alive, pid, qout, countOut);
     uint32_t sslot = wave_reserve(shadow, shadowCount);
     if (shadow) {
       P.sPath[sslot] = pid;
-      P.sdx[sslot] = wdir[0];
-      P.sdy[sslot] = wdir[1];
-      P.sdz[sslot] = wdir[2];
-      P.sTmax[sslot] = sTmax;
-      P.sCr[sslot] = sc[0];
-      P.sCg[sslot] = sc[1];
-      P.sCb[sslot] = sc[2];
+      P.sdx[sslot] = nee.w[0];
+      P.sdy[sslot] = nee.w[1];
+      P.sdz[sslot] = nee.w[2];
+      P.sTmax[sslot] = nee.tMax;
+      P.sCr[sslot] = nee.c[0];
+      P.sCg[sslot] = nee.c[1];
+      P.sCb[sslot] = nee.c[2];
     }
   }
 }
@@ -796,6 +854,146 @@ k_shadow(PathSoA P, const JobDev* __restrict__ jobs, const uint32_t* __restrict_
   }
 }
 
+// One lane's closest hit over the quantized 4-wide nodes (ct_qwalk.h), each
+// leaf tested when the walk reaches it; a stack overflow continues with the
+// skip-link walk under the current bound, which is exact.  Exact-t ties are
+// marked (TIES_DETECT) for settle_ties.
+template <int STACK, bool COUNT>
+__device__ __forceinline__ void closest_wide_lane(const DevScene& S, double ox, double oy,
+                                                  double oz, double dx, double dy, double dz,
+                                                  Hit& h, int* __restrict__ sNode,
+                                                  qw::Entry* __restrict__ sEntry, int lanes,
+                                                  TraversalCounts& cnt) {
+  const qw::QRay Q = qw::qray64(S, ox, oy, oz, dx, dy, dz);
+  int cur = 0, sp = 0;
+  while (cur != -1) {
+    if (cur <= -2) {
+      test_leaf<COUNT>(S, -2 - cur, ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
+      cur = -1;
+    } else if (!qw::qwide_step<true, STACK, COUNT>(S, Q, qw::cull_bound(h.t), cur, sp, sNode,
+                                                   sEntry, lanes, cnt)) {
+      if (COUNT) cnt.fallbacks++;
+      closest_stackless<COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, h, cnt);
+      return;
+    }
+    if (cur == -1) qw::qpop(cur, sp, sNode, sEntry, lanes, qw::cull_bound(h.t));
+  }
+}
+
+// One lane's any hit on (T_MIN, tMax) over the quantized 4-wide nodes; a
+// stack overflow answers with the skip-link any-hit walk.
+template <int STACK, bool COUNT>
+__device__ __forceinline__ bool occluded_wide_lane(const DevScene& S, double ox, double oy,
+                                                   double oz, double dx, double dy, double dz,
+                                                   double tMax, int* __restrict__ sNode,
+                                                   int lanes, TraversalCounts& cnt) {
+  const qw::QRay Q = qw::qray64(S, ox, oy, oz, dx, dy, dz);
+  int cur = 0, sp = 0;
+  while (cur != -1) {
+    if (cur <= -2) {
+      if (leaf_blocks(S, -2 - cur, ox, oy, oz, dx, dy, dz, tMax, cnt, COUNT)) return true;
+      cur = -1;
+    } else if (!qw::qwide_step<false, STACK, COUNT>(S, Q, qw::cull_bound(tMax), cur, sp, sNode,
+                                                    (qw::Entry*)nullptr, lanes, cnt)) {
+      if (COUNT) cnt.fallbacks++;
+      return occluded<COUNT>(S, ox, oy, oz, dx, dy, dz, T_MIN, tMax, cnt);
+    }
+    if (cur == -1 && sp > 0) {
+      sp--;
+      cur = sNode[sp * lanes];
+    }
+  }
+  return false;
+}
+
+// LDS stack entries of the per-path walks.
+constexpr int PATHS_STACK = 24;
+
+// Whole paths, one per lane: the path's start (start_path), then per bounce
+// the closest hit (the 4-wide walk, or with WIDE false the skip-link walk;
+// exact-t ties settled in intersectClosest's order), one bounce of shading
+// (shade_bounce) and the NEE ray's any-hit test, in the reference's order --
+// the same arithmetic as the wavefront launches, so the same radiance bit for
+// bit.  The radiance and the RNG state are left in the path's slot for
+// k_finalize.  The bounce loop is wave-uniform (a lane whose path has ended
+// idles until the wave's last path ends), so each bounce's ray counters take
+// one atomic per wave.  For rounds over small scenes, where the wavefront's
+// per-bounce launches (three per bounce and path group) cost more than the
+// work they carry.
+template <bool MULTI, bool COUNT, bool WIDE>
+__global__ void __launch_bounds__(RBLOCK)
+k_paths(PathSoA P, const JobDev* __restrict__ jobs, int njobs, size_t base, size_t count,
+        int nb) {
+  constexpr int SD = WIDE ? PATHS_STACK : 1;
+  __shared__ int sNodeAll[SD * RBLOCK];
+  __shared__ qw::Entry sEntryAll[SD * RBLOCK];
+  int* sNode = sNodeAll + threadIdx.x;
+  qw::Entry* sEntry = sEntryAll + threadIdx.x;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < count;
+  const size_t pid = base + i;
+  const uint32_t lane = threadIdx.x & 63u;
+  const JobDev* J = jobs;
+  PathState ps;
+  bool alive = false;
+  if (valid) {
+    J = jobs + job_of<MULTI, false>(jobs, njobs, pid);
+    ps = start_path(*J, pid - J->pathBase);
+    alive = true;
+  }
+  TraversalCounts cnt = {0, 0, 0, 0}, scnt = {0, 0, 0, 0};
+  for (int b = 0; b < nb; b++) {
+    const uint64_t live = __ballot(alive);
+    if (live == 0) break;
+    if (lane == (uint32_t)__ffsll((unsigned long long)live) - 1u)
+      atomicAdd(P.ctr + b, (uint32_t)__popcll(live));
+    bool neeOn = false;
+    NeeRay nee;
+    if (alive) {
+      const DevScene& S = J->S;
+      Hit h;
+      h.t = __builtin_inf(); h.u = 0; h.v = 0; h.prim = -1; h.tie = false;
+      if (WIDE)
+        closest_wide_lane<SD, COUNT>(S, ps.ox, ps.oy, ps.oz, ps.dx, ps.dy, ps.dz, h, sNode,
+                                     sEntry, RBLOCK, cnt);
+      else
+        closest_stackless<COUNT>(S, ps.ox, ps.oy, ps.oz, ps.dx, ps.dy, ps.dz, T_MIN, h, cnt);
+      settle_ties<COUNT>(S, ps.ox, ps.oy, ps.oz, ps.dx, ps.dy, ps.dz, T_MIN, __builtin_inf(), h,
+                         cnt);
+      const bool cont = shade_bounce(S, J->opts, ps, h.prim, h.t, b, neeOn, nee);
+      alive = cont && b < J->opts.maxBounces;
+    }
+    const uint64_t sh = __ballot(neeOn);
+    if (sh != 0 && lane == (uint32_t)__ffsll((unsigned long long)sh) - 1u)
+      atomicAdd(P.ctr + MAX_BOUNCE_SLOTS + b, (uint32_t)__popcll(sh));
+    if (neeOn &&
+        !(WIDE ? occluded_wide_lane<SD, COUNT>(J->S, ps.ox, ps.oy, ps.oz, nee.w[0], nee.w[1],
+                                               nee.w[2], nee.tMax, sNode, RBLOCK, scnt)
+               : occluded<COUNT>(J->S, ps.ox, ps.oy, ps.oz, nee.w[0], nee.w[1], nee.w[2], T_MIN,
+                                 nee.tMax, scnt))) {
+      ps.Lr += nee.c[0];
+      ps.Lg += nee.c[1];
+      ps.Lb += nee.c[2];
+    }
+  }
+  if (valid) {
+    P.Lr[pid] = ps.Lr;
+    P.Lg[pid] = ps.Lg;
+    P.Lb[pid] = ps.Lb;
+    P.rng[pid] = ps.rng.s;
+  }
+  if (COUNT) {
+    wave_add64(P.ctr64 + 0, cnt.nodes);
+    wave_add64(P.ctr64 + 1, cnt.tris);
+    wave_add64(P.ctr64 + 2, cnt.spheres);
+    wave_add64(P.ctr64 + 3, scnt.nodes);
+    wave_add64(P.ctr64 + 4, scnt.tris);
+    wave_add64(P.ctr64 + 5, scnt.spheres);
+    wave_add64(P.ctr64 + 6, cnt.fallbacks);
+    wave_add64(P.ctr64 + 7, scnt.fallbacks);
+  }
+}
+
 // Adds each pass's sample in pass order, dropping non-finite samples
 // (tracer.ts:386-392); optional final scale (tracer.ts:352-353).
 template <bool MULTI>
@@ -875,6 +1073,8 @@ struct ct_tracer {
   hipStream_t ownStream = nullptr;   // capture stream
   bool overlap = true;               // NEE launches on side streams (SideStream)
   int groups = 2;                    // path groups per round, each on its own stream
+  int pathKernel = CT_PATHS_AUTO;    // wavefront launches or one per-path launch (k_paths)
+  bool roundPaths = false;           // the current round runs k_paths
   SideStream side, sideB, groupB;    // streams and events, created on first use
   hipGraphExec_t graphExec[2] = {nullptr, nullptr};  // one per path group
   std::vector<char> graphKey;
@@ -885,6 +1085,7 @@ struct ct_tracer {
   uint64_t lastShadow = 0;
   uint32_t lastRounds = 0;
   uint32_t lastExtendLaunches = 0;
+  uint32_t lastPathRounds = 0;  // rounds of the last call traced by k_paths
   std::vector<hipEvent_t> events;  // (begin, end) pairs per launch when profiling
   std::vector<int> eventKind;      // 0 extend, 1 shade, 2 shadow
   std::vector<hipEvent_t> eventPool;  // recorded before, reused by later calls
@@ -1117,6 +1318,16 @@ int ct::tracer_streams(ct_tracer* T) {
 // Smallest round traced as two path groups.
 constexpr size_t SPLIT_MIN_PATHS = size_t(1) << 20;
 
+// Rounds CT_PATHS_AUTO traces with k_paths: at most this many paths, every
+// scene at most this many primitives.  Measured against the wavefront (ms per
+// frame): 1M-path frames of soups 2.6x faster at 2k triangles, 1.6x at 8k,
+// 1.25x at 16k, 0.69x at 64k; 65k-path frames 1.9x at 16k; but 8.3M-path
+// frames only 1.7x at 2k, 0.98x at 8k, and the mirror room (deep paths) 0.78x
+// -- as rounds grow, the wavefront's per-bounce compaction outweighs its
+// launch costs, while a lane of k_paths idles once its own path has ended.
+constexpr size_t PATHS_AUTO_MAX_PATHS = size_t(1) << 21;
+constexpr int PATHS_AUTO_MAX_PRIMS = 16384;
+
 // The streams of a round of `paths` paths.  Profiling events are recorded on
 // each launch's own stream, so they bracket that launch as it runs beside the
 // others.
@@ -1167,6 +1378,25 @@ template <bool MULTI>
 static int launch_group_body(ct_tracer* T, const PathSoA& P, int njobs, size_t base,
                              size_t count, int nb, hipStream_t st, const SideStream& side) {
   const JobDev* jobs = T->jobsDev;
+  if (T->roundPaths) {  // the whole group in one launch, a lane per path
+    const unsigned grid = (unsigned)((count + RBLOCK - 1) / RBLOCK);
+    mark(T, st, 0, true);
+    const bool wide = T->walk != CT_WALK_STACKLESS;
+#define CT_PATHS(C, W)                                                                       \
+  hipLaunchKernelGGL((k_paths<MULTI, C, W>), dim3(grid), dim3(RBLOCK), 0, st, P, jobs, njobs, \
+                     base, count, nb)
+    if (T->counting) {
+      if (wide) CT_PATHS(true, true);
+      else CT_PATHS(true, false);
+    } else {
+      if (wide) CT_PATHS(false, true);
+      else CT_PATHS(false, false);
+    }
+#undef CT_PATHS
+    CT_LAUNCH_CHECK("k_paths");
+    mark(T, st, 0, false);
+    return CT_OK;
+  }
   unsigned gridPaths = grid_for(count, RBLOCK, 256 * 16);
   hipLaunchKernelGGL((k_generate<MULTI>), dim3(gridPaths), dim3(RBLOCK), 0, st, P, jobs, njobs,
                      base, count);
@@ -1264,6 +1494,20 @@ struct BatchTotals {
   uint64_t ext = 0, shadow = 0;
 };
 
+// Whether a round of `paths` paths runs k_paths instead of the wavefront
+// launches: asked for, or (CT_PATHS_AUTO) when the round is small and every
+// scene of it too (PATHS_AUTO_MAX_*), where a bounce's launches cost more
+// than the traversal they carry.  k_paths is the binary64 arithmetic; the
+// FP32 precision form always runs its wavefront.
+static bool round_uses_paths(const ct_tracer* T, const std::vector<RoundJob>& R, size_t paths) {
+  if (T->precision != CT_PRECISION_F64 || T->pathKernel == CT_PATHS_WAVEFRONT) return false;
+  if (T->pathKernel == CT_PATHS_PER_PATH) return true;
+  if (paths > PATHS_AUTO_MAX_PATHS) return false;
+  for (const RoundJob& j : R)
+    if (j.scene->dev.primCount > PATHS_AUTO_MAX_PRIMS) return false;
+  return true;
+}
+
 // Traces one round: every job's passes, all jobs' paths sharing the launches.
 static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t st,
                      BatchTotals& tot) {
@@ -1311,6 +1555,7 @@ static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t s
                         hipMemcpyHostToDevice, st));
   int nb = maxB + 1;
   bool multi = R.size() > 1;
+  T->roundPaths = round_uses_paths(T, R, paths);
   RoundStreams rs;
   rc = round_streams(T, paths, rs);
   if (rc) return rc;
@@ -1318,11 +1563,11 @@ static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t s
     // each path group's launches are one captured graph, replayed on the
     // group's stream; they depend only on the round's shape and the tracer's
     // settings, and read the job table from device memory at replay
-    std::vector<char> key(sizeof(size_t) * 2 + sizeof(int) * 10);
+    std::vector<char> key(sizeof(size_t) * 2 + sizeof(int) * 11);
     size_t kk[2] = {paths, pix};
-    int ki[10] = {nb, (int)R.size(), T->walk, T->shadowWalk, T->stack, (int)traversal_blocks(T),
+    int ki[11] = {nb, (int)R.size(), T->walk, T->shadowWalk, T->stack, (int)traversal_blocks(T),
                   T->counting, multi, T->block,
-                  T->precision * 64 + (T->overlap ? 32 : 0) + rs.groups};
+                  T->precision * 64 + (T->overlap ? 32 : 0) + rs.groups, T->roundPaths ? 1 : 0};
     memcpy(key.data(), kk, sizeof kk);
     memcpy(key.data() + sizeof kk, ki, sizeof ki);
     if (!T->graphExec[0] || key != T->graphKey) {
@@ -1380,7 +1625,8 @@ static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t s
     }
   T->lastBounces = std::max(T->lastBounces, nb);
   T->lastRounds++;
-  T->lastExtendLaunches += (uint32_t)(nb * rs.groups);
+  T->lastExtendLaunches += (uint32_t)((T->roundPaths ? 1 : nb) * rs.groups);
+  T->lastPathRounds += T->roundPaths ? 1 : 0;
   return CT_OK;
 }
 
@@ -1392,6 +1638,7 @@ int ct::run_plan(ct_tracer* T, std::vector<PlanJob> jobs, hipStream_t st) {
   T->lastBounces = 0;
   T->lastRounds = 0;
   T->lastExtendLaunches = 0;
+  T->lastPathRounds = 0;
   BatchTotals tot;
   uint64_t paths = 0;
   for (const PlanJob& j : jobs) {
@@ -1638,6 +1885,21 @@ int ct_tracer_set_overlap(ct_tracer* T, int32_t enable) {
   if (!T) return set_error(CT_EINVAL, "null tracer");
   T->overlap = enable != 0;
   drop_graphs(T);  // a captured round keeps the streams it was captured with
+  return CT_OK;
+}
+
+int ct_tracer_set_path_kernel(ct_tracer* T, int32_t mode) {
+  if (!T) return set_error(CT_EINVAL, "null tracer");
+  if (mode != CT_PATHS_AUTO && mode != CT_PATHS_WAVEFRONT && mode != CT_PATHS_PER_PATH)
+    return set_error(CT_EINVAL, "ct_tracer_set_path_kernel: unknown mode");
+  T->pathKernel = mode;
+  return CT_OK;
+}
+
+int ct_tracer_get_path_kernel(const ct_tracer* T, int32_t* mode, uint32_t* perPathRounds) {
+  if (!T) return set_error(CT_EINVAL, "null tracer");
+  if (mode) *mode = T->pathKernel;
+  if (perPathRounds) *perPathRounds = T->lastPathRounds;
   return CT_OK;
 }
 

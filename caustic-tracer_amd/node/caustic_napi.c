@@ -17,6 +17,9 @@
  *                  russianRoulette, nee, accum)               (tracer.ts:361)
  *   renderImage(scene, frame16, width, height, spp, seed, maxBounces,
  *               russianRoulette, nee) -> Float32Array          (tracer.ts:348)
+ *   radiance(scene, rays Float64Array(6n), states Uint32Array(n), maxBounces,
+ *            russianRoulette, nee) -> Float64Array(3n); states advanced in
+ *            place                                              (tracer.ts:191)
  *   setPrecision(0|1)            arithmetic of the two calls above
  *   createServer({hbmBudget, maxPaths, maxActive, slicePasses, policy,
  *                 precision}) -> server handle                 (serving loop)
@@ -409,6 +412,44 @@ static napi_value render_image(napi_env env, napi_callback_info info) {
   return res;
 }
 
+/* radiance(scene, rays Float64Array(6n), states Uint32Array(n), maxBounces, rr, nee)
+ * -> Float64Array(3n); the states are updated in place (radianceInto's Rng). */
+static napi_value radiance(napi_env env, napi_callback_info info) {
+  size_t argc = 6;
+  napi_value a[6];
+  CALL(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  ct_scene *s = get_scene(env, a[0]);
+  if (!s) return NULL;
+  void *rays, *states;
+  size_t nr, ns;
+  if (!typed(env, a[1], napi_float64_array, &rays, &nr)) return NULL;
+  if (!typed(env, a[2], napi_uint32_array, &states, &ns)) return NULL;
+  if (nr % 6 != 0 || nr / 6 != ns || ns > 0xffffffffu) {
+    napi_throw_range_error(env, NULL, "radiance: rays needs 6 values per state");
+    return NULL;
+  }
+  const uint32_t n = (uint32_t)ns;
+  ct_trace_opts o = {(int32_t)num(env, a[3]), (int32_t)num(env, a[4]), (int32_t)num(env, a[5])};
+  void *h;
+  napi_value res = new_typed(env, napi_float64_array, (size_t)n * 3, 8, &h);
+  if (!res || n == 0) return res;
+  int rc = 0;
+  if (!g_tracer) rc = ct_tracer_create(&g_tracer);
+  void *dr = NULL, *ds = NULL, *dout = NULL;
+  if (!rc) dr = to_device(rays, (size_t)n * 48, &rc);
+  if (!rc) ds = to_device(states, (size_t)n * 4, &rc);
+  if (!rc) rc = ct_device_alloc(&dout, (size_t)n * 24);
+  if (!rc) rc = ct_radiance(g_tracer, s, (const double *)dr, n, (uint32_t *)ds, &o,
+                            (double *)dout, NULL);
+  if (!rc) rc = ct_copy_d2h(h, dout, (size_t)n * 24);
+  if (!rc) rc = ct_copy_d2h(states, ds, (size_t)n * 4);
+  ct_device_free(dr);
+  ct_device_free(ds);
+  ct_device_free(dout);
+  if (rc) return throw_ct(env, rc, "ct_radiance");
+  return res;
+}
+
 static napi_value set_precision(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value a[1];
@@ -696,6 +737,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"occluded", NULL, occluded, NULL, NULL, NULL, napi_default, NULL},
       {"accumulatePass", NULL, accumulate_pass, NULL, NULL, NULL, napi_default, NULL},
       {"renderImage", NULL, render_image, NULL, NULL, NULL, napi_default, NULL},
+      {"radiance", NULL, radiance, NULL, NULL, NULL, napi_default, NULL},
       {"setPrecision", NULL, set_precision, NULL, NULL, NULL, napi_default, NULL},
       {"createServer", NULL, create_server, NULL, NULL, NULL, napi_default, NULL},
       {"submitJob", NULL, submit_job, NULL, NULL, NULL, napi_default, NULL},

@@ -141,6 +141,67 @@ function pick(ps, frame, sx, sy) {
   }
 }
 
+// rng.ts:10-64 -- the per-path random stream the radiance estimates draw from:
+// a PCG output permutation turns (pixel, frame) into a seed, xorshift32 steps
+// produce the numbers; all arithmetic in uint32.
+function pcgHash(input) {
+  const lcg = (Math.imul(input >>> 0, 747796405) + 2891336453) >>> 0
+  const x = ((lcg >>> ((lcg >>> 28) + 4)) ^ lcg) >>> 0
+  const w = Math.imul(x, 277803737) >>> 0
+  return ((w >>> 22) ^ w) >>> 0
+}
+
+function seedFor(pixelIndex, frame) {
+  const h = pcgHash(pcgHash(pixelIndex >>> 0) ^ Math.imul(pcgHash((frame >>> 0) ^ 0x9e3779b9), 0x85ebca6b))
+  return h === 0 ? 0x1234567 : h
+}
+
+class Rng {
+  constructor(seed) {
+    const h = pcgHash(seed >>> 0)
+    this.state = h === 0 ? 0x9e3779b9 : h
+  }
+  static fromPixel(pixelIndex, frame) {
+    const r = new Rng(0)
+    r.state = seedFor(pixelIndex, frame)
+    return r
+  }
+  nextU32() {
+    let x = this.state
+    x = (x ^ (x << 13)) >>> 0
+    x = (x ^ (x >>> 17)) >>> 0
+    x = (x ^ (x << 5)) >>> 0
+    this.state = x
+    return x
+  }
+  next() { return this.nextU32() / 4294967296 }
+  range(min, max) { return min + (max - min) * this.next() }
+  int(n) { return Math.min(n - 1, Math.floor(this.next() * n)) }
+}
+
+/**
+ * radianceInto (tracer.ts:191-328) for many rays on the GPU: rays is a
+ * Float64Array of 6 values per ray (origin, unit direction), states a
+ * Uint32Array of xorshift32 states (one Rng per ray), advanced in place as the
+ * reference advances each Rng.  Returns a Float64Array of 3 values per ray.
+ */
+function radianceBatch(ps, rays, states, opts) {
+  return addon.radiance(deviceScene(packedOf(ps)), rays, states, opts.maxBounces,
+    opts.russianRoulette ? 1 : 0, opts.nee ? 1 : 0)
+}
+
+/** radianceInto (tracer.ts:191): one ray, drawing from rng (an Rng); out gets linear RGB. */
+function radianceInto(ps, ox, oy, oz, dx, dy, dz, rng, opts, out) {
+  const st = new Uint32Array([rng.state >>> 0])
+  const L = radianceBatch(ps, new Float64Array([ox, oy, oz, dx, dy, dz]), st, opts)
+  rng.state = st[0]
+  const o = out || new Float64Array(3)
+  o[0] = L[0]
+  o[1] = L[1]
+  o[2] = L[2]
+  return o
+}
+
 /** Arithmetic of accumulatePass / renderImage: 0 binary64 (reference), 1 FP32 precision form. */
 function setPrecision(p) {
   addon.setPrecision(p)
@@ -171,5 +232,5 @@ class Server {
 module.exports = {
   WALK_STACKLESS, WALK_ORDERED, T_MIN, buildBvh, packScene, intersectClosest, intersectClosestStackless,
   occluded, closestHitBatch, occludedBatch, accumulatePass, renderImage, frameArray,
-  setPrecision, Server, tonemapBuffer, pick,
+  setPrecision, Server, tonemapBuffer, pick, pcgHash, seedFor, Rng, radianceInto, radianceBatch,
 }

@@ -21,6 +21,9 @@ WALK_WIDE = 2
 WALK_BRUTE = 3
 PRECISION_F64 = 0
 PRECISION_F32 = 1
+PATHS_AUTO = 0  # per-path launches for small rounds over small scenes, else the wavefront
+PATHS_WAVEFRONT = 1
+PATHS_PER_PATH = 2
 
 
 class DeviceScene:
@@ -324,6 +327,20 @@ class Tracer:
 
     def set_graph(self, on):
         check(lib().ct_tracer_set_graph(self.handle, int(bool(on))), "set_graph")
+
+    def set_path_kernel(self, mode):
+        """PATHS_WAVEFRONT (per-bounce launches), PATHS_PER_PATH (one launch, a
+        lane per path) or PATHS_AUTO (per-path for rounds of at most 2^21 paths
+        over scenes of at most 16384 primitives); the binary64 results are
+        identical under each."""
+        check(lib().ct_tracer_set_path_kernel(self.handle, int(mode)), "set_path_kernel")
+
+    def get_path_kernel(self):
+        """(mode, rounds of the last call that ran the per-path launch)."""
+        m, r = C.c_int32(0), C.c_uint32(0)
+        check(lib().ct_tracer_get_path_kernel(self.handle, C.byref(m), C.byref(r)),
+              "get_path_kernel")
+        return m.value, r.value
 
     def set_overlap(self, on):
         """NEE launches beside the next extension launch (True) or in order."""

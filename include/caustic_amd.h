@@ -256,6 +256,27 @@ int ct_tracer_set_graph(ct_tracer *tracer, int32_t enable);
  * one path group and no overlap every launch of a round runs alone, which
  * per-launch profiling uses; results never depend on it. */
 int ct_tracer_set_overlap(ct_tracer *tracer, int32_t enable);
+/* How a binary64 round is traced: CT_PATHS_WAVEFRONT runs the launches above
+ * (per bounce an extension, a shading and an NEE launch per path group);
+ * CT_PATHS_PER_PATH runs one launch per path group in which each lane carries
+ * one path through all its bounces (the skip-link walks of intersect.ts:
+ * 290-420 for the closest hit and the NEE test, the same shading), which
+ * suits small scenes, where a bounce's launches cost more than the work they
+ * carry; CT_PATHS_AUTO (default) takes the per-path launch for rounds of at
+ * most 2^21 paths whose scenes have at most 16384 primitives each.  The
+ * per-path walks are the 4-wide ones (LDS stack, leaves tested as reached),
+ * or the skip-link walks when the tracer's walk is CT_WALK_STACKLESS.
+ * Results are identical either way.
+ * The FP32 precision form always runs its wavefront.  With the per-path
+ * launch, a round's profiling events and extendLaunches count the per-path
+ * launches (no shading or NEE launches are recorded).  No counterpart in the
+ * reference, which traces one path at a time (tracer.ts:361-395). */
+#define CT_PATHS_AUTO 0
+#define CT_PATHS_WAVEFRONT 1
+#define CT_PATHS_PER_PATH 2
+int ct_tracer_set_path_kernel(ct_tracer *tracer, int32_t mode);
+/* The mode set, and how many rounds of the last call ran the per-path launch. */
+int ct_tracer_get_path_kernel(const ct_tracer *tracer, int32_t *mode, uint32_t *perPathRounds);
 /* CT_PRECISION_F64 or CT_PRECISION_F32 (which always walks the quantized
  * 4-wide nodes; the walk settings then only pick the stack depth). */
 int ct_tracer_set_precision(ct_tracer *tracer, int32_t precision);

@@ -358,6 +358,7 @@ def test_render_c1_cornell_variant(ct, O, walk):
     o = O.render_image(ps, fr, w, h, 4, seed=1, max_bounces=8)
     tr = ct.Tracer(walk=walk)
     tr.set_traversal(walk, 16)
+    tr.set_path_kernel(ct.PATHS_WAVEFRONT)  # the walks of the wavefront launches
     d = ct.DeviceBuffer.empty((w * h * 3,), np.float32)
     tr.render_image(ps, fr, {"width": w, "height": h, "spp": 4, "seed": 1, "maxBounces": 8}, d)
     g = d.numpy()

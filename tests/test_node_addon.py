@@ -26,7 +26,8 @@ def test_addon_loads_and_exports_reference_names():
     assert out.returncode == 0, out.stderr
     names = set(out.stdout.strip().split(","))
     for n in ("buildBvh", "packScene", "intersectClosest", "intersectClosestStackless", "occluded",
-              "accumulatePass", "renderImage", "setPrecision", "Server", "tonemapBuffer", "pick"):
+              "accumulatePass", "renderImage", "setPrecision", "Server", "tonemapBuffer", "pick",
+              "radianceInto", "Rng", "seedFor", "pcgHash"):
         assert n in names
 
 
@@ -130,3 +131,66 @@ def test_node_pack_scene_and_render_a_scene(ct, O, preset, tmp_path):
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     o = O.render_image(ps, fr, w, h, spp, seed=7, max_bounces=6)
     assert np.array_equal(a.view(np.uint32), o.view(np.uint32))
+
+
+@needs_node
+def test_node_rng_matches_the_pinned_host_rng(ct):
+    """The mirror's pcgHash / seedFor / Rng (rng.ts:10-64) draw the sequence
+    the host Rng draws (itself pinned by hand-worked known answers)."""
+    script = ("const m = require(%r);"
+              "const out = {hash: [0, 1, 4294967295, 123456789].map(m.pcgHash),"
+              " seed: [[0, 0], [5, 9], [1919, 4242]].map(a => m.seedFor(a[0], a[1])),"
+              " seq: (() => { const r = new m.Rng(7); return [r.nextU32(), r.nextU32(), r.next(),"
+              " r.range(-2, 3), r.int(10)] })(),"
+              " pix: (() => { const r = m.Rng.fromPixel(5, 9); return [r.state, r.nextU32()] })()};"
+              "console.log(JSON.stringify(out))") % INDEX
+    r = subprocess.run([NODE, "-e", script], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    got = json.loads(r.stdout)
+    P = ct.presets
+    assert got["hash"] == [P.pcg_hash(x) for x in (0, 1, 4294967295, 123456789)]
+    assert got["seed"] == [P.seed_for(a, b) for a, b in ((0, 0), (5, 9), (1919, 4242))]
+    h = P.Rng(7)
+    assert got["seq"] == [h.nextU32(), h.nextU32(), h.next(), h.range(-2, 3), h.int(10)]
+    p = P.Rng.fromPixel(5, 9)
+    assert got["pix"] == [p.state, p.nextU32()]
+
+
+@needs_node
+@pytest.mark.gpu
+def test_node_radiance_into_matches_oracle(O, tmp_path):
+    """radianceInto (tracer.ts:191-328) from Node: radiance bit for bit and
+    the callers' RNG states advanced as the oracle advances them."""
+    sc = scenes.cornell_box()
+    ps = O.pack_scene(sc)
+    fr = O.camera_frame(sc["camera"], 1.0)
+    rs = np.random.default_rng(23)
+    rays, states = [], []
+    for k in range(64):
+        x, y = int(rs.integers(0, 32)), int(rs.integers(0, 32))
+        o, d = O.camera_sample(fr, 32, 32, x, y, k)
+        rays.append(np.concatenate([o, d]))
+        states.append(O.seed_for(y * 32 + x, 50 + k))
+    rays = np.array(rays)
+    states = np.array(states, np.uint32)
+    opts = {"maxBounces": 8, "russianRoulette": True, "nee": True}
+    for name, arr in (("prims", ps.prims), ("nodes", ps.nodes), ("mats", ps.mats),
+                      ("lights", ps.lights), ("primObject", ps.primObject), ("rays", rays),
+                      ("states", states)):
+        arr.tofile(tmp_path / f"{name}.bin")
+    (tmp_path / "meta.json").write_text(json.dumps(
+        {"primCount": ps.primCount, "nodeCount": ps.nodeCount, "matCount": ps.matCount,
+         "opts": opts}))
+    script = os.path.join(ROOT, "tests", "node", "radiance.js")
+    r = subprocess.run([NODE, script, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    L = np.fromfile(tmp_path / "L.bin", np.float64).reshape(-1, 3)
+    st = np.fromfile(tmp_path / "states_after.bin", np.uint32)
+    for i in range(len(rays)):
+        oL, ost = O.radiance(ps, rays[i, :3], rays[i, 3:], int(states[i]), max_bounces=8,
+                             with_state=True)
+        assert np.array_equal(L[i], oL) and st[i] == ost, i
+    one = json.loads((tmp_path / "one.json").read_text())
+    oL, ost = O.radiance(ps, rays[0, :3], rays[0, 3:], int(O.seed_for(5, 9)), max_bounces=8,
+                         with_state=True)
+    assert one["same"] and one["L"] == list(oL) and one["state"] == ost
