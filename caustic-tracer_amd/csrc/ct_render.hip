@@ -24,6 +24,8 @@
 #include <algorithm>
 #include <limits>
 #include <type_traits>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "ct_engine.h"
@@ -908,49 +910,62 @@ __device__ __forceinline__ bool occluded_wide_lane(const DevScene& S, double ox,
 
 // LDS stack entries of the per-path walks.
 constexpr int PATHS_STACK = 24;
+// Waves per SIMD of k_paths: four (127 VGPRs, 4 spilled) against three
+// uncapped (139): 1.19x instead of 1.05x the wavefront on a 16k-triangle soup
+// at 1920x1080x4, 1.00x instead of 0.88x on the mirror room.
+#ifndef CT_PATHS_WAVES
+#define CT_PATHS_WAVES 4
+#endif
 
-// Whole paths, one per lane: the path's start (start_path), then per bounce
-// the closest hit (the 4-wide walk, or with WIDE false the skip-link walk;
+
+// Whole paths, one lane at a time: a lane takes the next path of the group
+// (one atomic per wave), starts it (start_path), then per bounce runs the
+// closest hit (the 4-wide walk, or with WIDE false the skip-link walk;
 // exact-t ties settled in intersectClosest's order), one bounce of shading
 // (shade_bounce) and the NEE ray's any-hit test, in the reference's order --
 // the same arithmetic as the wavefront launches, so the same radiance bit for
-// bit.  The radiance and the RNG state are left in the path's slot for
-// k_finalize.  The bounce loop is wave-uniform (a lane whose path has ended
-// idles until the wave's last path ends), so each bounce's ray counters take
-// one atomic per wave.  For rounds over small scenes, where the wavefront's
-// per-bounce launches (three per bounce and path group) cost more than the
-// work they carry.
+// bit -- and when the path ends leaves its radiance and RNG state in the
+// path's slot for k_finalize and takes the next path.  Lanes of a wave are at
+// different bounces; each bounce's extension and NEE rays are counted in LDS
+// and added to the group's counters once per workgroup.  A persistent grid
+// (the workgroups resident at once).  For rounds over small scenes, where the
+// wavefront's per-bounce launches (three per bounce and path group) cost more
+// than the work they carry.
 template <bool MULTI, bool COUNT, bool WIDE>
-__global__ void __launch_bounds__(RBLOCK)
+__global__ void __launch_bounds__(RBLOCK) __attribute__((amdgpu_waves_per_eu(CT_PATHS_WAVES)))
 k_paths(PathSoA P, const JobDev* __restrict__ jobs, int njobs, size_t base, size_t count,
         int nb) {
   constexpr int SD = WIDE ? PATHS_STACK : 1;
   __shared__ int sNodeAll[SD * RBLOCK];
   __shared__ qw::Entry sEntryAll[SD * RBLOCK];
+  __shared__ uint32_t sRays[2 * MAX_BOUNCE_SLOTS];  // [b] extension, [MAX_BOUNCE_SLOTS + b] NEE
+  for (int k = threadIdx.x; k < 2 * MAX_BOUNCE_SLOTS; k += RBLOCK) sRays[k] = 0;
+  __syncthreads();
   int* sNode = sNodeAll + threadIdx.x;
   qw::Entry* sEntry = sEntryAll + threadIdx.x;
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = i < count;
-  const size_t pid = base + i;
-  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t* fetch = P.ctr + 2 * MAX_BOUNCE_SLOTS;  // the group's path cursor
+  const uint32_t n = (uint32_t)count;
+  bool have = false, exhausted = false;
+  size_t pid = 0;
   const JobDev* J = jobs;
   PathState ps;
-  bool alive = false;
-  if (valid) {
-    J = jobs + job_of<MULTI, false>(jobs, njobs, pid);
-    ps = start_path(*J, pid - J->pathBase);
-    alive = true;
-  }
+  int b = 0;
   TraversalCounts cnt = {0, 0, 0, 0}, scnt = {0, 0, 0, 0};
-  for (int b = 0; b < nb; b++) {
-    const uint64_t live = __ballot(alive);
-    if (live == 0) break;
-    if (lane == (uint32_t)__ffsll((unsigned long long)live) - 1u)
-      atomicAdd(P.ctr + b, (uint32_t)__popcll(live));
-    bool neeOn = false;
-    NeeRay nee;
-    if (alive) {
+  for (;;) {
+    if (!exhausted) {
+      const uint32_t slot = refill(!have, n, fetch, exhausted);
+      if (slot != 0xffffffffu) {
+        pid = base + slot;
+        J = jobs + job_of<MULTI, false>(jobs, njobs, pid);
+        ps = start_path(*J, pid - J->pathBase);
+        b = 0;
+        have = true;
+      }
+    }
+    if (!__any(have)) break;
+    if (have) {
       const DevScene& S = J->S;
+      atomicAdd(sRays + b, 1u);
       Hit h;
       h.t = __builtin_inf(); h.u = 0; h.v = 0; h.prim = -1; h.tie = false;
       if (WIDE)
@@ -960,27 +975,32 @@ k_paths(PathSoA P, const JobDev* __restrict__ jobs, int njobs, size_t base, size
         closest_stackless<COUNT>(S, ps.ox, ps.oy, ps.oz, ps.dx, ps.dy, ps.dz, T_MIN, h, cnt);
       settle_ties<COUNT>(S, ps.ox, ps.oy, ps.oz, ps.dx, ps.dy, ps.dz, T_MIN, __builtin_inf(), h,
                          cnt);
+      bool neeOn = false;
+      NeeRay nee;
       const bool cont = shade_bounce(S, J->opts, ps, h.prim, h.t, b, neeOn, nee);
-      alive = cont && b < J->opts.maxBounces;
+      if (neeOn) {
+        atomicAdd(sRays + MAX_BOUNCE_SLOTS + b, 1u);
+        const bool blocked =
+            WIDE ? occluded_wide_lane<SD, COUNT>(S, ps.ox, ps.oy, ps.oz, nee.w[0], nee.w[1],
+                                                 nee.w[2], nee.tMax, sNode, RBLOCK, scnt)
+                 : occluded<COUNT>(S, ps.ox, ps.oy, ps.oz, nee.w[0], nee.w[1], nee.w[2], T_MIN,
+                                   nee.tMax, scnt);
+        if (!blocked) {
+          ps.Lr += nee.c[0];
+          ps.Lg += nee.c[1];
+          ps.Lb += nee.c[2];
+        }
+      }
+      if (cont && b < J->opts.maxBounces) {
+        b++;
+      } else {
+        P.Lr[pid] = ps.Lr;
+        P.Lg[pid] = ps.Lg;
+        P.Lb[pid] = ps.Lb;
+        P.rng[pid] = ps.rng.s;
+        have = false;
+      }
     }
-    const uint64_t sh = __ballot(neeOn);
-    if (sh != 0 && lane == (uint32_t)__ffsll((unsigned long long)sh) - 1u)
-      atomicAdd(P.ctr + MAX_BOUNCE_SLOTS + b, (uint32_t)__popcll(sh));
-    if (neeOn &&
-        !(WIDE ? occluded_wide_lane<SD, COUNT>(J->S, ps.ox, ps.oy, ps.oz, nee.w[0], nee.w[1],
-                                               nee.w[2], nee.tMax, sNode, RBLOCK, scnt)
-               : occluded<COUNT>(J->S, ps.ox, ps.oy, ps.oz, nee.w[0], nee.w[1], nee.w[2], T_MIN,
-                                 nee.tMax, scnt))) {
-      ps.Lr += nee.c[0];
-      ps.Lg += nee.c[1];
-      ps.Lb += nee.c[2];
-    }
-  }
-  if (valid) {
-    P.Lr[pid] = ps.Lr;
-    P.Lg[pid] = ps.Lg;
-    P.Lb[pid] = ps.Lb;
-    P.rng[pid] = ps.rng.s;
   }
   if (COUNT) {
     wave_add64(P.ctr64 + 0, cnt.nodes);
@@ -991,6 +1011,11 @@ k_paths(PathSoA P, const JobDev* __restrict__ jobs, int njobs, size_t base, size
     wave_add64(P.ctr64 + 5, scnt.spheres);
     wave_add64(P.ctr64 + 6, cnt.fallbacks);
     wave_add64(P.ctr64 + 7, scnt.fallbacks);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nb; k += RBLOCK) {
+    if (sRays[k]) atomicAdd(P.ctr + k, sRays[k]);
+    if (sRays[MAX_BOUNCE_SLOTS + k]) atomicAdd(P.ctr + MAX_BOUNCE_SLOTS + k, sRays[MAX_BOUNCE_SLOTS + k]);
   }
 }
 
@@ -1074,7 +1099,7 @@ struct ct_tracer {
   bool overlap = true;               // NEE launches on side streams (SideStream)
   int groups = 2;                    // path groups per round, each on its own stream
   int pathKernel = CT_PATHS_AUTO;    // wavefront launches or one per-path launch (k_paths)
-  bool roundPaths = false;           // the current round runs k_paths
+  bool groupPaths[2] = {false, false};  // path group g of the current round runs k_paths
   SideStream side, sideB, groupB;    // streams and events, created on first use
   hipGraphExec_t graphExec[2] = {nullptr, nullptr};  // one per path group
   std::vector<char> graphKey;
@@ -1318,23 +1343,22 @@ int ct::tracer_streams(ct_tracer* T) {
 // Smallest round traced as two path groups.
 constexpr size_t SPLIT_MIN_PATHS = size_t(1) << 20;
 
-// Rounds CT_PATHS_AUTO traces with k_paths: at most this many paths, every
-// scene at most this many primitives.  Measured against the wavefront (ms per
-// frame): 1M-path frames of soups 2.6x faster at 2k triangles, 1.6x at 8k,
-// 1.25x at 16k, 0.69x at 64k; 65k-path frames 1.9x at 16k; but 8.3M-path
-// frames only 1.7x at 2k, 0.98x at 8k, and the mirror room (deep paths) 0.78x
-// -- as rounds grow, the wavefront's per-bounce compaction outweighs its
-// launch costs, while a lane of k_paths idles once its own path has ended.
-constexpr size_t PATHS_AUTO_MAX_PATHS = size_t(1) << 21;
+// Largest scene (primitives) of a round CT_PATHS_AUTO traces with k_paths.
+// Measured against the wavefront (ms per frame, 256x256x16 / 1920x1080x4):
+// soups of 2k triangles 3.0x / 1.6x faster, 8k 1.9x / 1.6x, 16k 1.6x / 1.2x,
+// 64k 0.93x; the Cornell variant 2.6x at 256x256x16, 2.4x at 1024x1024x4;
+// the mirror room (deep specular paths, 2k triangles) 1.27x / 1.00x.
 constexpr int PATHS_AUTO_MAX_PRIMS = 16384;
 
 // The streams of a round of `paths` paths.  Profiling events are recorded on
 // each launch's own stream, so they bracket that launch as it runs beside the
 // others.
-static int round_streams(ct_tracer* T, size_t paths, RoundStreams& rs) {
+static int round_streams(ct_tracer* T, size_t paths, bool perPath, RoundStreams& rs) {
   rs = RoundStreams();
   int rc;
-  if (T->groups > 1 && paths >= SPLIT_MIN_PATHS) {
+  // a per-path launch is persistent and holds the whole chip by itself (one
+  // group: Cornell 256x256x16 1.02 ms against 1.12 ms as two)
+  if (T->groups > 1 && paths >= SPLIT_MIN_PATHS && !perPath) {
     if ((rc = make_side(T->groupB))) return rc;
     rs.groups = 2;
     rs.split = T->groupB;
@@ -1354,19 +1378,48 @@ static int round_streams(ct_tracer* T, size_t paths, RoundStreams& rs) {
 }
 
 
+// Workgroups of a persistent per-path launch of `count` paths: as many as the
+// device holds at once (the occupancy of `kernel`, asked once per kernel and
+// device), fewer when the paths do not need them.
+static unsigned paths_grid(const void* kernel, size_t count) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, unsigned> resident;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  unsigned r;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = resident.find({kernel, dev});
+    if (it == resident.end()) {
+      int perCU = 0, cus = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, RBLOCK, 0) != hipSuccess ||
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+          perCU < 1 || cus < 1) {
+        perCU = 1;
+        cus = 256;
+      }
+      it = resident.emplace(std::make_pair(kernel, dev), (unsigned)(perCU * cus)).first;
+    }
+    r = it->second;
+  }
+  const size_t need = (count + RBLOCK - 1) / RBLOCK;
+  return (unsigned)std::max<size_t>(1, std::min<size_t>(need, r));
+}
+
 // Kernel launches of one path group: generate, then per bounce extend, shade
 // and the NEE launch (on the group's side stream when it has one).
 template <bool MULTI>
 static int launch_group_body(ct_tracer* T, const PathSoA& P, int njobs, size_t base,
-                             size_t count, int nb, hipStream_t st, const SideStream& side);
+                             size_t count, int nb, hipStream_t st, const SideStream& side,
+                             bool perPath);
 
 // A group's launches; its NEE side stream is joined back to `st` on every
 // path out, so no later work on `st` can overlap a launch still on the side
 // stream.
 template <bool MULTI>
 static int launch_group(ct_tracer* T, const PathSoA& P, int njobs, size_t base, size_t count,
-                        int nb, hipStream_t st, const SideStream& side) {
-  int rc = launch_group_body<MULTI>(T, P, njobs, base, count, nb, st, side);
+                        int nb, hipStream_t st, const SideStream& side, bool perPath) {
+  int rc = launch_group_body<MULTI>(T, P, njobs, base, count, nb, st, side, perPath);
   if (side.side) {
     int rj = check_hip(side_join(side, st), "side_join");
     if (!rc) rc = rj;
@@ -1376,15 +1429,16 @@ static int launch_group(ct_tracer* T, const PathSoA& P, int njobs, size_t base, 
 
 template <bool MULTI>
 static int launch_group_body(ct_tracer* T, const PathSoA& P, int njobs, size_t base,
-                             size_t count, int nb, hipStream_t st, const SideStream& side) {
+                             size_t count, int nb, hipStream_t st, const SideStream& side,
+                             bool perPath) {
   const JobDev* jobs = T->jobsDev;
-  if (T->roundPaths) {  // the whole group in one launch, a lane per path
-    const unsigned grid = (unsigned)((count + RBLOCK - 1) / RBLOCK);
+  if (perPath) {  // the whole group in one persistent launch
     mark(T, st, 0, true);
     const bool wide = T->walk != CT_WALK_STACKLESS;
-#define CT_PATHS(C, W)                                                                       \
-  hipLaunchKernelGGL((k_paths<MULTI, C, W>), dim3(grid), dim3(RBLOCK), 0, st, P, jobs, njobs, \
-                     base, count, nb)
+#define CT_PATHS(C, W)                                                                          \
+  hipLaunchKernelGGL((k_paths<MULTI, C, W>), dim3(paths_grid((const void*)k_paths<MULTI, C, W>, \
+                                                             count)),                           \
+                     dim3(RBLOCK), 0, st, P, jobs, njobs, base, count, nb)
     if (T->counting) {
       if (wide) CT_PATHS(true, true);
       else CT_PATHS(true, false);
@@ -1458,7 +1512,7 @@ static int launch_part(ct_tracer* T, const RoundStreams& rs, int njobs, size_t p
     size_t base, count;
     group_range(rs, paths, g, base, count);
     return launch_group<MULTI>(T, group_view(P, base, g), njobs, base, count, nb, st,
-                               rs.shadow[g]);
+                               rs.shadow[g], T->groupPaths[g]);
   }
   return CT_OK;
 }
@@ -1494,23 +1548,30 @@ struct BatchTotals {
   uint64_t ext = 0, shadow = 0;
 };
 
-// Whether a round of `paths` paths runs k_paths instead of the wavefront
-// launches: asked for, or (CT_PATHS_AUTO) when the round is small and every
-// scene of it too (PATHS_AUTO_MAX_*), where a bounce's launches cost more
-// than the traversal they carry.  k_paths is the binary64 arithmetic; the
-// FP32 precision form always runs its wavefront.
-static bool round_uses_paths(const ct_tracer* T, const std::vector<RoundJob>& R, size_t paths) {
+// Whether job j's paths would run k_paths instead of the wavefront launches:
+// asked for, or (CT_PATHS_AUTO) when its scene has at most PATHS_AUTO_MAX_PRIMS
+// primitives, where a bounce's launches and tails cost more than the
+// per-bounce compaction of the wavefront saves.  k_paths is the binary64
+// arithmetic; the FP32 precision form always runs its wavefront.
+static bool scene_uses_paths(const ct_tracer* T, const ct_scene* scene) {
   if (T->precision != CT_PRECISION_F64 || T->pathKernel == CT_PATHS_WAVEFRONT) return false;
-  if (T->pathKernel == CT_PATHS_PER_PATH) return true;
-  if (paths > PATHS_AUTO_MAX_PATHS) return false;
-  for (const RoundJob& j : R)
-    if (j.scene->dev.primCount > PATHS_AUTO_MAX_PRIMS) return false;
-  return true;
+  return T->pathKernel == CT_PATHS_PER_PATH || scene->dev.primCount <= PATHS_AUTO_MAX_PRIMS;
+}
+static bool job_uses_paths(const ct_tracer* T, const RoundJob& j) {
+  return scene_uses_paths(T, j.scene);
 }
 
 // Traces one round: every job's passes, all jobs' paths sharing the launches.
+// The round runs k_paths when every job of it would (job_uses_paths): within
+// a round that also holds larger scenes the wavefront launches amortize
+// their costs over all paths, and a persistent per-path launch beside them
+// would take the CU slots their path groups overlap in (measured: the
+// 64-scene batch 160 -> 204 ms, the serving stream 1424 -> 1290 spp*Mpix/s
+// with the small scenes' paths split off into a per-path group).
 static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t st,
                      BatchTotals& tot) {
+  bool allPaths = !R.empty();
+  for (const RoundJob& j : R) allPaths = allPaths && job_uses_paths(T, j);
   size_t paths = 0, pix = 0;
   int maxB = 0;
   for (const RoundJob& j : R) {
@@ -1555,10 +1616,10 @@ static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t s
                         hipMemcpyHostToDevice, st));
   int nb = maxB + 1;
   bool multi = R.size() > 1;
-  T->roundPaths = round_uses_paths(T, R, paths);
   RoundStreams rs;
-  rc = round_streams(T, paths, rs);
+  rc = round_streams(T, paths, allPaths, rs);
   if (rc) return rc;
+  T->groupPaths[0] = T->groupPaths[1] = allPaths;
   if (T->graph) {
     // each path group's launches are one captured graph, replayed on the
     // group's stream; they depend only on the round's shape and the tracer's
@@ -1567,7 +1628,8 @@ static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t s
     size_t kk[2] = {paths, pix};
     int ki[11] = {nb, (int)R.size(), T->walk, T->shadowWalk, T->stack, (int)traversal_blocks(T),
                   T->counting, multi, T->block,
-                  T->precision * 64 + (T->overlap ? 32 : 0) + rs.groups, T->roundPaths ? 1 : 0};
+                  T->precision * 64 + (T->overlap ? 32 : 0) + rs.groups,
+                  (T->groupPaths[0] ? 1 : 0) + (T->groupPaths[1] ? 2 : 0)};
     memcpy(key.data(), kk, sizeof kk);
     memcpy(key.data() + sizeof kk, ki, sizeof ki);
     if (!T->graphExec[0] || key != T->graphKey) {
@@ -1625,8 +1687,8 @@ static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t s
     }
   T->lastBounces = std::max(T->lastBounces, nb);
   T->lastRounds++;
-  T->lastExtendLaunches += (uint32_t)((T->roundPaths ? 1 : nb) * rs.groups);
-  T->lastPathRounds += T->roundPaths ? 1 : 0;
+  for (int g = 0; g < rs.groups; g++) T->lastExtendLaunches += T->groupPaths[g] ? 1u : (uint32_t)nb;
+  T->lastPathRounds += T->groupPaths[0] ? 1 : 0;
   return CT_OK;
 }
 
@@ -1681,11 +1743,28 @@ int ct::run_plan(ct_tracer* T, std::vector<PlanJob> jobs, hipStream_t st) {
     }
   }
   std::vector<uint32_t> done(jobs.size(), 0);
-  for (;;) {
+  // Jobs whose paths run k_paths (job_uses_paths) are packed into rounds of
+  // their own, traced before the others' wavefront rounds, so that neither
+  // kind waits on the other's launches.  A tracer in graph mode keeps one
+  // round shape (it replays the graph of the last one) and packs all jobs
+  // together.
+  std::vector<int> kind(jobs.size(), 0);
+  bool split = false;
+  if (!T->graph) {
+    bool anyPaths = false, anyWave = false;
+    for (size_t k = 0; k < jobs.size(); k++) {
+      kind[k] = scene_uses_paths(T, jobs[k].scene) ? 0 : 1;
+      anyPaths = anyPaths || kind[k] == 0;
+      anyWave = anyWave || kind[k] == 1;
+    }
+    split = anyPaths && anyWave;
+  }
+  for (int pass = split ? 0 : 1; pass < 2;) {
     std::vector<RoundJob> R;
     size_t budget = T->maxPaths;
     for (size_t k = 0; k < jobs.size(); k++) {
       const PlanJob& j = jobs[k];
+      if (split && kind[k] != pass) continue;
       uint32_t left = j.count - done[k];
       size_t npix = (size_t)j.width * j.height;
       if (left == 0 || npix == 0) continue;
@@ -1712,7 +1791,10 @@ int ct::run_plan(ct_tracer* T, std::vector<PlanJob> jobs, hipStream_t st) {
       done[k] += p;
       budget -= p * npix;
     }
-    if (R.empty()) break;
+    if (R.empty()) {
+      pass++;
+      continue;
+    }
     int rc = run_round(T, R, st, tot);
     if (rc) return rc;
   }

@@ -21,7 +21,7 @@ WALK_WIDE = 2
 WALK_BRUTE = 3
 PRECISION_F64 = 0
 PRECISION_F32 = 1
-PATHS_AUTO = 0  # per-path launches for small rounds over small scenes, else the wavefront
+PATHS_AUTO = 0  # per-path launches for rounds over small scenes, else the wavefront
 PATHS_WAVEFRONT = 1
 PATHS_PER_PATH = 2
 
@@ -330,9 +330,9 @@ class Tracer:
 
     def set_path_kernel(self, mode):
         """PATHS_WAVEFRONT (per-bounce launches), PATHS_PER_PATH (one launch, a
-        lane per path) or PATHS_AUTO (per-path for rounds of at most 2^21 paths
-        over scenes of at most 16384 primitives); the binary64 results are
-        identical under each."""
+        lane per path at a time) or PATHS_AUTO (per-path for rounds over
+        scenes of at most 16384 primitives); the binary64 results are identical
+        under each."""
         check(lib().ct_tracer_set_path_kernel(self.handle, int(mode)), "set_path_kernel")
 
     def get_path_kernel(self):

@@ -50,7 +50,8 @@ def test_batch_equals_alone_and_oracle(ct, O):
         o = O.render_image(j["scene"], j["frame"], j["width"], j["height"], j["spp"],
                            seed=j["seed"], max_bounces=j["maxBounces"])
         assert rel_err(m, o).max() <= 1e-4
-    assert st["rounds"] == 1
+    # the two small scenes' per-path round, then the soup's wavefront round
+    assert st["rounds"] == 2
 
 
 def test_batch_split_into_rounds(ct, O):

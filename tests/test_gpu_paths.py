@@ -4,8 +4,8 @@ bounces) against the wavefront launches and the oracle.
 Both forms run the same binary64 arithmetic in the reference's order, so the
 images, radiance values, RNG states and ray counts must be identical bit for
 bit under every option, and within 1e-4 per pixel of the oracle (in practice
-equal); CT_PATHS_AUTO takes the per-path launch for rounds of at most 2^21
-paths over scenes of at most 16384 primitives, and the wavefront otherwise."""
+equal); CT_PATHS_AUTO takes the per-path launch for rounds over scenes of at
+most 16384 primitives, and the wavefront otherwise."""
 import numpy as np
 import pytest
 
@@ -89,8 +89,9 @@ def test_per_path_equals_wavefront_and_oracle(ct, O, name, w, h, spp, seed, extr
 
 
 def test_two_path_groups_and_several_rounds(ct, O):
-    """A round of 2^20 paths (two path groups on two streams) and the same job
-    in rounds of 2^18 paths: the per-path launches give the wavefront's image."""
+    """A round of 2^20 paths (one per-path launch; the wavefront traces it as
+    two path groups on two streams) and the same job in rounds of 2^18 paths:
+    the per-path launches give the wavefront's image."""
     sc, ps = _scene(ct, O, "c1")
     fr = O.camera_frame(sc["camera"], 1.0)
     opts = {"width": 256, "height": 256, "spp": 16, "seed": 1, "maxBounces": 8}
@@ -103,11 +104,11 @@ def test_two_path_groups_and_several_rounds(ct, O):
         assert s["extensionRays"] == sw["extensionRays"] and s["shadowRays"] == sw["shadowRays"]
 
 
-def test_auto_picks_per_path_for_small_rounds_and_scenes(ct, O):
+def test_auto_picks_per_path_for_small_scenes(ct, O):
     small = ct.synth.soup_scene(16300, n_lights=16, n_materials=8, seed=3)  # 16316 primitives
     large = ct.synth.soup_scene(16400, n_lights=16, n_materials=8, seed=3)  # 16416 primitives
     for sc, (w, h), want in ((small, (48, 32), 1), (large, (48, 32), 0),
-                             (small, (1024, 1024), 0)):  # 2^21 + 2^20 paths: wavefront
+                             (small, (1024, 1024), 1)):  # 3 x 2^20 paths, two path groups
         ps = ct.packScene(sc)
         fr = ct.cameraFrame(sc["camera"], w / h)
         opts = {"width": w, "height": h, "spp": 2 if w < 1024 else 3, "seed": 5, "maxBounces": 6}
@@ -210,3 +211,47 @@ def test_per_path_skip_link_walk(ct, O, name):
     assert rs == 1
     assert np.array_equal(gs.view(np.uint32), gw.view(np.uint32))
     assert ss["extensionRays"] == sw["extensionRays"] and ss["shadowRays"] == sw["shadowRays"]
+
+
+@pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
+def test_mixed_batch_packs_each_form_in_its_own_rounds(ct, O, graph):
+    """Small scenes and a 20k-triangle soup in one batch: under CT_PATHS_AUTO
+    the small scenes' jobs run a per-path round before the soup's wavefront
+    round (eager; a graph-mode tracer keeps one wavefront round), and forced
+    per-path everything runs per-path.  Every job's image equals the
+    wavefront batch's and the job rendered alone."""
+    soup = ct.synth.soup_scene(20000, n_lights=16, n_materials=20, seed=9)
+    specs = [(soup, 48, 32, 3, 2, 6), (scenes.cornell_box(), 40, 30, 3, 1, 8),
+             (scenes.night_emissives(), 32, 24, 2, 7, 5)]
+    packed = [O.pack_scene(sc) for sc, *_ in specs]
+    out = {}
+    for mode in (ct.PATHS_AUTO, ct.PATHS_PER_PATH, ct.PATHS_WAVEFRONT):
+        js = [{"scene": ps, "frame": O.camera_frame(sc["camera"], w / h), "width": w, "height": h,
+               "spp": spp, "seed": seed, "maxBounces": depth,
+               "out": ct.DeviceBuffer.empty((w * h * 3,), np.float32)}
+              for ps, (sc, w, h, spp, seed, depth) in zip(packed, specs)]
+        tr = ct.Tracer()
+        tr.set_path_kernel(mode)
+        tr.set_graph(graph)
+        for rep in range(3 if graph else 1):  # capture, then replays
+            tr.render_batch(js)
+            want = {ct.PATHS_PER_PATH: 1, ct.PATHS_WAVEFRONT: 0, ct.PATHS_AUTO: 0 if graph else 1}
+            assert tr.get_path_kernel()[1] == want[mode]
+            assert tr.stats()["rounds"] == (2 if mode == ct.PATHS_AUTO and not graph else 1)
+            imgs = [j["out"].numpy() for j in js]
+            if rep == 0:
+                out[mode] = imgs
+            else:
+                for a, b in zip(imgs, out[mode]):
+                    assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), rep
+        st = tr.stats()
+        out[(mode, "rays")] = (st["extensionRays"], st["shadowRays"])
+    for mode in (ct.PATHS_AUTO, ct.PATHS_PER_PATH):
+        for a, b in zip(out[mode], out[ct.PATHS_WAVEFRONT]):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        assert out[(mode, "rays")] == out[(ct.PATHS_WAVEFRONT, "rays")]
+    for ps, (sc, w, h, spp, seed, depth), img in zip(packed, specs, out[ct.PATHS_PER_PATH]):
+        opts = {"width": w, "height": h, "spp": spp, "seed": seed, "maxBounces": depth}
+        alone, _, _, _ = _render(ct, ps, O.camera_frame(sc["camera"], w / h), opts,
+                                 ct.PATHS_WAVEFRONT)
+        assert np.array_equal(alone.view(np.uint32), img.view(np.uint32))
