@@ -1343,12 +1343,17 @@ int ct::tracer_streams(ct_tracer* T) {
 // Smallest round traced as two path groups.
 constexpr size_t SPLIT_MIN_PATHS = size_t(1) << 20;
 
-// Largest scene (primitives) of a round CT_PATHS_AUTO traces with k_paths.
+// Rounds CT_PATHS_AUTO traces with k_paths: every scene of at most
+// PATHS_AUTO_MAX_PRIMS primitives, at most PATHS_AUTO_MAX_PATHS paths.
 // Measured against the wavefront (ms per frame, 256x256x16 / 1920x1080x4):
 // soups of 2k triangles 3.0x / 1.6x faster, 8k 1.9x / 1.6x, 16k 1.6x / 1.2x,
-// 64k 0.93x; the Cornell variant 2.6x at 256x256x16, 2.4x at 1024x1024x4;
-// the mirror room (deep specular paths, 2k triangles) 1.27x / 1.00x.
+// 64k 0.93x; the Cornell variant 2.9x at 256x256x16, 2.5x at 1024x1024x4;
+// the mirror room (deep specular paths, 2k triangles) 1.27x / 1.00x.  The
+// wavefront's per-bounce compaction wins back huge rounds: the 29 jobs on
+// scenes of 1k-16k triangles of the 64-scene batch (95M paths in one round)
+// take 46.7 ms in the wavefront, 49.8 ms per path.
 constexpr int PATHS_AUTO_MAX_PRIMS = 16384;
+constexpr size_t PATHS_AUTO_MAX_PATHS = size_t(1) << 24;
 
 // The streams of a round of `paths` paths.  Profiling events are recorded on
 // each launch's own stream, so they bracket that launch as it runs beside the
@@ -1571,7 +1576,12 @@ static bool job_uses_paths(const ct_tracer* T, const RoundJob& j) {
 static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t st,
                      BatchTotals& tot) {
   bool allPaths = !R.empty();
-  for (const RoundJob& j : R) allPaths = allPaths && job_uses_paths(T, j);
+  size_t roundPaths = 0;
+  for (const RoundJob& j : R) {
+    allPaths = allPaths && job_uses_paths(T, j);
+    roundPaths += (size_t)j.width * j.height * j.passes;
+  }
+  if (T->pathKernel == CT_PATHS_AUTO && roundPaths > PATHS_AUTO_MAX_PATHS) allPaths = false;
   size_t paths = 0, pix = 0;
   int maxB = 0;
   for (const RoundJob& j : R) {
@@ -1745,19 +1755,21 @@ int ct::run_plan(ct_tracer* T, std::vector<PlanJob> jobs, hipStream_t st) {
   std::vector<uint32_t> done(jobs.size(), 0);
   // Jobs whose paths run k_paths (job_uses_paths) are packed into rounds of
   // their own, traced before the others' wavefront rounds, so that neither
-  // kind waits on the other's launches.  A tracer in graph mode keeps one
-  // round shape (it replays the graph of the last one) and packs all jobs
-  // together.
+  // kind waits on the other's launches -- when they fit the per-path rounds'
+  // size limit.  A tracer in graph mode keeps one round shape (it replays the
+  // graph of the last one) and packs all jobs together.
   std::vector<int> kind(jobs.size(), 0);
   bool split = false;
   if (!T->graph) {
-    bool anyPaths = false, anyWave = false;
+    bool anyWave = false;
+    uint64_t smallPaths = 0;
     for (size_t k = 0; k < jobs.size(); k++) {
       kind[k] = scene_uses_paths(T, jobs[k].scene) ? 0 : 1;
-      anyPaths = anyPaths || kind[k] == 0;
+      if (kind[k] == 0) smallPaths += (uint64_t)jobs[k].width * jobs[k].height * jobs[k].count;
       anyWave = anyWave || kind[k] == 1;
     }
-    split = anyPaths && anyWave;
+    split = smallPaths > 0 && anyWave &&
+            (T->pathKernel != CT_PATHS_AUTO || smallPaths <= PATHS_AUTO_MAX_PATHS);
   }
   for (int pass = split ? 0 : 1; pass < 2;) {
     std::vector<RoundJob> R;

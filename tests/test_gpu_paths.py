@@ -4,8 +4,8 @@ bounces) against the wavefront launches and the oracle.
 Both forms run the same binary64 arithmetic in the reference's order, so the
 images, radiance values, RNG states and ray counts must be identical bit for
 bit under every option, and within 1e-4 per pixel of the oracle (in practice
-equal); CT_PATHS_AUTO takes the per-path launch for rounds over scenes of at
-most 16384 primitives, and the wavefront otherwise."""
+equal); CT_PATHS_AUTO takes the per-path launch for rounds of at most 2^24
+paths over scenes of at most 16384 primitives, and the wavefront otherwise."""
 import numpy as np
 import pytest
 
@@ -116,6 +116,14 @@ def test_auto_picks_per_path_for_small_scenes(ct, O):
         assert tr.get_path_kernel() == (ct.PATHS_AUTO, want)
         gw, _, _, _ = _render(ct, ps, fr, opts, ct.PATHS_WAVEFRONT)
         assert np.array_equal(ga.view(np.uint32), gw.view(np.uint32))
+    # rounds above 2^24 paths run the wavefront, whatever the scene
+    sc, ps = _scene(ct, O, "cornell")
+    opts = {"width": 1024, "height": 1024, "spp": 17, "seed": 3, "maxBounces": 4}
+    fr = O.camera_frame(sc["camera"], 1.0)
+    ga, _, ra, _ = _render(ct, ps, fr, opts, ct.PATHS_AUTO)
+    gp, _, rp, _ = _render(ct, ps, fr, opts, ct.PATHS_PER_PATH)
+    assert ra == 0 and rp == 1
+    assert np.array_equal(ga.view(np.uint32), gp.view(np.uint32))
     # the FP32 precision form always runs its wavefront
     tr = ct.Tracer()
     tr.set_path_kernel(ct.PATHS_PER_PATH)
