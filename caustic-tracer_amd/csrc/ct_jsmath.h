@@ -172,6 +172,30 @@ __device__ __forceinline__ double cos(double x) {
   }
 }
 
+// sin(x) and cos(x) from one argument reduction: per quadrant the same kernel
+// calls as sin and cos above, so the same two results bit for bit.
+__device__ __forceinline__ void sincos(double x, double& s, double& c) {
+  uint32_t ix = hi_word(x) & 0x7fffffffu;
+  if (ix <= 0x3fe921fbu) {
+    s = kernel_sin(x, 0.0, 0);
+    c = kernel_cos(x, 0.0);
+    return;
+  }
+  if (ix >= 0x7ff00000u) {
+    s = c = x - x;
+    return;
+  }
+  double y0, y1;
+  int n = rem_pio2(x, y0, y1);
+  const double ks = kernel_sin(y0, y1, 1), kc = kernel_cos(y0, y1);
+  switch (n & 3) {
+    case 0: s = ks; c = kc; break;
+    case 1: s = kc; c = -ks; break;
+    case 2: s = -ks; c = -kc; break;
+    default: s = -kc; c = ks; break;
+  }
+}
+
 // cube root: a 5-bit estimate from the exponent, a polynomial to 23 bits, a
 // rounding to 23 bits away from zero, one Newton step to 53 bits
 __device__ __forceinline__ double cbrt(double x) {

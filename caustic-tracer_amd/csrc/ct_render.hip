@@ -80,8 +80,10 @@ __device__ __forceinline__ PathState start_path(const JobDev& J, size_t local) {
   if (F.lensRadius > 0) {
     double r = sqrt(rng.next());
     double theta = TWO_PI * rng.next();
-    lx = r * jsm::cos(theta);
-    ly = r * jsm::sin(theta);
+    double st, ct;
+    jsm::sincos(theta, st, ct);
+    lx = r * ct;
+    ly = r * st;
   }
   double px = (2 * sx - 1) * F.tanHalfFov * F.aspect;
   double py = (2 * sy - 1) * F.tanHalfFov;
@@ -176,8 +178,10 @@ __device__ __forceinline__ void unit_vector(double u1, double u2, double* out) {
   double z = 1 - 2 * u1;
   double r = sqrt(z * z < 1 ? 1 - z * z : 0);
   double phi = TWO_PI * u2;
-  out[0] = r * jsm::cos(phi);
-  out[1] = r * jsm::sin(phi);
+  double sp, cp;
+  jsm::sincos(phi, sp, cp);
+  out[0] = r * cp;
+  out[1] = r * sp;
   out[2] = z;
 }
 
@@ -208,7 +212,9 @@ __device__ bool sample_light(const DevScene& S, double px, double py, double pz,
       double ax = cx * invD, ay = cy * invD, az = cz * invD;
       double b[6];
       basis(ax, ay, az, b);
-      double sx = sinT * jsm::cos(phi), sy = sinT * jsm::sin(phi);
+      double sp, cp;
+      jsm::sincos(phi, sp, cp);
+      double sx = sinT * cp, sy = sinT * sp;
       double wx = b[0] * sx + b[3] * sy + ax * cosT;
       double wy = b[1] * sx + b[4] * sy + ay * cosT;
       double wz = b[2] * sx + b[5] * sy + az * cosT;
@@ -397,8 +403,10 @@ __device__ __forceinline__ bool shade_bounce(const DevScene& S, const TraceOpts&
       double u2 = rng.next();
       double r = sqrt(u1);
       double phi = TWO_PI * u2;
-      double lx = r * jsm::cos(phi);
-      double ly = r * jsm::sin(phi);
+      double sp, cp;
+      jsm::sincos(phi, sp, cp);
+      double lx = r * cp;
+      double ly = r * sp;
       double lz = sqrt(u1 < 1 ? 1 - u1 : 0);
       double b[6];
       basis(sx, sy, sz, b);
