@@ -25,6 +25,8 @@ beside it the same jobs rendered one after another with renderImage.
 (4,259,842 triangles), 1024x1024, 64 spp, maxBounces 16, rendered by the
 precision form (quantized nodes, FP32, compensated sums) and by the binary64
 form; each image's RMS error against a binary64 render at --ref-spp.
+--form c1: the 256x256, 16 spp, maxBounces 8 Cornell variant alone (the
+small plain workload; rounds over such scenes run the per-path launch).
 --form serve: 128 resident scenes (1k-1M triangles) and a Poisson request
 stream (random camera, 128^2-1024^2, 1-64 spp, maxBounces 4-12) through the
 serving loop; p50/p99 time-to-complete and aggregate throughput, beside the
@@ -249,7 +251,8 @@ def cpu_baseline(ct, ps, frame, w, h, spp, depth, seed, budget_s=12.0):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--form", choices=["plain", "batch", "serve", "precision", "copy", "traffic"],
+    ap.add_argument("--form", choices=["plain", "c1", "batch", "serve", "precision", "copy",
+                                       "traffic"],
                     default="plain")
     ap.add_argument("--pmc-dir", default=None,
                     help="traffic: directory holding the rocprofv3 --pmc outputs in fetch/, "
@@ -328,6 +331,15 @@ def main():
         res = precision_form(ct, args)
     elif args.form == "serve":
         res = serve_form(ct, args)
+    elif args.form == "c1":
+        r = c1_form(ct, args)
+        res = {"metric": METRIC, "value": r.pop("value"), "unit": r.pop("unit"), "n_gpus": 1,
+               "steps": r.pop("steps"), "warmup": args.warmup,
+               "ms_per_step": r.pop("ms_per_step"), "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": r.pop("dtype"), "data": "synthetic",
+               "config": {"workload": r.pop("workload"), "width": 256, "height": 256, "spp": 16,
+                          "max_bounces": 8, "parallelism": "single GPU"},
+               "roofline": None, "cpu_baseline": r.pop("cpu_baseline", None), **r}
     else:
         res = plain_form(ct, args)
         if args.forms:
