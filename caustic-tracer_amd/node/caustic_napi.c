@@ -21,6 +21,7 @@
  *            russianRoulette, nee) -> Float64Array(3n); states advanced in
  *            place                                              (tracer.ts:191)
  *   setPrecision(0|1)            arithmetic of the two calls above
+ *   setPathKernel(0|1|2)         auto / wavefront / per-path launches
  *   createServer({hbmBudget, maxPaths, maxActive, slicePasses, policy,
  *                 precision}) -> server handle                 (serving loop)
  *   submitJob(server, scene, frame16, width, height, spp, seed, maxBounces,
@@ -450,6 +451,18 @@ static napi_value radiance(napi_env env, napi_callback_info info) {
   return res;
 }
 
+static napi_value set_path_kernel(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value a[1];
+  CALL(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  if (!ensure_init(env)) return NULL;
+  int rc = 0;
+  if (!g_tracer) rc = ct_tracer_create(&g_tracer);
+  if (!rc) rc = ct_tracer_set_path_kernel(g_tracer, (int32_t)num(env, a[0]));
+  if (rc) return throw_ct(env, rc, "ct_tracer_set_path_kernel");
+  return NULL;
+}
+
 static napi_value set_precision(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value a[1];
@@ -739,6 +752,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"renderImage", NULL, render_image, NULL, NULL, NULL, napi_default, NULL},
       {"radiance", NULL, radiance, NULL, NULL, NULL, napi_default, NULL},
       {"setPrecision", NULL, set_precision, NULL, NULL, NULL, napi_default, NULL},
+      {"setPathKernel", NULL, set_path_kernel, NULL, NULL, NULL, napi_default, NULL},
       {"createServer", NULL, create_server, NULL, NULL, NULL, napi_default, NULL},
       {"submitJob", NULL, submit_job, NULL, NULL, NULL, napi_default, NULL},
       {"pollJob", NULL, poll_job, NULL, NULL, NULL, napi_default, NULL},

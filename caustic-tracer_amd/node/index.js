@@ -208,6 +208,15 @@ function setPrecision(p) {
 }
 
 /**
+ * How renders are launched: 0 auto (one per-path launch for scenes of at most
+ * 16384 primitives, the wavefront above), 1 the wavefront, 2 per path.  The
+ * images are the same under each.
+ */
+function setPathKernel(mode) {
+  addon.setPathKernel(mode)
+}
+
+/**
  * Serving loop over resident scenes: jobs are renderImage calls that run with
  * continuous batching on the library's own thread; wait() returns a Promise
  * (N-API async work) so the event loop keeps accepting requests meanwhile.
@@ -232,5 +241,6 @@ class Server {
 module.exports = {
   WALK_STACKLESS, WALK_ORDERED, T_MIN, buildBvh, packScene, intersectClosest, intersectClosestStackless,
   occluded, closestHitBatch, occludedBatch, accumulatePass, renderImage, frameArray,
-  setPrecision, Server, tonemapBuffer, pick, pcgHash, seedFor, Rng, radianceInto, radianceBatch,
+  setPrecision, setPathKernel, Server, tonemapBuffer, pick, pcgHash, seedFor, Rng, radianceInto,
+  radianceBatch,
 }

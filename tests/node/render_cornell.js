@@ -18,6 +18,11 @@ const ps = {
 const frame = rd('frame.bin', Float64Array)
 const img = ct.renderImage(ps, frame, { width: w, height: h, spp, seed, maxBounces: 8, russianRoulette: true, nee: true })
 fs.writeFileSync(path.join(dir, 'out.bin'), Buffer.from(img.buffer))
+// the same image through the wavefront launches (the default runs this small scene per path)
+ct.setPathKernel(1)
+const imgW = ct.renderImage(ps, frame, { width: w, height: h, spp, seed, maxBounces: 8, russianRoulette: true, nee: true })
+ct.setPathKernel(0)
+const sameWave = Buffer.compare(Buffer.from(img.buffer), Buffer.from(imgW.buffer)) === 0
 // single-ray API through the same addon
 const hit = { t: Infinity, prim: -1, u: 0, v: 0 }
 const r = ct.intersectClosest(ps, 0, 0.5, 1.6, 0, 0, -1, 1e-4, Infinity, hit)
@@ -25,4 +30,4 @@ const blocked = ct.occluded(ps, 0, 0.5, 1.6, 0, 0, -1, 1e-4, 100)
 const rgba = ct.tonemapBuffer(img, w, h, 1, 0, 'aces')
 fs.writeFileSync(path.join(dir, 'rgba.bin'), Buffer.from(rgba.buffer))
 const picked = ct.pick(ps, frame, 0.74, 0.15)
-fs.writeFileSync(path.join(dir, 'hit.json'), JSON.stringify({ r, hit, blocked, picked }))
+fs.writeFileSync(path.join(dir, 'hit.json'), JSON.stringify({ r, hit, blocked, picked, sameWave }))

@@ -27,7 +27,7 @@ def test_addon_loads_and_exports_reference_names():
     names = set(out.stdout.strip().split(","))
     for n in ("buildBvh", "packScene", "intersectClosest", "intersectClosestStackless", "occluded",
               "accumulatePass", "renderImage", "setPrecision", "Server", "tonemapBuffer", "pick",
-              "radianceInto", "Rng", "seedFor", "pcgHash"):
+              "radianceInto", "Rng", "seedFor", "pcgHash", "setPathKernel"):
         assert n in names
 
 
@@ -53,6 +53,7 @@ def test_node_render_matches_oracle(O, tmp_path):
     err = np.where(den > 0, np.abs(g - o) / np.where(den > 0, den, 1), 0)
     assert err.max() <= 1e-4
     res = json.loads((tmp_path / "hit.json").read_text())
+    assert res["sameWave"]  # setPathKernel: the wavefront gives the per-path image
     ref = O.closest_batch(ps, np.array([[0, 0.5, 1.6, 0, 0, -1.0]]))
     assert res["r"] and res["hit"]["prim"] == int(ref["prim"][0]) and res["hit"]["t"] == ref["t"][0]
     occ, _ = O.occluded_batch(ps, np.array([[0, 0.5, 1.6, 0, 0, -1.0]]), 100.0)
