@@ -63,3 +63,40 @@ def test_fixed_cost_fit():
     assert f["fixed_ms"] == pytest.approx(0.4, abs=1e-6)
     assert f["ns_per_ray"] == pytest.approx(0.6, abs=1e-6)
     assert f["small"]["requests"] == 3  # under 2^20 samples: 128^2x1, 256^2x4, 128^2x2
+
+
+def test_other_forms_summaries_and_failures(monkeypatch):
+    """The default line's "forms" object: each form's summary keeps its own
+    figures, the shorter settings reach the forms, and a form that fails is
+    recorded with its error while the others still report."""
+    import argparse
+
+    seen = {}
+
+    def fake_batch(ct, a):
+        seen["batch"] = (a.steps, a.warmup, a.no_cpu)
+        return {"config": {"workload": "batch w"}, "value": 1.0, "unit": "Mrays/s",
+                "dtype": "f64", "ms_per_step": 2.0, "spp_mpix_per_s": 3.0, "rounds": 1,
+                "plain_sequential": {"ms": 4.0}, "speedup_vs_plain": 1.5,
+                "identical_to_plain": True, "extra": "dropped",
+                "autotune": {"trials": 2, "ms": 2.0, "config": {}, "initialMs": 3.0},
+                "roofline": {"kernel": "k", "achieved": 1.0, "frac": 0.1, "peak": 8000.0}}
+
+    def fake_serve(ct, a):
+        seen["serve"] = (a.duration, a.rates)
+        raise RuntimeError("no GPU here")
+
+    monkeypatch.setattr(bench, "batch_form", fake_batch)
+    monkeypatch.setattr(bench, "serve_form", fake_serve)
+    args = argparse.Namespace(forms=["batch", "serve", "bogus"], steps=5, warmup=1, no_cpu=False,
+                              forms_serve_duration=5.0, duration=2.0, rates=[20.0])
+    out = bench.other_forms(None, args)
+    assert seen["batch"] == (3, 1, True) and seen["serve"] == (5.0, [])
+    b = out["batch"]
+    assert b["workload"] == "batch w" and b["speedup_vs_plain"] == 1.5 and "extra" not in b
+    assert b["autotune"] == {"trials": 2, "ms": 2.0, "config": {}}
+    assert b["roofline"] == {"kernel": "k", "achieved": 1.0, "frac": 0.1}
+    assert out["serve"]["error"] == "RuntimeError: no GPU here"
+    assert out["bogus"]["error"].startswith("ValueError")
+    assert all("run_s" in v for v in out.values())
+    assert args.steps == 5 and args.duration == 2.0  # the caller's settings are untouched
