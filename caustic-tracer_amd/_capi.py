@@ -210,8 +210,16 @@ class DeviceBuffer:
         if a.nbytes:
             check(lib().ct_copy_h2d(self.ptr, a.ctypes.data_as(vp), a.nbytes), "ct_copy_h2d")
 
-    def numpy(self):
-        out = np.empty(self.shape, self.dtype)
+    def numpy(self, count=None):
+        """The buffer on the host; with `count`, only its first `count` elements
+        (a 1-D array)."""
+        if count is None:
+            out = np.empty(self.shape, self.dtype)
+        else:
+            total = int(np.prod(self.shape))
+            if not 0 <= count <= total:
+                raise ValueError(f"numpy(count={count}): the buffer holds {total} elements")
+            out = np.empty((int(count),), self.dtype)
         if out.nbytes:
             check(lib().ct_copy_d2h(out.ctypes.data_as(vp), self.ptr, out.nbytes), "ct_copy_d2h")
         return out

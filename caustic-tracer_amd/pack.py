@@ -201,10 +201,11 @@ def buildBvh(bmin, bmax, bcen, bins=12, leaf_words=False):
     else:
         check(lib().ct_bvh_build(dev[0].ptr, dev[1].ptr, dev[2].ptr, n, bins, d_nodes.ptr,
                                  C.byref(nc), d_order.ptr, None), "ct_bvh_build")
-    nodes = d_nodes.numpy()[: nc.value * NODE_STRIDE].copy()
-    order = d_order.numpy()[:n].copy()
+    # only the nodes written come back (the buffer holds room for 2n - 1)
+    nodes = d_nodes.numpy(nc.value * NODE_STRIDE)
+    order = d_order.numpy(n)
     if leaf_words:
-        return nodes, nc.value, order, d_words.numpy()[: nc.value].copy()
+        return nodes, nc.value, order, d_words.numpy(nc.value)
     return nodes, nc.value, order
 
 
