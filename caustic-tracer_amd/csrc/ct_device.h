@@ -144,9 +144,27 @@ __device__ __forceinline__ int32_t gload(const int32_t* p) {
   return *(const CT_GLOBAL int32_t*)p;
 }
 
+// Loads through the constant address space: the scene arrays are not written
+// while a kernel runs, and a load whose address is the same on every lane of
+// a wave then goes through the scalar cache (other addresses: vector loads).
+#define CT_CONST __attribute__((address_space(4)))
+__device__ __forceinline__ float4 cload(const float4* p) {
+  ct_nf4 v = *(const CT_CONST ct_nf4*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// UNIFORM: callers whose primitive index is the same on every lane of a wave
+// (the scans over all primitives) read the record through the scalar cache.
+template <bool UNIFORM = false>
 __device__ __forceinline__ PrimGeom load_prim(const DevScene& S, int p) {
   const float4* r = S.prims + (size_t)p * 4;
   PrimGeom g;
+  if (UNIFORM) {
+    g.a = cload(r);
+    g.b = cload(r + 1);
+    g.c = cload(r + 2);
+    return g;
+  }
   g.a = gload(r);
   g.b = gload(r + 1);
   g.c = gload(r + 2);
@@ -373,7 +391,7 @@ __device__ __forceinline__ double tie_bound(double t) {
 // finds nothing more), and settle_ties walks that ray again with
 // TIES_ORDERED, where such a tie goes to the primitive intersectClosest
 // reaches first (ordered_first) whatever the order of the tests.
-template <bool COUNT, int TIES = TIES_DETECT>
+template <bool COUNT, int TIES = TIES_DETECT, bool UNIFORM = false>
 __device__ __forceinline__ void test_leaf(const DevScene& S, int leafWord, double ox, double oy,
                                           double oz, double dx, double dy, double dz,
                                           double tMin, Hit& h, TraversalCounts& cnt) {
@@ -381,7 +399,7 @@ __device__ __forceinline__ void test_leaf(const DevScene& S, int leafWord, doubl
   int count = li & 7;
   int start = li >> 3;
   for (int p = start; p < start + count; p++) {
-    PrimGeom g = load_prim(S, p);
+    PrimGeom g = load_prim<UNIFORM>(S, p);
     // TIES_ORDERED sees hits up to and including h.t; the others keep the
     // strict bound, TIES_DETECT noting a primitive rejected only by a tie
     const double tMax = TIES == TIES_ORDERED ? tie_bound(h.t) : h.t;

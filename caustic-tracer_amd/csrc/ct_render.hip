@@ -731,12 +731,13 @@ __global__ void __launch_bounds__(BLOCK)
   }
 }
 
+template <bool UNIFORM = false>
 __device__ __forceinline__ bool leaf_blocks(const DevScene& S, int leafWord, double ox, double oy,
                                             double oz, double dx, double dy, double dz,
                                             double tMax, TraversalCounts& cnt, bool count) {
   int c = leafWord & 7, start = leafWord >> 3;
   for (int p = start; p < start + c; p++) {
-    PrimGeom g = load_prim(S, p);
+    PrimGeom g = load_prim<UNIFORM>(S, p);
     if (g.a.w == 0.0f) {
       if (count) cnt.spheres++;
       if (ray_sphere(ox, oy, oz, dx, dy, dz, g.a.x, g.a.y, g.a.z, g.b.x, T_MIN, tMax) > 0)
@@ -916,6 +917,45 @@ __device__ __forceinline__ bool occluded_wide_lane(const DevScene& S, double ox,
   return false;
 }
 
+// Scenes of at most PATHS_SCAN_MAX primitives: the per-path launch tests
+// every primitive in packed order (runs of up to seven through the leaf
+// tests) instead of walking.  The walks' contract -- every box on the way to
+// a primitive whose binary64 test could return a nearer hit is entered -- makes
+// the closest t the minimum over all primitives, which the scan computes with
+// the same tests; exact-t ties are marked the same way (TIES_DETECT) and
+// settled by settle_ties, and whether (tMin, tMax) is blocked does not depend
+// on the order.  Lanes on one scene then run the same tests on the same
+// records: no divergence between walks of different lengths, and a wave's
+// record loads are scalar loads of one address.  Cornell box (14 primitives)
+// 256x256x16: 0.99 -> 0.93 ms; the box with 8 to 50 more spheres (22-64
+// primitives) 1.14-1.29x; 128 primitives 0.88x.
+#ifndef CT_PATHS_SCAN_MAX
+#define CT_PATHS_SCAN_MAX 64
+#endif
+constexpr int PATHS_SCAN_MAX = CT_PATHS_SCAN_MAX;
+
+template <bool COUNT>
+__device__ __forceinline__ void closest_scan_lane(const DevScene& S, double ox, double oy,
+                                                  double oz, double dx, double dy, double dz,
+                                                  Hit& h, TraversalCounts& cnt) {
+  const int n = S.primCount;
+  for (int p = 0; p < n; p += 7)
+    test_leaf<COUNT, TIES_DETECT, true>(S, (p << 3) | min(7, n - p), ox, oy, oz, dx, dy, dz,
+                                        T_MIN, h, cnt);
+}
+
+__device__ __forceinline__ bool occluded_scan_lane(const DevScene& S, double ox, double oy,
+                                                   double oz, double dx, double dy, double dz,
+                                                   double tMax, TraversalCounts& cnt,
+                                                   bool count) {
+  const int n = S.primCount;
+  for (int p = 0; p < n; p += 7)
+    if (leaf_blocks<true>(S, (p << 3) | min(7, n - p), ox, oy, oz, dx, dy, dz, tMax, cnt,
+                          count))
+      return true;
+  return false;
+}
+
 // LDS stack entries of the per-path walks.
 constexpr int PATHS_STACK = 24;
 // Waves per SIMD of k_paths: four (127 VGPRs, 4 spilled) against three
@@ -924,7 +964,6 @@ constexpr int PATHS_STACK = 24;
 #ifndef CT_PATHS_WAVES
 #define CT_PATHS_WAVES 4
 #endif
-
 
 // Whole paths, one lane at a time: a lane takes the next path of the group
 // (one atomic per wave), starts it (start_path), then per bounce runs the
@@ -976,7 +1015,10 @@ k_paths(PathSoA P, const JobDev* __restrict__ jobs, int njobs, size_t base, size
       atomicAdd(sRays + b, 1u);
       Hit h;
       h.t = __builtin_inf(); h.u = 0; h.v = 0; h.prim = -1; h.tie = false;
-      if (WIDE)
+      const bool scan = WIDE && S.primCount <= PATHS_SCAN_MAX;
+      if (scan)
+        closest_scan_lane<COUNT>(S, ps.ox, ps.oy, ps.oz, ps.dx, ps.dy, ps.dz, h, cnt);
+      else if (WIDE)
         closest_wide_lane<SD, COUNT>(S, ps.ox, ps.oy, ps.oz, ps.dx, ps.dy, ps.dz, h, sNode,
                                      sEntry, RBLOCK, cnt);
       else
@@ -989,10 +1031,12 @@ k_paths(PathSoA P, const JobDev* __restrict__ jobs, int njobs, size_t base, size
       if (neeOn) {
         atomicAdd(sRays + MAX_BOUNCE_SLOTS + b, 1u);
         const bool blocked =
-            WIDE ? occluded_wide_lane<SD, COUNT>(S, ps.ox, ps.oy, ps.oz, nee.w[0], nee.w[1],
-                                                 nee.w[2], nee.tMax, sNode, RBLOCK, scnt)
-                 : occluded<COUNT>(S, ps.ox, ps.oy, ps.oz, nee.w[0], nee.w[1], nee.w[2], T_MIN,
-                                   nee.tMax, scnt);
+            scan   ? occluded_scan_lane(S, ps.ox, ps.oy, ps.oz, nee.w[0], nee.w[1], nee.w[2],
+                                        nee.tMax, scnt, COUNT)
+            : WIDE ? occluded_wide_lane<SD, COUNT>(S, ps.ox, ps.oy, ps.oz, nee.w[0], nee.w[1],
+                                                   nee.w[2], nee.tMax, sNode, RBLOCK, scnt)
+                   : occluded<COUNT>(S, ps.ox, ps.oy, ps.oz, nee.w[0], nee.w[1], nee.w[2],
+                                     T_MIN, nee.tMax, scnt);
         if (!blocked) {
           ps.Lr += nee.c[0];
           ps.Lg += nee.c[1];
@@ -1022,8 +1066,9 @@ k_paths(PathSoA P, const JobDev* __restrict__ jobs, int njobs, size_t base, size
   }
   __syncthreads();
   for (int k = threadIdx.x; k < nb; k += RBLOCK) {
-    if (sRays[k]) atomicAdd(P.ctr + k, sRays[k]);
-    if (sRays[MAX_BOUNCE_SLOTS + k]) atomicAdd(P.ctr + MAX_BOUNCE_SLOTS + k, sRays[MAX_BOUNCE_SLOTS + k]);
+    const uint32_t ext = sRays[k], nee = sRays[MAX_BOUNCE_SLOTS + k];
+    if (ext) atomicAdd(P.ctr + k, ext);
+    if (nee) atomicAdd(P.ctr + MAX_BOUNCE_SLOTS + k, nee);
   }
 }
 

@@ -219,3 +219,20 @@ def ray_set_bvh_test(rng, n, lo=-14, hi=14, tlo=-8, thi=8):
     dirv = tgt - o
     ln = np.array([js_hypot(*row) for row in dirv])
     return np.concatenate([o, dirv / ln[:, None]], axis=1)
+
+
+def tie_rays(ps, seed, per):
+    """Rays aimed at the vertices, edge midpoints and an edge quarter point of
+    every triangle: where meshes and quads share edges and vertices, several
+    primitives are hit at bit-identical t."""
+    rs = np.random.default_rng(seed)
+    prims = ps.prims.reshape(-1, 16)
+    rays = []
+    for p in np.flatnonzero(prims[:, 3] == 1):
+        a, b, c = (prims[p, k:k + 3].astype(np.float64) for k in (0, 4, 8))
+        for tgt in (a, b, c, 0.5 * (a + b), 0.5 * (b + c), 0.5 * (a + c), a + 0.25 * (b - a)):
+            for _ in range(per):
+                o = tgt + rs.uniform(-6, 6, 3)
+                d = tgt - o
+                rays.append(np.concatenate([o, d / np.linalg.norm(d)]))
+    return np.array(rays)

@@ -148,23 +148,6 @@ def test_wide_walk_adversarial_rays(ct, O, off):
     assert (o["prim"] >= 0).mean() > 0.7
 
 
-def _tie_rays(ps, seed, per):
-    """Rays aimed at the vertices, edge midpoints and an edge quarter point of
-    every triangle: where meshes and quads share edges and vertices, several
-    primitives are hit at bit-identical t."""
-    rs = np.random.default_rng(seed)
-    prims = ps.prims.reshape(-1, 16)
-    rays = []
-    for p in np.flatnonzero(prims[:, 3] == 1):
-        a, b, c = (prims[p, k:k + 3].astype(np.float64) for k in (0, 4, 8))
-        for tgt in (a, b, c, 0.5 * (a + b), 0.5 * (b + c), 0.5 * (a + c), a + 0.25 * (b - a)):
-            for _ in range(per):
-                o = tgt + rs.uniform(-6, 6, 3)
-                d = tgt - o
-                rays.append(np.concatenate([o, d / np.linalg.norm(d)]))
-    return np.array(rays)
-
-
 @pytest.mark.parametrize("preset", ["cornell", "mirror"])
 def test_exact_ties_follow_the_reference_walks(ct, O, preset):
     """On exact-t ties the ordered walk (intersect.ts:202-282) keeps the
@@ -174,7 +157,7 @@ def test_exact_ties_follow_the_reference_walks(ct, O, preset):
     stackless walk the latter."""
     sc = ct.presetById(preset)["scene"]
     ps = O.pack_scene(sc)
-    rays = _tie_rays(ps, 3, 12 if preset == "cornell" else 2)
+    rays = scenes.tie_rays(ps, 3, 12 if preset == "cornell" else 2)
     ordered = O.closest_batch(ps, rays, walk=O.WALK_ORDERED)
     stackless = O.closest_batch(ps, rays, walk=O.WALK_STACKLESS)
     ties = int((ordered["prim"] != stackless["prim"]).sum())

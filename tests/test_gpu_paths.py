@@ -176,8 +176,19 @@ def test_graph_replays_and_counting(ct, O):
         assert np.array_equal(d.numpy().view(np.uint32), ref.view(np.uint32)), k
         ct.renderImage(ps, fr, dict(opts, seed=k + 10))
     _, sc_, rc, _ = _render(ct, ps, fr, opts, ct.PATHS_PER_PATH, counting=True)
-    assert rc == 1 and sc_["nodeVisits"] > 0 and sc_["shadowNodeVisits"] > 0
-    assert sc_["extensionRays"] == sref["extensionRays"]
+    assert rc == 1 and sc_["extensionRays"] == sref["extensionRays"]
+    # the box (14 primitives) is scanned: every extension ray tests every
+    # primitive, NEE rays stop at their first blocker, no node is visited
+    # unless a tie sends a ray through the ordered walk
+    tests = sc_["triTests"] + sc_["sphereTests"]
+    assert tests >= sc_["extensionRays"] * ps.primCount
+    assert 0 < sc_["shadowTriTests"] + sc_["shadowSphereTests"] <= sc_["shadowRays"] * ps.primCount
+    assert sc_["shadowNodeVisits"] == 0
+    # above the scan's limit (the mirror room, 2016 primitives) the rays walk
+    sm, pm = _scene(ct, O, "mirror")
+    fm = O.camera_frame(sm["camera"], 1.5)
+    _, sm_, rm, _ = _render(ct, pm, fm, opts, ct.PATHS_PER_PATH, counting=True)
+    assert rm == 1 and sm_["nodeVisits"] > 0 and sm_["shadowNodeVisits"] > 0
 
 
 def test_radiance_per_path_equals_wavefront(ct, O):
@@ -205,6 +216,46 @@ def test_radiance_per_path_equals_wavefront(ct, O):
         oL, ost = O.radiance(ps, rays[i, :3], rays[i, 3:], int(states[i]), max_bounces=8,
                              with_state=True)
         assert np.array_equal(La.reshape(-1, 3)[i], oL) and Sa[i] == ost
+
+
+def test_scan_settles_ties_like_the_ordered_walk(ct, O):
+    """Scenes of at most 64 primitives are scanned in packed order by the
+    per-path launch, which on an exact-t tie meets the lower index first.
+    Rays aimed at the Cornell box's shared edges and vertices, where triangles
+    of different walls are hit at the same t, must shade the primitive
+    intersectClosest keeps (intersect.ts:202-282): radiance and RNG states
+    equal to the wavefront's (4-wide walk) and to the oracle's."""
+    sc = ct.presetById("cornell")["scene"]
+    ps = O.pack_scene(sc)
+    assert ps.primCount <= 64
+    rays = scenes.tie_rays(ps, 5, 12)
+    ordered = O.closest_batch(ps, rays, walk=O.WALK_ORDERED)
+    lowest = O.closest_batch(ps, rays, walk=O.WALK_STACKLESS)  # lowest index on ties
+    tied = np.flatnonzero(ordered["prim"] != lowest["prim"])
+    obj = ps.primObject
+    cross = [i for i in tied if obj[ordered["prim"][i]] != obj[lowest["prim"][i]]]
+    assert len(cross) >= 10  # ties between walls: the winner changes the shading
+    n = len(rays)
+    states = np.random.default_rng(6).integers(1, 1 << 32, n).astype(np.uint32)
+    out = {}
+    for mode in (ct.PATHS_PER_PATH, ct.PATHS_WAVEFRONT):
+        tr = ct.Tracer()
+        tr.set_path_kernel(mode)
+        tr.set_counting(True)
+        d_rays = ct.DeviceBuffer.from_array(np.ascontiguousarray(rays))
+        d_rng = ct.DeviceBuffer.from_array(states.copy())
+        d_out = ct.DeviceBuffer.empty((n * 3,), np.float64)
+        tr.radiance(ps, d_rays, n, d_rng, {"maxBounces": 8}, d_out)
+        out[mode] = (d_out.numpy().reshape(-1, 3), d_rng.numpy(), tr.stats(),
+                     tr.get_path_kernel()[1])
+    (La, Sa, sa, ra), (Lb, Sb, _, rb) = out[ct.PATHS_PER_PATH], out[ct.PATHS_WAVEFRONT]
+    assert ra == 1 and rb == 0
+    assert sa["nodeVisits"] > 0  # tied rays were walked again in the reference's order
+    assert np.array_equal(La.view(np.uint64), Lb.view(np.uint64)) and np.array_equal(Sa, Sb)
+    for i in cross + list(range(0, n, 97)):
+        oL, ost = O.radiance(ps, rays[i, :3], rays[i, 3:], int(states[i]), max_bounces=8,
+                             with_state=True)
+        assert np.array_equal(La[i], oL) and Sa[i] == ost, i
 
 
 @pytest.mark.parametrize("name", ["cornell", "mirror", "glass"])
