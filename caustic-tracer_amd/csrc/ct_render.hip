@@ -964,10 +964,24 @@ constexpr int PATHS_STACK = 24;
 #ifndef CT_PATHS_WAVES
 #define CT_PATHS_WAVES 4
 #endif
+// The scan-only build (below) as well: shading holds the registers, so it
+// needs 128 VGPRs like the others; at five waves (96 VGPRs, 32 spilled) the
+// Cornell frame takes 3% longer, at six (80, 118 spilled) 8%.
+#ifndef CT_PATHS_SCAN_WAVES
+#define CT_PATHS_SCAN_WAVES 4
+#endif
+
+// Closest-hit and any-hit form of a k_paths build: the skip-link walk (the
+// tracer's walk set to CT_WALK_STACKLESS), the 4-wide walk with the scan for
+// lanes on scenes of at most PATHS_SCAN_MAX primitives, or the scan alone
+// for rounds whose scenes all have at most PATHS_SCAN_MAX primitives (no
+// walk code: 1% faster on the Cornell frame and the boxes of 22-64
+// primitives).
+enum PathsWalk : int { PATHS_SKIP = 0, PATHS_WIDE = 1, PATHS_SCAN = 2 };
 
 // Whole paths, one lane at a time: a lane takes the next path of the group
 // (one atomic per wave), starts it (start_path), then per bounce runs the
-// closest hit (the 4-wide walk, or with WIDE false the skip-link walk;
+// closest hit (the scan, the 4-wide walk or the skip-link walk, as WALK says;
 // exact-t ties settled in intersectClosest's order), one bounce of shading
 // (shade_bounce) and the NEE ray's any-hit test, in the reference's order --
 // the same arithmetic as the wavefront launches, so the same radiance bit for
@@ -978,10 +992,12 @@ constexpr int PATHS_STACK = 24;
 // (the workgroups resident at once).  For rounds over small scenes, where the
 // wavefront's per-bounce launches (three per bounce and path group) cost more
 // than the work they carry.
-template <bool MULTI, bool COUNT, bool WIDE>
-__global__ void __launch_bounds__(RBLOCK) __attribute__((amdgpu_waves_per_eu(CT_PATHS_WAVES)))
+template <bool MULTI, bool COUNT, int WALK>
+__global__ void __launch_bounds__(RBLOCK) __attribute__((
+    amdgpu_waves_per_eu(WALK == PATHS_SCAN ? CT_PATHS_SCAN_WAVES : CT_PATHS_WAVES)))
 k_paths(PathSoA P, const JobDev* __restrict__ jobs, int njobs, size_t base, size_t count,
         int nb) {
+  constexpr bool WIDE = WALK == PATHS_WIDE;
   constexpr int SD = WIDE ? PATHS_STACK : 1;
   __shared__ int sNodeAll[SD * RBLOCK];
   __shared__ qw::Entry sEntryAll[SD * RBLOCK];
@@ -1015,7 +1031,7 @@ k_paths(PathSoA P, const JobDev* __restrict__ jobs, int njobs, size_t base, size
       atomicAdd(sRays + b, 1u);
       Hit h;
       h.t = __builtin_inf(); h.u = 0; h.v = 0; h.prim = -1; h.tie = false;
-      const bool scan = WIDE && S.primCount <= PATHS_SCAN_MAX;
+      const bool scan = WALK == PATHS_SCAN || (WIDE && S.primCount <= PATHS_SCAN_MAX);
       if (scan)
         closest_scan_lane<COUNT>(S, ps.ox, ps.oy, ps.oz, ps.dx, ps.dy, ps.dz, h, cnt);
       else if (WIDE)
@@ -1152,6 +1168,7 @@ struct ct_tracer {
   bool overlap = true;               // NEE launches on side streams (SideStream)
   int groups = 2;                    // path groups per round, each on its own stream
   int pathKernel = CT_PATHS_AUTO;    // wavefront launches or one per-path launch (k_paths)
+  bool roundScan = false;  // every scene of the current round has at most PATHS_SCAN_MAX prims
   bool groupPaths[2] = {false, false};  // path group g of the current round runs k_paths
   SideStream side, sideB, groupB;    // streams and events, created on first use
   hipGraphExec_t graphExec[2] = {nullptr, nullptr};  // one per path group
@@ -1492,17 +1509,21 @@ static int launch_group_body(ct_tracer* T, const PathSoA& P, int njobs, size_t b
   const JobDev* jobs = T->jobsDev;
   if (perPath) {  // the whole group in one persistent launch
     mark(T, st, 0, true);
-    const bool wide = T->walk != CT_WALK_STACKLESS;
+    const int walk = T->walk == CT_WALK_STACKLESS ? PATHS_SKIP
+                     : T->roundScan               ? PATHS_SCAN
+                                                  : PATHS_WIDE;
 #define CT_PATHS(C, W)                                                                          \
   hipLaunchKernelGGL((k_paths<MULTI, C, W>), dim3(paths_grid((const void*)k_paths<MULTI, C, W>, \
                                                              count)),                           \
                      dim3(RBLOCK), 0, st, P, jobs, njobs, base, count, nb)
     if (T->counting) {
-      if (wide) CT_PATHS(true, true);
-      else CT_PATHS(true, false);
+      if (walk == PATHS_SCAN) CT_PATHS(true, PATHS_SCAN);
+      else if (walk == PATHS_WIDE) CT_PATHS(true, PATHS_WIDE);
+      else CT_PATHS(true, PATHS_SKIP);
     } else {
-      if (wide) CT_PATHS(false, true);
-      else CT_PATHS(false, false);
+      if (walk == PATHS_SCAN) CT_PATHS(false, PATHS_SCAN);
+      else if (walk == PATHS_WIDE) CT_PATHS(false, PATHS_WIDE);
+      else CT_PATHS(false, PATHS_SKIP);
     }
 #undef CT_PATHS
     CT_LAUNCH_CHECK("k_paths");
@@ -1683,6 +1704,9 @@ static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t s
   rc = round_streams(T, paths, allPaths, rs);
   if (rc) return rc;
   T->groupPaths[0] = T->groupPaths[1] = allPaths;
+  T->roundScan = allPaths;
+  for (const RoundJob& j : R)
+    T->roundScan = T->roundScan && j.scene->dev.primCount <= PATHS_SCAN_MAX;
   if (T->graph) {
     // each path group's launches are one captured graph, replayed on the
     // group's stream; they depend only on the round's shape and the tracer's
@@ -1692,7 +1716,8 @@ static int run_round(ct_tracer* T, const std::vector<RoundJob>& R, hipStream_t s
     int ki[11] = {nb, (int)R.size(), T->walk, T->shadowWalk, T->stack, (int)traversal_blocks(T),
                   T->counting, multi, T->block,
                   T->precision * 64 + (T->overlap ? 32 : 0) + rs.groups,
-                  (T->groupPaths[0] ? 1 : 0) + (T->groupPaths[1] ? 2 : 0)};
+                  (T->groupPaths[0] ? 1 : 0) + (T->groupPaths[1] ? 2 : 0) +
+                      (T->roundScan ? 4 : 0)};
     memcpy(key.data(), kk, sizeof kk);
     memcpy(key.data() + sizeof kk, ki, sizeof ki);
     if (!T->graphExec[0] || key != T->graphKey) {

@@ -138,7 +138,10 @@ def test_auto_picks_per_path_for_small_scenes(ct, O):
 
 def test_batch_of_small_scenes(ct, O):
     """Several scenes, cameras, sizes and depths in shared per-path launches:
-    each job's image equals the wavefront batch's and the job rendered alone."""
+    each job's image equals the wavefront batch's and the job rendered alone.
+    In the batch the glass scene's lanes (65 primitives) walk while the box's
+    and the night scene's lanes scan; alone, those two run the scan-only
+    build."""
     specs = [(scenes.cornell_box(), 40, 30, 3, 1, 8), (scenes.night_emissives(), 32, 24, 2, 7, 5),
              (scenes.glass_spheres(O.Rng(20240607)), 36, 24, 2, 3, 6)]
     jobs = {}
