@@ -128,28 +128,85 @@ static bool leaf_words(const float* nodes, uint32_t nodeCount, const uint32_t* g
   return exact;
 }
 
+// Box of primitive p as the reference computes the box of a leaf that holds
+// p alone: the padded primitive bounds of pack.ts:152-154 (spheres) and
+// pack.ts:177-187 (triangles, from the float32 vertices the tests use), then
+// the outward nudge of bvh.ts:139-145, rounded outward to float32.
+static void prim_box(const float* prims, uint32_t p, float lo[3], float hi[3]) {
+  const float* r = prims + (size_t)p * PRIM_STRIDE;
+  double mn[3], mx[3];
+  if (r[3] == 0.0f) {
+    const double rad = r[4], rp = rad + 1e-6 + std::fabs(rad) * 1e-6;
+    for (int a = 0; a < 3; a++) {
+      mn[a] = (double)r[a] - rp;
+      mx[a] = (double)r[a] + rp;
+    }
+  } else {
+    for (int a = 0; a < 3; a++) {
+      mn[a] = std::min({(double)r[a], (double)r[4 + a], (double)r[8 + a]}) - 1e-6;
+      mx[a] = std::max({(double)r[a], (double)r[4 + a], (double)r[8 + a]}) + 1e-6;
+    }
+  }
+  for (int a = 0; a < 3; a++) {
+    const double l = mn[a] - std::fabs(mn[a]) * 1.2e-7 - 1e-9;
+    const double h = mx[a] + std::fabs(mx[a]) * 1.2e-7 + 1e-9;
+    float fl = (float)l, fh = (float)h;
+    if ((double)fl > l) fl = std::nextafter(fl, -INFINITY);
+    if ((double)fh < h) fh = std::nextafter(fh, INFINITY);
+    lo[a] = fl;
+    hi[a] = fh;
+  }
+}
+
+static double half_area(const float lo[3], const float hi[3]) {
+  double x = (double)hi[0] - lo[0], y = (double)hi[1] - lo[1], z = (double)hi[2] - lo[2];
+  return x * y + y * z + z * x;
+}
+
 // 4-wide nodes from the binary tree by the surface-area cost model, collapsed
 // optimally by dynamic programming over the binary tree (the collapse of
 // Ylitie, Karras and Laine, "Efficient incoherent ray traversal on GPUs through
-// compressed wide BVHs", HPG 2017, for four children):
+// compressed wide BVHs", HPG 2017, for four children), with each reference
+// leaf's primitives also offered one box each:
 //   D(n, k)  cheapest cover of the subtree of n by at most k roots, a root
-//            being a reference leaf or a wide node
+//            being a wide node, a reference leaf or one primitive
 //   wide(n)  = A(n) + min over k of D(left, k) + D(right, 4 - k)
-//   leaf(n)  = A(n) * C_PRIM * count(n)
-// with A the surface area of the reference node's float32 box, in units of one
-// 4-wide node step.  Every primitive stays in its reference leaf and every box
-// is a reference node's box, so the walks over these nodes visit a superset
-// of the boxes on the path to any primitive and return the reference's
-// closest hit.  On the 1M-triangle soup this takes 1.7% fewer node visits per
-// ray than opening the largest child first.  Leaves are not merged: merging
-// subtrees of up to 7 primitives (one run of the leaf order) at C_PRIM = 0.3
-// cost 4% on that soup.  Wide nodes are numbered breadth first until the first
-// TOP_NODES slots are taken (the block traversal launches copy into LDS), then
-// depth first.
-constexpr double C_PRIM = 0.5;  // one binary64 primitive test, in node steps
+//   leaf(n)  = min(A(n) * C_PRIM * count(n),       the leaf tested whole
+//                  A(n) + sum_p A(p) * C_PRIM)     a wide node of its primitives
+//   D(leaf, k >= count) may also be sum_p A(p) * C_PRIM: its primitives as roots
+// (the primitive boxes' terms weighted by SPLIT_MARGIN)
+// with A the surface area of a box, in units of one 4-wide node step.  A
+// primitive's own box is the box the reference gives a leaf of that primitive
+// alone (prim_box), so it holds every hit on it, as the reference leaf box
+// does.  Every box is such a box or a reference node's box, so the walks over
+// these nodes visit a superset of the boxes on the path to any primitive and
+// return the reference's closest hit.  Wide nodes are numbered breadth first
+// until the first TOP_NODES slots are taken (the block traversal launches copy
+// into LDS), then depth first.
+//
+// Measured on the 1M-triangle soup at 1920x1080x4 (binary64 / FP32 Mrays/s):
+// reference leaves only 1249 / 1351 (the collapse itself takes 1.7% fewer
+// node visits per ray than opening the largest child first; merging subtrees
+// of up to 7 primitives into one leaf cost 4%); every leaf of two or more
+// primitives a wide node of its primitives 1355 / 1439 (27.5 -> 4.4 triangle
+// tests and 167 -> 223 node slots per ray); the cost model above 1383 / 1488
+// with C_PRIM 0.5 (0.75, 1.0, 1.5: the same within 0.5%).  The caustic
+// scene's tessellated spheres, whose neighbouring triangles' boxes overlap,
+// lost 1-2% with it (FP32 3051 vs 3080, binary64 2682 vs 2748); counting the
+// primitive boxes at SPLIT_MARGIN = 1.5 times their cost keeps the caustic
+// scene where it was (3089 / 2743) and the soup at 1370 / 1479 (2 or 3:
+// 1360 / 1474, 1344 / 1457).
+#ifndef CT_C_PRIM
+#define CT_C_PRIM 0.5
+#endif
+constexpr double C_PRIM = CT_C_PRIM;  // one binary64 primitive test, in node steps
+#ifndef CT_SPLIT_MARGIN
+#define CT_SPLIT_MARGIN 1.5
+#endif
+constexpr double SPLIT_MARGIN = CT_SPLIT_MARGIN;  // weight of the primitive boxes' cost
 
 static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* words,
-                       std::vector<float>& out) {
+                       const float* prims, std::vector<float>& out) {
   out.clear();
   const uint32_t N = nodeCount;
   auto isLeaf = [&](uint32_t i) { return words[i] != 0xffffffffu; };
@@ -162,8 +219,11 @@ static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* w
   // Children have larger indices than their parent (skip links move forward),
   // so one pass from the last node to the first sees children before parents.
   std::vector<float> D((size_t)N * 4);    // D[4n + k-1] = D(n, k)
-  std::vector<uint8_t> how((size_t)N * 4);  // k = 1: 0 leaf, 1 wide; k > 1: roots given
-                                            // to the left child (0: as for k - 1)
+  std::vector<uint8_t> how((size_t)N * 4);  // interior, k = 1: 1 (wide); k > 1: roots given
+                                            // to the left child (0: as for k - 1); leaf,
+                                            // k = 1: 0 tested whole, 2 wide node of its
+                                            // primitives; k > 1: 3 primitives as roots
+                                            // (0: as for k - 1)
   std::vector<uint8_t> split(N);          // wide(n): slots given to the left child
   const float INF = __builtin_inff();
   for (int64_t ii = (int64_t)N - 1; ii >= 0; ii--) {
@@ -172,8 +232,28 @@ static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* w
     uint8_t* h = &how[(size_t)i * 4];
     const double a = area(i);
     if (isLeaf(i)) {
-      d[0] = d[1] = d[2] = d[3] = (float)(a * C_PRIM * (words[i] & 7));
+      const uint32_t c = words[i] & 7, start = words[i] >> 3;
+      d[0] = d[1] = d[2] = d[3] = (float)(a * C_PRIM * c);
       h[0] = h[1] = h[2] = h[3] = 0;
+      if (c >= 2) {
+        double sp = 0;
+        for (uint32_t j = 0; j < c; j++) {
+          float lo[3], hi[3];
+          prim_box(prims, start + j, lo, hi);
+          sp += half_area(lo, hi) * C_PRIM * SPLIT_MARGIN;
+        }
+        if ((float)(a + sp) < d[0]) {
+          d[0] = (float)(a + sp);
+          h[0] = 2;
+        }
+        for (int k = 2; k <= 4; k++) {
+          d[k - 1] = d[k - 2];
+          if (k >= (int)c && (float)sp < d[k - 1]) {
+            d[k - 1] = (float)sp;
+            h[k - 1] = 3;
+          }
+        }
+      }
       continue;
     }
     const uint32_t l = i + 1, r = rightOf(i);
@@ -203,8 +283,13 @@ static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* w
       }
     }
   }
-  // the roots covering subtree m with at most k of them
-  std::vector<uint32_t> roots;
+  // the roots covering subtree m with at most k of them: a reference node, or
+  // one primitive (prim >= 0) of a leaf whose primitives are roots
+  struct Root {
+    uint32_t n;
+    int64_t prim;
+  };
+  std::vector<Root> roots;
   std::vector<std::pair<uint32_t, int>> pend;
   auto expand = [&](uint32_t m, int k) {
     pend.clear();
@@ -214,7 +299,10 @@ static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* w
       pend.pop_back();
       while (kk > 1 && how[(size_t)n * 4 + kk - 1] == 0) kk--;
       if (kk == 1) {
-        roots.push_back(n);
+        roots.push_back({n, -1});
+      } else if (isLeaf(n)) {  // its primitives as roots
+        const uint32_t c = words[n] & 7, start = words[n] >> 3;
+        for (uint32_t j = 0; j < c; j++) roots.push_back({n, (int64_t)start + j});
       } else {
         int kl = how[(size_t)n * 4 + kk - 1];
         pend.push_back({rightOf(n), kk - kl});
@@ -242,6 +330,17 @@ static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* w
     rr[12 + k] = b[4]; rr[16 + k] = b[5]; rr[20 + k] = b[6];
     memcpy(&rr[24 + k], &c, 4);
   };
+  auto putPrim = [&](uint32_t slot, int k, uint32_t p) {  // one primitive: leaf word p*8 + 1
+    float lo[3], hi[3];
+    prim_box(prims, p, lo, hi);
+    float* rr = &out[(size_t)slot * 32];
+    for (int a = 0; a < 3; a++) {
+      rr[4 * a + k] = lo[a];
+      rr[12 + 4 * a + k] = hi[a];
+    }
+    int32_t c = -2 - (int32_t)((p << 3) | 1);
+    memcpy(&rr[24 + k], &c, 4);
+  };
   uint32_t root = alloc();
   if (isLeaf(0)) {
     put(root, 0, 0, -2 - (int32_t)words[0]);
@@ -263,8 +362,15 @@ static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* w
     expand(it.n2 + 1, split[it.n2]);
     expand(rightOf(it.n2), 4 - split[it.n2]);
     for (size_t k = 0; k < roots.size(); k++) {
-      uint32_t g = roots[k];
-      if (isLeaf(g)) {
+      const uint32_t g = roots[k].n;
+      if (roots[k].prim >= 0) {
+        putPrim(it.slot, (int)k, (uint32_t)roots[k].prim);
+      } else if (isLeaf(g) && how[(size_t)g * 4] == 2) {  // a wide node of its primitives
+        const uint32_t ws = alloc();
+        put(it.slot, (int)k, g, (int32_t)ws);
+        const uint32_t c = words[g] & 7, start = words[g] >> 3;
+        for (uint32_t j = 0; j < c; j++) putPrim(ws, (int)j, start + j);
+      } else if (isLeaf(g)) {
         put(it.slot, (int)k, g, -2 - (int32_t)words[g]);
       } else {
         uint32_t ws = alloc();
@@ -380,7 +486,7 @@ static int scene_create_impl(const float* prims, uint32_t primCount, const float
     int v = validate_scene(nodes, nodeCount, primCount, matCount, lights, lightCount, primObject,
                            words.data());
     if (v) return v;
-    build_wide(nodes, nodeCount, words.data(), wide);
+    build_wide(nodes, nodeCount, words.data(), prims, wide);
   } else {
     std::vector<float> hn((size_t)nodeCount * NODE_STRIDE);
     std::vector<int32_t> hl(lightCount);
@@ -390,12 +496,15 @@ static int scene_create_impl(const float* prims, uint32_t primCount, const float
       CT_HIP(hipMemcpyAsync(hl.data(), lights, hl.size() * 4, hipMemcpyDeviceToHost, stream));
     if (primCount)
       CT_HIP(hipMemcpyAsync(ho.data(), primObject, ho.size() * 4, hipMemcpyDeviceToHost, stream));
+    std::vector<float> hp((size_t)primCount * PRIM_STRIDE);
+    if (primCount)
+      CT_HIP(hipMemcpyAsync(hp.data(), prims, hp.size() * 4, hipMemcpyDeviceToHost, stream));
     CT_HIP(hipStreamSynchronize(stream));
     binaryExact = leaf_words(hn.data(), nodeCount, leafWordsIn, words);
     int v = validate_scene(hn.data(), nodeCount, primCount, matCount, hl.data(), lightCount,
                            ho.data(), words.data());
     if (v) return v;
-    build_wide(hn.data(), nodeCount, words.data(), wide);
+    build_wide(hn.data(), nodeCount, words.data(), hp.data(), wide);
   }
   quantize_wide(wide, qwide);
   // The array lengths PackedScene carries: max(1, n) records each (pack.ts:136-138, 111).
