@@ -1415,14 +1415,20 @@ constexpr size_t SPLIT_MIN_PATHS = size_t(1) << 20;
 
 // Rounds CT_PATHS_AUTO traces with k_paths: every scene of at most
 // PATHS_AUTO_MAX_PRIMS primitives, at most PATHS_AUTO_MAX_PATHS paths.
-// Measured against the wavefront (ms per frame, 256x256x16 / 1920x1080x4):
-// soups of 2k triangles 3.0x / 1.6x faster, 8k 1.9x / 1.6x, 16k 1.6x / 1.2x,
-// 64k 0.93x; the Cornell variant 2.9x at 256x256x16, 2.5x at 1024x1024x4;
-// the mirror room (deep specular paths, 2k triangles) 1.27x / 1.00x.  The
-// wavefront's per-bounce compaction wins back huge rounds: the 29 jobs on
+// Measured against the wavefront (ms per frame, 256x256x16 / 1920x1080x4,
+// sparse soups over the 4-wide tree with primitive boxes): 2k triangles
+// 3.4x / 1.4x faster, 8k 2.9x / 1.6x, 16k 2.3x / 1.5x, 32k 1.8x / 1.1x, 64k
+// 1.4x / 0.87x; the Cornell variant 2.9x at 256x256x16, 2.5x at 1024x1024x4;
+// the mirror room (deep specular paths, 2k triangles) 1.27x / 1.00x.  With
+// the limit at 32k instead of 16k the batching form's 64 jobs rendered one by
+// one take 7% less time and the serving stream gains 1-2% (64k: 10% and 0%).
+// The wavefront's per-bounce compaction wins back huge rounds: the 29 jobs on
 // scenes of 1k-16k triangles of the 64-scene batch (95M paths in one round)
 // take 46.7 ms in the wavefront, 49.8 ms per path.
-constexpr int PATHS_AUTO_MAX_PRIMS = 16384;
+#ifndef CT_PATHS_AUTO_MAX_PRIMS
+#define CT_PATHS_AUTO_MAX_PRIMS 32768
+#endif
+constexpr int PATHS_AUTO_MAX_PRIMS = CT_PATHS_AUTO_MAX_PRIMS;
 constexpr size_t PATHS_AUTO_MAX_PATHS = size_t(1) << 24;
 
 // The streams of a round of `paths` paths.  Profiling events are recorded on

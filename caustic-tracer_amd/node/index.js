@@ -209,7 +209,7 @@ function setPrecision(p) {
 
 /**
  * How renders are launched: 0 auto (one per-path launch for scenes of at most
- * 16384 primitives, the wavefront above), 1 the wavefront, 2 per path.  The
+ * 32768 primitives, the wavefront above), 1 the wavefront, 2 per path.  The
  * images are the same under each.
  */
 function setPathKernel(mode) {

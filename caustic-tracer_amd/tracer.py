@@ -331,7 +331,7 @@ class Tracer:
     def set_path_kernel(self, mode):
         """PATHS_WAVEFRONT (per-bounce launches), PATHS_PER_PATH (one launch, a
         lane per path at a time) or PATHS_AUTO (per-path for rounds of at most
-        2^24 paths over scenes of at most 16384 primitives); the binary64
+        2^24 paths over scenes of at most 32768 primitives); the binary64
         results are identical under each."""
         check(lib().ct_tracer_set_path_kernel(self.handle, int(mode)), "set_path_kernel")
 

@@ -263,7 +263,7 @@ int ct_tracer_set_overlap(ct_tracer *tracer, int32_t enable);
  * 290-420 for the closest hit and the NEE test, the same shading), which
  * suits small scenes, where a bounce's launches cost more than the work they
  * carry; CT_PATHS_AUTO (default) takes the per-path launch for rounds of at
- * most 2^24 paths whose scenes have at most 16384 primitives each (and, not
+ * most 2^24 paths whose scenes have at most 32768 primitives each (and, not
  * in graph mode, packs such jobs into rounds of their own).  The per-path launch is
  * persistent (a lane takes the next path when its own ends); its walks are
  * the 4-wide ones (LDS stack, leaves tested as reached), or the skip-link

@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 def _jobs(ct, O):
     sc1 = scenes.cornell_box()
     sc2 = scenes.night_emissives()
-    sc3 = ct.synth.soup_scene(20000, n_lights=16, n_materials=20, seed=9)
+    sc3 = ct.synth.soup_scene(40000, n_lights=16, n_materials=20, seed=9)
     out = []
     for sc, (w, h, spp, seed, depth) in ((sc1, (64, 48, 4, 1, 8)), (sc2, (40, 30, 2, 7, 5)),
                                          (sc3, (48, 32, 3, 2, 6))):

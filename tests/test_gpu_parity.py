@@ -409,6 +409,7 @@ def test_render_soup_all_walks(ct, O, walk, stack):
     o = O.render_image(ps, fr, w, h, 2, seed=3, max_bounces=6)
     tr = ct.Tracer(walk=walk)
     tr.set_traversal(walk, stack)
+    tr.set_path_kernel(ct.PATHS_WAVEFRONT)  # the walks of the wavefront launches
     tr.set_counting(True)
     d = ct.DeviceBuffer.empty((w * h * 3,), np.float32)
     tr.render_image(ps, fr, {"width": w, "height": h, "spp": 2, "seed": 3, "maxBounces": 6}, d)

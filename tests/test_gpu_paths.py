@@ -5,7 +5,7 @@ Both forms run the same binary64 arithmetic in the reference's order, so the
 images, radiance values, RNG states and ray counts must be identical bit for
 bit under every option, and within 1e-4 per pixel of the oracle (in practice
 equal); CT_PATHS_AUTO takes the per-path launch for rounds of at most 2^24
-paths over scenes of at most 16384 primitives, and the wavefront otherwise."""
+paths over scenes of at most 32768 primitives, and the wavefront otherwise."""
 import numpy as np
 import pytest
 
@@ -105,8 +105,8 @@ def test_two_path_groups_and_several_rounds(ct, O):
 
 
 def test_auto_picks_per_path_for_small_scenes(ct, O):
-    small = ct.synth.soup_scene(16300, n_lights=16, n_materials=8, seed=3)  # 16316 primitives
-    large = ct.synth.soup_scene(16400, n_lights=16, n_materials=8, seed=3)  # 16416 primitives
+    small = ct.synth.soup_scene(32700, n_lights=16, n_materials=8, seed=3)  # 32716 primitives
+    large = ct.synth.soup_scene(32800, n_lights=16, n_materials=8, seed=3)  # 32816 primitives
     for sc, (w, h), want in ((small, (48, 32), 1), (large, (48, 32), 0),
                              (small, (1024, 1024), 1)):  # 3 x 2^20 paths, two path groups
         ps = ct.packScene(sc)
@@ -277,12 +277,12 @@ def test_per_path_skip_link_walk(ct, O, name):
 
 @pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
 def test_mixed_batch_packs_each_form_in_its_own_rounds(ct, O, graph):
-    """Small scenes and a 20k-triangle soup in one batch: under CT_PATHS_AUTO
+    """Small scenes and a 40k-triangle soup in one batch: under CT_PATHS_AUTO
     the small scenes' jobs run a per-path round before the soup's wavefront
     round (eager; a graph-mode tracer keeps one wavefront round), and forced
     per-path everything runs per-path.  Every job's image equals the
     wavefront batch's and the job rendered alone."""
-    soup = ct.synth.soup_scene(20000, n_lights=16, n_materials=20, seed=9)
+    soup = ct.synth.soup_scene(40000, n_lights=16, n_materials=20, seed=9)
     specs = [(soup, 48, 32, 3, 2, 6), (scenes.cornell_box(), 40, 30, 3, 1, 8),
              (scenes.night_emissives(), 32, 24, 2, 7, 5)]
     packed = [O.pack_scene(sc) for sc, *_ in specs]
