@@ -5,6 +5,7 @@
 #include <cmath>
 
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ct_internal.h"
@@ -25,6 +26,24 @@ int check_hip(hipError_t e, const char* what) {
 }
 
 static size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+// f(i) for i in [0, n), in contiguous chunks on up to 16 host threads (the
+// upload's per-primitive and per-node passes, each i independent of the rest).
+template <typename F>
+static void parallel_for(size_t n, F f) {
+  const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nt = std::min<size_t>({hw, 16, (n + 4095) / 4096});
+  if (nt <= 1) {
+    for (size_t i = 0; i < n; i++) f(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (size_t t = 0; t < nt; t++)
+    th.emplace_back([&, t] {
+      for (size_t i = n * t / nt, e = n * (t + 1) / nt; i < e; i++) f(i);
+    });
+  for (auto& x : th) x.join();
+}
 
 }  // namespace ct
 
@@ -206,9 +225,13 @@ constexpr double C_PRIM = CT_C_PRIM;  // one binary64 primitive test, in node st
 constexpr double SPLIT_MARGIN = CT_SPLIT_MARGIN;  // weight of the primitive boxes' cost
 
 static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* words,
-                       const float* prims, std::vector<float>& out) {
+                       const float* prims, uint32_t primCount, std::vector<float>& out) {
   out.clear();
   const uint32_t N = nodeCount;
+  std::vector<float> pbox((size_t)primCount * 6);  // prim_box of each primitive: lo, hi
+  parallel_for(primCount, [&](size_t p) {
+    prim_box(prims, (uint32_t)p, &pbox[p * 6], &pbox[p * 6 + 3]);
+  });
   auto isLeaf = [&](uint32_t i) { return words[i] != 0xffffffffu; };
   auto rightOf = [&](uint32_t i) { return (uint32_t)nodes[(size_t)(i + 1) * NODE_STRIDE + 3]; };
   auto area = [&](uint32_t i) {
@@ -238,9 +261,8 @@ static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* w
       if (c >= 2) {
         double sp = 0;
         for (uint32_t j = 0; j < c; j++) {
-          float lo[3], hi[3];
-          prim_box(prims, start + j, lo, hi);
-          sp += half_area(lo, hi) * C_PRIM * SPLIT_MARGIN;
+          const float* b = &pbox[(size_t)(start + j) * 6];
+          sp += half_area(b, b + 3) * C_PRIM * SPLIT_MARGIN;
         }
         if ((float)(a + sp) < d[0]) {
           d[0] = (float)(a + sp);
@@ -331,12 +353,11 @@ static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* w
     memcpy(&rr[24 + k], &c, 4);
   };
   auto putPrim = [&](uint32_t slot, int k, uint32_t p) {  // one primitive: leaf word p*8 + 1
-    float lo[3], hi[3];
-    prim_box(prims, p, lo, hi);
+    const float* b = &pbox[(size_t)p * 6];
     float* rr = &out[(size_t)slot * 32];
     for (int a = 0; a < 3; a++) {
-      rr[4 * a + k] = lo[a];
-      rr[12 + 4 * a + k] = hi[a];
+      rr[4 * a + k] = b[a];
+      rr[12 + 4 * a + k] = b[3 + a];
     }
     int32_t c = -2 - (int32_t)((p << 3) | 1);
     memcpy(&rr[24 + k], &c, 4);
@@ -390,6 +411,21 @@ static bool exact_diff(float a, float b, double& d) {
   return err == 0.0;
 }
 
+// Smallest e in [-126, 127] with 255 * 2^e >= ext (127 when there is none, -126
+// for ext <= 0 or NaN), as a search upwards from -126 finds it: frexp gives the
+// candidate and the exact comparisons (255 * 2^e is exact in binary64) settle
+// it in a step or two instead of up to 253.
+static int scale_exponent(double ext) {
+  if (!(ext > 0)) return -126;
+  if (ext == INFINITY) return 127;
+  int k;
+  std::frexp(ext / 255.0, &k);
+  int e = std::max(-126, std::min(127, k));
+  while (e > -126 && 255.0 * std::ldexp(1.0, e - 1) >= ext) e--;
+  while (e < 127 && 255.0 * std::ldexp(1.0, e) < ext) e++;
+  return e;
+}
+
 // 8-bit parent-relative quantization of the 4-wide nodes (precision form).
 // Per node and axis the origin is the children's minimum and the scale the
 // smallest power of two with 255*scale covering the children's extent; child
@@ -400,7 +436,7 @@ static bool exact_diff(float a, float b, double& d) {
 static void quantize_wide(const std::vector<float>& wide, std::vector<float>& out) {
   size_t nw = wide.size() / 32;
   out.assign(nw * 16, 0.0f);
-  for (size_t n = 0; n < nw; n++) {
+  parallel_for(nw, [&](size_t n) {
     const float* w = &wide[n * 32];
     float* q = &out[n * 16];
     int32_t codes[4];
@@ -422,8 +458,7 @@ static void quantize_wide(const std::vector<float>& wide, std::vector<float>& ou
         origin[a] = lo[a];
         double ext;
         bool ok = exact_diff(hi[a], lo[a], ext);
-        int e = -126;
-        while (e < 127 && 255.0 * std::ldexp(1.0, e) < ext) e++;
+        int e = scale_exponent(ext);
         if (!ok && e < 127) e++;  // inexact extent: one more doubling of margin
         scale[a] = std::ldexp(1.0f, e);
       }
@@ -457,7 +492,7 @@ static void quantize_wide(const std::vector<float>& wide, std::vector<float>& ou
     memcpy(&q[10], &qhi[1], 4);
     memcpy(&q[11], &qhi[2], 4);
     memcpy(&q[12], codes, 16);
-  }
+  });
 }
 
 static int scene_create_impl(const float* prims, uint32_t primCount, const float* nodes,
@@ -486,7 +521,7 @@ static int scene_create_impl(const float* prims, uint32_t primCount, const float
     int v = validate_scene(nodes, nodeCount, primCount, matCount, lights, lightCount, primObject,
                            words.data());
     if (v) return v;
-    build_wide(nodes, nodeCount, words.data(), prims, wide);
+    build_wide(nodes, nodeCount, words.data(), prims, primCount, wide);
   } else {
     std::vector<float> hn((size_t)nodeCount * NODE_STRIDE);
     std::vector<int32_t> hl(lightCount);
@@ -504,7 +539,7 @@ static int scene_create_impl(const float* prims, uint32_t primCount, const float
     int v = validate_scene(hn.data(), nodeCount, primCount, matCount, hl.data(), lightCount,
                            ho.data(), words.data());
     if (v) return v;
-    build_wide(hn.data(), nodeCount, words.data(), hp.data(), wide);
+    build_wide(hn.data(), nodeCount, words.data(), hp.data(), primCount, wide);
   }
   quantize_wide(wide, qwide);
   // The array lengths PackedScene carries: max(1, n) records each (pack.ts:136-138, 111).
