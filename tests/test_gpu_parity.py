@@ -148,6 +148,49 @@ def test_wide_walk_adversarial_rays(ct, O, off):
     assert (o["prim"] >= 0).mean() > 0.7
 
 
+def test_scene_from_device_arrays(ct, O):
+    """ct_scene_create_device: the packed arrays handed over in device memory
+    give the scene the host arrays give (its 4-wide tree, primitive boxes
+    included, is derived from a host copy): closest hits equal to the host
+    scene's and the oracle's, the same image."""
+    import ctypes as C
+    A = ct._capi
+    sc = scenes.random_scene(O.Rng(77), 50, 3000)
+    sc["objects"].append(scenes.obj("light", scenes.sphere((0, 12, 0), 2.0),
+                                    scenes.light(scenes.rgb(1, 1, 1), 10)))
+    ps = O.pack_scene(sc)
+    host = ct.DeviceScene(ps)
+    arrays = [ct.DeviceBuffer.from_array(np.ascontiguousarray(a, t)) for a, t in
+              ((ps.prims, np.float32), (ps.nodes, np.float32), (ps.mats, np.float32),
+               (ps.lights, np.int32), (ps.primObject, np.uint32))]
+    sky = A.Sky()
+    h = C.c_void_p()
+    A.check(A.lib().ct_scene_create_device(
+        arrays[0].ptr, ps.primCount, arrays[1].ptr, ps.nodeCount, arrays[2].ptr, ps.matCount,
+        arrays[3].ptr, len(ps.lights), arrays[4].ptr, C.byref(sky), None, C.byref(h)),
+        "ct_scene_create_device")
+    dev = object.__new__(ct.DeviceScene)
+    dev.handle, dev.primCount, dev.nodeCount = h, ps.primCount, ps.nodeCount
+    rays = _aimed_rays(ps, 13, 4000)
+    gd = ct.closest_hit(dev, rays, walk=ct.WALK_WIDE)
+    gh = ct.closest_hit(host, rays, walk=ct.WALK_WIDE)
+    o = O.closest_batch(ps, rays, walk=O.WALK_ORDERED)
+    assert np.array_equal(gd["prim"], gh["prim"]) and np.array_equal(gd["prim"], o["prim"])
+    assert np.array_equal(gd["t"], o["t"])
+    fr = O.camera_frame(sc["camera"], 1.5)
+    opts = {"width": 48, "height": 32, "spp": 2, "seed": 4, "maxBounces": 6}
+    imgs = []
+    for s_ in (dev, host):
+        d = ct.DeviceBuffer.empty((48 * 32 * 3,), np.float32)
+        tr = ct.Tracer()
+        tr.set_path_kernel(ct.PATHS_WAVEFRONT)
+        tr.render_image(s_, fr, opts, d)
+        imgs.append(d.numpy())
+    assert np.array_equal(imgs[0].view(np.uint32), imgs[1].view(np.uint32))
+    assert (imgs[0] > 0).mean() > 0.1
+    dev.close()
+
+
 @pytest.mark.parametrize("preset", ["cornell", "mirror"])
 def test_exact_ties_follow_the_reference_walks(ct, O, preset):
     """On exact-t ties the ordered walk (intersect.ts:202-282) keeps the
