@@ -177,6 +177,19 @@ static void prim_box(const float* prims, uint32_t p, float lo[3], float hi[3]) {
   }
 }
 
+// Union of the boxes of primitives p .. p + len - 1.
+static void run_box(const float* pbox, uint32_t p, uint32_t len, float lo[3], float hi[3]) {
+  for (int a = 0; a < 3; a++) {
+    lo[a] = pbox[(size_t)p * 6 + a];
+    hi[a] = pbox[(size_t)p * 6 + 3 + a];
+  }
+  for (uint32_t j = 1; j < len; j++)
+    for (int a = 0; a < 3; a++) {
+      lo[a] = std::min(lo[a], pbox[(size_t)(p + j) * 6 + a]);
+      hi[a] = std::max(hi[a], pbox[(size_t)(p + j) * 6 + 3 + a]);
+    }
+}
+
 static double half_area(const float lo[3], const float hi[3]) {
   double x = (double)hi[0] - lo[0], y = (double)hi[1] - lo[1], z = (double)hi[2] - lo[2];
   return x * y + y * z + z * x;
@@ -186,20 +199,23 @@ static double half_area(const float lo[3], const float hi[3]) {
 // optimally by dynamic programming over the binary tree (the collapse of
 // Ylitie, Karras and Laine, "Efficient incoherent ray traversal on GPUs through
 // compressed wide BVHs", HPG 2017, for four children), with each reference
-// leaf's primitives also offered one box each:
+// leaf's primitives also offered in runs of consecutive ones, a box each:
 //   D(n, k)  cheapest cover of the subtree of n by at most k roots, a root
-//            being a wide node, a reference leaf or one primitive
+//            being a wide node, a reference leaf or a run of its primitives
 //   wide(n)  = A(n) + min over k of D(left, k) + D(right, 4 - k)
-//   leaf(n)  = min(A(n) * C_PRIM * count(n),       the leaf tested whole
-//                  A(n) + sum_p A(p) * C_PRIM)     a wide node of its primitives
-//   D(leaf, k >= count) may also be sum_p A(p) * C_PRIM: its primitives as roots
-// (the primitive boxes' terms weighted by SPLIT_MARGIN)
-// with A the surface area of a box, in units of one 4-wide node step.  A
-// primitive's own box is the box the reference gives a leaf of that primitive
-// alone (prim_box), so it holds every hit on it, as the reference leaf box
-// does.  Every box is such a box or a reference node's box, so the walks over
-// these nodes visit a superset of the boxes on the path to any primitive and
-// return the reference's closest hit.  Wide nodes are numbered breadth first
+//   R(n, r)  cheapest split of leaf n into r runs: sum over the runs of
+//            A(run) * C_PRIM * length, A(run) the union of its primitives' boxes
+//   leaf(n)  = min(A(n) * C_PRIM * count(n),          the leaf tested whole
+//                  A(n) + min over r <= 4 of R(n, r))  a wide node of its runs
+//   D(leaf, k) may also be R(n, r) for 2 <= r <= k:    its runs as roots
+// (the runs' terms weighted by SPLIT_MARGIN) with A the surface area of a
+// box, in units of one 4-wide node step.  A primitive's own box is the box
+// the reference gives a leaf of that primitive alone (prim_box), so it holds
+// every hit on it, as the reference leaf box does, and a run's box holds its
+// primitives' boxes.  Every box is such a box or a reference node's box, so
+// the walks over these nodes visit a superset of the boxes on the path to any
+// primitive and return the reference's closest hit.  A wide node holds at most
+// four runs, whatever the leaf's count (leaf words allow seven).  Wide nodes are numbered breadth first
 // until the first TOP_NODES slots are taken (the block traversal launches copy
 // into LDS), then depth first.
 //
@@ -214,7 +230,9 @@ static double half_area(const float lo[3], const float hi[3]) {
 // lost 1-2% with it (FP32 3051 vs 3080, binary64 2682 vs 2748); counting the
 // primitive boxes at SPLIT_MARGIN = 1.5 times their cost keeps the caustic
 // scene where it was (3089 / 2743) and the soup at 1370 / 1479 (2 or 3:
-// 1360 / 1474, 1344 / 1457).
+// 1360 / 1474, 1344 / 1457).  Runs of several primitives, beside single ones:
+// soup 1381 / 1499 against 1368 / 1480, caustic scene 2742 / 3091 against
+// 2738 / 3086 (314k instead of 340k wide nodes on the soup).
 #ifndef CT_C_PRIM
 #define CT_C_PRIM 0.5
 #endif
@@ -223,6 +241,10 @@ constexpr double C_PRIM = CT_C_PRIM;  // one binary64 primitive test, in node st
 #define CT_SPLIT_MARGIN 1.5
 #endif
 constexpr double SPLIT_MARGIN = CT_SPLIT_MARGIN;  // weight of the primitive boxes' cost
+#ifndef CT_LEAF_RUNS
+#define CT_LEAF_RUNS 1
+#endif
+constexpr bool LEAF_RUNS = CT_LEAF_RUNS;  // runs of several primitives, not only single ones
 
 static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* words,
                        const float* prims, uint32_t primCount, std::vector<float>& out) {
@@ -244,9 +266,9 @@ static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* w
   std::vector<float> D((size_t)N * 4);    // D[4n + k-1] = D(n, k)
   std::vector<uint8_t> how((size_t)N * 4);  // interior, k = 1: 1 (wide); k > 1: roots given
                                             // to the left child (0: as for k - 1); leaf,
-                                            // k = 1: 0 tested whole, 2 wide node of its
-                                            // primitives; k > 1: 3 primitives as roots
-                                            // (0: as for k - 1)
+                                            // k = 1: 0 tested whole, 2 wide node of runs;
+                                            // k > 1: 3 runs as roots (0: as for k - 1)
+  std::vector<uint8_t> cuts((size_t)N * 4);  // leaf runs: bit j = a run ends at primitive j
   std::vector<uint8_t> split(N);          // wide(n): slots given to the left child
   const float INF = __builtin_inff();
   for (int64_t ii = (int64_t)N - 1; ii >= 0; ii--) {
@@ -259,20 +281,40 @@ static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* w
       d[0] = d[1] = d[2] = d[3] = (float)(a * C_PRIM * c);
       h[0] = h[1] = h[2] = h[3] = 0;
       if (c >= 2) {
-        double sp = 0;
-        for (uint32_t j = 0; j < c; j++) {
-          const float* b = &pbox[(size_t)(start + j) * 6];
-          sp += half_area(b, b + 3) * C_PRIM * SPLIT_MARGIN;
+        // the leaf's primitives in 2-4 runs of consecutive ones (a box each,
+        // the union of its primitives' boxes; leaf word start*8 + length):
+        // the cheapest partition for each number of runs
+        double best[5] = {INF, INF, INF, INF, INF};
+        uint8_t bestCuts[5] = {0, 0, 0, 0, 0};
+        const uint32_t last = 1u << (c - 1);  // bit c-1: the last run ends at the last primitive
+        for (uint32_t m = last; m < 2 * last; m++) {
+          const int runs = __builtin_popcount(m);
+          if (runs < 2 || runs > 4 || (!LEAF_RUNS && runs != (int)c)) continue;
+          double cost = 0;
+          for (uint32_t j = 0, rs = 0; j < c; j++) {
+            if (!((m >> j) & 1)) continue;
+            float lo[3], hi[3];
+            run_box(pbox.data(), start + rs, j - rs + 1, lo, hi);
+            cost += half_area(lo, hi) * C_PRIM * SPLIT_MARGIN * (j - rs + 1);
+            rs = j + 1;
+          }
+          if (cost < best[runs]) {
+            best[runs] = cost;
+            bestCuts[runs] = (uint8_t)m;
+          }
         }
-        if ((float)(a + sp) < d[0]) {
-          d[0] = (float)(a + sp);
-          h[0] = 2;
-        }
-        for (int k = 2; k <= 4; k++) {
+        for (int r = 2; r <= 4; r++)  // a wide node of the runs
+          if ((float)(a + best[r]) < d[0]) {
+            d[0] = (float)(a + best[r]);
+            h[0] = 2;
+            cuts[(size_t)i * 4] = bestCuts[r];
+          }
+        for (int k = 2; k <= 4; k++) {  // the runs as roots
           d[k - 1] = d[k - 2];
-          if (k >= (int)c && (float)sp < d[k - 1]) {
-            d[k - 1] = (float)sp;
+          if ((float)best[k] < d[k - 1]) {
+            d[k - 1] = (float)best[k];
             h[k - 1] = 3;
+            cuts[(size_t)i * 4 + k - 1] = bestCuts[k];
           }
         }
       }
@@ -305,11 +347,11 @@ static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* w
       }
     }
   }
-  // the roots covering subtree m with at most k of them: a reference node, or
-  // one primitive (prim >= 0) of a leaf whose primitives are roots
+  // the roots covering subtree m with at most k of them: a reference node
+  // (count 0), or a run of `count` primitives from `start` of a leaf whose runs
+  // are roots
   struct Root {
-    uint32_t n;
-    int64_t prim;
+    uint32_t n, start, count;
   };
   std::vector<Root> roots;
   std::vector<std::pair<uint32_t, int>> pend;
@@ -321,10 +363,14 @@ static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* w
       pend.pop_back();
       while (kk > 1 && how[(size_t)n * 4 + kk - 1] == 0) kk--;
       if (kk == 1) {
-        roots.push_back({n, -1});
-      } else if (isLeaf(n)) {  // its primitives as roots
-        const uint32_t c = words[n] & 7, start = words[n] >> 3;
-        for (uint32_t j = 0; j < c; j++) roots.push_back({n, (int64_t)start + j});
+        roots.push_back({n, 0, 0});
+      } else if (isLeaf(n)) {  // its runs as roots
+        const uint32_t c = words[n] & 7, start = words[n] >> 3, m = cuts[(size_t)n * 4 + kk - 1];
+        for (uint32_t j = 0, rs = 0; j < c; j++)
+          if ((m >> j) & 1) {
+            roots.push_back({n, start + rs, j - rs + 1});
+            rs = j + 1;
+          }
       } else {
         int kl = how[(size_t)n * 4 + kk - 1];
         pend.push_back({rightOf(n), kk - kl});
@@ -352,14 +398,15 @@ static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* w
     rr[12 + k] = b[4]; rr[16 + k] = b[5]; rr[20 + k] = b[6];
     memcpy(&rr[24 + k], &c, 4);
   };
-  auto putPrim = [&](uint32_t slot, int k, uint32_t p) {  // one primitive: leaf word p*8 + 1
-    const float* b = &pbox[(size_t)p * 6];
+  auto putRun = [&](uint32_t slot, int k, uint32_t p, uint32_t len) {  // leaf word p*8 + len
+    float lo[3], hi[3];
+    run_box(pbox.data(), p, len, lo, hi);
     float* rr = &out[(size_t)slot * 32];
     for (int a = 0; a < 3; a++) {
-      rr[4 * a + k] = b[a];
-      rr[12 + 4 * a + k] = b[3 + a];
+      rr[4 * a + k] = lo[a];
+      rr[12 + 4 * a + k] = hi[a];
     }
-    int32_t c = -2 - (int32_t)((p << 3) | 1);
+    int32_t c = -2 - (int32_t)((p << 3) | len);
     memcpy(&rr[24 + k], &c, 4);
   };
   uint32_t root = alloc();
@@ -384,13 +431,18 @@ static void build_wide(const float* nodes, uint32_t nodeCount, const uint32_t* w
     expand(rightOf(it.n2), 4 - split[it.n2]);
     for (size_t k = 0; k < roots.size(); k++) {
       const uint32_t g = roots[k].n;
-      if (roots[k].prim >= 0) {
-        putPrim(it.slot, (int)k, (uint32_t)roots[k].prim);
-      } else if (isLeaf(g) && how[(size_t)g * 4] == 2) {  // a wide node of its primitives
+      if (roots[k].count) {
+        putRun(it.slot, (int)k, roots[k].start, roots[k].count);
+      } else if (isLeaf(g) && how[(size_t)g * 4] == 2) {  // a wide node of its runs
         const uint32_t ws = alloc();
         put(it.slot, (int)k, g, (int32_t)ws);
-        const uint32_t c = words[g] & 7, start = words[g] >> 3;
-        for (uint32_t j = 0; j < c; j++) putPrim(ws, (int)j, start + j);
+        const uint32_t c = words[g] & 7, start = words[g] >> 3, m = cuts[(size_t)g * 4];
+        int slot = 0;
+        for (uint32_t j = 0, rs = 0; j < c; j++)
+          if ((m >> j) & 1) {
+            putRun(ws, slot++, start + rs, j - rs + 1);
+            rs = j + 1;
+          }
       } else if (isLeaf(g)) {
         put(it.slot, (int)k, g, -2 - (int32_t)words[g]);
       } else {

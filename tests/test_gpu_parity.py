@@ -191,6 +191,43 @@ def test_scene_from_device_arrays(ct, O):
     dev.close()
 
 
+def test_leaves_of_up_to_seven_primitives(ct, O):
+    """Leaf words hold up to seven primitives (bvh.ts:60-68); the reference
+    builder stops at four, a packed tree from elsewhere need not.  A tree
+    whose leaf holds six scattered primitives: the 4-wide tree derived at
+    upload, which may give that leaf's primitives up to four boxes of runs,
+    returns the reference walks' closest hits."""
+    rng = np.random.default_rng(12)
+    grey = scenes.lambert(scenes.rgb(0.5, 0.5, 0.5))
+    objs = []
+    for i in range(7):
+        c = rng.uniform(-2, 2, 3)
+        a, b, d = (tuple(c + rng.uniform(-0.15, 0.15, 3)) for _ in range(3))
+        objs.append(scenes.obj(f"t{i}", {"kind": "triangle", "a": a, "b": b, "c": d}, grey))
+    base = O.pack_scene({"objects": objs, "sky": {"kind": "none"}})
+    prims = np.asarray(base.prims, np.float32).copy()
+    pr = prims.reshape(-1, 16)[:7].astype(np.float64)
+    nodes = np.zeros(3 * 8, np.float32)
+    for n, ix, skip, leaf in ((0, range(7), 3, -1), (1, range(6), 2, 0 * 8 + 6),
+                              (2, [6], 3, 6 * 8 + 1)):
+        v = np.concatenate([pr[list(ix), k:k + 3] for k in (0, 4, 8)])
+        nodes[n * 8:n * 8 + 3] = v.min(0) - 1e-3
+        nodes[n * 8 + 3] = skip
+        nodes[n * 8 + 4:n * 8 + 7] = v.max(0) + 1e-3
+        nodes[n * 8 + 7] = leaf
+    ps = O.Packed(prims, nodes, 3, np.asarray(base.mats, np.float32).copy(),
+                  np.asarray(base.lights, np.int32).copy(),
+                  np.asarray(base.primObject, np.uint32).copy(), np.asarray(base.order).copy(),
+                  {"kind": "none"})
+    rays = _aimed_rays(ps, 21, 3000)
+    for walk, owalk in ((ct.WALK_WIDE, O.WALK_ORDERED), (ct.WALK_ORDERED, O.WALK_ORDERED),
+                        (ct.WALK_STACKLESS, O.WALK_STACKLESS)):
+        g = ct.closest_hit(ps, rays, walk=walk)
+        o = O.closest_batch(ps, rays, walk=owalk)
+        assert np.array_equal(g["prim"], o["prim"]) and np.array_equal(g["t"], o["t"]), walk
+    assert (o["prim"] >= 0).mean() > 0.3
+
+
 @pytest.mark.parametrize("preset", ["cornell", "mirror"])
 def test_exact_ties_follow_the_reference_walks(ct, O, preset):
     """On exact-t ties the ordered walk (intersect.ts:202-282) keeps the
